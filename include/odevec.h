@@ -1,0 +1,152 @@
+/*
+ * libodevec -- C ABI of the MI355X-native OdeVec BDF integrator.
+ *
+ * The reference's host interface is the Fortran module `odevec_main`
+ * (src/odevec.F90): a `type(odevec)` handle with caller-set tolerance fields,
+ * `InitOdevec(this)`, `SolveODE(this,time,dt,t_stop,y,GetRHS,GetJac,GetLU,Mask)`
+ * and `CloseOdevec(this)`.  Each entry point below names the reference
+ * interface it replaces.  All sizes are plain integers, all arrays plain
+ * pointers; no other library's types appear.
+ *
+ * State layout: y is the reference's Fortran y(nvector,neq) -- neq planes of
+ * ncells doubles, cells fastest: y[j*ncells + i].  Parameters likewise:
+ * params[p*ncells + i].
+ *
+ * Error handling: every call returns ODEVEC_OK (0) or a negative code;
+ * odevec_last_error() gives the message of the calling thread's last error.
+ * Per-cell solver failures (the reference prints and STOPs,
+ * src/odevec.F90:380-389) are reported as per-cell status codes and make the
+ * call return ODEVEC_ESOLVE; with opts.abort_on_error the library prints and
+ * aborts instead, like the reference.
+ *
+ * Threading: a handle owns its device buffers and HIP stream; use one host
+ * thread per handle.  There is no global mutable state besides the
+ * thread-local error message and the read-only network registry.
+ */
+#ifndef ODEVEC_H
+#define ODEVEC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ODEVEC_ABI_VERSION 1
+
+/* return codes */
+#define ODEVEC_OK 0
+#define ODEVEC_EINVAL (-1)    /* bad argument */
+#define ODEVEC_ENOMEM (-2)    /* device or host allocation failed */
+#define ODEVEC_EDEVICE (-3)   /* HIP runtime error */
+#define ODEVEC_ENOTFOUND (-4) /* unknown network id */
+#define ODEVEC_ESOLVE (-5)    /* at least one cell did not finish with status OK/SKIPPED */
+
+/* per-cell status codes */
+#define ODEVEC_STATUS_OK 0
+#define ODEVEC_STATUS_DT_MIN 1     /* step below MinimalTimestep (src/odevec.F90:272-275, 387-390) */
+#define ODEVEC_STATUS_CONV_FAIL 2  /* > 10 consecutive convergence failures (src/odevec.F90:380-383) */
+#define ODEVEC_STATUS_ORDER 3      /* order/step mismatch (src/odevec.F90:584-587, 595-598) */
+#define ODEVEC_STATUS_MAX_STEPS 4  /* attempt budget exhausted (no reference counterpart) */
+#define ODEVEC_STATUS_SKIPPED 5    /* masked out or t_stop <= t0: y unchanged */
+
+/* precision forms */
+#define ODEVEC_FP64 0   /* everything FP64 */
+#define ODEVEC_MIXED 1  /* P, LU factors and triangular solves FP32; residual, history, norms, control FP64 */
+
+/* execution modes */
+#define ODEVEC_PERCELL 0     /* one lane per cell, per-cell step control (reference at nvector=1) */
+#define ODEVEC_LOCKSTEP 1    /* groups of `group` cells share dt/order/verdicts (reference at nvector=group) */
+#define ODEVEC_PERSISTENT 2  /* per-cell control, persistent grid with lane refill (work compaction) */
+
+typedef struct odevec_s odevec_t;
+
+typedef struct {
+  int32_t precision;      /* ODEVEC_FP64 | ODEVEC_MIXED */
+  int32_t mode;           /* ODEVEC_PERCELL | ODEVEC_LOCKSTEP | ODEVEC_PERSISTENT */
+  int32_t group;          /* lockstep group size, 1..1024; 0 = min(nvector, 1024) */
+  int32_t block;          /* threads per workgroup (percell/persistent), 0 = 256 */
+  int32_t grid;           /* persistent: workgroups, 0 = fill the device */
+  int32_t max_attempts;   /* per-cell step-attempt budget, 0 = 1000000 */
+  int32_t device_ptrs;    /* 1: array arguments are device pointers (no copies) */
+  int32_t abort_on_error; /* 1: print and abort on a failed cell, like the reference's STOP */
+} odevec_opts_t;
+
+typedef struct {
+  int64_t attempts, steps, newton, jac, errfail, convfail, stepsel, qsum; /* summed over cells */
+  int64_t cells_ok, cells_failed, cells_skipped;
+  double flops64, flops32; /* algorithmic flops of the solve (op-count model) */
+  double kernel_ms;        /* device time of the solve kernel(s) */
+} odevec_stats_t;
+
+typedef struct {
+  int32_t neq, nnz, nrate, nparam, rate_kind, tgas_index;
+  int32_t flops_rhs, flops_jac, flops_lu, flops_solve, flops_lu_r, flops_solve_r;
+} odevec_network_info_t;
+
+typedef struct {
+  int32_t vgprs, spill_bytes, blocks_per_cu, block;
+} odevec_kernel_info_t;
+
+/* Library / registry ------------------------------------------------------ */
+int32_t odevec_abi_version(void);
+const char* odevec_last_error(void);
+int32_t odevec_network_count(void);
+const char* odevec_network_name(int32_t index);           /* e.g. "PRIMORDIAL/CMK/CSC" */
+int32_t odevec_network_info(const char* network, odevec_network_info_t* info);
+int32_t odevec_kernel_info(const char* network, int32_t mode, int32_t precision, int32_t block,
+                           odevec_kernel_info_t* info);
+void odevec_default_opts(odevec_opts_t* opts);
+
+/* Replaces InitOdevec(this) (src/odevec.F90:94-154): a handle for `nvector`
+ * cells of the generated network `network` (pre_odevec.py --example /
+ * --ordering / --packaging select it). opts may be NULL (defaults). */
+int32_t odevec_create(const char* network, int64_t nvector, const odevec_opts_t* opts, odevec_t** out);
+
+/* Replace the caller-set fields RelativeTolerance, AbsoluteTolerance,
+ * MinimalTimestep and CheckNegatives of type(odevec) (src/odevec.F90:48-52).
+ * Defaults: rtol 1e-6, atol 1e-10, dt_min 1e-10 (single-precision literal as
+ * pre_odevec.py:258-259 emits it), check_negatives 0 (src/odevec.F90:146). */
+int32_t odevec_set_tolerances(odevec_t* h, double rtol, double atol);
+int32_t odevec_set_dt_min(odevec_t* h, double dt_min);
+int32_t odevec_set_check_negatives(odevec_t* h, int32_t flag);
+
+/* Replaces SolveODE(this,time,dt,t_stop,y,GetRHS,GetJac,GetLU,Mask)
+ * (src/odevec.F90:158-214) for the handle's nvector cells: integrate y from
+ * *time to t_stop; *dt is the initial-step hint on entry and the last step
+ * size (cell 0's in per-cell modes) on return; *time returns as t_stop.
+ * mask (NULL = all cells) and status (NULL = not wanted) have nvector
+ * entries. The network's RHS/Jacobian/LU are fixed at generation time, so
+ * the reference's procedure arguments have no counterpart here. */
+int32_t odevec_solve(odevec_t* h, double* time, double* dt, double t_stop, double* y,
+                     const int8_t* mask, int32_t* status);
+
+/* Per-cell form (no reference counterpart; the reference shares one t_stop
+ * per call): cell i integrates from t0[i] to t_stop[i] with dt hint dt0[i].
+ * t0 NULL -> 0, dt0 NULL -> 1e-10, params NULL -> network defaults, status /
+ * dt_out / stats NULL -> not wanted. ncells may exceed the handle's nvector
+ * (buffers grow). */
+int32_t odevec_solve_cells(odevec_t* h, int64_t ncells, const double* t0, const double* t_stop,
+                           const double* dt0, double* y, const double* params, int32_t* status,
+                           double* dt_out, odevec_stats_t* stats);
+
+/* Asynchronous per-cell form on a caller stream (a hipStream_t passed as
+ * void*; NULL = the handle's stream). All pointers are device pointers;
+ * strides of 0 broadcast one value to every cell (params: stride between
+ * cells, planes are ncells apart when the cell stride is 1, else 1 apart).
+ * counters (optional) receives 8 int32 planes of ncells: attempts, steps,
+ * newton, jac, errfail, convfail, stepsel, qsum. flops (optional) receives
+ * 2 doubles (FP64, FP32), accumulated. The call returns after enqueueing. */
+int32_t odevec_solve_cells_async(odevec_t* h, int64_t ncells, const double* d_t0, int64_t t0_stride,
+                                 const double* d_t_stop, int64_t t_stop_stride, const double* d_dt0,
+                                 int64_t dt0_stride, double* d_y, const double* d_params,
+                                 int64_t params_cell_stride, int32_t* d_status, double* d_dt_out,
+                                 int32_t* d_counters, double* d_flops, void* stream);
+
+/* Replaces CloseOdevec(this) (src/odevec.F90:1058-1074). */
+int32_t odevec_destroy(odevec_t* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ODEVEC_H */

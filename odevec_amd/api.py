@@ -1,0 +1,191 @@
+"""Python host interface mirroring the reference's Fortran module.
+
+``Odevec`` mirrors ``type(odevec)`` of src/odevec.F90:33-74 with the
+procedures that act on it:
+
+=====================================  ====================================
+reference (src/odevec.F90)             here
+=====================================  ====================================
+``call InitOdevec(ode)`` (:94)         ``ode = Odevec(network, nvector)`` /
+                                       ``ode.InitOdevec()``
+``ode%RelativeTolerance`` etc. (:48-52) same attribute names
+``ode%y(nvector,neq)`` (:57)           ``ode.y`` (numpy, shape (nvector, neq))
+``call SolveODE(ode,t,dt,t_stop,y,
+GetRHS,GetJac,GetLU[,Mask])`` (:158)   ``t, dt = ode.SolveODE(t, dt, t_stop[, y, Mask])``
+``call CloseOdevec(ode)`` (:1058)      ``ode.CloseOdevec()``
+=====================================  ====================================
+
+The network (RHS, Jacobian, LU) is fixed when the library is generated, so
+the procedure arguments GetRHS/GetJac/GetLU have no counterpart; they are
+accepted and ignored so that call sites read like the reference's.
+Failures that make the reference print and STOP raise ``OdevecError`` here
+(with per-cell status in ``ode.status``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Odevec:
+    def __init__(self, network: str, nvector: int, precision: int = L.FP64, mode: int = L.PERCELL,
+                 group: int = 0, block: int = 0, grid: int = 0, max_attempts: int = 0,
+                 abort_on_error: bool = False, init: bool = True):
+        self.network = network
+        self.nvector = int(nvector)
+        self._opts = L.Opts(precision, mode, group, block, grid, max_attempts, 0, 1 if abort_on_error else 0)
+        self._h = None
+        info = L.network_info(network)
+        self.neq = info["neq"]
+        self.nnz = info["nnz"]
+        self.nparam = info["nparam"]
+        self.info = info
+        self._rtol, self._atol = 1e-6, 1e-10
+        self._dtmin = float(np.float32(1e-10))
+        self._checkneg = False
+        self.y = np.zeros((self.nvector, self.neq), order="F")
+        self.status = np.zeros(self.nvector, dtype=np.int32)
+        if init:
+            self.InitOdevec()
+
+    # -- reference procedures
+    def InitOdevec(self):
+        if self._h is not None:
+            return
+        h = C.c_void_p()
+        L.check(L.lib().odevec_create(self.network.encode(), self.nvector, C.byref(self._opts), C.byref(h)))
+        self._h = h
+        self._push()
+
+    def CloseOdevec(self):
+        if self._h is not None:
+            L.lib().odevec_destroy(self._h)
+            self._h = None
+
+    def SolveODE(self, time: float, dt: float, t_stop: float, y: Optional[np.ndarray] = None,
+                 GetRHS=None, GetJac=None, GetLU=None, Mask: Optional[np.ndarray] = None):
+        """Integrate y (default: self.y) from time to t_stop; returns (t_stop, last dt)."""
+        self._require()
+        if y is None:
+            y = self.y
+        if y.shape != (self.nvector, self.neq):
+            raise ValueError("y must have shape (nvector, neq) = (%d, %d)" % (self.nvector, self.neq))
+        buf = np.asfortranarray(y, dtype=np.float64)
+        t = C.c_double(time)
+        d = C.c_double(dt)
+        mask = None
+        if Mask is not None:
+            mask = np.ascontiguousarray(np.asarray(Mask, dtype=np.int8))
+            if mask.shape != (self.nvector,):
+                raise ValueError("Mask must have nvector entries")
+        rc = L.lib().odevec_solve(self._h, C.byref(t), C.byref(d), float(t_stop), buf.ctypes.data,
+                                  None if mask is None else mask.ctypes.data, self.status.ctypes.data)
+        if buf is not y:
+            y[...] = buf
+        L.check(rc)
+        return t.value, d.value
+
+    # -- caller-set fields of type(odevec)
+    @property
+    def RelativeTolerance(self):
+        return self._rtol
+
+    @RelativeTolerance.setter
+    def RelativeTolerance(self, v):
+        self._rtol = float(v)
+        self._push()
+
+    @property
+    def AbsoluteTolerance(self):
+        return self._atol
+
+    @AbsoluteTolerance.setter
+    def AbsoluteTolerance(self, v):
+        self._atol = float(v)
+        self._push()
+
+    @property
+    def MinimalTimestep(self):
+        return self._dtmin
+
+    @MinimalTimestep.setter
+    def MinimalTimestep(self, v):
+        self._dtmin = float(v)
+        self._push()
+
+    @property
+    def CheckNegatives(self):
+        return self._checkneg
+
+    @CheckNegatives.setter
+    def CheckNegatives(self, v):
+        self._checkneg = bool(v)
+        self._push()
+
+    # -- per-cell form
+    def solve_cells(self, y: np.ndarray, t_stop, t0=None, dt0=None, params=None, stats: bool = False,
+                    raise_on_fail: bool = True):
+        """Per-cell solves of y (shape (neq, ncells), modified in place);
+        t_stop/t0/dt0 scalars or arrays of ncells; params shape (nparam, ncells).
+        Returns (status, dt_out, stats-dict or None)."""
+        self._require()
+        if y.ndim != 2 or y.shape[0] != self.neq or y.dtype != np.float64 or not y.flags.c_contiguous:
+            raise ValueError("y must be a C-contiguous float64 array of shape (neq, ncells)")
+        n = y.shape[1]
+
+        def arr(v):
+            if v is None:
+                return None
+            return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (n,)))
+
+        ts, t0a, d0 = arr(t_stop), arr(t0), arr(dt0)
+        p = None
+        if params is not None and self.nparam:
+            p = np.ascontiguousarray(np.broadcast_to(np.asarray(params, dtype=np.float64), (self.nparam, n)))
+        st = np.zeros(n, dtype=np.int32)
+        dto = np.zeros(n)
+        s = L.Stats() if stats else None
+        ptr = lambda a: None if a is None else a.ctypes.data
+        rc = L.lib().odevec_solve_cells(self._h, n, ptr(t0a), ptr(ts), ptr(d0), y.ctypes.data, ptr(p),
+                                        st.ctypes.data, dto.ctypes.data, C.byref(s) if s is not None else None)
+        if rc != L.OK and (raise_on_fail or rc != L.ESOLVE):
+            L.check(rc)
+        return st, dto, (s.as_dict() if s is not None else None)
+
+    def solve_cells_async(self, ncells, t_stop_ptr, y_ptr, t0_ptr=None, dt0_ptr=None, params_ptr=None,
+                          t_stop_stride=1, t0_stride=1, dt0_stride=1, params_stride=1, status_ptr=None,
+                          dt_out_ptr=None, counters_ptr=None, flops_ptr=None, stream=None):
+        """Enqueue a per-cell solve on device pointers (e.g. torch tensors' data_ptr())."""
+        self._require()
+        L.check(L.lib().odevec_solve_cells_async(
+            self._h, int(ncells), t0_ptr, t0_stride, t_stop_ptr, t_stop_stride, dt0_ptr, dt0_stride, y_ptr,
+            params_ptr, params_stride, status_ptr, dt_out_ptr, counters_ptr, flops_ptr, stream))
+
+    # -- helpers
+    def _require(self):
+        if self._h is None:
+            raise RuntimeError("call InitOdevec() first")
+
+    def _push(self):
+        if self._h is None:
+            return
+        lib = L.lib()
+        L.check(lib.odevec_set_tolerances(self._h, self._rtol, self._atol))
+        L.check(lib.odevec_set_dt_min(self._h, self._dtmin))
+        L.check(lib.odevec_set_check_negatives(self._h, 1 if self._checkneg else 0))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.CloseOdevec()
+
+    def __del__(self):
+        try:
+            self.CloseOdevec()
+        except Exception:
+            pass

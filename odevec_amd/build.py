@@ -1,0 +1,108 @@
+"""Build libodevec.so for gfx950.
+
+1. Run the preprocessor for the built-in network variants (and any extra
+   ``NETWORK/ORDERING/PACKAGING`` ids) into ``csrc/generated``.
+2. Compile each instantiation unit and the C ABI with hipcc (one process per
+   unit, in parallel) and link ``odevec_amd/libodevec.so``.
+
+    python -m odevec_amd.build [--extra PRIMORDIAL/INVERT/CSC] [-j 8]
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from typing import List
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+GEN = os.path.join(CSRC, "generated")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libodevec.so")
+
+BUILTIN_VARIANTS = [
+    "ROBER/NONE/DENSE",
+    "ROBER/NONE/CSC",
+    "OREGO/NONE/DENSE",
+    "OREGO/NONE/CSC",
+    "PRIMORDIAL/CMK/CSC",
+    "PRIMORDIAL/NONE/DENSE",
+    "PRIMORDIAL/FF/CSC",
+]
+
+# the reproducible build: no contraction of a*b+c into FMA, so device results
+# equal the oracle's bit for bit
+REPRO_FLAGS = ["-ffp-contract=off", "-DODV_REPRO=1"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # no packed FP32 instructions (hipcc's host pass warns about the feature)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def generate(variants: List[str]) -> List[str]:
+    from .pre_odevec import cli
+    bases = []
+    for v in variants:
+        name, ordering, packaging = v.split("/")
+        ordering = "None" if ordering.upper() == "NONE" else ordering
+        cli.generate(name, ordering, packaging, GEN)
+        bases.append("%s_%s_%s" % (name, ordering.upper(), packaging))
+    return bases
+
+
+def _compile(src: str, obj: str, extra: List[str]) -> str:
+    cmd = [hipcc()] + HIPCC_FLAGS + extra + ["-I", CSRC, "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr[-6000:]))
+    return obj
+
+
+def build(variants: List[str] = None, jobs: int = 8, verbose: bool = True, repro: bool = True) -> str:
+    variants = list(BUILTIN_VARIANTS if variants is None else variants)
+    bases = generate(variants)
+    os.makedirs(BUILD, exist_ok=True)
+    units = [(os.path.join(CSRC, "odevec_capi.hip"), os.path.join(BUILD, "odevec_capi.o"), [])]
+    for b in bases:
+        src = os.path.join(GEN, "inst_%s.hip" % b)
+        units.append((src, os.path.join(BUILD, "inst_%s.o" % b), []))
+        if repro:
+            units.append((src, os.path.join(BUILD, "inst_%s_repro.o" % b), REPRO_FLAGS))
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        futs = [ex.submit(_compile, s, o, f) for s, o, f in units]
+        objs = [f.result() for f in futs]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built %s (%d network variants%s)" % (LIB, len(bases), ", each also +REPRO" if repro else ""))
+    return LIB
+
+
+def main(argv=None) -> int:
+    p = argparse.ArgumentParser()
+    p.add_argument("--extra", action="append", default=[], help="NETWORK/ORDERING/PACKAGING to add")
+    p.add_argument("-j", "--jobs", type=int, default=8)
+    a = p.parse_args(argv)
+    build(BUILTIN_VARIANTS + a.extra, a.jobs)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,110 @@
+// Generated by odevec_amd.pre_odevec -- do not edit.
+// network OREGO, ordering None, packaging CSC
+// neq 3, nnz(J) 7, nnz(P) 7, nnz(LU) 8, terms 9
+// flops: rhs 17, jac 25, lu 8, solve 13, lu_r 11, solve_r 13
+#pragma once
+#include "odevec_netcommon.h"
+
+struct odv_net_OREGO_NONE_CSC {
+  static constexpr const char* ID = "OREGO/NONE/CSC";
+  static constexpr int NEQ = 3;
+  static constexpr int NNZ = 8;
+  static constexpr int NRATE = 1;
+  static constexpr int NPARAM = 0;
+  static constexpr int RATE_KIND = 0;
+  static constexpr int TGAS = -1;
+  static constexpr bool CROUT = true;
+  static constexpr int FLOPS_RHS = 17, FLOPS_JAC = 25, FLOPS_LU = 8, FLOPS_SOLVE = 13;
+  static constexpr int FLOPS_LU_R = 11, FLOPS_SOLVE_R = 13;
+  // permutation Perm (0-based): P_order[i][j] = P[Perm[i]][Perm[j]]
+  // Perm = {0, 1, 2}
+
+  static inline void param_defaults(double* p) {
+    (void)p;
+  }
+
+  ODV_HD static inline void rates(const double* __restrict__ y0, const double* __restrict__ prm,
+                                 double* __restrict__ k) {
+    (void)y0; (void)prm; (void)k;
+  }
+
+  ODV_HD static inline void rhs(const double* __restrict__ y, const double* __restrict__ k,
+                               double* __restrict__ f) {
+    const double F0 = y[0];
+    const double F1 = y[0] * y[0];
+    const double F2 = y[0] * y[1];
+    const double F3 = y[1];
+    const double F4 = y[2];
+    f[0] = (77.27 * F0) - 0.00064713625 * F1 - 77.27 * F2 + (77.27 * F3);
+    f[1] = (-0.012941633234114146 * F2) - 0.012941633234114146 * F3 + (0.012941633234114146 * F4);
+    f[2] = (0.161 * F0) - 0.161 * F4;
+  }
+
+  template <class M>
+  ODV_HD static inline void jac(const double* __restrict__ y, const double* __restrict__ k,
+                               double hb, M* __restrict__ a) {
+    const double D0 = 1.0;
+    const double D1 = y[0];
+    const double D2 = y[1];
+    const double J0_0 = (77.27 * D0) - 0.0012942725 * D1 - 77.27 * D2;
+    const double J1_0 = (-0.012941633234114146 * D2);
+    const double J2_0 = (0.161 * D0);
+    const double J0_1 = (-77.27 * D1) + (77.27 * D0);
+    const double J1_1 = (-0.012941633234114146 * D1) - 0.012941633234114146 * D0;
+    const double J1_2 = (0.012941633234114146 * D0);
+    const double J2_2 = (-0.161 * D0);
+    a[0] = (M)(1.0 - hb * J0_0);
+    a[1] = (M)(-(hb * J1_0));
+    a[2] = (M)(-(hb * J2_0));
+    a[3] = (M)(-(hb * J0_1));
+    a[4] = (M)(1.0 - hb * J1_1);
+    a[5] = (M)(0.0);
+    a[6] = (M)(-(hb * J1_2));
+    a[7] = (M)(1.0 - hb * J2_2);
+  }
+
+  template <class M>
+  ODV_HD static inline void lu(M* __restrict__ a) {
+    a[3] = a[3] / a[0];
+    a[4] = a[4] - a[3] * a[1];
+    a[5] = a[5] - a[3] * a[2];
+    a[6] = a[6] / a[4];
+    a[7] = a[7] - a[6] * a[5];
+  }
+
+  template <class M>
+  ODV_HD static inline void lu_r(M* __restrict__ a) {
+    a[0] = (M)1 / a[0];
+    a[3] = a[3] * a[0];
+    a[4] = a[4] - a[3] * a[1];
+    a[5] = a[5] - a[3] * a[2];
+    a[4] = (M)1 / a[4];
+    a[6] = a[6] * a[4];
+    a[7] = a[7] - a[6] * a[5];
+    a[7] = (M)1 / a[7];
+  }
+
+  template <class M, class V>
+  ODV_HD static inline void solve(const M* __restrict__ a, V* __restrict__ x) {
+    x[0] = x[0] / a[0];
+    x[1] = x[1] - x[0] * a[1];
+    x[2] = x[2] - x[0] * a[2];
+    x[1] = x[1] / a[4];
+    x[2] = x[2] - x[1] * a[5];
+    x[2] = x[2] / a[7];
+    x[1] = x[1] - x[2] * a[6];
+    x[0] = x[0] - x[1] * a[3];
+  }
+
+  template <class M, class V>
+  ODV_HD static inline void solve_r(const M* __restrict__ a, V* __restrict__ x) {
+    x[0] = x[0] * a[0];
+    x[1] = x[1] - x[0] * a[1];
+    x[2] = x[2] - x[0] * a[2];
+    x[1] = x[1] * a[4];
+    x[2] = x[2] - x[1] * a[5];
+    x[2] = x[2] * a[7];
+    x[1] = x[1] - x[2] * a[6];
+    x[0] = x[0] - x[1] * a[3];
+  }
+};

@@ -1,0 +1,503 @@
+// C ABI of libodevec (declarations and reference citations in include/odevec.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/odevec.h"
+#include "odevec_registry.h"
+
+namespace odv {
+
+std::vector<const NetEntry*>& registry() {
+  static std::vector<const NetEntry*> r;
+  return r;
+}
+
+__global__ void k_setq(unsigned int* q, unsigned int v) { *q = v; }
+
+__global__ void k_fill_f64(double* p, double v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace odv
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(ODEVEC_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// "NET/ORDERING/PACKAGING" selects the default build of a network,
+// "NET/ORDERING/PACKAGING+REPRO" its reproducible build.
+const odv::NetEntry* find_net(const char* id) {
+  if (!id) return nullptr;
+  std::string s(id);
+  int repro = 0;
+  const std::string suf = "+REPRO";
+  if (s.size() > suf.size() && s.compare(s.size() - suf.size(), suf.size(), suf) == 0) {
+    repro = 1;
+    s.resize(s.size() - suf.size());
+  }
+  for (const odv::NetEntry* e : odv::registry())
+    if (e->repro == repro && s == e->id) return e;
+  return nullptr;
+}
+
+}  // namespace
+
+struct odevec_s {
+  const odv::NetEntry* net = nullptr;
+  int64_t nvector = 0;
+  odevec_opts_t opts{};
+  double rtol = 1e-6, atol = 1e-10, dt_min = (double)1e-10f;
+  int check_neg = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int64_t cap = 0;
+  double* d_y = nullptr;
+  double* d_t0 = nullptr;
+  double* d_ts = nullptr;
+  double* d_dt0 = nullptr;
+  double* d_prm = nullptr;
+  double* d_dtout = nullptr;
+  int* d_status = nullptr;
+  int* d_cnt = nullptr;
+  int8_t* d_mask = nullptr;
+  double* d_scal = nullptr;    // [0..2] uniform t0, t_stop, dt0; [3..] default params; [..] flops
+  unsigned int* d_queue = nullptr;   // 64 queue counters, one per in-flight persistent launch
+  int qslot = 0;
+  int nscal = 0;
+  std::vector<int> h_status;
+  std::vector<int> h_cnt;
+};
+
+namespace {
+
+void free_bufs(odevec_s* h) {
+  void* ps[] = {h->d_y, h->d_t0, h->d_ts, h->d_dt0, h->d_prm, h->d_dtout, h->d_status, h->d_cnt, h->d_mask};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  h->d_y = h->d_t0 = h->d_ts = h->d_dt0 = h->d_prm = h->d_dtout = nullptr;
+  h->d_status = h->d_cnt = nullptr;
+  h->d_mask = nullptr;
+  h->cap = 0;
+}
+
+int ensure_cap(odevec_s* h, int64_t n) {
+  if (n <= h->cap) return ODEVEC_OK;
+  free_bufs(h);
+  const int64_t neq = h->net->neq, np = h->net->nparam;
+  hipError_t e = hipSuccess;
+#define ODV_ALLOC(p, bytes)                                        \
+  if (e == hipSuccess) e = hipMalloc((void**)&(p), (size_t)(bytes));
+  ODV_ALLOC(h->d_y, neq * n * 8);
+  ODV_ALLOC(h->d_t0, n * 8);
+  ODV_ALLOC(h->d_ts, n * 8);
+  ODV_ALLOC(h->d_dt0, n * 8);
+  ODV_ALLOC(h->d_prm, (np > 0 ? np : 1) * n * 8);
+  ODV_ALLOC(h->d_dtout, n * 8);
+  ODV_ALLOC(h->d_status, n * 4);
+  ODV_ALLOC(h->d_cnt, 8 * n * 4);
+  ODV_ALLOC(h->d_mask, n);
+#undef ODV_ALLOC
+  if (e != hipSuccess) {
+    free_bufs(h);
+    return fail(ODEVEC_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
+  }
+  h->cap = n;
+  return ODEVEC_OK;
+}
+
+struct Req {
+  int64_t ncells;
+  const double* t0; int64_t t0_s;
+  const double* ts; int64_t ts_s;
+  const double* dt0; int64_t dt_s;
+  double* y;
+  const double* prm; int64_t ps_cell;
+  const int8_t* mask;
+  int* status;
+  double* dt_out;
+  int* counters;
+  double* flops;
+  hipStream_t stream;
+};
+
+int launch(odevec_s* h, const Req& r) {
+  odv::KArgs a{};
+  a.y = r.y;
+  a.ncells = r.ncells;
+  a.prm = r.prm;
+  a.ps_cell = r.ps_cell;
+  a.ps_plane = (r.ps_cell == 1) ? r.ncells : 1;
+  a.t0 = r.t0; a.t0_s = r.t0_s;
+  a.tstop = r.ts; a.ts_s = r.ts_s;
+  a.dt0 = r.dt0; a.dt_s = r.dt_s;
+  a.mask = reinterpret_cast<const signed char*>(r.mask);
+  a.status = r.status;
+  a.dt_out = r.dt_out;
+  a.counters = r.counters;
+  a.flops = r.flops;
+  h->qslot = (h->qslot + 1) % 64;
+  a.queue = h->d_queue + h->qslot;
+  a.ctl.rtol = h->rtol;
+  a.ctl.atol = h->atol;
+  a.ctl.dt_min = h->dt_min;
+  a.ctl.check_negatives = h->check_neg;
+  a.ctl.max_attempts = h->opts.max_attempts > 0 ? h->opts.max_attempts : 1000000;
+  int grp = h->opts.group;
+  if (grp <= 0) grp = (int)(h->nvector < 1024 ? h->nvector : 1024);
+  if (grp < 1) grp = 1;
+  a.group = grp;
+  hipError_t e = h->net->launch(a, h->opts.mode, h->opts.precision, h->opts.block, h->opts.grid, r.stream);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return ODEVEC_OK;
+}
+
+int check_status(odevec_s* h, const int* hs, int64_t n, odevec_stats_t* st) {
+  int64_t ok = 0, bad = 0, skip = 0;
+  int first_bad = -1;
+  int64_t first_i = -1;
+  for (int64_t i = 0; i < n; ++i) {
+    if (hs[i] == ODEVEC_STATUS_OK) ok++;
+    else if (hs[i] == ODEVEC_STATUS_SKIPPED) skip++;
+    else {
+      if (first_bad < 0) { first_bad = hs[i]; first_i = i; }
+      bad++;
+    }
+  }
+  if (st) { st->cells_ok = ok; st->cells_failed = bad; st->cells_skipped = skip; }
+  if (bad) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%lld of %lld cells failed (first: cell %lld, status %d)",
+                  (long long)bad, (long long)n, (long long)first_i, first_bad);
+    if (h->opts.abort_on_error) {
+      std::fprintf(stderr, "ODEVEC: %s. Aborting.\n", buf);
+      std::abort();
+    }
+    return fail(ODEVEC_ESOLVE, buf);
+  }
+  return ODEVEC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t odevec_abi_version(void) { return ODEVEC_ABI_VERSION; }
+
+const char* odevec_last_error(void) { return g_err.c_str(); }
+
+int32_t odevec_network_count(void) { return (int32_t)odv::registry().size(); }
+
+const char* odevec_network_name(int32_t i) {
+  static std::vector<std::string> names;
+  if (i < 0 || i >= (int32_t)odv::registry().size()) return nullptr;
+  if (names.size() != odv::registry().size()) {
+    names.clear();
+    for (const odv::NetEntry* e : odv::registry()) names.push_back(std::string(e->id) + (e->repro ? "+REPRO" : ""));
+  }
+  return names[(size_t)i].c_str();
+}
+
+int32_t odevec_network_info(const char* network, odevec_network_info_t* info) {
+  const odv::NetEntry* e = find_net(network);
+  if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown network ") + (network ? network : "(null)"));
+  if (!info) return fail(ODEVEC_EINVAL, "info is NULL");
+  info->neq = e->neq; info->nnz = e->nnz; info->nrate = e->nrate; info->nparam = e->nparam;
+  info->rate_kind = e->rate_kind; info->tgas_index = e->tgas;
+  info->flops_rhs = e->flops_rhs; info->flops_jac = e->flops_jac; info->flops_lu = e->flops_lu;
+  info->flops_solve = e->flops_solve; info->flops_lu_r = e->flops_lu_r; info->flops_solve_r = e->flops_solve_r;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_kernel_info(const char* network, int32_t mode, int32_t precision, int32_t block,
+                           odevec_kernel_info_t* info) {
+  const odv::NetEntry* e = find_net(network);
+  if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown network ") + (network ? network : "(null)"));
+  if (!info) return fail(ODEVEC_EINVAL, "info is NULL");
+  odv::KernelInfo k{};
+  hipError_t he = e->info(mode, precision, block, &k);
+  if (he != hipSuccess) return hip_fail(he, "kernel info");
+  info->vgprs = k.regs; info->spill_bytes = k.spill_bytes;
+  info->blocks_per_cu = k.blocks_per_cu; info->block = k.block;
+  return ODEVEC_OK;
+}
+
+void odevec_default_opts(odevec_opts_t* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->precision = ODEVEC_FP64;
+  o->mode = ODEVEC_PERCELL;
+}
+
+int32_t odevec_create(const char* network, int64_t nvector, const odevec_opts_t* opts, odevec_t** out) {
+  if (!out) return fail(ODEVEC_EINVAL, "out is NULL");
+  *out = nullptr;
+  const odv::NetEntry* e = find_net(network);
+  if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown network ") + (network ? network : "(null)"));
+  if (nvector < 1) return fail(ODEVEC_EINVAL, "nvector must be >= 1");
+  odevec_opts_t o;
+  odevec_default_opts(&o);
+  if (opts) o = *opts;
+  if (o.precision != ODEVEC_FP64 && o.precision != ODEVEC_MIXED) return fail(ODEVEC_EINVAL, "bad precision");
+  if (o.mode < ODEVEC_PERCELL || o.mode > ODEVEC_PERSISTENT) return fail(ODEVEC_EINVAL, "bad mode");
+  if (o.group < 0 || o.group > 1024) return fail(ODEVEC_EINVAL, "group must be in 0..1024");
+  if (o.block != 0 && (o.block < 64 || o.block > 256 || o.block % 64)) return fail(ODEVEC_EINVAL, "block must be 64..256, a multiple of 64");
+  int ndev = 0;
+  hipError_t he = hipGetDeviceCount(&ndev);
+  if (he != hipSuccess || ndev < 1) return fail(ODEVEC_EDEVICE, "no HIP device available");
+  odevec_s* h = new odevec_s();
+  h->net = e;
+  h->nvector = nvector;
+  h->opts = o;
+  hipGetDevice(&h->device);
+  if ((he = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (he = hipEventCreate(&h->e0)) != hipSuccess || (he = hipEventCreate(&h->e1)) != hipSuccess) {
+    odevec_destroy(h);
+    return hip_fail(he, "stream/event creation");
+  }
+  const int np = e->nparam > 0 ? e->nparam : 1;
+  h->nscal = 3 + np + 2;
+  if ((he = hipMalloc((void**)&h->d_scal, h->nscal * 8)) != hipSuccess ||
+      (he = hipMalloc((void**)&h->d_queue, 64 * sizeof(unsigned int))) != hipSuccess) {
+    odevec_destroy(h);
+    return fail(ODEVEC_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(he));
+  }
+  std::vector<double> sc(h->nscal, 0.0);
+  sc[2] = 1e-10;
+  e->param_defaults(sc.data() + 3);
+  if ((he = hipMemcpy(h->d_scal, sc.data(), h->nscal * 8, hipMemcpyHostToDevice)) != hipSuccess) {
+    odevec_destroy(h);
+    return hip_fail(he, "upload defaults");
+  }
+  int rc = ensure_cap(h, nvector);
+  if (rc != ODEVEC_OK) {
+    std::string m = g_err;
+    odevec_destroy(h);
+    return fail(rc, m);
+  }
+  *out = h;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_set_tolerances(odevec_t* h, double rtol, double atol) {
+  if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
+  if (!(rtol > 0.0) || !(atol >= 0.0) || (atol == 0.0 && rtol == 0.0))
+    return fail(ODEVEC_EINVAL, "need rtol > 0 and atol >= 0");
+  h->rtol = rtol;
+  h->atol = atol;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_set_dt_min(odevec_t* h, double dt_min) {
+  if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
+  h->dt_min = dt_min;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_set_check_negatives(odevec_t* h, int32_t flag) {
+  if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
+  h->check_neg = flag ? 1 : 0;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_solve(odevec_t* h, double* time, double* dt, double t_stop, double* y, const int8_t* mask,
+                     int32_t* status) {
+  if (!h || !time || !dt || !y) return fail(ODEVEC_EINVAL, "NULL argument");
+  const int64_t n = h->nvector;
+  const int64_t neq = h->net->neq;
+  hipStream_t st = h->stream;
+  hipError_t e;
+  double sc[3] = {*time, t_stop, *dt};
+  if ((e = hipMemcpyAsync(h->d_scal, sc, 24, hipMemcpyHostToDevice, st)) != hipSuccess) return hip_fail(e, "H2D");
+  double* dy = h->opts.device_ptrs ? y : h->d_y;
+  if (!h->opts.device_ptrs &&
+      (e = hipMemcpyAsync(h->d_y, y, (size_t)(neq * n * 8), hipMemcpyHostToDevice, st)) != hipSuccess)
+    return hip_fail(e, "H2D y");
+  const int8_t* dmask = nullptr;
+  if (mask) {
+    if (h->opts.device_ptrs) dmask = mask;
+    else {
+      if ((e = hipMemcpyAsync(h->d_mask, mask, (size_t)n, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "H2D mask");
+      dmask = h->d_mask;
+    }
+  }
+  Req r{};
+  r.ncells = n;
+  r.t0 = h->d_scal; r.t0_s = 0;
+  r.ts = h->d_scal + 1; r.ts_s = 0;
+  r.dt0 = h->d_scal + 2; r.dt_s = 0;
+  r.y = dy;
+  r.prm = h->d_scal + 3; r.ps_cell = 0;
+  r.mask = dmask;
+  r.status = h->d_status;
+  r.dt_out = h->d_dtout;
+  r.stream = st;
+  hipEventRecord(h->e0, st);
+  int rc = launch(h, r);
+  hipEventRecord(h->e1, st);
+  if (rc != ODEVEC_OK) return rc;
+  if (!h->opts.device_ptrs &&
+      (e = hipMemcpyAsync(y, h->d_y, (size_t)(neq * n * 8), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return hip_fail(e, "D2H y");
+  h->h_status.resize((size_t)n);
+  double dt0 = 0.0;
+  if ((e = hipMemcpyAsync(h->h_status.data(), h->d_status, (size_t)n * 4, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(&dt0, h->d_dtout, 8, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return hip_fail(e, "D2H status");
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "solve");
+  if (status) std::memcpy(status, h->h_status.data(), (size_t)n * 4);
+  *time = t_stop;
+  *dt = dt0;
+  return check_status(h, h->h_status.data(), n, nullptr);
+}
+
+int32_t odevec_solve_cells(odevec_t* h, int64_t ncells, const double* t0, const double* t_stop,
+                           const double* dt0, double* y, const double* params, int32_t* status,
+                           double* dt_out, odevec_stats_t* stats) {
+  if (!h || !t_stop || !y) return fail(ODEVEC_EINVAL, "NULL argument");
+  if (ncells < 0) return fail(ODEVEC_EINVAL, "ncells < 0");
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  if (ncells == 0) return ODEVEC_OK;
+  const bool dev = h->opts.device_ptrs != 0;
+  int rc = ensure_cap(h, ncells);
+  if (rc != ODEVEC_OK) return rc;
+  const int64_t neq = h->net->neq, np = h->net->nparam;
+  hipStream_t st = h->stream;
+  hipError_t e;
+  auto h2d = [&](double* dst, const double* src, int64_t cnt) {
+    return hipMemcpyAsync(dst, src, (size_t)cnt * 8, hipMemcpyHostToDevice, st);
+  };
+  Req r{};
+  r.ncells = ncells;
+  r.stream = st;
+  if (dev) {
+    r.y = y;
+    r.ts = t_stop; r.ts_s = 1;
+  } else {
+    if ((e = h2d(h->d_y, y, neq * ncells)) != hipSuccess) return hip_fail(e, "H2D y");
+    if ((e = h2d(h->d_ts, t_stop, ncells)) != hipSuccess) return hip_fail(e, "H2D t_stop");
+    r.y = h->d_y;
+    r.ts = h->d_ts; r.ts_s = 1;
+  }
+  if (t0) {
+    if (dev) { r.t0 = t0; }
+    else { if ((e = h2d(h->d_t0, t0, ncells)) != hipSuccess) return hip_fail(e, "H2D t0"); r.t0 = h->d_t0; }
+    r.t0_s = 1;
+  } else { r.t0 = h->d_scal; r.t0_s = 0; }
+  double zero_and_hint[3] = {0.0, 0.0, 1e-10};
+  if ((e = h2d(h->d_scal, zero_and_hint, 3)) != hipSuccess) return hip_fail(e, "H2D scalars");
+  if (dt0) {
+    if (dev) { r.dt0 = dt0; }
+    else { if ((e = h2d(h->d_dt0, dt0, ncells)) != hipSuccess) return hip_fail(e, "H2D dt0"); r.dt0 = h->d_dt0; }
+    r.dt_s = 1;
+  } else { r.dt0 = h->d_scal + 2; r.dt_s = 0; }
+  if (params && np > 0) {
+    if (dev) { r.prm = params; }
+    else { if ((e = h2d(h->d_prm, params, np * ncells)) != hipSuccess) return hip_fail(e, "H2D params"); r.prm = h->d_prm; }
+    r.ps_cell = 1;
+  } else { r.prm = h->d_scal + 3; r.ps_cell = 0; }
+  r.status = h->d_status;
+  r.dt_out = (dt_out && dev) ? dt_out : h->d_dtout;
+  r.counters = stats ? h->d_cnt : nullptr;
+  double* dfl = h->d_scal + 3 + (np > 0 ? np : 1);
+  if (stats) {
+    hipLaunchKernelGGL(odv::k_fill_f64, dim3(1), dim3(64), 0, st, dfl, 0.0, 2LL);
+    r.flops = dfl;
+  }
+  hipEventRecord(h->e0, st);
+  rc = launch(h, r);
+  hipEventRecord(h->e1, st);
+  if (rc != ODEVEC_OK) return rc;
+  if (!dev && (e = hipMemcpyAsync(y, h->d_y, (size_t)(neq * ncells * 8), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return hip_fail(e, "D2H y");
+  if (dt_out && !dev &&
+      (e = hipMemcpyAsync(dt_out, h->d_dtout, (size_t)ncells * 8, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return hip_fail(e, "D2H dt");
+  h->h_status.resize((size_t)ncells);
+  if ((e = hipMemcpyAsync(h->h_status.data(), h->d_status, (size_t)ncells * 4, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return hip_fail(e, "D2H status");
+  double fl[2] = {0.0, 0.0};
+  if (stats) {
+    h->h_cnt.resize((size_t)(8 * ncells));
+    if ((e = hipMemcpyAsync(h->h_cnt.data(), h->d_cnt, (size_t)(8 * ncells * 4), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(fl, dfl, 16, hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return hip_fail(e, "D2H stats");
+  }
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "solve");
+  if (status) std::memcpy(status, h->h_status.data(), (size_t)ncells * 4);
+  if (stats) {
+    int64_t* dst = &stats->attempts;
+    for (int m = 0; m < 8; ++m) {
+      int64_t s = 0;
+      const int* c = h->h_cnt.data() + (size_t)m * ncells;
+      for (int64_t i = 0; i < ncells; ++i) s += c[i];
+      dst[m] = s;
+    }
+    stats->flops64 = fl[0];
+    stats->flops32 = fl[1];
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, h->e0, h->e1);
+    stats->kernel_ms = ms;
+  }
+  return check_status(h, h->h_status.data(), ncells, stats);
+}
+
+int32_t odevec_solve_cells_async(odevec_t* h, int64_t ncells, const double* d_t0, int64_t t0_stride,
+                                 const double* d_t_stop, int64_t t_stop_stride, const double* d_dt0,
+                                 int64_t dt0_stride, double* d_y, const double* d_params,
+                                 int64_t params_cell_stride, int32_t* d_status, double* d_dt_out,
+                                 int32_t* d_counters, double* d_flops, void* stream) {
+  if (!h || !d_t_stop || !d_y) return fail(ODEVEC_EINVAL, "NULL argument");
+  if (ncells < 0) return fail(ODEVEC_EINVAL, "ncells < 0");
+  if (ncells == 0) return ODEVEC_OK;
+  if (t0_stride < 0 || t0_stride > 1 || t_stop_stride < 0 || t_stop_stride > 1 || dt0_stride < 0 ||
+      dt0_stride > 1 || params_cell_stride < 0 || params_cell_stride > 1)
+    return fail(ODEVEC_EINVAL, "strides must be 0 or 1");
+  Req r{};
+  r.ncells = ncells;
+  r.stream = stream ? (hipStream_t)stream : h->stream;
+  r.y = d_y;
+  r.ts = d_t_stop; r.ts_s = t_stop_stride;
+  if (d_t0) { r.t0 = d_t0; r.t0_s = t0_stride; }
+  else { r.t0 = h->d_scal; r.t0_s = 0; }
+  if (d_dt0) { r.dt0 = d_dt0; r.dt_s = dt0_stride; }
+  else { r.dt0 = h->d_scal + 2; r.dt_s = 0; }
+  if (d_params && h->net->nparam > 0) { r.prm = d_params; r.ps_cell = params_cell_stride; }
+  else { r.prm = h->d_scal + 3; r.ps_cell = 0; }
+  r.status = d_status;
+  r.dt_out = d_dt_out;
+  r.counters = d_counters;
+  r.flops = d_flops;
+  return launch(h, r);
+}
+
+int32_t odevec_destroy(odevec_t* h) {
+  if (!h) return ODEVEC_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_bufs(h);
+  if (h->d_scal) (void)hipFree(h->d_scal);
+  if (h->d_queue) (void)hipFree(h->d_queue);
+  if (h->e0) (void)hipEventDestroy(h->e0);
+  if (h->e1) (void)hipEventDestroy(h->e1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return ODEVEC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,545 @@
+// Per-lane LSODE-style BDF integrator in Nordsieck form.
+//
+// One lane integrates one cell (or, with a group policy, one lane of a
+// lockstep group).  The algorithm is the reference's SolveODE
+// (src/odevec.F90:158-214) and SolveLinearSystem (src/odevec.F90:242-465),
+// restated as a per-lane state machine: `odv_attempt` performs one
+// predictor / Jacobian-refresh / corrector / error-test pass (one iteration of
+// the reference's `predictor:` loop) and reports whether the cell is done.
+//
+// All loops over species and Nordsieck columns are fully unrolled against
+// compile-time sizes, and the runtime order q only predicates them, so the
+// state stays in registers.  The control constants that are default-real
+// literals in the reference (1.3, 1.2, 1.4, 1.1, 0.3) are used at their
+// single-precision values, as the reference promotes them.
+#pragma once
+#include "odevec_netcommon.h"
+
+namespace odv {
+inline namespace ODV_VARIANT_NS {
+
+enum Status : int {
+  ST_RUNNING = -1,
+  ST_OK = 0,
+  ST_DT_MIN = 1,       // step below MinimalTimestep (src/odevec.F90:272-275, 387-390)
+  ST_CONV_FAIL = 2,    // > 10 consecutive convergence failures (src/odevec.F90:380-383)
+  ST_ORDER = 3,        // order/step-size mismatch (src/odevec.F90:584-587, 595-598)
+  ST_MAX_STEPS = 4,    // attempt budget exhausted (no reference counterpart)
+  ST_SKIPPED = 5,      // masked out / empty span: y returned unchanged
+};
+
+constexpr int MAXORD = 5;          // pre_odevec.py:83
+constexpr int NCOL = MAXORD + 1;   // Nordsieck columns 0..MAXORD are used
+
+struct Controls {
+  double rtol, atol, dt_min;
+  int check_negatives;
+  int max_attempts;
+};
+
+// Per-cell event counters (feed the flop model and diagnostics).
+struct Counters {
+  int attempts;    // predictor passes
+  int steps;       // accepted steps
+  int newton;      // corrector iterations (RHS evaluations in the corrector)
+  int jac;         // Jacobian + LU refreshes
+  int errfail;     // error-test failures
+  int convfail;    // convergence failures
+  int stepsel;     // step-size/order selections
+  int qsum;        // sum of q over attempts
+};
+
+// l-vector of the order-q BDF method in Nordsieck form (src/odevec.F90:127-134);
+// lq[k] = l_k(q), lq[0] = beta.
+ODV_HD inline void odv_load_l(int q, double* lq) {
+#pragma unroll
+  for (int i = 0; i < 7; ++i) lq[i] = 0.0;
+  lq[1] = 1.0;
+  switch (q) {
+    case 1: lq[0] = 1.0; break;
+    case 2: lq[0] = 2.0 / 3.0; lq[2] = 1.0 / 3.0; break;
+    case 3: lq[0] = 6.0 / 11.0; lq[2] = 6.0 / 11.0; lq[3] = 1.0 / 11.0; break;
+    case 4: lq[0] = 12.0 / 25.0; lq[2] = 7.0 / 10.0; lq[3] = 1.0 / 5.0; lq[4] = 1.0 / 50.0; break;
+    default: lq[0] = 60.0 / 137.0; lq[2] = 225.0 / 274.0; lq[3] = 85.0 / 274.0;
+             lq[4] = 15.0 / 274.0; lq[5] = 1.0 / 274.0; break;
+  }
+}
+
+ODV_HD inline double odv_fact(int q) {
+  double f = 1.0;
+  for (int i = 1; i <= q; ++i) f = f * i;
+  return f;
+}
+
+// tau(order, order2) (src/odevec.F90:1034-1054), lqq = l_q(q)
+ODV_HD inline double odv_tau(int q, int q2, double lqq) {
+  const double f = odv_fact(q);
+  if (q - 1 == q2) return (q2 + 1.0) / f;
+  return (q2 + 1.0) / (f * lqq);
+}
+
+// Lane-local reductions (per-cell semantics: the reference at nvector = 1).
+struct LaneGroup {
+  ODV_HD double max_active(double v, bool) const { return v; }
+  ODV_HD bool any_valid(bool b) const { return b; }
+  ODV_HD bool any_active(bool b) const { return b; }
+};
+
+template <class Net, class M, bool RCP>
+struct Lane {
+  static constexpr int N = Net::NEQ;
+  double z[NCOL][N];     // Nordsieck history y_NS(:,:,0..MAXORD)
+  double en[N];          // accumulated correction
+  double enold[N];       // correction of the previous accepted step
+  double w[N];           // inverse squared error weights
+  M a[Net::NNZ];         // P = I - dt*beta*J, then its LU factors
+  double k[Net::NRATE];  // rate coefficients
+  double lq[7];          // l-vector of the current order
+  double t, dt, tstop, rate, jchange, cerr;
+  int q, swu, lju, fet, fc, status;
+  bool upd_jac, neglect_up, first;
+  bool active;           // lane takes part in norms / y updates (lockstep mask)
+  Counters cnt;
+  double flops64, flops32;
+};
+
+template <class L>
+ODV_HD inline void odv_set_order(L& s, int q) {
+  s.q = q;
+  odv_load_l(q, s.lq);
+}
+
+// 1/(rtol*|y| + atol)^2 (src/odevec.F90:706-716)
+template <int N>
+ODV_HD inline void odv_weights(const double* y, double rtol, double atol, double* w) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const double v = 1.0 / (rtol * fabs(y[j]) + atol);
+    w[j] = v * v;
+  }
+}
+
+// sqrt(sum_j x_j^2 w_j / neq) for one cell (src/odevec.F90:669-702)
+template <int N>
+ODV_HD inline double odv_wnorm(const double* x, const double* w) {
+  double s = (x[0] * x[0]) * w[0];
+#pragma unroll
+  for (int j = 1; j < N; ++j) s = s + (x[j] * x[j]) * w[j];
+  return sqrt(s / (double)N);
+}
+
+template <int N>
+ODV_HD inline bool odv_has_nan(const double* x) {
+  bool b = false;
+#pragma unroll
+  for (int j = 0; j < N; ++j) b = b || (x[j] != x[j]);
+  return b;
+}
+
+// PredictSolution: Pascal triangle with additions only (src/odevec.F90:769-793)
+template <class L>
+ODV_HD inline void odv_predict(L& s) {
+  constexpr int N = L::N;
+#pragma unroll
+  for (int l = 0; l < MAXORD; ++l) {
+#pragma unroll
+    for (int kk = MAXORD; kk > l; --kk) {
+      if (l < s.q && kk <= s.q) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) s.z[kk - 1][j] = s.z[kk][j] + s.z[kk - 1][j];
+      }
+    }
+  }
+}
+
+// ResetSystem: exact inverse sequence of the predictor (src/odevec.F90:526-544)
+template <class L>
+ODV_HD inline void odv_reset(L& s) {
+  constexpr int N = L::N;
+#pragma unroll
+  for (int kk = 0; kk < MAXORD; ++kk) {
+#pragma unroll
+    for (int jj = MAXORD; jj > kk; --jj) {
+      if (kk < s.q && jj <= s.q) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) s.z[jj - 1][j] = s.z[jj - 1][j] - s.z[jj][j];
+      }
+    }
+  }
+}
+
+// SetStepSize (src/odevec.F90:720-743)
+template <class L>
+ODV_HD inline void odv_set_step(L& s, double scale) {
+  constexpr int N = L::N;
+  double d = 1.0;
+#pragma unroll
+  for (int kk = 1; kk <= MAXORD; ++kk) {
+    if (kk <= s.q) {
+      d = scale * d;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s.z[kk][j] = s.z[kk][j] * d;
+    }
+  }
+  s.dt = s.dt * scale;
+  s.jchange = s.jchange * scale;
+}
+
+// CalcStepSizeOrder (src/odevec.F90:547-618); returns the step scale or a
+// negative value on the (unreachable in practice) order error.
+template <class L, class G>
+ODV_HD inline double odv_step_order(L& s, const G& g) {
+  constexpr int N = L::N;
+  const int q = s.q;
+  const double lqq = s.lq[q];
+  double down, same, up;
+  if (q == 1) {
+    down = 0.0;
+  } else {
+    double col[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double v = 0.0;
+#pragma unroll
+      for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.z[kk][j] : v;
+      col[j] = v;
+    }
+    const double nq = g.max_active(odv_wnorm<N>(col, s.w), s.active);
+    down = 1.0 / (ODV_F32(1.3) * odv::m_pow(nq / odv_tau(q, q - 1, lqq), 1.0 / q) + 1e-6);
+  }
+  const double ne = g.max_active(odv_wnorm<N>(s.en, s.w), s.active);
+  same = 1.0 / (ODV_F32(1.2) * odv::m_pow(ne / odv_tau(q, q, lqq), 1.0 / (q + 1.0)) + 1e-6);
+  if (q == MAXORD || s.neglect_up) {
+    up = 0.0;
+  } else {
+    double d[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) d[j] = s.en[j] - s.enold[j];
+    const double nd = g.max_active(odv_wnorm<N>(d, s.w), s.active);
+    up = 1.0 / (ODV_F32(1.4) * odv::m_pow(nd / odv_tau(q, q + 1, lqq), 1.0 / (q + 2.0)) + 1e-6);
+  }
+  int loc = 1;
+  double best = down;
+  if (same > best) { loc = 2; best = same; }
+  if (up > best) { loc = 3; best = up; }
+  double sc = best;
+  if (sc >= ODV_F32(1.1)) {
+    if (loc == 1) {
+      if (q == 1) return -1.0;
+      odv_set_order(s, q - 1);
+    } else if (loc == 2) {
+      sc = same;
+    } else {
+      if (q == MAXORD) return -1.0;
+#pragma unroll
+      for (int kk = 2; kk <= MAXORD; ++kk) {
+        if (kk == q + 1) {
+#pragma unroll
+          for (int j = 0; j < N; ++j) s.z[kk][j] = (lqq * s.en[j]) / (q + 1.0);
+        }
+      }
+      odv_set_order(s, q + 1);
+    }
+  } else {
+    sc = same;
+  }
+  const double cap = s.first ? 1e4 : 10.0;
+  s.first = false;
+  sc = fmin(cap, sc);
+  s.jchange = s.jchange * sc;
+  s.swu = 0;
+  s.neglect_up = false;
+  s.cnt.stepsel++;
+  s.flops64 += 10.0 * N + 20.0 + q * N;
+  return sc;
+}
+
+template <class Net, class M, bool RCP>
+ODV_HD inline void odv_factor(Lane<Net, M, RCP>& s, const double* y) {
+  const double hb = s.lq[0] * s.dt;
+  Net::template jac<M>(y, s.k, hb, s.a);
+  if (RCP) Net::template lu_r<M>(s.a);
+  else Net::template lu<M>(s.a);
+}
+
+template <class Net, class M, bool RCP>
+ODV_HD inline void odv_lin_solve(const Lane<Net, M, RCP>& s, double* x) {
+  constexpr int N = Net::NEQ;
+  if constexpr (sizeof(M) == sizeof(double)) {
+    if (RCP) Net::template solve_r<M, double>(s.a, x);
+    else Net::template solve<M, double>(s.a, x);
+  } else {
+    M xm[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) xm[j] = (M)x[j];
+    if (RCP) Net::template solve_r<M, M>(s.a, xm);
+    else Net::template solve<M, M>(s.a, xm);
+#pragma unroll
+    for (int j = 0; j < N; ++j) x[j] = (double)xm[j];
+  }
+}
+
+// Start of SolveODE for one lane (src/odevec.F90:170-204). `y` is the caller's
+// state, `dt_hint` the caller's dt (it only scales y_NS(:,:,1), Q2).
+// Returns true when nothing is to be integrated (status set).
+template <class Net, class M, bool RCP, class G>
+ODV_HD inline bool odv_begin(Lane<Net, M, RCP>& s, const double* y, const double* prm, double t0,
+                             double tstop, double dt_hint, const Controls& c, const G& g) {
+  constexpr int N = Net::NEQ;
+  s.first = true;
+  odv_set_order(s, 1);
+  s.swu = 0;
+  s.lju = 0;
+  s.rate = 0.7;
+  s.jchange = 0.0;
+  s.neglect_up = false;
+  s.fet = 0;
+  s.fc = 0;
+  s.upd_jac = true;
+  s.cerr = 0.0;
+  s.status = ST_RUNNING;
+  s.t = t0;
+  s.tstop = tstop;
+  s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
+  s.flops64 = 0.0;
+  s.flops32 = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) { s.en[j] = 0.0; s.enold[j] = 0.0; }
+  Net::rates(y, prm, s.k);
+  odv_weights<N>(y, c.rtol, c.atol, s.w);
+#pragma unroll
+  for (int kk = 0; kk < NCOL; ++kk)
+#pragma unroll
+    for (int j = 0; j < N; ++j) s.z[kk][j] = (kk == 0) ? y[j] : 0.0;
+  double f[N];
+  Net::rhs(y, s.k, f);
+#pragma unroll
+  for (int j = 0; j < N; ++j) s.z[1][j] = dt_hint * f[j];
+  // CalcInitialStepSize (src/odevec.F90:216-238)
+  const double tol = c.rtol;
+  double acc = (f[0] * f[0]) * ((s.w[0] * tol) * tol);
+#pragma unroll
+  for (int j = 1; j < N; ++j) acc = acc + (f[j] * f[j]) * ((s.w[j] * tol) * tol);
+  acc = g.max_active(acc, s.active);
+  const double W0 = fmax(fabs(t0), fabs(tstop));
+  double dt = sqrt(tol / (odv::m_pow(W0, -2.0) + acc / (double)N));
+  s.dt = fmin(dt, fabs(tstop - t0));
+  s.flops64 += Net::FLOPS_RHS + 12.0 * N + 8.0;
+  return false;
+}
+
+// InterpolateSolution (src/odevec.F90:998-1030): Horner back to t_stop.
+template <class Net, class M, bool RCP>
+ODV_HD inline void odv_interpolate(const Lane<Net, M, RCP>& s, double* y) {
+  constexpr int N = Net::NEQ;
+  const double r = (s.tstop - s.t) / s.dt;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double v = 0.0;
+#pragma unroll
+    for (int kk = 1; kk <= MAXORD; ++kk) v = (kk == s.q) ? s.z[kk][j] : v;
+#pragma unroll
+    for (int kk = MAXORD - 1; kk >= 0; --kk)
+      if (kk < s.q) v = s.z[kk][j] + r * v;
+    y[j] = v;
+  }
+}
+
+// One pass of the reference's `predictor:` loop (src/odevec.F90:259-456) plus,
+// on acceptance, the Jacobian-refresh policy (src/odevec.F90:458-464).
+// `y` is the lane's working copy of the caller's state (updated as the
+// reference updates its y argument). Returns true when the lane is finished.
+template <class Net, class M, bool RCP, class G>
+ODV_HD inline bool odv_attempt(Lane<Net, M, RCP>& s, double* y, const Controls& c, const G& g) {
+  constexpr int N = Net::NEQ;
+  constexpr bool MIXED = sizeof(M) != sizeof(double);
+  const double told = s.t;
+  s.cnt.attempts++;
+  s.cnt.qsum += s.q;
+  if (s.cnt.attempts > c.max_attempts) {
+    s.status = ST_MAX_STEPS;
+    return true;
+  }
+  s.t = s.t + s.dt;
+  odv_predict(s);
+  s.flops64 += 0.5 * s.q * (s.q + 1) * N;
+#pragma unroll
+  for (int j = 0; j < N; ++j) s.en[j] = 0.0;
+
+  if (c.check_negatives) {
+    bool neg = false;
+#pragma unroll
+    for (int j = 0; j < N; ++j) neg = neg || (s.z[0][j] < 0.0);
+    if (g.any_active(neg && s.active)) {
+      s.t = told;
+      odv_reset(s);
+      odv_set_step(s, 0.25);
+      if (s.dt <= c.dt_min) {
+        s.status = ST_DT_MIN;
+        return true;
+      }
+      return false;
+    }
+  }
+  if (s.active) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) y[j] = s.z[0][j];
+  }
+  const double beta = s.lq[0];
+  if (s.upd_jac) {
+    odv_factor(s, y);
+    s.upd_jac = false;
+    s.lju = 0;
+    s.rate = 0.7;
+    s.jchange = 1.0;
+    s.cnt.jac++;
+    s.flops64 += Net::FLOPS_JAC;
+    if (MIXED) s.flops32 += RCP ? Net::FLOPS_LU_R : Net::FLOPS_LU;
+    else s.flops64 += RCP ? Net::FLOPS_LU_R : Net::FLOPS_LU;
+  }
+
+  // corrector (src/odevec.F90:318-394) with CheckConvergence (src/odevec.F90:622-665)
+  int it = 0;
+  bool converged = false;
+  bool failed = false;
+  const double tauq = odv_tau(s.q, s.q, s.lq[s.q]);
+  while (true) {
+    double f[N];
+    Net::rhs(y, s.k, f);
+#pragma unroll
+    for (int j = 0; j < N; ++j) f[j] = s.dt * f[j] - (s.z[1][j] + s.en[j]);
+    odv_lin_solve(s, f);
+#pragma unroll
+    for (int j = 0; j < N; ++j) s.en[j] = s.en[j] + f[j];
+    if (s.active) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) y[j] = s.z[0][j] + beta * s.en[j];
+    }
+    s.cnt.newton++;
+    s.flops64 += Net::FLOPS_RHS + 9.0 * N + 8.0;
+    if (MIXED) s.flops32 += RCP ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
+    else s.flops64 += RCP ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
+
+    const double e = g.max_active(odv_wnorm<N>(f, s.w), s.active);
+    double rtmp = 0.0;
+    if (it != 0) {
+      rtmp = e / s.cerr;
+      s.rate = fmax(0.2 * s.rate, rtmp);
+    }
+    const double crit = e * fmin(1.0, 1.5 * s.rate) / (tauq / (2.0 * (s.q + 2.0)));
+    if (g.any_valid(odv_has_nan<N>(f) || odv_has_nan<N>(s.w))) {
+      failed = true;
+    } else if (crit < 1.0) {
+      converged = true;
+    } else if (it >= 2 && s.rate > 2.0 * rtmp) {
+      failed = true;
+    } else if (it == 3) {
+      failed = true;
+    } else {
+      failed = false;
+      s.cerr = e;
+      it++;
+    }
+    if (c.check_negatives) {
+      bool neg = false;
+#pragma unroll
+      for (int j = 0; j < N; ++j) neg = neg || (y[j] < 0.0);
+      if (g.any_active(neg && s.active)) {
+        failed = true;
+        converged = false;
+      }
+    }
+    if (failed && !converged) {
+      s.t = told;
+      s.fc++;
+      s.cnt.convfail++;
+      if (s.fc > 10) {
+        s.status = ST_CONV_FAIL;
+        return true;
+      }
+      odv_reset(s);
+      odv_set_step(s, 0.25);
+      s.flops64 += 0.5 * s.q * (s.q + 1) * N + s.q * N;
+      if (s.dt < c.dt_min) {
+        s.status = ST_DT_MIN;
+        return true;
+      }
+      s.upd_jac = true;
+      return false;
+    }
+    if (converged) break;
+  }
+  s.fc = 0;
+
+  // local truncation error test (src/odevec.F90:400-455)
+  const double err = g.max_active(odv_wnorm<N>(s.en, s.w), s.active) / tauq;
+  odv_weights<N>(y, c.rtol, c.atol, s.w);
+  s.flops64 += 7.0 * N + 2.0;
+  if (err > 1.0) {
+    s.cnt.errfail++;
+    if (s.fet >= 3) {
+      s.t = told;
+      odv_set_order(s, 1);
+      if (s.active) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) y[j] = s.z[0][j];
+      }
+      double f[N];
+      Net::rhs(y, s.k, f);
+#pragma unroll
+      for (int j = 0; j < N; ++j) s.z[1][j] = s.dt * f[j];
+      s.upd_jac = true;
+      s.flops64 += Net::FLOPS_RHS + N;
+    } else {
+      s.t = told;
+      odv_reset(s);
+      s.upd_jac = true;
+      s.swu = 0;
+      s.neglect_up = true;
+      s.flops64 += 0.5 * s.q * (s.q + 1) * N;
+    }
+    const double sc = odv_step_order(s, g);
+    if (sc < 0.0) {
+      s.status = ST_ORDER;
+      return true;
+    }
+    odv_set_step(s, sc);
+    s.fet++;
+    return false;
+  }
+  s.swu++;
+  s.lju++;
+  // UpdateNordsieck (src/odevec.F90:747-765)
+#pragma unroll
+  for (int kk = 0; kk <= MAXORD; ++kk) {
+    if (kk <= s.q) {
+      const double lk = s.lq[kk];
+#pragma unroll
+      for (int j = 0; j < N; ++j) s.z[kk][j] = s.z[kk][j] + s.en[j] * lk;
+    }
+  }
+  s.flops64 += 2.0 * (s.q + 1) * N;
+  if (s.swu == s.q + 1) {
+    const double sc = odv_step_order(s, g);
+    if (sc < 0.0) {
+      s.status = ST_ORDER;
+      return true;
+    }
+    odv_set_step(s, sc);
+  }
+  s.fet = 0;
+  s.cnt.steps++;
+  if (s.lju >= 20) s.upd_jac = true;
+  else if (fabs(s.jchange - 1.0) > ODV_F32(0.3)) s.upd_jac = true;
+#pragma unroll
+  for (int j = 0; j < N; ++j) s.enold[j] = s.en[j];
+  if (!(s.t < s.tstop)) {
+    odv_interpolate(s, y);
+    s.status = ST_OK;
+    return true;
+  }
+  return false;
+}
+
+}  // namespace ODV_VARIANT_NS
+}  // namespace odv

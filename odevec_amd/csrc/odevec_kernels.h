@@ -1,0 +1,256 @@
+// HIP kernels around the per-lane integrator.
+//
+//  * odv_k_percell     -- one lane per cell, one cell per lane (the plain form);
+//  * odv_k_persistent  -- a grid sized to the machine; a lane that finishes its
+//                         cell takes the next one from a queue (one atomic per
+//                         wavefront per refill round), so lanes whose cells need
+//                         few steps do not idle while the wavefront's slowest
+//                         cell finishes;
+//  * odv_k_lockstep    -- one workgroup per group of `group` cells sharing one
+//                         step size, order and verdicts: the reference's
+//                         semantics at nvector = group (max-norms over the
+//                         group, src/odevec.F90:669-702).
+//
+// Data layout in HBM (struct of arrays, cells fastest, as the reference's
+// Fortran y(nvector,neq)): y[j*ncells + i], params[p*ncells + i].
+#pragma once
+#include "odevec_integrator.h"
+
+namespace odv {
+inline namespace ODV_VARIANT_NS {
+
+struct KArgs {
+  double* y;              // [NEQ][ncells], in/out
+  long long ncells;
+  const double* prm;      // params: prm[p*ps_plane + i*ps_cell]
+  long long ps_plane, ps_cell;
+  const double* t0;       // t0[i*t0_s]
+  const double* tstop;    // tstop[i*ts_s]
+  const double* dt0;      // dt hint dt0[i*dt_s]
+  long long t0_s, ts_s, dt_s;
+  const signed char* mask;  // optional
+  int* status;            // optional [ncells]
+  double* dt_out;         // optional [ncells]: last step size
+  int* counters;          // optional [8][ncells]
+  double* flops;          // optional [2]: FP64, FP32 algorithmic flops (atomic)
+  unsigned int* queue;    // persistent: [0] = next cell (must start at grid lanes)
+  Controls ctl;
+  int group;              // lockstep group size
+};
+
+__device__ inline double odv_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <class Net, class M, bool RCP>
+__device__ inline void odv_store(const KArgs& a, long long i, const Lane<Net, M, RCP>& s,
+                                 const double* y) {
+  constexpr int N = Net::NEQ;
+#pragma unroll
+  for (int j = 0; j < N; ++j) a.y[(long long)j * a.ncells + i] = y[j];
+  if (a.status) a.status[i] = s.status;
+  if (a.dt_out) a.dt_out[i] = s.dt;
+  if (a.counters) {
+    const int* c = &s.cnt.attempts;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a.counters[(long long)m * a.ncells + i] = c[m];
+  }
+}
+
+template <class Net, class M, bool RCP>
+__device__ inline bool odv_load_begin(const KArgs& a, long long i, Lane<Net, M, RCP>& s, double* y) {
+  constexpr int N = Net::NEQ;
+  constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
+  double prm[NP];
+#pragma unroll
+  for (int j = 0; j < N; ++j) y[j] = a.y[(long long)j * a.ncells + i];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) prm[p] = (Net::NPARAM > 0) ? a.prm[p * a.ps_plane + i * a.ps_cell] : 0.0;
+  const double t0 = a.t0[i * a.t0_s];
+  const double ts = a.tstop[i * a.ts_s];
+  const double dh = a.dt0[i * a.dt_s];
+  s.active = true;
+  s.flops64 = 0.0;
+  s.flops32 = 0.0;
+  s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
+  s.dt = dh;
+  if ((a.mask && !a.mask[i]) || !(t0 < ts)) {
+    s.status = ST_SKIPPED;
+    return true;
+  }
+  LaneGroup g;
+  return odv_begin(s, y, prm, t0, ts, dh, a.ctl, g);
+}
+
+template <class Net, class M, bool RCP>
+__global__ void __launch_bounds__(256) odv_k_percell(KArgs a) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double fl64 = 0.0, fl32 = 0.0;
+  if (i < a.ncells) {
+    Lane<Net, M, RCP> s;
+    double y[Net::NEQ];
+    LaneGroup g;
+    bool done = odv_load_begin(a, i, s, y);
+    while (!done) done = odv_attempt(s, y, a.ctl, g);
+    odv_store(a, i, s, y);
+    fl64 = s.flops64;
+    fl32 = s.flops32;
+  }
+  if (a.flops) {
+    fl64 = odv_wave_sum(fl64);
+    fl32 = odv_wave_sum(fl32);
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&a.flops[0], fl64);
+      atomicAdd(&a.flops[1], fl32);
+    }
+  }
+}
+
+// Persistent form. Every lane starts on cell (global lane id); the queue
+// counter starts at the number of launched lanes. Whenever at least one lane
+// of a wavefront has finished, the wavefront reserves that many cells with a
+// single atomic and hands them out in lane order. A lane exits when the
+// queue is exhausted; the loop ends for every lane because each iteration
+// either finishes a cell attempt-bounded by max_attempts or exits.
+template <class Net, class M, bool RCP>
+__global__ void __launch_bounds__(256) odv_k_persistent(KArgs a) {
+  const long long nlanes = (long long)gridDim.x * blockDim.x;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  Lane<Net, M, RCP> s;
+  double y[Net::NEQ];
+  LaneGroup g;
+  double fl64 = 0.0, fl32 = 0.0;
+  bool have = i < a.ncells;
+  bool done = true;
+  bool exhausted = nlanes >= a.ncells;   // wave-uniform: no cell left to hand out
+  if (have) done = odv_load_begin(a, i, s, y);
+  while (true) {
+    // finish / refill
+    const bool need = !have || done;
+    if (have && done) {
+      odv_store(a, i, s, y);
+      fl64 += s.flops64;
+      fl32 += s.flops32;
+      have = false;
+    }
+    const unsigned long long ball = __ballot(need);
+    if (ball && !exhausted) {
+      const int cnt = __popcll(ball);
+      const int rank = __popcll(ball & ((1ull << lane) - 1ull));
+      const int leader = __ffsll((long long)ball) - 1;
+      unsigned int base = 0;
+      if (lane == leader) base = atomicAdd(a.queue, (unsigned int)cnt);
+      base = __shfl(base, leader, 64);
+      if ((long long)base + cnt >= a.ncells) exhausted = true;
+      if (need) {
+        const long long ni = (long long)base + rank;
+        if (ni < a.ncells) {
+          i = ni;
+          have = true;
+          done = odv_load_begin(a, i, s, y);
+        }
+      }
+    }
+    // all lanes of the wavefront without a cell: leave
+    if (__ballot(have) == 0ull) break;
+    if (have && !done) done = odv_attempt(s, y, a.ctl, g);
+  }
+  if (a.flops) {
+    fl64 = odv_wave_sum(fl64);
+    fl32 = odv_wave_sum(fl32);
+    if (lane == 0) {
+      atomicAdd(&a.flops[0], fl64);
+      atomicAdd(&a.flops[1], fl32);
+    }
+  }
+}
+
+// Lockstep group reductions over one workgroup (group <= blockDim.x).
+struct BlockGroup {
+  double* red;   // LDS scratch [blockDim/64]
+  int* redi;
+  bool valid;
+  __device__ double max_active(double v, bool act) const {
+    double x = act ? v : -INFINITY;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = x;
+    __syncthreads();
+    double r = red[0];
+    for (int k = 1; k < nw; ++k) r = fmax(r, red[k]);
+    return r;
+  }
+  __device__ bool any_of(bool b) const {
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const bool wb = __ballot(b) != 0ull;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) redi[w] = wb ? 1 : 0;
+    __syncthreads();
+    int r = 0;
+    for (int k = 0; k < nw; ++k) r |= redi[k];
+    return r != 0;
+  }
+  __device__ bool any_valid(bool b) const { return any_of(b && valid); }
+  __device__ bool any_active(bool b) const { return any_of(b); }
+};
+
+template <class Net, class M, bool RCP>
+__global__ void __launch_bounds__(1024) odv_k_lockstep(KArgs a) {
+  __shared__ double red[16];
+  __shared__ int redi[16];
+  constexpr int N = Net::NEQ;
+  constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
+  const long long g0 = (long long)blockIdx.x * a.group;
+  const long long i = g0 + threadIdx.x;
+  const bool valid = threadIdx.x < a.group && i < a.ncells;
+  BlockGroup g{red, redi, valid};
+  Lane<Net, M, RCP> s;
+  double y[N];
+  double prm[NP];
+  const long long ii = valid ? i : g0;   // padding lanes mirror the group's first cell
+#pragma unroll
+  for (int j = 0; j < N; ++j) y[j] = a.y[(long long)j * a.ncells + ii];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) prm[p] = (Net::NPARAM > 0) ? a.prm[p * a.ps_plane + ii * a.ps_cell] : 0.0;
+  const double t0 = a.t0[g0 * a.t0_s];
+  const double ts = a.tstop[g0 * a.ts_s];
+  const double dh = a.dt0[g0 * a.dt_s];
+  s.active = valid && (!a.mask || a.mask[ii]);
+  s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
+  s.flops64 = s.flops32 = 0.0;
+  s.dt = dh;
+  s.status = ST_SKIPPED;
+  // the group shares time, t_stop and the dt hint: those of its first cell
+  const bool any_act = g.any_of(s.active);
+  if (any_act && t0 < ts) {
+    odv_begin(s, y, prm, t0, ts, dh, a.ctl, g);
+    bool done = false;
+    while (!done) done = odv_attempt(s, y, a.ctl, g);
+    if (!s.active) s.status = ST_SKIPPED;
+  }
+  if (valid) {
+    if (s.active) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) a.y[(long long)j * a.ncells + i] = y[j];
+    }
+    if (a.status) a.status[i] = s.status;
+    if (a.dt_out) a.dt_out[i] = s.dt;
+    if (a.counters) {
+      const int* c = &s.cnt.attempts;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) a.counters[(long long)m * a.ncells + i] = c[m];
+    }
+  }
+  if (a.flops && threadIdx.x == 0) {
+    atomicAdd(&a.flops[0], s.flops64 * (double)min((long long)a.group, a.ncells - g0));
+    atomicAdd(&a.flops[1], s.flops32 * (double)min((long long)a.group, a.ncells - g0));
+  }
+}
+
+}  // namespace ODV_VARIANT_NS
+}  // namespace odv

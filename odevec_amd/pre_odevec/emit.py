@@ -1,0 +1,425 @@
+"""Emitters: straight-line HIP/C++ network code and the network IR (JSON).
+
+The emitted header defines one struct per (network, ordering, packaging)
+with static ``__host__ __device__`` members:
+
+* ``rates(y0, prm, k)``  -- rate coefficients, once per cell-solve;
+* ``rhs(y, k, f)``       -- f(y), the RHS (src/odevec_commons.F90:78-95);
+* ``jac(y, k, hb, a)``   -- the entries of ``P_order = I - hb*J`` in LU storage
+  order, fill entries zero (src/odevec_commons.F90:32-51,
+  pre_odevec.py:144-180);
+* ``lu(a)``              -- the pivot-free in-place factorization, unrolled for
+  the pattern: left-looking Crout for CSC (src/odevec.F90:929-971), Doolittle
+  for DENSE (src/odevec.F90:898-922);
+* ``solve(a, x)``        -- the triangular solves in place, with the ordering's
+  permutation folded into the indexing (src/odevec.F90:328-342, 797-891);
+* ``lu_r`` / ``solve_r`` -- the same with reciprocal pivots stored in the
+  diagonal slots (used by the FP32 form, where the pivot division is an
+  approximation anyway).
+
+Every evaluator in this project evaluates the network in one canonical
+order: a term is ``coef * (k * y_f0 * y_f1 * ...)`` (left to right), a row
+sums its terms left to right, a Jacobian entry sums ``(coef*e_j) * (k * y...)``
+over the row's terms, and ``P = 1 - hb*J`` on the diagonal, ``-(hb*J)`` off
+it.  The factorizations and solves follow the reference loops statement by
+statement with the structurally-zero operations (exact no-ops) removed.
+"""
+from __future__ import annotations
+
+import json
+import math
+from typing import Dict, List, Tuple
+
+from .structure import Structure, Term
+
+RATE_KIND_ID = {"none": 0, "param": 1, "primordial": 2, "arrhenius": 3}
+
+
+def _lit(v: float) -> str:
+    """Exact C literal of a double."""
+    if v == int(v) and abs(v) < 2 ** 53:
+        return "%d.0" % int(v)
+    r = repr(float(v))
+    if "e" not in r and "." not in r:
+        r += ".0"
+    return r
+
+
+def struct_name(st: Structure) -> str:
+    return "odv_net_%s_%s_%s" % (st.net.name, st.ordering.upper(), st.packaging)
+
+
+def network_id(st: Structure) -> str:
+    return "%s/%s/%s" % (st.net.name, st.ordering.upper(), st.packaging)
+
+
+class _Counter:
+    def __init__(self):
+        self.add = 0
+        self.mul = 0
+        self.div = 0
+
+    @property
+    def flops(self) -> int:
+        return self.add + self.mul + self.div
+
+    def as_dict(self) -> Dict[str, int]:
+        return {"add": self.add, "mul": self.mul, "div": self.div, "flops": self.flops}
+
+
+def _monomial(rate: int, factors: Tuple[int, ...], yname: str, cnt: _Counter) -> str:
+    parts = []
+    if rate >= 0:
+        parts.append("k[%d]" % rate)
+    parts += ["%s[%d]" % (yname, f) for f in factors]
+    if not parts:
+        return "1.0"
+    cnt.mul += len(parts) - 1
+    return " * ".join(parts)
+
+
+def _scaled(coef: float, var: str, cnt: _Counter) -> str:
+    if coef == 1.0:
+        return var
+    if coef == -1.0:
+        return "(-%s)" % var
+    cnt.mul += 1
+    return "(%s * %s)" % (_lit(coef), var)
+
+
+def _emit_rhs(st: Structure, cnt: _Counter) -> List[str]:
+    lines = []
+    flux: Dict[Tuple[int, Tuple[int, ...]], str] = {}
+    rows: Dict[int, List[str]] = {i: [] for i in range(st.neq)}
+    for t in st.terms:
+        key = (t.rate, t.factors)
+        if key not in flux:
+            name = "F%d" % len(flux)
+            flux[key] = name
+            lines.append("    const double %s = %s;" % (name, _monomial(t.rate, t.factors, "y", cnt)))
+        rows[t.row].append(_scaled(t.coef, flux[key], cnt))
+    for i in range(st.neq):
+        terms = rows[i]
+        if not terms:
+            lines.append("    f[%d] = 0.0;" % i)
+            continue
+        cnt.add += len(terms) - 1
+        expr = terms[0]
+        for tt in terms[1:]:
+            expr = "%s + %s" % (expr, tt) if not tt.startswith("(-") else "%s - %s" % (expr, tt[2:-1])
+        lines.append("    f[%d] = %s;" % (i, expr))
+    return lines
+
+
+def jacobian_contributions(st: Structure):
+    """{(row, col): [(coef*e, rate, reduced factors), ...]} in canonical order."""
+    contrib: Dict[Tuple[int, int], List[Tuple[float, int, Tuple[int, ...]]]] = {}
+    for t in st.terms:
+        for j in sorted(set(t.factors)):
+            e = t.factors.count(j)
+            red = list(t.factors)
+            red.remove(j)
+            contrib.setdefault((t.row, j), []).append((t.coef * e, t.rate, tuple(red)))
+    return contrib
+
+
+def _emit_jac(st: Structure, cnt: _Counter) -> List[str]:
+    lines = []
+    contrib = jacobian_contributions(st)
+    mono: Dict[Tuple[int, Tuple[int, ...]], str] = {}
+    jname: Dict[Tuple[int, int], str] = {}
+    for (r, c), lst in sorted(contrib.items(), key=lambda kv: (kv[0][1], kv[0][0])):
+        vals = []
+        for coef, rate, red in lst:
+            key = (rate, red)
+            if key not in mono:
+                nm = "D%d" % len(mono)
+                mono[key] = nm
+                lines.append("    const double %s = %s;" % (nm, _monomial(rate, red, "y", cnt)))
+            vals.append(_scaled(coef, mono[key], cnt))
+        nm = "J%d_%d" % (r, c)
+        jname[(r, c)] = nm
+        cnt.add += len(vals) - 1
+        expr = vals[0]
+        for v in vals[1:]:
+            expr = "%s + %s" % (expr, v) if not v.startswith("(-") else "%s - %s" % (expr, v[2:-1])
+        lines.append("    const double %s = %s;" % (nm, expr))
+    perm = [int(p) for p in st.perm]
+    for s, (r, c) in enumerate(st.csc.entries):
+        R, C = perm[r], perm[c]
+        if (R, C) in jname:
+            cnt.mul += 1
+            if R == C:
+                cnt.add += 1
+                val = "1.0 - hb * %s" % jname[(R, C)]
+            else:
+                val = "-(hb * %s)" % jname[(R, C)]
+        elif R == C:
+            val = "1.0"
+        else:
+            val = "0.0"      # fill entry (pre_odevec.py:175-177)
+        lines.append("    a[%d] = (M)(%s);" % (s, val))
+    return lines
+
+
+def _emit_lu_crout(st: Structure, cnt: _Counter, rcp: bool) -> List[str]:
+    cs = st.csc
+    idx = cs.index()
+    lines = []
+    n = st.neq
+    for k in range(n):
+        for kk in range(cs.u_col_start[k], cs.l_col_start[k]):
+            j = cs.row_index[kk]
+            dj = cs.l_col_start[j]
+            if rcp:
+                cnt.mul += 1
+                lines.append("    a[%d] = a[%d] * a[%d];" % (kk, kk, dj))
+            else:
+                cnt.div += 1
+                lines.append("    a[%d] = a[%d] / a[%d];" % (kk, kk, dj))
+            for jj in range(cs.l_col_start[j] + 1, cs.u_col_start[j + 1]):
+                r = cs.row_index[jj]
+                tgt = idx[(r, k)]
+                cnt.mul += 1
+                cnt.add += 1
+                lines.append("    a[%d] = a[%d] - a[%d] * a[%d];" % (tgt, tgt, kk, jj))
+        if rcp:
+            cnt.div += 1
+            d = cs.l_col_start[k]
+            lines.append("    a[%d] = (M)1 / a[%d];" % (d, d))
+    return lines
+
+
+def _emit_lu_doolittle(st: Structure, cnt: _Counter, rcp: bool) -> List[str]:
+    idx = st.csc.index()
+    n = st.neq
+    lines = []
+    for k in range(n):
+        dk = idx[(k, k)]
+        if rcp:
+            cnt.div += 1
+            lines.append("    a[%d] = (M)1 / a[%d];" % (dk, dk))
+        if k == n - 1:
+            break
+        for j in range(k + 1, n):
+            if (j, k) in idx:
+                s = idx[(j, k)]
+                if rcp:
+                    cnt.mul += 1
+                    lines.append("    a[%d] = a[%d] * a[%d];" % (s, s, dk))
+                else:
+                    cnt.div += 1
+                    lines.append("    a[%d] = a[%d] / a[%d];" % (s, s, dk))
+        for j in range(k + 1, n):
+            if (k, j) not in idx:
+                continue
+            for m in range(k + 1, n):
+                if (m, k) in idx:
+                    tgt = idx[(m, j)]
+                    cnt.mul += 1
+                    cnt.add += 1
+                    lines.append("    a[%d] = a[%d] - a[%d] * a[%d];" % (tgt, tgt, idx[(m, k)], idx[(k, j)]))
+    return lines
+
+
+def _emit_solve_crout(st: Structure, cnt: _Counter, rcp: bool) -> List[str]:
+    cs = st.csc
+    perm = [int(p) for p in st.perm]
+    X = lambda i: "x[%d]" % perm[i]
+    lines = []
+    n = st.neq
+    for k in range(n):
+        d = cs.l_col_start[k]
+        if rcp:
+            cnt.mul += 1
+            lines.append("    %s = %s * a[%d];" % (X(k), X(k), d))
+        else:
+            cnt.div += 1
+            lines.append("    %s = %s / a[%d];" % (X(k), X(k), d))
+        for kk in range(cs.l_col_start[k] + 1, cs.u_col_start[k + 1]):
+            r = cs.row_index[kk]
+            cnt.mul += 1
+            cnt.add += 1
+            lines.append("    %s = %s - %s * a[%d];" % (X(r), X(r), X(k), kk))
+    for k in range(n - 1, -1, -1):
+        for kk in range(cs.u_col_start[k], cs.l_col_start[k]):
+            r = cs.row_index[kk]
+            cnt.mul += 1
+            cnt.add += 1
+            lines.append("    %s = %s - %s * a[%d];" % (X(r), X(r), X(k), kk))
+    return lines
+
+
+def _emit_solve_doolittle(st: Structure, cnt: _Counter, rcp: bool) -> List[str]:
+    idx = st.csc.index()
+    perm = [int(p) for p in st.perm]
+    X = lambda i: "x[%d]" % perm[i]
+    n = st.neq
+    lines = []
+    for k in range(n):
+        prods = ["a[%d] * %s" % (idx[(k, j)], X(j)) for j in range(k) if (k, j) in idx]
+        if prods:
+            cnt.mul += len(prods)
+            cnt.add += len(prods)     # (len-1) sums + the subtraction from res
+            lines.append("    %s = %s - (%s);" % (X(k), X(k), " + ".join(prods)))
+    for k in range(n - 1, -1, -1):
+        prods = ["a[%d] * %s" % (idx[(k, j)], X(j)) for j in range(k + 1, n) if (k, j) in idx]
+        d = idx[(k, k)]
+        if prods:
+            cnt.mul += len(prods)
+            cnt.add += len(prods)
+            inner = "(%s - (%s))" % (X(k), " + ".join(prods))
+        else:
+            inner = X(k)
+        if rcp:
+            cnt.mul += 1
+            lines.append("    %s = %s * a[%d];" % (X(k), inner, d))
+        else:
+            cnt.div += 1
+            lines.append("    %s = %s / a[%d];" % (X(k), inner, d))
+    return lines
+
+
+def _emit_rates(st: Structure, cnt: _Counter) -> List[str]:
+    net = st.net
+    kind = net.rate_kind
+    if kind == "none":
+        return ["    (void)y0; (void)prm; (void)k;"]
+    if kind == "param":
+        return ["    (void)y0;"] + ["    k[%d] = prm[%d];" % (i, i) for i in range(net.nrate)]
+    if kind == "primordial":
+        return ["    (void)prm;",
+                "    odv_primordial_rates(y0, NEQ, TGAS, k);"]
+    if kind == "arrhenius":
+        lines = ["#pragma clang fp contract(off)", "    (void)y0;", "    const double invT = 1.0 / prm[0];"]
+        for r, (A, Ea) in enumerate(zip(net.arrhenius_A, net.arrhenius_Ea)):
+            lines.append("    k[%d] = %s * odv::m_exp(-(%s * invT));" % (r, _lit(A), _lit(Ea)))
+        return lines
+    raise ValueError(kind)
+
+
+def opcounts(st: Structure) -> Dict[str, Dict[str, int]]:
+    out = {}
+    for name, fn in (("rhs", lambda c: _emit_rhs(st, c)), ("jac", lambda c: _emit_jac(st, c)),
+                     ("lu", lambda c: (_emit_lu_crout if st.packaging == "CSC" else _emit_lu_doolittle)(st, c, False)),
+                     ("solve", lambda c: (_emit_solve_crout if st.packaging == "CSC" else _emit_solve_doolittle)(st, c, False)),
+                     ("lu_r", lambda c: (_emit_lu_crout if st.packaging == "CSC" else _emit_lu_doolittle)(st, c, True)),
+                     ("solve_r", lambda c: (_emit_solve_crout if st.packaging == "CSC" else _emit_solve_doolittle)(st, c, True))):
+        c = _Counter()
+        fn(c)
+        out[name] = c.as_dict()
+    return out
+
+
+def emit_header(st: Structure) -> str:
+    net = st.net
+    sn = struct_name(st)
+    crout = st.packaging == "CSC"
+    lu_fn = _emit_lu_crout if crout else _emit_lu_doolittle
+    sv_fn = _emit_solve_crout if crout else _emit_solve_doolittle
+    c = _Counter()
+    rates = _emit_rates(st, c)
+    rhs = _emit_rhs(st, _Counter())
+    jac = _emit_jac(st, _Counter())
+    lu = lu_fn(st, _Counter(), False)
+    sv = sv_fn(st, _Counter(), False)
+    lur = lu_fn(st, _Counter(), True)
+    svr = sv_fn(st, _Counter(), True)
+    ops = opcounts(st)
+    perm = ", ".join(str(int(p)) for p in st.perm)
+    hdr = []
+    hdr.append("// Generated by odevec_amd.pre_odevec -- do not edit.")
+    hdr.append("// network %s, ordering %s, packaging %s" % (net.name, st.ordering, st.packaging))
+    hdr.append("// neq %d, nnz(J) %d, nnz(P) %d, nnz(LU) %d, terms %d" % (
+        st.neq, int(st.J.sum()), int(st.P.sum()), st.csc.nnz, len(st.terms)))
+    hdr.append("// flops: " + ", ".join("%s %d" % (k, v["flops"]) for k, v in ops.items()))
+    hdr.append("#pragma once")
+    hdr.append('#include "odevec_netcommon.h"')
+    hdr.append("")
+    hdr.append("struct %s {" % sn)
+    hdr.append('  static constexpr const char* ID = "%s";' % network_id(st))
+    hdr.append("  static constexpr int NEQ = %d;" % st.neq)
+    hdr.append("  static constexpr int NNZ = %d;" % st.csc.nnz)
+    hdr.append("  static constexpr int NRATE = %d;" % max(net.nrate, 1))
+    hdr.append("  static constexpr int NPARAM = %d;" % net.nparam)
+    hdr.append("  static constexpr int RATE_KIND = %d;" % RATE_KIND_ID[net.rate_kind])
+    hdr.append("  static constexpr int TGAS = %d;" % net.tgas_index)
+    hdr.append("  static constexpr bool CROUT = %s;" % ("true" if crout else "false"))
+    hdr.append("  static constexpr int FLOPS_RHS = %d, FLOPS_JAC = %d, FLOPS_LU = %d, FLOPS_SOLVE = %d;" % (
+        ops["rhs"]["flops"], ops["jac"]["flops"], ops["lu"]["flops"], ops["solve"]["flops"]))
+    hdr.append("  static constexpr int FLOPS_LU_R = %d, FLOPS_SOLVE_R = %d;" % (
+        ops["lu_r"]["flops"], ops["solve_r"]["flops"]))
+    hdr.append("  // permutation Perm (0-based): P_order[i][j] = P[Perm[i]][Perm[j]]")
+    hdr.append("  // Perm = {%s}" % perm)
+    hdr.append("")
+    hdr.append("  static inline void param_defaults(double* p) {")
+    if net.nparam == 0:
+        hdr.append("    (void)p;")
+    for i, v in enumerate(net.param_defaults):
+        hdr.append("    p[%d] = %s;" % (i, _lit(v)))
+    hdr.append("  }")
+    hdr.append("")
+    hdr.append("  ODV_HD static inline void rates(const double* __restrict__ y0, const double* __restrict__ prm,")
+    hdr.append("                                 double* __restrict__ k) {")
+    hdr += rates
+    hdr.append("  }")
+    hdr.append("")
+    hdr.append("  ODV_HD static inline void rhs(const double* __restrict__ y, const double* __restrict__ k,")
+    hdr.append("                               double* __restrict__ f) {")
+    hdr += rhs
+    hdr.append("  }")
+    hdr.append("")
+    hdr.append("  template <class M>")
+    hdr.append("  ODV_HD static inline void jac(const double* __restrict__ y, const double* __restrict__ k,")
+    hdr.append("                               double hb, M* __restrict__ a) {")
+    hdr += jac
+    hdr.append("  }")
+    for name, body, sig in (("lu", lu, "M* __restrict__ a"), ("lu_r", lur, "M* __restrict__ a")):
+        hdr.append("")
+        hdr.append("  template <class M>")
+        hdr.append("  ODV_HD static inline void %s(%s) {" % (name, sig))
+        hdr += body
+        hdr.append("  }")
+    for name, body in (("solve", sv), ("solve_r", svr)):
+        hdr.append("")
+        hdr.append("  template <class M, class V>")
+        hdr.append("  ODV_HD static inline void %s(const M* __restrict__ a, V* __restrict__ x) {" % name)
+        hdr += body
+        hdr.append("  }")
+    hdr.append("};")
+    hdr.append("")
+    return "\n".join(hdr)
+
+
+def ir_dict(st: Structure) -> dict:
+    net = st.net
+    return {
+        "id": network_id(st),
+        "name": net.name,
+        "ordering": st.ordering,
+        "packaging": st.packaging,
+        "neq": st.neq,
+        "nrate": net.nrate,
+        "nparam": net.nparam,
+        "rate_kind": net.rate_kind,
+        "tgas_index": net.tgas_index,
+        "species": list(net.species),
+        "param_names": list(net.param_names),
+        "param_defaults": list(net.param_defaults),
+        "arrhenius_A": net.arrhenius_A,
+        "arrhenius_Ea": net.arrhenius_Ea,
+        "terms": [[t.row, t.coef, t.rate, list(t.factors)] for t in st.terms],
+        "perm": [int(p) for p in st.perm],
+        "nnz_J": int(st.J.sum()),
+        "nnz_P": int(st.P.sum()),
+        "lu_entries": [list(e) for e in st.csc.entries],
+        "row_index": st.csc.row_index,
+        "u_col_start": st.csc.u_col_start,
+        "l_col_start": st.csc.l_col_start,
+        "nnz": st.csc.nnz,
+        "opcounts": opcounts(st),
+    }
+
+
+def emit_ir_json(st: Structure) -> str:
+    return json.dumps(ir_dict(st), indent=1)

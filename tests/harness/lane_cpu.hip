@@ -1,0 +1,97 @@
+// Test harness: the product's per-lane integrator (odevec_integrator.h and the
+// generated network headers) compiled for the host, so tests can compare its
+// control flow and arithmetic with the oracle on the CPU. Not part of the
+// library; built by tests/conftest.py with --offload-host-only.
+#include <cstring>
+
+#include "odevec_integrator.h"
+#include "generated/net_ROBER_NONE_DENSE.h"
+#include "generated/net_ROBER_NONE_CSC.h"
+#include "generated/net_OREGO_NONE_DENSE.h"
+#include "generated/net_OREGO_NONE_CSC.h"
+#include "generated/net_PRIMORDIAL_CMK_CSC.h"
+#include "generated/net_PRIMORDIAL_NONE_DENSE.h"
+#include "generated/net_PRIMORDIAL_FF_CSC.h"
+
+namespace {
+
+template <class Net, class M, bool RCP>
+void run(long n, const double* t0, const double* ts, const double* dt0, double* y, const double* prm,
+         const odv::Controls& c, int* status, double* dtout, int* cnt) {
+  constexpr int N = Net::NEQ;
+  constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
+  for (long i = 0; i < n; ++i) {
+    odv::Lane<Net, M, RCP> s;
+    odv::LaneGroup g;
+    double yy[N], p[NP];
+    for (int j = 0; j < N; ++j) yy[j] = y[j * n + i];
+    for (int q = 0; q < NP; ++q) p[q] = Net::NPARAM > 0 ? prm[q * n + i] : 0.0;
+    s.active = true;
+    s.cnt = odv::Counters{0, 0, 0, 0, 0, 0, 0, 0};
+    s.dt = dt0[i];
+    bool done;
+    if (!(t0[i] < ts[i])) {
+      s.status = odv::ST_SKIPPED;
+      done = true;
+    } else {
+      done = odv::odv_begin(s, yy, p, t0[i], ts[i], dt0[i], c, g);
+    }
+    while (!done) done = odv::odv_attempt(s, yy, c, g);
+    for (int j = 0; j < N; ++j) y[j * n + i] = yy[j];
+    status[i] = s.status;
+    dtout[i] = s.dt;
+    const int* cc = &s.cnt.attempts;
+    for (int m = 0; m < 8; ++m) cnt[m * n + i] = cc[m];
+  }
+}
+
+template <class Net>
+void run_prec(int mixed, long n, const double* t0, const double* ts, const double* dt0, double* y,
+              const double* prm, const odv::Controls& c, int* status, double* dtout, int* cnt) {
+  if (mixed) run<Net, float, true>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+  else run<Net, double, false>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+}
+
+}  // namespace
+
+extern "C" int lane_solve_cells(const char* id, int mixed, long n, const double* t0, const double* ts,
+                                const double* dt0, double* y, const double* prm, double rtol, double atol,
+                                double dt_min, int check_neg, int max_att, int* status, double* dtout,
+                                int* cnt) {
+  odv::Controls c{rtol, atol, dt_min, check_neg, max_att};
+#define ODV_CASE(NET) \
+  if (!std::strcmp(id, NET::ID)) { run_prec<NET>(mixed, n, t0, ts, dt0, y, prm, c, status, dtout, cnt); return 0; }
+  ODV_CASE(odv_net_ROBER_NONE_DENSE)
+  ODV_CASE(odv_net_ROBER_NONE_CSC)
+  ODV_CASE(odv_net_OREGO_NONE_DENSE)
+  ODV_CASE(odv_net_OREGO_NONE_CSC)
+  ODV_CASE(odv_net_PRIMORDIAL_CMK_CSC)
+  ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE)
+  ODV_CASE(odv_net_PRIMORDIAL_FF_CSC)
+#undef ODV_CASE
+  return -1;
+}
+
+// generated network functions, host build, for unit tests of the emitted code
+extern "C" int lane_netfun(const char* id, int what, const double* y, const double* k, double hb, double* out) {
+#define ODV_CASE(NET)                                                      \
+  if (!std::strcmp(id, NET::ID)) {                                         \
+    if (what == 0) NET::rhs(y, k, out);                                    \
+    else if (what == 1) NET::rates(y, k, out);                             \
+    else if (what == 2) NET::template jac<double>(y, k, hb, out);          \
+    else if (what == 3) { NET::template jac<double>(y, k, hb, out); NET::template lu<double>(out); } \
+    else if (what == 4) { double r[NET::NEQ]; for (int j = 0; j < NET::NEQ; ++j) r[j] = y[j];          \
+                          NET::template solve<double, double>(k, r);                                   \
+                          for (int j = 0; j < NET::NEQ; ++j) out[j] = r[j]; }                          \
+    return 0;                                                              \
+  }
+  ODV_CASE(odv_net_ROBER_NONE_DENSE)
+  ODV_CASE(odv_net_ROBER_NONE_CSC)
+  ODV_CASE(odv_net_OREGO_NONE_DENSE)
+  ODV_CASE(odv_net_OREGO_NONE_CSC)
+  ODV_CASE(odv_net_PRIMORDIAL_CMK_CSC)
+  ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE)
+  ODV_CASE(odv_net_PRIMORDIAL_FF_CSC)
+#undef ODV_CASE
+  return -1;
+}
