@@ -64,7 +64,7 @@ typedef struct odevec_s odevec_t;
 typedef struct {
   int32_t precision;      /* ODEVEC_FP64 | ODEVEC_MIXED */
   int32_t mode;           /* ODEVEC_PERCELL | ODEVEC_LOCKSTEP | ODEVEC_PERSISTENT */
-  int32_t group;          /* lockstep group size, 1..1024; 0 = min(nvector, 1024) */
+  int32_t group;          /* lockstep group size, 1..256; 0 = min(nvector, 256) */
   int32_t block;          /* threads per workgroup (percell/persistent), 0 = 256 */
   int32_t grid;           /* persistent: workgroups, 0 = fill the device */
   int32_t max_attempts;   /* per-cell step-attempt budget, 0 = 1000000 */

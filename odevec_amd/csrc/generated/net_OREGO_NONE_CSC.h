@@ -28,8 +28,11 @@ struct odv_net_OREGO_NONE_CSC {
     (void)y0; (void)prm; (void)k;
   }
 
+  // k[r * S]: rate r of this cell (S = stride between a cell's rates)
+  template <int S = 1>
   ODV_HD static inline void rhs(const double* __restrict__ y, const double* __restrict__ k,
                                double* __restrict__ f) {
+    (void)k;
     const double F0 = y[0];
     const double F1 = y[0] * y[0];
     const double F2 = y[0] * y[1];
@@ -40,26 +43,27 @@ struct odv_net_OREGO_NONE_CSC {
     f[2] = (0.161 * F0) - 0.161 * F4;
   }
 
-  template <class M>
+  template <class M, int S = 1>
   ODV_HD static inline void jac(const double* __restrict__ y, const double* __restrict__ k,
                                double hb, M* __restrict__ a) {
+    (void)k; (void)y; (void)hb;
     const double D0 = 1.0;
     const double D1 = y[0];
     const double D2 = y[1];
     const double J0_0 = (77.27 * D0) - 0.0012942725 * D1 - 77.27 * D2;
-    const double J1_0 = (-0.012941633234114146 * D2);
-    const double J2_0 = (0.161 * D0);
-    const double J0_1 = (-77.27 * D1) + (77.27 * D0);
-    const double J1_1 = (-0.012941633234114146 * D1) - 0.012941633234114146 * D0;
-    const double J1_2 = (0.012941633234114146 * D0);
-    const double J2_2 = (-0.161 * D0);
     a[0] = (M)(1.0 - hb * J0_0);
+    const double J1_0 = (-0.012941633234114146 * D2);
     a[1] = (M)(-(hb * J1_0));
+    const double J2_0 = (0.161 * D0);
     a[2] = (M)(-(hb * J2_0));
+    const double J0_1 = (-77.27 * D1) + (77.27 * D0);
     a[3] = (M)(-(hb * J0_1));
+    const double J1_1 = (-0.012941633234114146 * D1) - 0.012941633234114146 * D0;
     a[4] = (M)(1.0 - hb * J1_1);
     a[5] = (M)(0.0);
+    const double J1_2 = (0.012941633234114146 * D0);
     a[6] = (M)(-(hb * J1_2));
+    const double J2_2 = (-0.161 * D0);
     a[7] = (M)(1.0 - hb * J2_2);
   }
 

@@ -33,36 +33,40 @@ struct odv_net_ROBER_NONE_CSC {
     k[2] = prm[2];
   }
 
+  // k[r * S]: rate r of this cell (S = stride between a cell's rates)
+  template <int S = 1>
   ODV_HD static inline void rhs(const double* __restrict__ y, const double* __restrict__ k,
                                double* __restrict__ f) {
-    const double F0 = k[0] * y[0];
-    const double F1 = k[2] * y[1] * y[2];
-    const double F2 = k[1] * y[1] * y[1];
+    (void)k;
+    const double F0 = k[0 * S] * y[0];
+    const double F1 = k[2 * S] * y[1] * y[2];
+    const double F2 = k[1 * S] * y[1] * y[1];
     f[0] = (-F0) + F1;
     f[1] = F0 - F2 - F1;
     f[2] = F2;
   }
 
-  template <class M>
+  template <class M, int S = 1>
   ODV_HD static inline void jac(const double* __restrict__ y, const double* __restrict__ k,
                                double hb, M* __restrict__ a) {
-    const double D0 = k[0];
+    (void)k; (void)y; (void)hb;
+    const double D0 = k[0 * S];
     const double J0_0 = (-D0);
-    const double J1_0 = D0;
-    const double D1 = k[2] * y[2];
-    const double J0_1 = D1;
-    const double D2 = k[1] * y[1];
-    const double J1_1 = (-2.0 * D2) - D1;
-    const double J2_1 = (2.0 * D2);
-    const double D3 = k[2] * y[1];
-    const double J0_2 = D3;
-    const double J1_2 = (-D3);
     a[0] = (M)(1.0 - hb * J0_0);
+    const double J1_0 = D0;
     a[1] = (M)(-(hb * J1_0));
+    const double D1 = k[2 * S] * y[2];
+    const double J0_1 = D1;
     a[2] = (M)(-(hb * J0_1));
+    const double D2 = k[1 * S] * y[1];
+    const double J1_1 = (-2.0 * D2) - D1;
     a[3] = (M)(1.0 - hb * J1_1);
+    const double J2_1 = (2.0 * D2);
     a[4] = (M)(-(hb * J2_1));
+    const double D3 = k[2 * S] * y[1];
+    const double J0_2 = D3;
     a[5] = (M)(-(hb * J0_2));
+    const double J1_2 = (-D3);
     a[6] = (M)(-(hb * J1_2));
     a[7] = (M)(1.0);
   }

@@ -159,7 +159,7 @@ int launch(odevec_s* h, const Req& r) {
   a.ctl.check_negatives = h->check_neg;
   a.ctl.max_attempts = h->opts.max_attempts > 0 ? h->opts.max_attempts : 1000000;
   int grp = h->opts.group;
-  if (grp <= 0) grp = (int)(h->nvector < 1024 ? h->nvector : 1024);
+  if (grp <= 0) grp = (int)(h->nvector < 256 ? h->nvector : 256);
   if (grp < 1) grp = 1;
   a.group = grp;
   hipError_t e = h->net->launch(a, h->opts.mode, h->opts.precision, h->opts.block, h->opts.grid, r.stream);
@@ -255,7 +255,7 @@ int32_t odevec_create(const char* network, int64_t nvector, const odevec_opts_t*
   if (opts) o = *opts;
   if (o.precision != ODEVEC_FP64 && o.precision != ODEVEC_MIXED) return fail(ODEVEC_EINVAL, "bad precision");
   if (o.mode < ODEVEC_PERCELL || o.mode > ODEVEC_PERSISTENT) return fail(ODEVEC_EINVAL, "bad mode");
-  if (o.group < 0 || o.group > 1024) return fail(ODEVEC_EINVAL, "group must be in 0..1024");
+  if (o.group < 0 || o.group > 256) return fail(ODEVEC_EINVAL, "group must be in 0..256");
   if (o.block != 0 && (o.block < 64 || o.block > 256 || o.block % 64)) return fail(ODEVEC_EINVAL, "block must be 64..256, a multiple of 64");
   int ndev = 0;
   hipError_t he = hipGetDeviceCount(&ndev);

@@ -9,9 +9,17 @@
 //
 // All loops over species and Nordsieck columns are fully unrolled against
 // compile-time sizes, and the runtime order q only predicates them, so the
-// state stays in registers.  The control constants that are default-real
-// literals in the reference (1.3, 1.2, 1.4, 1.1, 0.3) are used at their
-// single-precision values, as the reference promotes them.
+// hot state (Nordsieck history, correction, weights, LU factors) stays in
+// registers.  The rate coefficients and the previous correction, read once
+// per RHS / once per step-size selection, can live outside the lane (LDS
+// columns with a stride, `Cfg::KS > 0`).  The current iterate y is not kept:
+// in the reference it always equals y_NS(:,:,0) + beta*en for the cells it
+// integrates (src/odevec.F90:293-299, 345-352); only a lockstep group keeps the
+// caller's y for cells outside the mask (`Cfg::KEEPY`).
+//
+// The control constants that are default-real literals in the reference
+// (1.3, 1.2, 1.4, 1.1, 0.3) are used at their single-precision values, as the
+// reference promotes them.
 #pragma once
 #include "odevec_netcommon.h"
 
@@ -47,6 +55,22 @@ struct Counters {
   int convfail;    // convergence failures
   int stepsel;     // step-size/order selections
   int qsum;        // sum of q over attempts
+};
+
+// Build-time configuration of one lane.
+//  Net   -- generated network struct
+//  M     -- element type of P and its LU factors (double, or float for the mixed form)
+//  RCP   -- reciprocal pivots stored in the LU diagonal (mixed form)
+//  KS    -- 0: rates and previous correction in the lane; > 0: in external
+//           columns, element r of this lane at ptr[r*KS]
+//  KEEPY -- keep the caller's y for lanes outside the mask (lockstep)
+template <class Net_, class M_, bool RCP_, int KS_, bool KEEPY_>
+struct Cfg {
+  using Net = Net_;
+  using M = M_;
+  static constexpr bool RCP = RCP_;
+  static constexpr int KS = KS_;
+  static constexpr bool KEEPY = KEEPY_;
 };
 
 // l-vector of the order-q BDF method in Nordsieck form (src/odevec.F90:127-134);
@@ -85,28 +109,77 @@ struct LaneGroup {
   ODV_HD bool any_active(bool b) const { return b; }
 };
 
-template <class Net, class M, bool RCP>
+template <class C>
 struct Lane {
+  using Net = typename C::Net;
+  using M = typename C::M;
   static constexpr int N = Net::NEQ;
+  static constexpr int KS = C::KS;
+  static constexpr bool RCP_ = C::RCP;
+  static constexpr bool KEEPY_ = C::KEEPY;
   double z[NCOL][N];     // Nordsieck history y_NS(:,:,0..MAXORD)
   double en[N];          // accumulated correction
-  double enold[N];       // correction of the previous accepted step
   double w[N];           // inverse squared error weights
   M a[Net::NNZ];         // P = I - dt*beta*J, then its LU factors
-  double k[Net::NRATE];  // rate coefficients
-  double lq[7];          // l-vector of the current order
+  double kl[KS == 0 ? Net::NRATE : 1];   // rate coefficients (KS == 0)
+  double eol[KS == 0 ? N : 1];           // previous accepted correction (KS == 0)
+  double yin[C::KEEPY ? N : 1];          // caller's y (KEEPY)
+  double* kx;            // external rate column (KS > 0)
+  double* eox;           // external previous-correction column (KS > 0)
+  double lq[7];          // l-vector of the current order (constant indices only)
+  double lqq;            // l_q(q)
   double t, dt, tstop, rate, jchange, cerr;
   int q, swu, lju, fet, fc, status;
   bool upd_jac, neglect_up, first;
   bool active;           // lane takes part in norms / y updates (lockstep mask)
   Counters cnt;
   double flops64, flops32;
+
+  ODV_HD const double* kptr() const { return KS == 0 ? kl : kx; }
+  ODV_HD double eo(int j) const { return KS == 0 ? eol[j] : eox[j * KS]; }
+  ODV_HD void set_eo(int j, double v) {
+    if (KS == 0) eol[j] = v;
+    else eox[j * KS] = v;
+  }
 };
 
 template <class L>
 ODV_HD inline void odv_set_order(L& s, int q) {
   s.q = q;
   odv_load_l(q, s.lq);
+  double v = s.lq[1];
+#pragma unroll
+  for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.lq[kk] : v;
+  s.lqq = v;
+}
+
+// the reference's working y after a corrector update: y_NS(:,:,0) + beta*en
+// (src/odevec.F90:345-352), or the caller's y outside the mask
+template <class L>
+ODV_HD inline void odv_cur_y(const L& s, double beta, double* y) {
+  constexpr int N = L::N;
+#pragma unroll
+  for (int j = 0; j < N; ++j) y[j] = s.z[0][j] + beta * s.en[j];
+  if constexpr (L::KEEPY_) {
+    if (!s.active) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) y[j] = s.yin[j];
+    }
+  }
+}
+
+// the reference's working y after the predictor: y_NS(:,:,0) (src/odevec.F90:293-299)
+template <class L>
+ODV_HD inline void odv_pred_y(const L& s, double* y) {
+  constexpr int N = L::N;
+#pragma unroll
+  for (int j = 0; j < N; ++j) y[j] = s.z[0][j];
+  if constexpr (L::KEEPY_) {
+    if (!s.active) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) y[j] = s.yin[j];
+    }
+  }
 }
 
 // 1/(rtol*|y| + atol)^2 (src/odevec.F90:706-716)
@@ -191,7 +264,7 @@ template <class L, class G>
 ODV_HD inline double odv_step_order(L& s, const G& g) {
   constexpr int N = L::N;
   const int q = s.q;
-  const double lqq = s.lq[q];
+  const double lqq = s.lqq;
   double down, same, up;
   if (q == 1) {
     down = 0.0;
@@ -214,7 +287,7 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
   } else {
     double d[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) d[j] = s.en[j] - s.enold[j];
+    for (int j = 0; j < N; ++j) d[j] = s.en[j] - s.eo(j);
     const double nd = g.max_active(odv_wnorm<N>(d, s.w), s.active);
     up = 1.0 / (ODV_F32(1.4) * odv::m_pow(nd / odv_tau(q, q + 1, lqq), 1.0 / (q + 2.0)) + 1e-6);
   }
@@ -254,37 +327,50 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
   return sc;
 }
 
-template <class Net, class M, bool RCP>
-ODV_HD inline void odv_factor(Lane<Net, M, RCP>& s, const double* y) {
+// GetJac + LUDecompose (src/odevec.F90:301-304) at y = y_NS(:,:,0)
+template <class L>
+ODV_HD inline void odv_factor(L& s, const double* y) {
+  using Net = typename L::Net;
+  using M = typename L::M;
+  constexpr int S = L::KS > 0 ? L::KS : 1;
   const double hb = s.lq[0] * s.dt;
-  Net::template jac<M>(y, s.k, hb, s.a);
-  if (RCP) Net::template lu_r<M>(s.a);
+  Net::template jac<M, S>(y, s.kptr(), hb, s.a);
+  if (L::RCP_) Net::template lu_r<M>(s.a);
   else Net::template lu<M>(s.a);
 }
 
-template <class Net, class M, bool RCP>
-ODV_HD inline void odv_lin_solve(const Lane<Net, M, RCP>& s, double* x) {
+template <class L>
+ODV_HD inline void odv_lin_solve(const L& s, double* x) {
+  using Net = typename L::Net;
+  using M = typename L::M;
   constexpr int N = Net::NEQ;
   if constexpr (sizeof(M) == sizeof(double)) {
-    if (RCP) Net::template solve_r<M, double>(s.a, x);
+    if (L::RCP_) Net::template solve_r<M, double>(s.a, x);
     else Net::template solve<M, double>(s.a, x);
   } else {
     M xm[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) xm[j] = (M)x[j];
-    if (RCP) Net::template solve_r<M, M>(s.a, xm);
+    if (L::RCP_) Net::template solve_r<M, M>(s.a, xm);
     else Net::template solve<M, M>(s.a, xm);
 #pragma unroll
     for (int j = 0; j < N; ++j) x[j] = (double)xm[j];
   }
 }
 
+template <class L>
+ODV_HD inline void odv_rhs(const L& s, const double* y, double* f) {
+  constexpr int S = L::KS > 0 ? L::KS : 1;
+  L::Net::template rhs<S>(y, s.kptr(), f);
+}
+
 // Start of SolveODE for one lane (src/odevec.F90:170-204). `y` is the caller's
-// state, `dt_hint` the caller's dt (it only scales y_NS(:,:,1), Q2).
-// Returns true when nothing is to be integrated (status set).
-template <class Net, class M, bool RCP, class G>
-ODV_HD inline bool odv_begin(Lane<Net, M, RCP>& s, const double* y, const double* prm, double t0,
-                             double tstop, double dt_hint, const Controls& c, const G& g) {
+// state, `dt_hint` the caller's dt (it only scales y_NS(:,:,1), Q2). The
+// external columns (KS > 0) must be set before the call.
+template <class L, class G>
+ODV_HD inline void odv_begin(L& s, const double* y, const double* prm, double t0, double tstop,
+                             double dt_hint, const Controls& c, const G& g) {
+  using Net = typename L::Net;
   constexpr int N = Net::NEQ;
   s.first = true;
   odv_set_order(s, 1);
@@ -303,16 +389,31 @@ ODV_HD inline bool odv_begin(Lane<Net, M, RCP>& s, const double* y, const double
   s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
   s.flops64 = 0.0;
   s.flops32 = 0.0;
+  {
+    double kk[Net::NRATE];
+    Net::rates(y, prm, kk);
 #pragma unroll
-  for (int j = 0; j < N; ++j) { s.en[j] = 0.0; s.enold[j] = 0.0; }
-  Net::rates(y, prm, s.k);
+    for (int r = 0; r < Net::NRATE; ++r) {
+      if (L::KS == 0) s.kl[r] = kk[r];
+      else s.kx[r * L::KS] = kk[r];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    s.en[j] = 0.0;
+    s.set_eo(j, 0.0);
+  }
+  if constexpr (L::KEEPY_) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) s.yin[j] = y[j];
+  }
   odv_weights<N>(y, c.rtol, c.atol, s.w);
 #pragma unroll
   for (int kk = 0; kk < NCOL; ++kk)
 #pragma unroll
     for (int j = 0; j < N; ++j) s.z[kk][j] = (kk == 0) ? y[j] : 0.0;
   double f[N];
-  Net::rhs(y, s.k, f);
+  odv_rhs(s, y, f);
 #pragma unroll
   for (int j = 0; j < N; ++j) s.z[1][j] = dt_hint * f[j];
   // CalcInitialStepSize (src/odevec.F90:216-238)
@@ -325,13 +426,12 @@ ODV_HD inline bool odv_begin(Lane<Net, M, RCP>& s, const double* y, const double
   double dt = sqrt(tol / (odv::m_pow(W0, -2.0) + acc / (double)N));
   s.dt = fmin(dt, fabs(tstop - t0));
   s.flops64 += Net::FLOPS_RHS + 12.0 * N + 8.0;
-  return false;
 }
 
 // InterpolateSolution (src/odevec.F90:998-1030): Horner back to t_stop.
-template <class Net, class M, bool RCP>
-ODV_HD inline void odv_interpolate(const Lane<Net, M, RCP>& s, double* y) {
-  constexpr int N = Net::NEQ;
+template <class L>
+ODV_HD inline void odv_interpolate(const L& s, double* y) {
+  constexpr int N = L::N;
   const double r = (s.tstop - s.t) / s.dt;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
@@ -345,12 +445,26 @@ ODV_HD inline void odv_interpolate(const Lane<Net, M, RCP>& s, double* y) {
   }
 }
 
+// Result of a finished lane: the interpolated solution at t_stop, or on a
+// failure status the Nordsieck value y_NS(:,:,0) at the moment of failure.
+template <class L>
+ODV_HD inline void odv_result(const L& s, double* y) {
+  constexpr int N = L::N;
+  if (s.status == ST_OK) {
+    odv_interpolate(s, y);
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) y[j] = s.z[0][j];
+  }
+}
+
 // One pass of the reference's `predictor:` loop (src/odevec.F90:259-456) plus,
 // on acceptance, the Jacobian-refresh policy (src/odevec.F90:458-464).
-// `y` is the lane's working copy of the caller's state (updated as the
-// reference updates its y argument). Returns true when the lane is finished.
-template <class Net, class M, bool RCP, class G>
-ODV_HD inline bool odv_attempt(Lane<Net, M, RCP>& s, double* y, const Controls& c, const G& g) {
+// Returns true when the lane is finished (status set).
+template <class L, class G>
+ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
+  using Net = typename L::Net;
+  using M = typename L::M;
   constexpr int N = Net::NEQ;
   constexpr bool MIXED = sizeof(M) != sizeof(double);
   const double told = s.t;
@@ -381,12 +495,10 @@ ODV_HD inline bool odv_attempt(Lane<Net, M, RCP>& s, double* y, const Controls& 
       return false;
     }
   }
-  if (s.active) {
-#pragma unroll
-    for (int j = 0; j < N; ++j) y[j] = s.z[0][j];
-  }
   const double beta = s.lq[0];
   if (s.upd_jac) {
+    double y[N];
+    odv_pred_y(s, y);
     odv_factor(s, y);
     s.upd_jac = false;
     s.lju = 0;
@@ -394,31 +506,34 @@ ODV_HD inline bool odv_attempt(Lane<Net, M, RCP>& s, double* y, const Controls& 
     s.jchange = 1.0;
     s.cnt.jac++;
     s.flops64 += Net::FLOPS_JAC;
-    if (MIXED) s.flops32 += RCP ? Net::FLOPS_LU_R : Net::FLOPS_LU;
-    else s.flops64 += RCP ? Net::FLOPS_LU_R : Net::FLOPS_LU;
+    if (MIXED) s.flops32 += L::RCP_ ? Net::FLOPS_LU_R : Net::FLOPS_LU;
+    else s.flops64 += L::RCP_ ? Net::FLOPS_LU_R : Net::FLOPS_LU;
   }
 
   // corrector (src/odevec.F90:318-394) with CheckConvergence (src/odevec.F90:622-665)
   int it = 0;
   bool converged = false;
   bool failed = false;
-  const double tauq = odv_tau(s.q, s.q, s.lq[s.q]);
+  const double tauq = odv_tau(s.q, s.q, s.lqq);
+  bool first_it = true;
   while (true) {
     double f[N];
-    Net::rhs(y, s.k, f);
+    {
+      double y[N];
+      if (first_it) odv_pred_y(s, y);
+      else odv_cur_y(s, beta, y);
+      odv_rhs(s, y, f);
+    }
+    first_it = false;
 #pragma unroll
     for (int j = 0; j < N; ++j) f[j] = s.dt * f[j] - (s.z[1][j] + s.en[j]);
     odv_lin_solve(s, f);
 #pragma unroll
     for (int j = 0; j < N; ++j) s.en[j] = s.en[j] + f[j];
-    if (s.active) {
-#pragma unroll
-      for (int j = 0; j < N; ++j) y[j] = s.z[0][j] + beta * s.en[j];
-    }
     s.cnt.newton++;
     s.flops64 += Net::FLOPS_RHS + 9.0 * N + 8.0;
-    if (MIXED) s.flops32 += RCP ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
-    else s.flops64 += RCP ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
+    if (MIXED) s.flops32 += L::RCP_ ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
+    else s.flops64 += L::RCP_ ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
 
     const double e = g.max_active(odv_wnorm<N>(f, s.w), s.active);
     double rtmp = 0.0;
@@ -441,6 +556,8 @@ ODV_HD inline bool odv_attempt(Lane<Net, M, RCP>& s, double* y, const Controls& 
       it++;
     }
     if (c.check_negatives) {
+      double y[N];
+      odv_cur_y(s, beta, y);
       bool neg = false;
 #pragma unroll
       for (int j = 0; j < N; ++j) neg = neg || (y[j] < 0.0);
@@ -473,19 +590,20 @@ ODV_HD inline bool odv_attempt(Lane<Net, M, RCP>& s, double* y, const Controls& 
 
   // local truncation error test (src/odevec.F90:400-455)
   const double err = g.max_active(odv_wnorm<N>(s.en, s.w), s.active) / tauq;
-  odv_weights<N>(y, c.rtol, c.atol, s.w);
+  {
+    double y[N];
+    odv_cur_y(s, beta, y);
+    odv_weights<N>(y, c.rtol, c.atol, s.w);
+  }
   s.flops64 += 7.0 * N + 2.0;
   if (err > 1.0) {
     s.cnt.errfail++;
     if (s.fet >= 3) {
       s.t = told;
       odv_set_order(s, 1);
-      if (s.active) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) y[j] = s.z[0][j];
-      }
-      double f[N];
-      Net::rhs(y, s.k, f);
+      double y[N], f[N];
+      odv_pred_y(s, y);   // y = y_NS(:,:,0), still the predicted value
+      odv_rhs(s, y, f);
 #pragma unroll
       for (int j = 0; j < N; ++j) s.z[1][j] = s.dt * f[j];
       s.upd_jac = true;
@@ -532,9 +650,8 @@ ODV_HD inline bool odv_attempt(Lane<Net, M, RCP>& s, double* y, const Controls& 
   if (s.lju >= 20) s.upd_jac = true;
   else if (fabs(s.jchange - 1.0) > ODV_F32(0.3)) s.upd_jac = true;
 #pragma unroll
-  for (int j = 0; j < N; ++j) s.enold[j] = s.en[j];
+  for (int j = 0; j < N; ++j) s.set_eo(j, s.en[j]);
   if (!(s.t < s.tstop)) {
-    odv_interpolate(s, y);
     s.status = ST_OK;
     return true;
   }

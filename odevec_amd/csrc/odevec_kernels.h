@@ -13,6 +13,10 @@
 //
 // Data layout in HBM (struct of arrays, cells fastest, as the reference's
 // Fortran y(nvector,neq)): y[j*ncells + i], params[p*ncells + i].
+// For networks with neq >= 8 the per-lane rate coefficients and previous
+// correction live in LDS as columns of the workgroup (element r of lane t at
+// [r*BLOCK + t]: consecutive lanes hit consecutive banks), which keeps the
+// register file for the Nordsieck history and the LU factors.
 #pragma once
 #include "odevec_integrator.h"
 
@@ -33,9 +37,19 @@ struct KArgs {
   double* dt_out;         // optional [ncells]: last step size
   int* counters;          // optional [8][ncells]
   double* flops;          // optional [2]: FP64, FP32 algorithmic flops (atomic)
-  unsigned int* queue;    // persistent: [0] = next cell (must start at grid lanes)
+  unsigned int* queue;    // persistent: next cell (set to the number of launched lanes)
   Controls ctl;
   int group;              // lockstep group size
+};
+
+constexpr int BLOCK = 256;   // lanes per workgroup of the per-cell kernels
+
+template <class Net>
+struct LdsPolicy {
+  static constexpr bool USE = Net::NEQ >= 8 && (Net::NRATE + Net::NEQ) * 8 * BLOCK <= 144 * 1024;
+  static constexpr int KS = USE ? BLOCK : 0;
+  static constexpr int KWORDS = USE ? Net::NRATE * BLOCK : 1;
+  static constexpr int EWORDS = USE ? Net::NEQ * BLOCK : 1;
 };
 
 __device__ inline double odv_wave_sum(double v) {
@@ -44,26 +58,36 @@ __device__ inline double odv_wave_sum(double v) {
   return v;
 }
 
-template <class Net, class M, bool RCP>
-__device__ inline void odv_store(const KArgs& a, long long i, const Lane<Net, M, RCP>& s,
-                                 const double* y) {
-  constexpr int N = Net::NEQ;
+__device__ inline void odv_store_counters(const KArgs& a, long long i, const Counters& c) {
+  const long long n = a.ncells;
+  a.counters[0 * n + i] = c.attempts;
+  a.counters[1 * n + i] = c.steps;
+  a.counters[2 * n + i] = c.newton;
+  a.counters[3 * n + i] = c.jac;
+  a.counters[4 * n + i] = c.errfail;
+  a.counters[5 * n + i] = c.convfail;
+  a.counters[6 * n + i] = c.stepsel;
+  a.counters[7 * n + i] = c.qsum;
+}
+
+template <class L>
+__device__ inline void odv_store(const KArgs& a, long long i, const L& s, const double* y) {
+  constexpr int N = L::N;
 #pragma unroll
   for (int j = 0; j < N; ++j) a.y[(long long)j * a.ncells + i] = y[j];
   if (a.status) a.status[i] = s.status;
   if (a.dt_out) a.dt_out[i] = s.dt;
-  if (a.counters) {
-    const int* c = &s.cnt.attempts;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) a.counters[(long long)m * a.ncells + i] = c[m];
-  }
+  if (a.counters) odv_store_counters(a, i, s.cnt);
 }
 
-template <class Net, class M, bool RCP>
-__device__ inline bool odv_load_begin(const KArgs& a, long long i, Lane<Net, M, RCP>& s, double* y) {
+// Load cell i into the lane and start its solve; returns true if there is
+// nothing to integrate (masked out or t_stop <= t0; y is then left as is).
+template <class L>
+__device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
+  using Net = typename L::Net;
   constexpr int N = Net::NEQ;
   constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
-  double prm[NP];
+  double y[N], prm[NP];
 #pragma unroll
   for (int j = 0; j < N; ++j) y[j] = a.y[(long long)j * a.ncells + i];
 #pragma unroll
@@ -81,20 +105,39 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, Lane<Net, M, 
     return true;
   }
   LaneGroup g;
-  return odv_begin(s, y, prm, t0, ts, dh, a.ctl, g);
+  odv_begin(s, y, prm, t0, ts, dh, a.ctl, g);
+  return false;
+}
+
+template <class L>
+__device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
+  if (s.status == ST_SKIPPED) {
+    if (a.status) a.status[i] = s.status;
+    if (a.dt_out) a.dt_out[i] = s.dt;
+    if (a.counters) odv_store_counters(a, i, s.cnt);
+    return;
+  }
+  double y[L::N];
+  odv_result(s, y);
+  odv_store(a, i, s, y);
 }
 
 template <class Net, class M, bool RCP>
-__global__ void __launch_bounds__(256) odv_k_percell(KArgs a) {
+__global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
+  using P = LdsPolicy<Net>;
+  using L = Lane<Cfg<Net, M, RCP, P::KS, false>>;
+  __shared__ double lds_k[P::KWORDS];
+  __shared__ double lds_e[P::EWORDS];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double fl64 = 0.0, fl32 = 0.0;
   if (i < a.ncells) {
-    Lane<Net, M, RCP> s;
-    double y[Net::NEQ];
+    L s;
+    s.kx = lds_k + threadIdx.x;
+    s.eox = lds_e + threadIdx.x;
     LaneGroup g;
-    bool done = odv_load_begin(a, i, s, y);
-    while (!done) done = odv_attempt(s, y, a.ctl, g);
-    odv_store(a, i, s, y);
+    bool done = odv_load_begin(a, i, s);
+    while (!done) done = odv_attempt(s, a.ctl, g);
+    odv_finish(a, i, s);
     fl64 = s.flops64;
     fl32 = s.flops32;
   }
@@ -111,27 +154,32 @@ __global__ void __launch_bounds__(256) odv_k_percell(KArgs a) {
 // Persistent form. Every lane starts on cell (global lane id); the queue
 // counter starts at the number of launched lanes. Whenever at least one lane
 // of a wavefront has finished, the wavefront reserves that many cells with a
-// single atomic and hands them out in lane order. A lane exits when the
-// queue is exhausted; the loop ends for every lane because each iteration
-// either finishes a cell attempt-bounded by max_attempts or exits.
+// single atomic and hands them out in lane order. Every lane's loop ends: each
+// iteration either advances a cell by one attempt (cells are bounded by
+// max_attempts) or, once the wavefront has seen the queue exhausted and holds
+// no cell, leaves.
 template <class Net, class M, bool RCP>
-__global__ void __launch_bounds__(256) odv_k_persistent(KArgs a) {
+__global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
+  using P = LdsPolicy<Net>;
+  using L = Lane<Cfg<Net, M, RCP, P::KS, false>>;
+  __shared__ double lds_k[P::KWORDS];
+  __shared__ double lds_e[P::EWORDS];
   const long long nlanes = (long long)gridDim.x * blockDim.x;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  Lane<Net, M, RCP> s;
-  double y[Net::NEQ];
+  L s;
+  s.kx = lds_k + threadIdx.x;
+  s.eox = lds_e + threadIdx.x;
   LaneGroup g;
   double fl64 = 0.0, fl32 = 0.0;
   bool have = i < a.ncells;
   bool done = true;
   bool exhausted = nlanes >= a.ncells;   // wave-uniform: no cell left to hand out
-  if (have) done = odv_load_begin(a, i, s, y);
+  if (have) done = odv_load_begin(a, i, s);
   while (true) {
-    // finish / refill
     const bool need = !have || done;
     if (have && done) {
-      odv_store(a, i, s, y);
+      odv_finish(a, i, s);
       fl64 += s.flops64;
       fl32 += s.flops32;
       have = false;
@@ -150,13 +198,12 @@ __global__ void __launch_bounds__(256) odv_k_persistent(KArgs a) {
         if (ni < a.ncells) {
           i = ni;
           have = true;
-          done = odv_load_begin(a, i, s, y);
+          done = odv_load_begin(a, i, s);
         }
       }
     }
-    // all lanes of the wavefront without a cell: leave
     if (__ballot(have) == 0ull) break;
-    if (have && !done) done = odv_attempt(s, y, a.ctl, g);
+    if (have && !done) done = odv_attempt(s, a.ctl, g);
   }
   if (a.flops) {
     fl64 = odv_wave_sum(fl64);
@@ -168,7 +215,7 @@ __global__ void __launch_bounds__(256) odv_k_persistent(KArgs a) {
   }
 }
 
-// Lockstep group reductions over one workgroup (group <= blockDim.x).
+// Lockstep group reductions over one workgroup (group <= blockDim.x <= 256).
 struct BlockGroup {
   double* red;   // LDS scratch [blockDim/64]
   int* redi;
@@ -199,17 +246,20 @@ struct BlockGroup {
   __device__ bool any_active(bool b) const { return any_of(b); }
 };
 
+// Lockstep form. Every decision is taken from group-reduced values, so all
+// lanes of the workgroup follow one control path and reach the same barriers.
 template <class Net, class M, bool RCP>
-__global__ void __launch_bounds__(1024) odv_k_lockstep(KArgs a) {
-  __shared__ double red[16];
-  __shared__ int redi[16];
+__global__ void __launch_bounds__(256) odv_k_lockstep(KArgs a) {
+  using L = Lane<Cfg<Net, M, RCP, 0, true>>;
+  __shared__ double red[4];
+  __shared__ int redi[4];
   constexpr int N = Net::NEQ;
   constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
   const long long g0 = (long long)blockIdx.x * a.group;
   const long long i = g0 + threadIdx.x;
   const bool valid = threadIdx.x < a.group && i < a.ncells;
   BlockGroup g{red, redi, valid};
-  Lane<Net, M, RCP> s;
+  L s;
   double y[N];
   double prm[NP];
   const long long ii = valid ? i : g0;   // padding lanes mirror the group's first cell
@@ -217,6 +267,7 @@ __global__ void __launch_bounds__(1024) odv_k_lockstep(KArgs a) {
   for (int j = 0; j < N; ++j) y[j] = a.y[(long long)j * a.ncells + ii];
 #pragma unroll
   for (int p = 0; p < NP; ++p) prm[p] = (Net::NPARAM > 0) ? a.prm[p * a.ps_plane + ii * a.ps_cell] : 0.0;
+  // the group shares time, t_stop and the dt hint: those of its first cell
   const double t0 = a.t0[g0 * a.t0_s];
   const double ts = a.tstop[g0 * a.ts_s];
   const double dh = a.dt0[g0 * a.dt_s];
@@ -225,13 +276,13 @@ __global__ void __launch_bounds__(1024) odv_k_lockstep(KArgs a) {
   s.flops64 = s.flops32 = 0.0;
   s.dt = dh;
   s.status = ST_SKIPPED;
-  // the group shares time, t_stop and the dt hint: those of its first cell
   const bool any_act = g.any_of(s.active);
   if (any_act && t0 < ts) {
     odv_begin(s, y, prm, t0, ts, dh, a.ctl, g);
     bool done = false;
-    while (!done) done = odv_attempt(s, y, a.ctl, g);
-    if (!s.active) s.status = ST_SKIPPED;
+    while (!done) done = odv_attempt(s, a.ctl, g);
+    if (s.active) odv_result(s, y);
+    else s.status = ST_SKIPPED;
   }
   if (valid) {
     if (s.active) {
@@ -240,15 +291,12 @@ __global__ void __launch_bounds__(1024) odv_k_lockstep(KArgs a) {
     }
     if (a.status) a.status[i] = s.status;
     if (a.dt_out) a.dt_out[i] = s.dt;
-    if (a.counters) {
-      const int* c = &s.cnt.attempts;
-#pragma unroll
-      for (int m = 0; m < 8; ++m) a.counters[(long long)m * a.ncells + i] = c[m];
-    }
+    if (a.counters) odv_store_counters(a, i, s.cnt);
   }
   if (a.flops && threadIdx.x == 0) {
-    atomicAdd(&a.flops[0], s.flops64 * (double)min((long long)a.group, a.ncells - g0));
-    atomicAdd(&a.flops[1], s.flops32 * (double)min((long long)a.group, a.ncells - g0));
+    const double nval = (double)min((long long)a.group, a.ncells - g0);
+    atomicAdd(&a.flops[0], s.flops64 * nval);
+    atomicAdd(&a.flops[1], s.flops32 * nval);
   }
 }
 

@@ -70,7 +70,7 @@ class _Counter:
 def _monomial(rate: int, factors: Tuple[int, ...], yname: str, cnt: _Counter) -> str:
     parts = []
     if rate >= 0:
-        parts.append("k[%d]" % rate)
+        parts.append("k[%d * S]" % rate)
     parts += ["%s[%d]" % (yname, f) for f in factors]
     if not parts:
         return "1.0"
@@ -124,36 +124,36 @@ def jacobian_contributions(st: Structure):
 
 
 def _emit_jac(st: Structure, cnt: _Counter) -> List[str]:
+    """Entries of P_order in storage order, each written as soon as it is
+    formed (derivative monomials shared between entries are formed at their
+    first use)."""
     lines = []
     contrib = jacobian_contributions(st)
     mono: Dict[Tuple[int, Tuple[int, ...]], str] = {}
-    jname: Dict[Tuple[int, int], str] = {}
-    for (r, c), lst in sorted(contrib.items(), key=lambda kv: (kv[0][1], kv[0][0])):
-        vals = []
-        for coef, rate, red in lst:
-            key = (rate, red)
-            if key not in mono:
-                nm = "D%d" % len(mono)
-                mono[key] = nm
-                lines.append("    const double %s = %s;" % (nm, _monomial(rate, red, "y", cnt)))
-            vals.append(_scaled(coef, mono[key], cnt))
-        nm = "J%d_%d" % (r, c)
-        jname[(r, c)] = nm
-        cnt.add += len(vals) - 1
-        expr = vals[0]
-        for v in vals[1:]:
-            expr = "%s + %s" % (expr, v) if not v.startswith("(-") else "%s - %s" % (expr, v[2:-1])
-        lines.append("    const double %s = %s;" % (nm, expr))
     perm = [int(p) for p in st.perm]
     for s, (r, c) in enumerate(st.csc.entries):
         R, C = perm[r], perm[c]
-        if (R, C) in jname:
+        if (R, C) in contrib:
+            vals = []
+            for coef, rate, red in contrib[(R, C)]:
+                key = (rate, red)
+                if key not in mono:
+                    nm = "D%d" % len(mono)
+                    mono[key] = nm
+                    lines.append("    const double %s = %s;" % (nm, _monomial(rate, red, "y", cnt)))
+                vals.append(_scaled(coef, mono[key], cnt))
+            cnt.add += len(vals) - 1
+            expr = vals[0]
+            for v in vals[1:]:
+                expr = "%s + %s" % (expr, v) if not v.startswith("(-") else "%s - %s" % (expr, v[2:-1])
+            jn = "J%d_%d" % (R, C)
+            lines.append("    const double %s = %s;" % (jn, expr))
             cnt.mul += 1
             if R == C:
                 cnt.add += 1
-                val = "1.0 - hb * %s" % jname[(R, C)]
+                val = "1.0 - hb * %s" % jn
             else:
-                val = "-(hb * %s)" % jname[(R, C)]
+                val = "-(hb * %s)" % jn
         elif R == C:
             val = "1.0"
         else:
@@ -364,14 +364,18 @@ def emit_header(st: Structure) -> str:
     hdr += rates
     hdr.append("  }")
     hdr.append("")
+    hdr.append("  // k[r * S]: rate r of this cell (S = stride between a cell's rates)")
+    hdr.append("  template <int S = 1>")
     hdr.append("  ODV_HD static inline void rhs(const double* __restrict__ y, const double* __restrict__ k,")
     hdr.append("                               double* __restrict__ f) {")
+    hdr.append("    (void)k;")
     hdr += rhs
     hdr.append("  }")
     hdr.append("")
-    hdr.append("  template <class M>")
+    hdr.append("  template <class M, int S = 1>")
     hdr.append("  ODV_HD static inline void jac(const double* __restrict__ y, const double* __restrict__ k,")
     hdr.append("                               double hb, M* __restrict__ a) {")
+    hdr.append("    (void)k; (void)y; (void)hb;")
     hdr += jac
     hdr.append("  }")
     for name, body, sig in (("lu", lu, "M* __restrict__ a"), ("lu_r", lur, "M* __restrict__ a")):

@@ -795,6 +795,9 @@ int oracle_solve_vector(const onet_t* n, int G, double* time, double* dt, double
       }
     }
     *time = t_stop;
+  } else {
+    /* the reference STOPs here; the solution at the failure is y_NS(:,:,0) */
+    copy_masked(&v, y, v.yNS);
   }
   free(mem);
   return st;

@@ -20,8 +20,9 @@ void run(long n, const double* t0, const double* ts, const double* dt0, double* 
          const odv::Controls& c, int* status, double* dtout, int* cnt) {
   constexpr int N = Net::NEQ;
   constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
+  using L = odv::Lane<odv::Cfg<Net, M, RCP, 0, false>>;
   for (long i = 0; i < n; ++i) {
-    odv::Lane<Net, M, RCP> s;
+    L s;
     odv::LaneGroup g;
     double yy[N], p[NP];
     for (int j = 0; j < N; ++j) yy[j] = y[j * n + i];
@@ -34,14 +35,17 @@ void run(long n, const double* t0, const double* ts, const double* dt0, double* 
       s.status = odv::ST_SKIPPED;
       done = true;
     } else {
-      done = odv::odv_begin(s, yy, p, t0[i], ts[i], dt0[i], c, g);
+      odv::odv_begin(s, yy, p, t0[i], ts[i], dt0[i], c, g);
+      done = false;
     }
-    while (!done) done = odv::odv_attempt(s, yy, c, g);
+    while (!done) done = odv::odv_attempt(s, c, g);
+    if (s.status != odv::ST_SKIPPED) odv::odv_result(s, yy);
     for (int j = 0; j < N; ++j) y[j * n + i] = yy[j];
     status[i] = s.status;
     dtout[i] = s.dt;
-    const int* cc = &s.cnt.attempts;
-    for (int m = 0; m < 8; ++m) cnt[m * n + i] = cc[m];
+    const odv::Counters& k = s.cnt;
+    const int v[8] = {k.attempts, k.steps, k.newton, k.jac, k.errfail, k.convfail, k.stepsel, k.qsum};
+    for (int m = 0; m < 8; ++m) cnt[m * n + i] = v[m];
   }
 }
 
@@ -76,10 +80,10 @@ extern "C" int lane_solve_cells(const char* id, int mixed, long n, const double*
 extern "C" int lane_netfun(const char* id, int what, const double* y, const double* k, double hb, double* out) {
 #define ODV_CASE(NET)                                                      \
   if (!std::strcmp(id, NET::ID)) {                                         \
-    if (what == 0) NET::rhs(y, k, out);                                    \
+    if (what == 0) NET::rhs<1>(y, k, out);                                    \
     else if (what == 1) NET::rates(y, k, out);                             \
-    else if (what == 2) NET::template jac<double>(y, k, hb, out);          \
-    else if (what == 3) { NET::template jac<double>(y, k, hb, out); NET::template lu<double>(out); } \
+    else if (what == 2) NET::template jac<double, 1>(y, k, hb, out);          \
+    else if (what == 3) { NET::template jac<double, 1>(y, k, hb, out); NET::template lu<double>(out); } \
     else if (what == 4) { double r[NET::NEQ]; for (int j = 0; j < NET::NEQ; ++j) r[j] = y[j];          \
                           NET::template solve<double, double>(k, r);                                   \
                           for (int j = 0; j < NET::NEQ; ++j) out[j] = r[j]; }                          \
