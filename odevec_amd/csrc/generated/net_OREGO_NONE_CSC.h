@@ -1,13 +1,16 @@
 // Generated by odevec_amd.pre_odevec -- do not edit.
 // network OREGO, ordering None, packaging CSC
-// neq 3, nnz(J) 7, nnz(P) 7, nnz(LU) 8, terms 9
+// neq 3 (integrated 3), nnz(J) 7, nnz(P) 7, nnz(LU) 8 (full system 8), terms 9
 // flops: rhs 17, jac 25, lu 8, solve 13, lu_r 11, solve_r 13
 #pragma once
 #include "odevec_netcommon.h"
 
 struct odv_net_OREGO_NONE_CSC {
   static constexpr const char* ID = "OREGO/NONE/CSC";
-  static constexpr int NEQ = 3;
+  static constexpr int NEQ = 3;   // species in the caller's state
+  static constexpr int NACT = 3;  // species integrated; the others stay constant
+  // ACT[a] = species index of integrated species a
+  static constexpr int ACT[3] = {0, 1, 2};
   static constexpr int NNZ = 8;
   static constexpr int NRATE = 1;
   static constexpr int NPARAM = 0;
@@ -33,37 +36,50 @@ struct odv_net_OREGO_NONE_CSC {
   ODV_HD static inline void rhs(const double* __restrict__ y, const double* __restrict__ k,
                                double* __restrict__ f) {
     (void)k;
-    const double F0 = y[0];
-    const double F1 = y[0] * y[0];
-    const double F2 = y[0] * y[1];
-    const double F3 = y[1];
-    const double F4 = y[2];
-    f[0] = (77.27 * F0) - 0.00064713625 * F1 - 77.27 * F2 + (77.27 * F3);
-    f[1] = (-0.012941633234114146 * F2) - 0.012941633234114146 * F3 + (0.012941633234114146 * F4);
-    f[2] = (0.161 * F0) - 0.161 * F4;
+    {
+      const double F = y[0];
+      f[0] = (77.27 * F);
+      f[2] = (0.161 * F);
+    }
+    {
+      const double F = y[0] * y[0];
+      f[0] = f[0] - 0.00064713625 * F;
+    }
+    {
+      const double F = y[0] * y[1];
+      f[0] = f[0] - 77.27 * F;
+      f[1] = (-0.012941633234114146 * F);
+    }
+    {
+      const double F = y[1];
+      f[0] = f[0] + (77.27 * F);
+      f[1] = f[1] - 0.012941633234114146 * F;
+    }
+    {
+      const double F = y[2];
+      f[1] = f[1] + (0.012941633234114146 * F);
+      f[2] = f[2] - 0.161 * F;
+    }
   }
 
   template <class M, int S = 1>
   ODV_HD static inline void jac(const double* __restrict__ y, const double* __restrict__ k,
                                double hb, M* __restrict__ a) {
     (void)k; (void)y; (void)hb;
-    const double D0 = 1.0;
-    const double D1 = y[0];
-    const double D2 = y[1];
-    const double J0_0 = (77.27 * D0) - 0.0012942725 * D1 - 77.27 * D2;
+    const double J0_0 = (77.27 * 1.0) - 0.0012942725 * y[0] - 77.27 * y[1];
     a[0] = (M)(1.0 - hb * J0_0);
-    const double J1_0 = (-0.012941633234114146 * D2);
+    const double J1_0 = (-0.012941633234114146 * y[1]);
     a[1] = (M)(-(hb * J1_0));
-    const double J2_0 = (0.161 * D0);
+    const double J2_0 = (0.161 * 1.0);
     a[2] = (M)(-(hb * J2_0));
-    const double J0_1 = (-77.27 * D1) + (77.27 * D0);
+    const double J0_1 = (-77.27 * y[0]) + (77.27 * 1.0);
     a[3] = (M)(-(hb * J0_1));
-    const double J1_1 = (-0.012941633234114146 * D1) - 0.012941633234114146 * D0;
+    const double J1_1 = (-0.012941633234114146 * y[0]) - 0.012941633234114146 * 1.0;
     a[4] = (M)(1.0 - hb * J1_1);
     a[5] = (M)(0.0);
-    const double J1_2 = (0.012941633234114146 * D0);
+    const double J1_2 = (0.012941633234114146 * 1.0);
     a[6] = (M)(-(hb * J1_2));
-    const double J2_2 = (-0.161 * D0);
+    const double J2_2 = (-0.161 * 1.0);
     a[7] = (M)(1.0 - hb * J2_2);
   }
 

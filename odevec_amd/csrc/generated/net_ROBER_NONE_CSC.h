@@ -1,20 +1,23 @@
 // Generated by odevec_amd.pre_odevec -- do not edit.
 // network ROBER, ordering None, packaging CSC
-// neq 3, nnz(J) 7, nnz(P) 8, nnz(LU) 8, terms 6
-// flops: rhs 8, jac 15, lu 9, solve 13, lu_r 12, solve_r 13
+// neq 3 (integrated 3), nnz(J) 7, nnz(P) 8, nnz(LU) 8 (full system 8), terms 6
+// flops: rhs 8, jac 18, lu 9, solve 13, lu_r 12, solve_r 13
 #pragma once
 #include "odevec_netcommon.h"
 
 struct odv_net_ROBER_NONE_CSC {
   static constexpr const char* ID = "ROBER/NONE/CSC";
-  static constexpr int NEQ = 3;
+  static constexpr int NEQ = 3;   // species in the caller's state
+  static constexpr int NACT = 3;  // species integrated; the others stay constant
+  // ACT[a] = species index of integrated species a
+  static constexpr int ACT[3] = {0, 1, 2};
   static constexpr int NNZ = 8;
   static constexpr int NRATE = 3;
   static constexpr int NPARAM = 3;
   static constexpr int RATE_KIND = 1;
   static constexpr int TGAS = -1;
   static constexpr bool CROUT = true;
-  static constexpr int FLOPS_RHS = 8, FLOPS_JAC = 15, FLOPS_LU = 9, FLOPS_SOLVE = 13;
+  static constexpr int FLOPS_RHS = 8, FLOPS_JAC = 18, FLOPS_LU = 9, FLOPS_SOLVE = 13;
   static constexpr int FLOPS_LU_R = 12, FLOPS_SOLVE_R = 13;
   // permutation Perm (0-based): P_order[i][j] = P[Perm[i]][Perm[j]]
   // Perm = {0, 1, 2}
@@ -38,35 +41,40 @@ struct odv_net_ROBER_NONE_CSC {
   ODV_HD static inline void rhs(const double* __restrict__ y, const double* __restrict__ k,
                                double* __restrict__ f) {
     (void)k;
-    const double F0 = k[0 * S] * y[0];
-    const double F1 = k[2 * S] * y[1] * y[2];
-    const double F2 = k[1 * S] * y[1] * y[1];
-    f[0] = (-F0) + F1;
-    f[1] = F0 - F2 - F1;
-    f[2] = F2;
+    {
+      const double F = k[0 * S] * y[0];
+      f[0] = (-F);
+      f[1] = F;
+    }
+    {
+      const double F = k[1 * S] * y[1] * y[1];
+      f[1] = f[1] - F;
+      f[2] = F;
+    }
+    {
+      const double F = k[2 * S] * y[1] * y[2];
+      f[0] = f[0] + F;
+      f[1] = f[1] - F;
+    }
   }
 
   template <class M, int S = 1>
   ODV_HD static inline void jac(const double* __restrict__ y, const double* __restrict__ k,
                                double hb, M* __restrict__ a) {
     (void)k; (void)y; (void)hb;
-    const double D0 = k[0 * S];
-    const double J0_0 = (-D0);
+    const double J0_0 = (-(k[0 * S]));
     a[0] = (M)(1.0 - hb * J0_0);
-    const double J1_0 = D0;
+    const double J1_0 = (k[0 * S]);
     a[1] = (M)(-(hb * J1_0));
-    const double D1 = k[2 * S] * y[2];
-    const double J0_1 = D1;
+    const double J0_1 = (k[2 * S] * y[2]);
     a[2] = (M)(-(hb * J0_1));
-    const double D2 = k[1 * S] * y[1];
-    const double J1_1 = (-2.0 * D2) - D1;
+    const double J1_1 = (-2.0 * (k[1 * S] * y[1])) - (k[2 * S] * y[2]);
     a[3] = (M)(1.0 - hb * J1_1);
-    const double J2_1 = (2.0 * D2);
+    const double J2_1 = (2.0 * (k[1 * S] * y[1]));
     a[4] = (M)(-(hb * J2_1));
-    const double D3 = k[2 * S] * y[1];
-    const double J0_2 = D3;
+    const double J0_2 = (k[2 * S] * y[1]);
     a[5] = (M)(-(hb * J0_2));
-    const double J1_2 = (-D3);
+    const double J1_2 = (-(k[2 * S] * y[1]));
     a[6] = (M)(-(hb * J1_2));
     a[7] = (M)(1.0);
   }

@@ -113,7 +113,8 @@ template <class C>
 struct Lane {
   using Net = typename C::Net;
   using M = typename C::M;
-  static constexpr int N = Net::NEQ;
+  static constexpr int N = Net::NACT;   // integrated species
+  static constexpr int NF = Net::NEQ;   // species of the caller's state (norms divide by it)
   static constexpr int KS = C::KS;
   static constexpr bool RCP_ = C::RCP;
   static constexpr bool KEEPY_ = C::KEEPY;
@@ -132,6 +133,10 @@ struct Lane {
   int q, swu, lju, fet, fc, status;
   bool upd_jac, neglect_up, first;
   bool active;           // lane takes part in norms / y updates (lockstep mask)
+  // what the frozen species contribute to the reference's reductions: bit 0 a
+  // frozen weight is inf or NaN (its 0*0*w norm term is NaN), bit 1 a frozen
+  // weight is NaN (the NaN check fires), bit 2 a frozen value is negative
+  int fz;
   Counters cnt;
   double flops64, flops32;
 
@@ -192,13 +197,16 @@ ODV_HD inline void odv_weights(const double* y, double rtol, double atol, double
   }
 }
 
-// sqrt(sum_j x_j^2 w_j / neq) for one cell (src/odevec.F90:669-702)
-template <int N>
-ODV_HD inline double odv_wnorm(const double* x, const double* w) {
-  double s = (x[0] * x[0]) * w[0];
+// sqrt(sum_j x_j^2 w_j / neq) for one cell (src/odevec.F90:669-702); the
+// frozen species' terms are exact zeros unless their weight is not finite
+template <class L>
+ODV_HD inline double odv_wnorm(const L& s, const double* x) {
+  constexpr int N = L::N;
+  double acc = (x[0] * x[0]) * s.w[0];
 #pragma unroll
-  for (int j = 1; j < N; ++j) s = s + (x[j] * x[j]) * w[j];
-  return sqrt(s / (double)N);
+  for (int j = 1; j < N; ++j) acc = acc + (x[j] * x[j]) * s.w[j];
+  if (s.fz & 1) acc = NAN;
+  return sqrt(acc / (double)L::NF);
 }
 
 template <int N>
@@ -277,10 +285,10 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
       for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.z[kk][j] : v;
       col[j] = v;
     }
-    const double nq = g.max_active(odv_wnorm<N>(col, s.w), s.active);
+    const double nq = g.max_active(odv_wnorm(s, col), s.active);
     down = 1.0 / (ODV_F32(1.3) * odv::m_pow(nq / odv_tau(q, q - 1, lqq), 1.0 / q) + 1e-6);
   }
-  const double ne = g.max_active(odv_wnorm<N>(s.en, s.w), s.active);
+  const double ne = g.max_active(odv_wnorm(s, s.en), s.active);
   same = 1.0 / (ODV_F32(1.2) * odv::m_pow(ne / odv_tau(q, q, lqq), 1.0 / (q + 1.0)) + 1e-6);
   if (q == MAXORD || s.neglect_up) {
     up = 0.0;
@@ -288,7 +296,7 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
     double d[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) d[j] = s.en[j] - s.eo(j);
-    const double nd = g.max_active(odv_wnorm<N>(d, s.w), s.active);
+    const double nd = g.max_active(odv_wnorm(s, d), s.active);
     up = 1.0 / (ODV_F32(1.4) * odv::m_pow(nd / odv_tau(q, q + 1, lqq), 1.0 / (q + 2.0)) + 1e-6);
   }
   int loc = 1;
@@ -343,7 +351,7 @@ template <class L>
 ODV_HD inline void odv_lin_solve(const L& s, double* x) {
   using Net = typename L::Net;
   using M = typename L::M;
-  constexpr int N = Net::NEQ;
+  constexpr int N = L::N;
   if constexpr (sizeof(M) == sizeof(double)) {
     if (L::RCP_) Net::template solve_r<M, double>(s.a, x);
     else Net::template solve<M, double>(s.a, x);
@@ -368,10 +376,32 @@ ODV_HD inline void odv_rhs(const L& s, const double* y, double* f) {
 // state, `dt_hint` the caller's dt (it only scales y_NS(:,:,1), Q2). The
 // external columns (KS > 0) must be set before the call.
 template <class L, class G>
-ODV_HD inline void odv_begin(L& s, const double* y, const double* prm, double t0, double tstop,
+ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, double t0, double tstop,
                              double dt_hint, const Controls& c, const G& g) {
   using Net = typename L::Net;
-  constexpr int N = Net::NEQ;
+  constexpr int N = L::N;
+  double y[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) y[j] = yfull[Net::ACT[j]];
+  // frozen species: constant values, constant weights, zero corrections
+  s.fz = 0;
+  {
+    int a = 0;
+#pragma unroll
+    for (int j = 0; j < L::NF; ++j) {
+      if (a < N && Net::ACT[a] == j) {
+        ++a;
+        continue;
+      }
+      const double v = 1.0 / (c.rtol * fabs(yfull[j]) + c.atol);
+      const double wj = v * v;
+      // inf or NaN (a comparison: an arithmetic test like wj - wj could be
+      // contracted with the product that formed wj)
+      if (!(fabs(wj) <= 1.7976931348623157e308)) s.fz |= 1;
+      if (wj != wj) s.fz |= 2;
+      if (yfull[j] < 0.0) s.fz |= 4;
+    }
+  }
   s.first = true;
   odv_set_order(s, 1);
   s.swu = 0;
@@ -391,7 +421,7 @@ ODV_HD inline void odv_begin(L& s, const double* y, const double* prm, double t0
   s.flops32 = 0.0;
   {
     double kk[Net::NRATE];
-    Net::rates(y, prm, kk);
+    Net::rates(yfull, prm, kk);
 #pragma unroll
     for (int r = 0; r < Net::NRATE; ++r) {
       if (L::KS == 0) s.kl[r] = kk[r];
@@ -421,9 +451,10 @@ ODV_HD inline void odv_begin(L& s, const double* y, const double* prm, double t0
   double acc = (f[0] * f[0]) * ((s.w[0] * tol) * tol);
 #pragma unroll
   for (int j = 1; j < N; ++j) acc = acc + (f[j] * f[j]) * ((s.w[j] * tol) * tol);
+  if (s.fz & 1) acc = NAN;
   acc = g.max_active(acc, s.active);
   const double W0 = fmax(fabs(t0), fabs(tstop));
-  double dt = sqrt(tol / (odv::m_pow(W0, -2.0) + acc / (double)N));
+  double dt = sqrt(tol / (odv::m_pow(W0, -2.0) + acc / (double)L::NF));
   s.dt = fmin(dt, fabs(tstop - t0));
   s.flops64 += Net::FLOPS_RHS + 12.0 * N + 8.0;
 }
@@ -465,7 +496,7 @@ template <class L, class G>
 ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   using Net = typename L::Net;
   using M = typename L::M;
-  constexpr int N = Net::NEQ;
+  constexpr int N = L::N;
   constexpr bool MIXED = sizeof(M) != sizeof(double);
   const double told = s.t;
   s.cnt.attempts++;
@@ -481,7 +512,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   for (int j = 0; j < N; ++j) s.en[j] = 0.0;
 
   if (c.check_negatives) {
-    bool neg = false;
+    bool neg = (s.fz & 4) != 0;
 #pragma unroll
     for (int j = 0; j < N; ++j) neg = neg || (s.z[0][j] < 0.0);
     if (g.any_active(neg && s.active)) {
@@ -535,14 +566,14 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
     if (MIXED) s.flops32 += L::RCP_ ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
     else s.flops64 += L::RCP_ ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
 
-    const double e = g.max_active(odv_wnorm<N>(f, s.w), s.active);
+    const double e = g.max_active(odv_wnorm(s, f), s.active);
     double rtmp = 0.0;
     if (it != 0) {
       rtmp = e / s.cerr;
       s.rate = fmax(0.2 * s.rate, rtmp);
     }
     const double crit = e * fmin(1.0, 1.5 * s.rate) / (tauq / (2.0 * (s.q + 2.0)));
-    if (g.any_valid(odv_has_nan<N>(f) || odv_has_nan<N>(s.w))) {
+    if (g.any_valid(odv_has_nan<N>(f) || odv_has_nan<N>(s.w) || (s.fz & 2))) {
       failed = true;
     } else if (crit < 1.0) {
       converged = true;
@@ -558,7 +589,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
     if (c.check_negatives) {
       double y[N];
       odv_cur_y(s, beta, y);
-      bool neg = false;
+      bool neg = (s.fz & 4) != 0;
 #pragma unroll
       for (int j = 0; j < N; ++j) neg = neg || (y[j] < 0.0);
       if (g.any_active(neg && s.active)) {
@@ -589,7 +620,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   s.fc = 0;
 
   // local truncation error test (src/odevec.F90:400-455)
-  const double err = g.max_active(odv_wnorm<N>(s.en, s.w), s.active) / tauq;
+  const double err = g.max_active(odv_wnorm(s, s.en), s.active) / tauq;
   {
     double y[N];
     odv_cur_y(s, beta, y);

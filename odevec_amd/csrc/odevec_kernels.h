@@ -70,11 +70,12 @@ __device__ inline void odv_store_counters(const KArgs& a, long long i, const Cou
   a.counters[7 * n + i] = c.qsum;
 }
 
+// y: the integrated species; frozen species keep their value in a.y
 template <class L>
 __device__ inline void odv_store(const KArgs& a, long long i, const L& s, const double* y) {
   constexpr int N = L::N;
 #pragma unroll
-  for (int j = 0; j < N; ++j) a.y[(long long)j * a.ncells + i] = y[j];
+  for (int j = 0; j < N; ++j) a.y[(long long)L::Net::ACT[j] * a.ncells + i] = y[j];
   if (a.status) a.status[i] = s.status;
   if (a.dt_out) a.dt_out[i] = s.dt;
   if (a.counters) odv_store_counters(a, i, s.cnt);
@@ -85,11 +86,11 @@ __device__ inline void odv_store(const KArgs& a, long long i, const L& s, const 
 template <class L>
 __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
   using Net = typename L::Net;
-  constexpr int N = Net::NEQ;
+  constexpr int NF = Net::NEQ;
   constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
-  double y[N], prm[NP];
+  double y[NF], prm[NP];
 #pragma unroll
-  for (int j = 0; j < N; ++j) y[j] = a.y[(long long)j * a.ncells + i];
+  for (int j = 0; j < NF; ++j) y[j] = a.y[(long long)j * a.ncells + i];
 #pragma unroll
   for (int p = 0; p < NP; ++p) prm[p] = (Net::NPARAM > 0) ? a.prm[p * a.ps_plane + i * a.ps_cell] : 0.0;
   const double t0 = a.t0[i * a.t0_s];
@@ -281,8 +282,14 @@ __global__ void __launch_bounds__(256) odv_k_lockstep(KArgs a) {
     odv_begin(s, y, prm, t0, ts, dh, a.ctl, g);
     bool done = false;
     while (!done) done = odv_attempt(s, a.ctl, g);
-    if (s.active) odv_result(s, y);
-    else s.status = ST_SKIPPED;
+    if (s.active) {
+      double ya[L::N];
+      odv_result(s, ya);
+#pragma unroll
+      for (int j = 0; j < L::N; ++j) y[Net::ACT[j]] = ya[j];
+    } else {
+      s.status = ST_SKIPPED;
+    }
   }
   if (valid) {
     if (s.active) {

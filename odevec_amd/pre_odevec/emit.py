@@ -30,7 +30,7 @@ import json
 import math
 from typing import Dict, List, Tuple
 
-from .structure import Structure, Term
+from .structure import Structure, Term, reduce_frozen
 
 RATE_KIND_ID = {"none": 0, "param": 1, "primordial": 2, "arrhenius": 3}
 
@@ -88,26 +88,33 @@ def _scaled(coef: float, var: str, cnt: _Counter) -> str:
 
 
 def _emit_rhs(st: Structure, cnt: _Counter) -> List[str]:
+    """Scatter form: each distinct flux k*prod(y) is formed once, in
+    (rate, factors) order, and added to every row that has it. Terms inside a
+    row are in that same order, so each row is summed in its canonical order
+    while only one flux is live at a time."""
     lines = []
-    flux: Dict[Tuple[int, Tuple[int, ...]], str] = {}
-    rows: Dict[int, List[str]] = {i: [] for i in range(st.neq)}
+    by_key: Dict[Tuple[int, Tuple[int, ...]], List[Term]] = {}
     for t in st.terms:
-        key = (t.rate, t.factors)
-        if key not in flux:
-            name = "F%d" % len(flux)
-            flux[key] = name
-            lines.append("    const double %s = %s;" % (name, _monomial(t.rate, t.factors, "y", cnt)))
-        rows[t.row].append(_scaled(t.coef, flux[key], cnt))
+        by_key.setdefault((t.rate, t.factors), []).append(t)
+    started = set()
+    for n, key in enumerate(sorted(by_key)):
+        lines.append("    {")
+        lines.append("      const double F = %s;" % _monomial(key[0], key[1], "y", cnt))
+        for t in by_key[key]:
+            v = _scaled(t.coef, "F", cnt)
+            if t.row not in started:
+                started.add(t.row)
+                lines.append("      f[%d] = %s;" % (t.row, v))
+            else:
+                cnt.add += 1
+                if v.startswith("(-"):
+                    lines.append("      f[%d] = f[%d] - %s;" % (t.row, t.row, v[2:-1]))
+                else:
+                    lines.append("      f[%d] = f[%d] + %s;" % (t.row, t.row, v))
+        lines.append("    }")
     for i in range(st.neq):
-        terms = rows[i]
-        if not terms:
-            lines.append("    f[%d] = 0.0;" % i)
-            continue
-        cnt.add += len(terms) - 1
-        expr = terms[0]
-        for tt in terms[1:]:
-            expr = "%s + %s" % (expr, tt) if not tt.startswith("(-") else "%s - %s" % (expr, tt[2:-1])
-        lines.append("    f[%d] = %s;" % (i, expr))
+        if i not in started:
+            lines.insert(0, "    f[%d] = 0.0;" % i)
     return lines
 
 
@@ -124,24 +131,20 @@ def jacobian_contributions(st: Structure):
 
 
 def _emit_jac(st: Structure, cnt: _Counter) -> List[str]:
-    """Entries of P_order in storage order, each written as soon as it is
-    formed (derivative monomials shared between entries are formed at their
-    first use)."""
+    """Entries of P_order in storage order, each formed and written on its
+    own: the derivative monomials of an entry are formed where the entry is
+    (a monomial two entries share is formed twice, which keeps the live set to
+    one entry's terms)."""
     lines = []
     contrib = jacobian_contributions(st)
-    mono: Dict[Tuple[int, Tuple[int, ...]], str] = {}
     perm = [int(p) for p in st.perm]
     for s, (r, c) in enumerate(st.csc.entries):
         R, C = perm[r], perm[c]
         if (R, C) in contrib:
             vals = []
             for coef, rate, red in contrib[(R, C)]:
-                key = (rate, red)
-                if key not in mono:
-                    nm = "D%d" % len(mono)
-                    mono[key] = nm
-                    lines.append("    const double %s = %s;" % (nm, _monomial(rate, red, "y", cnt)))
-                vals.append(_scaled(coef, mono[key], cnt))
+                mono = _monomial(rate, red, "y", cnt)
+                vals.append(_scaled(coef, "(%s)" % mono if " " in mono else mono, cnt))
             cnt.add += len(vals) - 1
             expr = vals[0]
             for v in vals[1:]:
@@ -311,7 +314,10 @@ def opcounts(st: Structure) -> Dict[str, Dict[str, int]]:
     return out
 
 
-def emit_header(st: Structure) -> str:
+def emit_header(full: Structure) -> str:
+    """Header for the structure's active species (frozen species -- zero RHS,
+    read by no term -- are carried unchanged by the caller's state)."""
+    st = reduce_frozen(full)
     net = st.net
     sn = struct_name(st)
     crout = st.packaging == "CSC"
@@ -327,18 +333,22 @@ def emit_header(st: Structure) -> str:
     svr = sv_fn(st, _Counter(), True)
     ops = opcounts(st)
     perm = ", ".join(str(int(p)) for p in st.perm)
+    act = st.active
     hdr = []
     hdr.append("// Generated by odevec_amd.pre_odevec -- do not edit.")
     hdr.append("// network %s, ordering %s, packaging %s" % (net.name, st.ordering, st.packaging))
-    hdr.append("// neq %d, nnz(J) %d, nnz(P) %d, nnz(LU) %d, terms %d" % (
-        st.neq, int(st.J.sum()), int(st.P.sum()), st.csc.nnz, len(st.terms)))
+    hdr.append("// neq %d (integrated %d), nnz(J) %d, nnz(P) %d, nnz(LU) %d (full system %d), terms %d" % (
+        full.neq, st.neq, int(full.J.sum()), int(full.P.sum()), st.csc.nnz, full.csc.nnz, len(st.terms)))
     hdr.append("// flops: " + ", ".join("%s %d" % (k, v["flops"]) for k, v in ops.items()))
     hdr.append("#pragma once")
     hdr.append('#include "odevec_netcommon.h"')
     hdr.append("")
     hdr.append("struct %s {" % sn)
     hdr.append('  static constexpr const char* ID = "%s";' % network_id(st))
-    hdr.append("  static constexpr int NEQ = %d;" % st.neq)
+    hdr.append("  static constexpr int NEQ = %d;   // species in the caller's state" % full.neq)
+    hdr.append("  static constexpr int NACT = %d;  // species integrated; the others stay constant" % st.neq)
+    hdr.append("  // ACT[a] = species index of integrated species a")
+    hdr.append("  static constexpr int ACT[%d] = {%s};" % (len(act), ", ".join(str(a) for a in act)))
     hdr.append("  static constexpr int NNZ = %d;" % st.csc.nnz)
     hdr.append("  static constexpr int NRATE = %d;" % max(net.nrate, 1))
     hdr.append("  static constexpr int NPARAM = %d;" % net.nparam)
@@ -395,6 +405,11 @@ def emit_header(st: Structure) -> str:
     return "\n".join(hdr)
 
 
+def analyse_copy(st: Structure) -> Structure:
+    import copy
+    return copy.copy(st)
+
+
 def ir_dict(st: Structure) -> dict:
     net = st.net
     return {
@@ -421,7 +436,10 @@ def ir_dict(st: Structure) -> dict:
         "u_col_start": st.csc.u_col_start,
         "l_col_start": st.csc.l_col_start,
         "nnz": st.csc.nnz,
+        # the generated code integrates the active species only
+        "active": reduce_frozen(analyse_copy(st)).active,
         "opcounts": opcounts(st),
+        "opcounts_active": opcounts(reduce_frozen(analyse_copy(st))),
     }
 
 

@@ -218,9 +218,59 @@ class Structure:
     LU: np.ndarray
     csc: CSC
 
+    # species integrated by the generated code (natural order); the others are
+    # frozen: their RHS is identically zero and no RHS term reads them
+    active: Optional[List[int]] = None
+
     @property
     def neq(self) -> int:
+        """Number of species in this structure's index space."""
+        return self.P.shape[0]
+
+    @property
+    def neq_full(self) -> int:
         return self.net.neq
+
+    @property
+    def frozen(self) -> List[int]:
+        act = set(self.active if self.active is not None else range(self.neq_full))
+        return [i for i in range(self.neq_full) if i not in act]
+
+
+def frozen_species(neq: int, terms: List[Term]) -> List[int]:
+    """Species whose RHS row has no term and that no term reads. Their
+    correction is exactly zero at every iteration (their row of P is the unit
+    row, so the LU solve returns 0 for them and every update they feed is
+    x - 0*a), hence y_NS(:,:,0) keeps their initial value and the higher
+    Nordsieck columns stay zero."""
+    has_row = {t.row for t in terms}
+    read = {f for t in terms for f in t.factors}
+    return [i for i in range(neq) if i not in has_row and i not in read]
+
+
+def reduce_frozen(st: Structure) -> Structure:
+    """The structure restricted to the active species: terms renumbered to
+    active indices, the ordering restricted (relative order kept), and the LU
+    pattern restricted to active rows and columns. Every elimination and solve
+    operation that is dropped touches a frozen row or column and is an exact
+    no-op (x - 0*a or x - a*0), so the remaining operations, in the same order,
+    give the same values."""
+    frozen = frozen_species(st.neq, st.terms)
+    if not frozen:
+        st.active = list(range(st.neq))
+        return st
+    active = [i for i in range(st.neq) if i not in frozen]
+    amap = {s: a for a, s in enumerate(active)}
+    terms = [Term(amap[t.row], t.coef, t.rate, tuple(amap[f] for f in t.factors)) for t in st.terms]
+    keep_pos = [p for p in range(st.neq) if int(st.perm[p]) in amap]
+    perm = np.array([amap[int(st.perm[p])] for p in keep_pos], dtype=np.int64)
+    J = st.J[np.ix_(active, active)]
+    P = st.P[np.ix_(active, active)]
+    Po = st.P_order[np.ix_(keep_pos, keep_pos)]
+    LU = st.LU[np.ix_(keep_pos, keep_pos)]
+    red = Structure(st.net, st.ordering, st.packaging, terms, J, P, perm, Po, LU, csc_structure(LU))
+    red.active = active
+    return red
 
 
 def analyse(net: Network, ordering: str = "None", packaging: str = "CSC",

@@ -15,14 +15,20 @@
 
 namespace {
 
-template <class Net, class M, bool RCP>
+// KS > 0 runs the lane with its rates and previous correction in external
+// strided columns, as the device kernels keep them in LDS.
+template <class Net, class M, bool RCP, int KS>
 void run(long n, const double* t0, const double* ts, const double* dt0, double* y, const double* prm,
          const odv::Controls& c, int* status, double* dtout, int* cnt) {
   constexpr int N = Net::NEQ;
   constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
-  using L = odv::Lane<odv::Cfg<Net, M, RCP, 0, false>>;
+  using L = odv::Lane<odv::Cfg<Net, M, RCP, KS, false>>;
+  static double kcol[Net::NRATE * (KS > 0 ? KS : 1)];
+  static double ecol[Net::NEQ * (KS > 0 ? KS : 1)];
   for (long i = 0; i < n; ++i) {
     L s;
+    s.kx = kcol + (KS > 0 ? (int)(i % KS) : 0);
+    s.eox = ecol + (KS > 0 ? (int)(i % KS) : 0);
     odv::LaneGroup g;
     double yy[N], p[NP];
     for (int j = 0; j < N; ++j) yy[j] = y[j * n + i];
@@ -39,7 +45,11 @@ void run(long n, const double* t0, const double* ts, const double* dt0, double* 
       done = false;
     }
     while (!done) done = odv::odv_attempt(s, c, g);
-    if (s.status != odv::ST_SKIPPED) odv::odv_result(s, yy);
+    if (s.status != odv::ST_SKIPPED) {
+      double ya[Net::NACT];
+      odv::odv_result(s, ya);
+      for (int j = 0; j < Net::NACT; ++j) yy[Net::ACT[j]] = ya[j];
+    }
     for (int j = 0; j < N; ++j) y[j * n + i] = yy[j];
     status[i] = s.status;
     dtout[i] = s.dt;
@@ -52,8 +62,13 @@ void run(long n, const double* t0, const double* ts, const double* dt0, double* 
 template <class Net>
 void run_prec(int mixed, long n, const double* t0, const double* ts, const double* dt0, double* y,
               const double* prm, const odv::Controls& c, int* status, double* dtout, int* cnt) {
-  if (mixed) run<Net, float, true>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
-  else run<Net, double, false>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+  if (mixed & 2) {
+    if (mixed & 1) run<Net, float, true, 7>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+    else run<Net, double, false, 7>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+  } else {
+    if (mixed & 1) run<Net, float, true, 0>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+    else run<Net, double, false, 0>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+  }
 }
 
 }  // namespace
