@@ -1,0 +1,189 @@
+"""Benchmark: PRIMORDIAL cell-solves/s, mixed precision vs the plain FP64
+kernel, on one GPU.
+
+A step integrates 1,048,576 PRIMORDIAL cells (16 species, CSC + CMK
+ordering, seeded inputs: n_H logU[1,1e12], T logU[10,1e4], x(H2) logU[1e-6,1e-2],
+x(e) logU[1e-8,1e-3], t_stop logU[1e8,1e14] s; rtol 1e-6, atol 1e-20) from
+its initial state (restored by a device copy at the start of the step) to
+each cell's end time. Inputs are resident in HBM before the timed region.
+The mixed form (FP32 iteration matrix, LU factors and triangular solves;
+FP64 residual, history, norms and control) with the persistent
+work-compacting kernel is the headline value; the plain FP64 kernel (one
+lane per cell, no compaction) runs the same steps for the ratio.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NCELLS = 1048576
+PEAK_FP64 = 78.6e12     # MI355X vector/matrix FP64, FLOP/s
+PEAK_FP32 = 157.3e12    # MI355X FP32, FLOP/s
+PEAK_HBM = 8.0e12       # B/s
+PROFILE = os.path.join(ROOT, "profiles", "primordial_1m_mixed_persistent.md")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--cells", type=int, default=NCELLS)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+class Runner:
+    """One configuration of the solver on device-resident inputs."""
+
+    def __init__(self, w, precision, mode):
+        import torch
+        from odevec_amd.api import Odevec
+        self.torch = torch
+        n = w["y"].shape[1]
+        self.n = n
+        dev = torch.device("cuda:0")
+        self.y0 = torch.tensor(w["y"], dtype=torch.float64, device=dev)
+        self.y = torch.empty_like(self.y0)
+        self.ts = torch.tensor(w["t_stop"], dtype=torch.float64, device=dev)
+        self.dt0 = torch.tensor(w["dt0"], dtype=torch.float64, device=dev)
+        self.st = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.flops = torch.zeros(2, dtype=torch.float64, device=dev)
+        self.ode = Odevec(w["network"], n, precision=precision, mode=mode)
+        self.ode.RelativeTolerance = w["rtol"]
+        self.ode.AbsoluteTolerance = w["atol"]
+        self.ode.MinimalTimestep = w["dt_min"]
+
+    def step(self, count_flops=False):
+        self.y.copy_(self.y0)
+        stream = self.torch.cuda.current_stream().cuda_stream
+        self.ode.solve_cells_async(self.n, self.ts.data_ptr(), self.y.data_ptr(), dt0_ptr=self.dt0.data_ptr(),
+                                   status_ptr=self.st.data_ptr(),
+                                   flops_ptr=self.flops.data_ptr() if count_flops else None, stream=stream)
+
+    def timed(self, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            self.step()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record()
+            self.step()
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_ms = [a.elapsed_time(b) for a, b in ev]
+        return wall, kernel_ms
+
+    def count_flops(self):
+        self.flops.zero_()
+        self.step(count_flops=True)
+        self.torch.cuda.synchronize()
+        st = self.st.cpu().numpy()
+        return self.flops.cpu().numpy(), st
+
+
+def cpu_baseline(w, seconds=12.0):
+    """The oracle (a C restatement of the reference solver) on the host cores,
+    per-cell mode, on a seeded sample of the same workload."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    net = O.Network.load(w["network"])
+    from odevec_amd import workloads as W
+    n = 512 * threads
+    done, t_used = 0, 0.0
+    rng = np.random.default_rng(123)
+    while t_used < seconds and done < 64 * n:
+        idx = rng.choice(w["y"].shape[1], n, replace=False)
+        s = W.subset(w, idx)
+        t = time.perf_counter()
+        y, st, _, _ = net.solve_cells(s["y"], s["t_stop"], t0=s["t0"], dt0=s["dt0"], rtol=s["rtol"], atol=s["atol"],
+                                      dt_min=s["dt_min"], threads=threads)
+        t_used += time.perf_counter() - t
+        done += n
+    return {"value": done / t_used, "unit": "cell-solves/s", "cores": threads, "kind": "port",
+            "sample": "%d random cells of the same 1,048,576-cell PRIMORDIAL workload, oracle (C restatement of the "
+                      "reference solver), per-cell mode, %d OpenMP threads, %.1f s" % (done, threads, t_used)}
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprof summary."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (cells are independent; "
+              "see DESIGN.md for how the work would shard).", file=sys.stderr)
+        sys.exit(2)
+    from odevec_amd import workloads as W
+    import torch
+    # a created stream: its handle is never the null stream, which the C ABI
+    # reads as "the handle's own stream"
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    w = W.primordial(a.cells, seed=3)
+    mixed = Runner(w, precision=1, mode=2)
+    fp64 = Runner(w, precision=0, mode=0)
+    wall_m, ms_m = mixed.timed(a.steps, a.warmup)
+    wall_d, ms_d = fp64.timed(a.steps, a.warmup)
+    fl_m, st_m = mixed.count_flops()
+    fl_d, st_d = fp64.count_flops()
+    ok_m, ok_d = int((st_m == 0).sum()), int((st_d == 0).sum())
+    value = a.steps * a.cells / wall_m
+    value_d = a.steps * a.cells / wall_d
+    t_kernel = float(np.mean(ms_m)) * 1e-3
+    achieved = (fl_m[0] + fl_m[1] * PEAK_FP64 / PEAK_FP32) / t_kernel
+    frac = fl_m[0] / (PEAK_FP64 * t_kernel) + fl_m[1] / (PEAK_FP32 * t_kernel)
+    res = {
+        "metric": "PRIMORDIAL cell-solves/s, mixed vs FP64 GPU at rtol=1e-6; serving p50/p99 ms",
+        "value": value,
+        "unit": "cell-solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall_m / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic (seeded PRIMORDIAL initial states, seed 3)",
+        "config": {"workload": "PRIMORDIAL 16-species, CSC+CMK, %d cells, per-cell t_stop logU[1e8,1e14] s, rtol 1e-6, "
+                               "mixed precision (FP32 P/LU/solves, FP64 residual/history/control), persistent "
+                               "work-compacting kernel" % a.cells,
+                   "cells": a.cells, "rtol": 1e-6, "atol": 1e-20, "network": w["network"]},
+        "fp64_plain": {"value": value_d, "unit": "cell-solves/s", "ms_per_step": wall_d / a.steps * 1e3,
+                       "kernel": "one lane per cell, FP64, no compaction"},
+        "mixed_over_fp64": value / value_d,
+        "cells_ok": {"mixed": ok_m, "fp64": ok_d},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": frac, "traffic": read_traffic(),
+                     "note": "compute (FP64/FP32 VALU) bound; achieved = FP64 + FP32*peak64/peak32 algorithmic "
+                             "flops per launch (op-count model) / mean launch time; no MFMA is used"},
+        "flops_per_launch": {"fp64": float(fl_m[0]), "fp32": float(fl_m[1])},
+        "kernel_ms": {"mixed": ms_m, "fp64": ms_d},
+    }
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -62,6 +62,13 @@ def lib():
         return _lib
     if not os.path.exists(LIBPATH):
         raise ImportError("libodevec.so not found at %s; build it with `python -m odevec_amd.build`" % LIBPATH)
+    # PyTorch ships its own HIP runtime under the same soname; loading it first
+    # lets this library bind to that one, so one process never holds two
+    # runtimes (which leaves the second unable to see the GPU).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIBPATH)
     sig = {
         "odevec_abi_version": (_I32, []),

@@ -1,0 +1,85 @@
+"""Write tests/golden/radau_known_answers.json: tight-tolerance solutions of
+seeded cells by SciPy's Radau IIA integrator (an implicit Runge-Kutta method,
+independent of the BDF code under test), rtol 1e-12.
+
+The right-hand sides and analytic Jacobians are evaluated from the network
+term tables (pinned to the reference's network definitions by
+reference_rhs.json); PRIMORDIAL's rate coefficients come from the oracle
+(pinned to the reference's rate text by reference_rates.json).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+from scipy.integrate import solve_ivp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+
+from odevec_amd import workloads as W  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def make_fun(ir, k):
+    n = ir["neq"]
+    terms = ir["terms"]
+
+    def f(t, y):
+        out = np.zeros(n)
+        for row, coef, rate, fac in terms:
+            v = coef * (k[rate] if rate >= 0 else 1.0)
+            for s in fac:
+                v *= y[s]
+            out[row] += v
+        return out
+
+    def jac(t, y):
+        J = np.zeros((n, n))
+        for row, coef, rate, fac in terms:
+            base = coef * (k[rate] if rate >= 0 else 1.0)
+            for j in set(fac):
+                red = list(fac)
+                red.remove(j)
+                v = base * fac.count(j)
+                for s in red:
+                    v *= y[s]
+                J[row, j] += v
+        return J
+
+    return f, jac
+
+
+def solve(net_id, w, idx, t_stop=None, rtol=1e-12, atol=1e-30):
+    net = O.Network.load(net_id)
+    out = []
+    for i in idx:
+        y0 = w["y"][:, i].copy()
+        prm = None if w["params"] is None else w["params"][:, i]
+        k = net.rates(y0, prm)
+        f, jac = make_fun(net.ir, k)
+        ts = float(w["t_stop"][i] if t_stop is None else t_stop)
+        r = solve_ivp(f, (0.0, ts), y0, method="Radau", rtol=rtol, atol=atol, jac=jac)
+        assert r.success, r.message
+        out.append({"cell": int(i), "y0": y0.tolist(), "t_stop": ts,
+                    "params": None if prm is None else prm.tolist(), "y": r.y[:, -1].tolist(),
+                    "nfev": int(r.nfev)})
+    return out
+
+
+def main():
+    res = {"method": "scipy.integrate.solve_ivp Radau", "rtol": 1e-12, "cases": {}}
+    res["cases"]["ROBER"] = {"network": "ROBER/NONE/DENSE", "atol": 1e-30,
+                             "cells": solve("ROBER/NONE/DENSE", W.rober(32), range(4))}
+    res["cases"]["OREGO"] = {"network": "OREGO/NONE/DENSE", "atol": 1e-30,
+                             "cells": solve("OREGO/NONE/DENSE", W.orego(32), range(4), t_stop=30.0)}
+    res["cases"]["PRIMORDIAL"] = {"network": "PRIMORDIAL/CMK/CSC", "atol": 1e-40,
+                                  "cells": solve("PRIMORDIAL/CMK/CSC", W.primordial(64), range(6), atol=1e-40)}
+    with open(os.path.join(HERE, "radau_known_answers.json"), "w") as fh:
+        json.dump(res, fh, indent=1)
+    print("wrote radau_known_answers.json")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,61 @@
+"""The C ABI library: it loads, exports every entry point include/odevec.h
+declares, and reports errors through return codes (no compute calls here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from odevec_amd import _lib
+
+HDR = os.path.join(os.path.dirname(__file__), "..", "include", "odevec.h")
+
+
+def declared_functions():
+    text = open(HDR).read()
+    return sorted(set(re.findall(r"^\s*(?:int32_t|void|const char\*)\s+(odevec_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_lib.LIBPATH)
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_lib.EXPORTED) == names
+
+
+def test_registry_and_network_info():
+    nets = _lib.networks()
+    for base in ("ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC", "PRIMORDIAL/FF/CSC"):
+        assert base in nets and base + "+REPRO" in nets
+    info = _lib.network_info("PRIMORDIAL/CMK/CSC")
+    assert info["neq"] == 16 and info["nnz"] == 89 and info["nrate"] == 38 and info["tgas_index"] == 14
+    assert _lib.network_info("ROBER/NONE/DENSE")["nparam"] == 3
+    with pytest.raises(_lib.OdevecError) as e:
+        _lib.network_info("NOPE/NONE/DENSE")
+    assert e.value.code == _lib.ENOTFOUND
+    assert _lib.lib().odevec_abi_version() == 1
+
+
+def test_create_argument_checks():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.odevec_create(b"NOPE", 4, None, C.byref(h)) == _lib.ENOTFOUND
+    assert L.odevec_create(b"ROBER/NONE/DENSE", 0, None, C.byref(h)) == _lib.EINVAL
+    bad = _lib.Opts(7, 0, 0, 0, 0, 0, 0, 0)
+    assert L.odevec_create(b"ROBER/NONE/DENSE", 4, C.byref(bad), C.byref(h)) == _lib.EINVAL
+    grp = _lib.Opts(0, 1, 999, 0, 0, 0, 0, 0)
+    assert L.odevec_create(b"ROBER/NONE/DENSE", 4, C.byref(grp), C.byref(h)) == _lib.EINVAL
+    assert L.odevec_set_tolerances(None, 1e-6, 1e-10) == _lib.EINVAL
+    assert L.odevec_destroy(None) == _lib.OK
+
+
+def test_create_without_device_fails_cleanly():
+    from conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    h = C.c_void_p()
+    rc = _lib.lib().odevec_create(b"ROBER/NONE/DENSE", 4, None, C.byref(h))
+    assert rc == _lib.EDEVICE and not h.value
+    assert "device" in _lib.last_error()

@@ -1,0 +1,89 @@
+"""The product's per-lane integrator and generated network code, compiled for
+the host, against the oracle. Compiled without contraction and with the same
+libm-free elementary functions, the two must agree bit for bit: states, last
+step sizes and every event counter."""
+import numpy as np
+import pytest
+
+from odevec_amd import workloads as W
+
+NETS = {
+    "ROBER/NONE/DENSE": lambda n: W.rober(n),
+    "ROBER/NONE/CSC": lambda n: dict(W.rober(n), network="ROBER/NONE/CSC"),
+    "OREGO/NONE/DENSE": lambda n: W.orego(n),
+    "OREGO/NONE/CSC": lambda n: dict(W.orego(n), network="OREGO/NONE/CSC"),
+    "PRIMORDIAL/CMK/CSC": lambda n: W.primordial(n),
+    "PRIMORDIAL/NONE/DENSE": lambda n: W.primordial(n, network="PRIMORDIAL/NONE/DENSE"),
+    "PRIMORDIAL/FF/CSC": lambda n: W.primordial(n, network="PRIMORDIAL/FF/CSC"),
+}
+
+
+def _oracle_solve(oracle, nid, w, **kw):
+    return oracle.Network.load(nid).solve_cells(w["y"], w["t_stop"], t0=w["t0"], dt0=w["dt0"], params=w["params"],
+                                                rtol=w["rtol"], atol=w["atol"], dt_min=w["dt_min"], **kw)
+
+
+@pytest.mark.parametrize("nid", sorted(NETS))
+@pytest.mark.parametrize("strided", [False, True])
+def test_lane_fp64_bitwise_equals_oracle(harness, oracle, nid, strided):
+    n = 24 if "PRIMORDIAL" in nid else 32
+    w = NETS[nid](n)
+    y, st, dto, cnt = harness.solve_cells(nid, w, strided=strided)
+    yr, sr, dr, cr = _oracle_solve(oracle, nid, w)
+    assert (st == 0).all()
+    assert np.array_equal(y, yr)
+    assert np.array_equal(dto, dr)
+    assert np.array_equal(cnt, cr)
+
+
+def test_lane_failure_statuses_match_oracle(harness, oracle):
+    nid = "PRIMORDIAL/CMK/CSC"
+    w = W.primordial(8)
+    for kw in ({"max_attempts": 5}, {"max_attempts": 40}):
+        y, st, dto, cnt = harness.solve_cells(nid, w, **kw)
+        yr, sr, dr, cr = _oracle_solve(oracle, nid, w, **kw)
+        assert np.array_equal(st, sr) and np.array_equal(y, yr) and np.array_equal(cnt, cr)
+    yneg = w["y"].copy()
+    yneg[4, :] = -1e-3      # negative H2 with CheckNegatives on
+    wn = dict(w, y=yneg)
+    y, st, dto, cnt = harness.solve_cells(nid, wn, check_negatives=True, dt_min=1e-3)
+    yr, sr, dr, cr = _oracle_solve(oracle, nid, dict(wn, dt_min=1e-3), check_negatives=True)
+    assert np.array_equal(st, sr) and (st == 1).all() and np.array_equal(y, yr)
+
+
+def test_lane_skips_empty_span(harness):
+    w = W.rober(4)
+    w["t_stop"] = np.array([0.0, -1.0, 1.0, 0.0])
+    y, st, dto, cnt = harness.solve_cells("ROBER/NONE/DENSE", w)
+    assert list(st) == [5, 5, 0, 5]
+    assert np.array_equal(y[:, [0, 1, 3]], w["y"][:, [0, 1, 3]])
+
+
+@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"])
+def test_lane_mixed_precision_accuracy(harness, oracle, nid):
+    """FP32 P/LU/solves, FP64 residual and control: every cell finishes, and
+    its distance to a tight FP64 solve is of the size of the FP64 run's own."""
+    n = 32
+    w = NETS[nid](n)
+    if "OREGO" in nid:
+        w["t_stop"] = np.full(n, 30.0)
+    ym, sm, _, _ = harness.solve_cells(nid, w, mixed=1)
+    yd, sd, _, _ = harness.solve_cells(nid, w, mixed=0)
+    yt, stt, _, _ = _oracle_solve(oracle, nid, dict(w, rtol=1e-10, atol=w["atol"] * 1e-4, dt_min=1e-30))
+    assert (sm == 0).all() and (sd == 0).all() and (stt == 0).all()
+    big = np.abs(yt) > 1e-8 * np.abs(yt).max(axis=0)
+    em = np.abs(ym - yt)[big] / np.abs(yt[big])
+    ed = np.abs(yd - yt)[big] / np.abs(yt[big])
+    assert np.quantile(em, 0.99) <= 10 * max(np.quantile(ed, 0.99), 1e-7)
+
+
+def test_generated_rhs_and_solve_bitwise(harness, oracle):
+    nid = "PRIMORDIAL/CMK/CSC"
+    net = oracle.Network.load(nid)
+    rng = np.random.default_rng(4)
+    for _ in range(10):
+        y = np.exp(rng.uniform(-5, 5, 16))
+        k = np.exp(rng.uniform(-5, 0, 38))
+        f_or = net.rhs(y, k)
+        f_pr = harness.netfun(nid, 0, y[:12], k, nout=12)
+        assert np.array_equal(f_pr, f_or[:12]) and np.all(f_or[12:] == 0.0)

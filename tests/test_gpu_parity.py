@@ -1,0 +1,257 @@
+"""GPU parity through the C ABI.
+
+* The +REPRO build (no FMA contraction) must equal the oracle bit for bit:
+  states, last step sizes and event counters, in every mode.
+* The default build differs only by contraction; there the bar is accuracy:
+  10*rtol*|y| + atol against the oracle where the problem is well conditioned
+  (ROBER), and against tight Radau solutions otherwise.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from odevec_amd import workloads as W
+from odevec_amd import _lib
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_solve(nid, w, n=None, precision=0, mode=0, group=0, **kw):
+    from odevec_amd.api import Odevec
+    n = w["y"].shape[1] if n is None else n
+    say("GPU solve", nid, "cells", n, "precision", precision, "mode", mode, "group", group)
+    ode = Odevec(nid, max(n, 1), precision=precision, mode=mode, group=group, **kw)
+    ode.RelativeTolerance = w["rtol"]
+    ode.AbsoluteTolerance = w["atol"]
+    ode.MinimalTimestep = w["dt_min"]
+    y = np.array(w["y"], order="C", copy=True)
+    st, dto, stats = ode.solve_cells(y, w["t_stop"], t0=w["t0"], dt0=w["dt0"], params=w["params"], stats=True,
+                                     raise_on_fail=False)
+    ode.CloseOdevec()
+    return y, st, dto, stats
+
+
+def oracle_cells(oracle, nid, w):
+    return oracle.Network.load(nid.replace("+REPRO", "")).solve_cells(
+        w["y"], w["t_stop"], t0=w["t0"], dt0=w["dt0"], params=w["params"], rtol=w["rtol"], atol=w["atol"],
+        dt_min=w["dt_min"])
+
+
+CASES = {
+    "ROBER": lambda: W.rober(32, seed=1),          # the CPU configuration: 32 cells
+    "OREGO": lambda: W.orego(512, seed=2),
+    "PRIMORDIAL": lambda: W.primordial(512, seed=3),
+    "PRIMORDIAL_FF": lambda: W.primordial(256, seed=3, network="PRIMORDIAL/FF/CSC"),
+    "PRIMORDIAL_DENSE": lambda: W.primordial(256, seed=3, network="PRIMORDIAL/NONE/DENSE"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_repro_bitwise_percell(oracle, case, mode):
+    w = CASES[case]()
+    nid = w["network"] + "+REPRO"
+    y, st, dto, stats = gpu_solve(nid, w, mode=mode)
+    yr, sr, dr, cr = oracle_cells(oracle, nid, w)
+    assert np.array_equal(st, sr) and (st == 0).all()
+    assert np.array_equal(y, yr)
+    assert np.array_equal(dto, dr)
+    assert stats["attempts"] == int(cr[0].sum()) and stats["newton"] == int(cr[2].sum())
+
+
+@pytest.mark.parametrize("case,G", [("ROBER", 32), ("OREGO", 64), ("PRIMORDIAL", 64)])
+def test_repro_bitwise_lockstep(oracle, case, G):
+    """Lockstep groups reproduce the reference at nvector = G (shared t_stop)."""
+    w = CASES[case]()
+    n = min(w["y"].shape[1], 4 * G) - 7      # a ragged last group
+    w = W.subset(w, np.arange(n))
+    w["t_stop"] = np.full(n, w["t_stop"][0])
+    nid = w["network"] + "+REPRO"
+    y, st, dto, stats = gpu_solve(nid, w, mode=_lib.LOCKSTEP, group=G)
+    net = oracle.Network.load(w["network"])
+    yr = np.zeros_like(y)
+    for g0 in range(0, n, G):
+        sl = slice(g0, min(g0 + G, n))
+        prm = None if w["params"] is None else w["params"][:, sl]
+        yv, s, t, d, cnt = net.solve_vector(w["y"][:, sl], 0.0, w["t_stop"][0], w["dt0"][0], w["rtol"], w["atol"],
+                                            w["dt_min"], params=prm)
+        assert s == 0
+        yr[:, sl] = yv
+    assert (st == 0).all() and np.array_equal(y, yr)
+
+
+def test_repro_lockstep_mask(oracle):
+    w = W.rober(32)
+    mask = np.zeros(32, np.int8)
+    mask[::3] = 1
+    from odevec_amd.api import Odevec
+    ode = Odevec("ROBER/NONE/DENSE+REPRO", 32, mode=_lib.LOCKSTEP, group=32)
+    ode.y[:] = w["y"].T
+    say("GPU SolveODE masked lockstep")
+    t, dt = ode.SolveODE(0.0, 1e-10, 1e3, Mask=mask)
+    yv, s, tt, dd, cnt = oracle.Network.load("ROBER/NONE/DENSE").solve_vector(
+        w["y"], 0.0, 1e3, 1e-10, 1e-6, 1e-10, params=np.array([0.04, 3e7, 1e4])[:, None] * np.ones((1, 32)),
+        mask=mask)
+    assert t == 1e3 and np.array_equal(ode.y.T, yv) and dt == dd
+    assert np.array_equal(ode.y[mask == 0], w["y"].T[mask == 0])
+    ode.CloseOdevec()
+
+
+def test_reference_rober_driver_pattern(oracle):
+    """tests/rober.f90:39-62: one handle, nvector 32, repeated SolveODE calls
+    over 21 output decades with dt = 1e-10 before every call."""
+    from odevec_amd.api import Odevec
+    ode = Odevec("ROBER/NONE/DENSE+REPRO", 32, mode=_lib.LOCKSTEP, group=32)
+    rtol = 10.0 ** (-2.0 - 24 * 0.25)
+    ode.RelativeTolerance = rtol
+    ode.AbsoluteTolerance = 1e-6 * rtol
+    ode.MinimalTimestep = 1e-20
+    ode.y[:] = np.array([1.0, 0.0, 0.0])
+    net = oracle.Network.load("ROBER/NONE/DENSE")
+    yref = np.zeros((3, 32))
+    yref[0] = 1.0
+    prm = np.array([0.04, 3e7, 1e4])[:, None] * np.ones((1, 32))
+    t_start, t_stop = 0.0, 1e-10
+    say("GPU reference driver pattern")
+    while t_stop < 1e11:
+        t_prev = t_start
+        t_start, dt = ode.SolveODE(t_start, 1e-10, t_stop)
+        yref, s, tr, dr, cnt = net.solve_vector(yref, t_prev, t_stop, 1e-10, rtol, 1e-6 * rtol, 1e-20, params=prm)
+        assert t_start == t_stop == tr
+        assert s == 0 and np.array_equal(ode.y.T, yref), t_stop
+        t_stop = t_stop * 10.0
+    assert abs(ode.y[0].sum() - 1.0) < 1e-6     # mass conservation of ROBER
+    ode.CloseOdevec()
+
+
+@pytest.mark.parametrize("n", [1, 63, 257, 1000])
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_ragged_sizes_and_skips(oracle, n, mode):
+    w = W.primordial(n, seed=9)
+    w["t_stop"][::5] = 0.0           # t_stop == t0: skipped, y unchanged
+    y, st, dto, stats = gpu_solve("PRIMORDIAL/CMK/CSC+REPRO", w, mode=mode)
+    yr, sr, dr, cr = oracle_cells(oracle, "PRIMORDIAL/CMK/CSC", w)
+    assert np.array_equal(st, sr) and np.array_equal(y, yr)
+    assert (st[::5] == 5).all() and np.array_equal(y[:, ::5], w["y"][:, ::5])
+
+
+def test_rober_default_build_within_10rtol(oracle):
+    w = W.rober(512)
+    for mode in (_lib.PERCELL, _lib.PERSISTENT):
+        y, st, dto, stats = gpu_solve("ROBER/NONE/DENSE", w, mode=mode)
+        yr, sr, dr, cr = oracle_cells(oracle, "ROBER/NONE/DENSE", w)
+        assert (st == 0).all()
+        assert np.all(np.abs(y - yr) <= 10 * w["rtol"] * np.abs(yr) + w["atol"])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_default_build_accuracy_vs_radau(oracle, precision):
+    """Distance to the rtol-1e-12 Radau answers is of the size of the
+    oracle's own (the problems amplify rounding differences, so two builds
+    that round differently cannot agree to 10*rtol with each other)."""
+    gold = json.load(open(os.path.join(GOLD, "radau_known_answers.json")))["cases"]
+    for name in ("ROBER", "PRIMORDIAL"):
+        case = gold[name]
+        cells = case["cells"]
+        n = len(cells)
+        w = {"network": case["network"], "y": np.array([c["y0"] for c in cells]).T.copy(),
+             "t_stop": np.array([c["t_stop"] for c in cells]), "t0": np.zeros(n),
+             "dt0": np.full(n, 1e-10 if name == "ROBER" else W.PRIMORDIAL_DT_HINT),
+             "params": None if cells[0]["params"] is None else np.array([c["params"] for c in cells]).T.copy(),
+             "rtol": 1e-6, "atol": 1e-10 if name == "ROBER" else 1e-20, "dt_min": 1e-20}
+        y, st, _, _ = gpu_solve(case["network"], w, precision=precision)
+        yo, so, _, _ = oracle_cells(oracle, case["network"], w)
+        ref = np.array([c["y"] for c in cells]).T
+        big = np.abs(ref) > 1e-6 * np.abs(ref).max(axis=0)
+        eg = np.max(np.abs(y - ref)[big] / np.abs(ref[big]))
+        eo = np.max(np.abs(yo - ref)[big] / np.abs(ref[big]))
+        say(name, "precision", precision, "GPU err vs Radau %.3g, oracle %.3g" % (eg, eo))
+        assert (st == 0).all() and eg <= max(10 * eo, 1e-5)
+
+
+def test_mixed_precision_primordial_all_cells_finish():
+    w = W.primordial(8192, seed=3)
+    y, st, dto, stats = gpu_solve("PRIMORDIAL/CMK/CSC", w, precision=1, mode=_lib.PERSISTENT)
+    assert (st == 0).all() and stats["flops32"] > 0
+
+
+def test_async_device_pointers_equal_sync():
+    import torch
+    w = W.primordial(4096, seed=12)
+    from odevec_amd.api import Odevec
+    ode = Odevec("PRIMORDIAL/CMK/CSC", 4096, mode=_lib.PERSISTENT)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = 1e-6, 1e-20, 1e-20
+    y_sync = w["y"].copy()
+    st_sync, _, _ = ode.solve_cells(y_sync, w["t_stop"], dt0=w["dt0"])
+    dev = torch.device("cuda:0")
+    y = torch.tensor(w["y"], dtype=torch.float64, device=dev)
+    ts = torch.tensor(w["t_stop"], dtype=torch.float64, device=dev)
+    dt0 = torch.tensor(w["dt0"], dtype=torch.float64, device=dev)
+    st = torch.zeros(4096, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    say("GPU async solve on torch tensors")
+    ode.solve_cells_async(4096, ts.data_ptr(), y.data_ptr(), dt0_ptr=dt0.data_ptr(), status_ptr=st.data_ptr(),
+                          stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), y_sync) and (st.cpu().numpy() == st_sync).all()
+    ode.CloseOdevec()
+
+
+def test_repro_reference_failures(oracle):
+    """The cells of the 1M workload on which the reference algorithm fails
+    (it would STOP; golden/primordial_fail_cells.json) fail on the GPU with
+    the same status and the same returned state."""
+    fail = json.load(open(os.path.join(GOLD, "primordial_fail_cells.json")))
+    full = W.primordial(fail["ncells"], seed=3)
+    idx = np.array(sorted(set(fail["cells"]) | set(range(0, fail["ncells"], fail["ncells"] // 25))))
+    w = W.subset(full, idx)
+    y, st, dto, stats = gpu_solve(w["network"] + "+REPRO", w, mode=_lib.PERSISTENT)
+    yr, sr, dr, cr = oracle_cells(oracle, w["network"], w)
+    assert np.array_equal(st, sr) and np.array_equal(y, yr)
+    assert set(idx[st != 0]) == set(fail["cells"])
+
+
+def test_primordial_full_size_conservation():
+    """Configuration size (1,048,576 cells, per-cell end times 1e8..1e14 s):
+    cells finish at the reference algorithm's rate (it fails 39 of them), and
+    the finished ones conserve H, He and D nuclei as well as the reference
+    algorithm does (it conserves them up to rounding and tolerances: the
+    oracle's largest drifts are recorded in the fixture)."""
+    fail = json.load(open(os.path.join(GOLD, "primordial_fail_cells.json")))
+    w = W.primordial(1048576, seed=3)
+    V = {"H": [0, 1, 1, 0, 2, 0, 1, 1, 0, 2, 0, 0, 0, 0, 0, 0],
+         "He": [0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0],
+         "D": [0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 0, 0, 0, 0, 0]}
+    for precision in (0, 1):
+        y, st, dto, stats = gpu_solve("PRIMORDIAL/CMK/CSC", w, precision=precision, mode=_lib.PERSISTENT)
+        ok = st == 0
+        say("precision", precision, "status counts", np.bincount(st))
+        assert ok.sum() >= 1048576 - 200 and set(np.unique(st[~ok])) <= {2}
+        for name, v in V.items():
+            v = np.array(v, dtype=float)
+            a, b = v @ w["y"][:, ok], v @ y[:, ok]
+            r = np.abs(a - b) / a
+            drift = r.max()
+            oq = fail["oracle_conservation_quantiles"][name]
+            say(name, "drift max %.3g q99.9 %.3g cells>1e-6 %d (oracle max %.3g q99.9 %.3g cells>1e-6 %d)" % (
+                drift, np.quantile(r, 0.999), (r > 1e-6).sum(), fail["oracle_conservation_max"][name], oq["q999"],
+                oq["n_gt_1e-6"]))
+            if precision == 0:
+                assert drift <= 10 * fail["oracle_conservation_max"][name] + 1e-14
+            else:
+                # FP32 factors of P: a correction keeps the invariants only to
+                # float rounding of its largest terms, which shows in the trace
+                # deuterium nuclei (measured: 99.9% of cells within 2.1e-3,
+                # worst 0.094); H and He stay within 1e-4
+                bound = 1e-4 if name != "D" else 1e-2
+                assert np.quantile(r, 0.999) <= bound and drift <= 0.2
+        assert np.array_equal(y[12:], w["y"][12:])    # frozen species untouched

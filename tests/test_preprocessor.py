@@ -1,0 +1,139 @@
+"""Preprocessor: network structure, orderings, LU fill, CSC conventions,
+frozen species, emitted code and the command line (pre_odevec.py)."""
+import json
+import os
+
+import numpy as np
+import pytest
+import sympy as sp
+from scipy import sparse
+from scipy.sparse import csgraph
+
+from odevec_amd.pre_odevec import cli, emit, networks as N, structure as S
+
+GEN = os.path.join(os.path.dirname(__file__), "..", "odevec_amd", "csrc", "generated")
+
+
+@pytest.mark.parametrize("ordering,nnz", [("None", 143), ("CMK", 93), ("FF", 88), ("INVERT", 90)])
+def test_primordial_structure_counts(ordering, nnz):
+    st = S.analyse(N.load("PRIMORDIAL"), ordering, "CSC")
+    assert int(st.J.sum()) == 81
+    assert int(st.P.sum()) == 85
+    assert len(st.terms) == 122
+    assert st.csc.nnz == nnz
+
+
+def test_small_networks_structure():
+    rober = S.analyse(N.load("ROBER"), "None", "DENSE")
+    assert (int(rober.P.sum()), rober.csc.nnz) == (8, 8)
+    orego = S.analyse(N.load("OREGO"), "None", "DENSE")
+    assert (int(orego.P.sum()), orego.csc.nnz) == (7, 8)
+    assert orego.LU[2, 1] and not orego.P[2, 1]      # the one fill entry (3,2)
+
+
+@pytest.mark.parametrize("name", ["ROBER", "OREGO", "PRIMORDIAL"])
+@pytest.mark.parametrize("ordering", ["None", "CMK", "FF", "INVERT"])
+def test_fill_equals_sympy_symbolic_lu(name, ordering):
+    """pre_odevec.py:494 fixes nnz(LU) with SymPy's symbolic LU; the structural
+    fill used here must give the same pattern."""
+    net = N.load(name)
+    st = S.analyse(net, ordering, "CSC")
+    J = sp.Matrix(net.rhs).jacobian(net.y)
+    dt, beta = sp.symbols("dt beta")
+    P = sp.eye(net.neq) - dt * beta * J
+    Po = sp.Matrix(net.neq, net.neq, lambda i, j: P[int(st.perm[i]), int(st.perm[j])])
+    assert (S.sympy_lu_pattern(Po) == st.LU).all()
+
+
+def test_orderings_follow_library_routines():
+    st = S.analyse(N.load("PRIMORDIAL"), "None", "CSC")
+    P = st.P
+    # CMK: SciPy reverse Cuthill-McKee of the pattern (symmetrised by SciPy)
+    rows, cols = np.nonzero(P)
+    M = sparse.csc_matrix((np.ones(len(rows), dtype=int), (rows, cols)), shape=P.shape)
+    assert list(S.perm_cmk(P)) == list(csgraph.reverse_cuthill_mckee(M.T.tocsc(), symmetric_mode=False))
+    # FF: stable order of nnz(col) + nnz(row) - 1 for n <= 16
+    key = P.sum(axis=0) + P.sum(axis=1) - 1
+    assert list(S.perm_ff(P)) == list(np.argsort(key, kind="stable"))
+    assert list(S.perm_invert(P)) == list(range(16))[::-1]
+
+
+def test_csc_conventions():
+    """u_col_start: first entry of each column (+ nnz), l_col_start: the
+    diagonal, rows ascending within a column (pre_odevec.py:524-540)."""
+    st = S.analyse(N.load("PRIMORDIAL"), "CMK", "CSC")
+    cs = st.csc
+    assert cs.u_col_start[0] == 0 and cs.u_col_start[-1] == cs.nnz
+    for j in range(16):
+        col = cs.entries[cs.u_col_start[j]:cs.u_col_start[j + 1]]
+        assert all(c == j for _, c in col)
+        assert [r for r, _ in col] == sorted(r for r, _ in col)
+        assert cs.entries[cs.l_col_start[j]] == (j, j)
+
+
+def test_frozen_species_primordial():
+    st = S.analyse(N.load("PRIMORDIAL"), "CMK", "CSC")
+    assert S.frozen_species(16, st.terms) == [12, 13, 14, 15]   # CR, g, Tgas, dummy
+    red = S.reduce_frozen(S.analyse(N.load("PRIMORDIAL"), "CMK", "CSC"))
+    assert red.neq == 12 and red.csc.nnz == 89 and red.active == list(range(12))
+    assert S.frozen_species(3, S.analyse(N.load("ROBER"), "None", "DENSE").terms) == []
+
+
+def test_primordial_reactions_conserve_elements_and_charge():
+    """Mass action of the reaction list conserves H, He, D nuclei and charge."""
+    H = [0, 1, 1, 0, 2, 0, 1, 1, 0, 2, 0, 0, 0, 0, 0, 0]
+    He = [0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0]
+    D = [0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 0, 0, 0, 0, 0]
+    Q = [-1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 0, 0, 0, 0]
+    for reac, prod in N.PRIMORDIAL_REACTIONS:
+        for vec in (H, He, D, Q):
+            assert sum(vec[s] for s in reac) == sum(vec[s] for s in prod), (reac, prod)
+
+
+def test_emitted_files_are_current(tmp_path):
+    """The committed generated headers/IR are what the preprocessor emits."""
+    for base in ("ROBER_NONE_DENSE", "OREGO_NONE_DENSE", "PRIMORDIAL_CMK_CSC", "PRIMORDIAL_FF_CSC"):
+        name, ordering, packaging = base.split("_")
+        cli.generate(name, "None" if ordering == "NONE" else ordering, packaging, str(tmp_path))
+        for f in ("net_%s.h" % base, "%s.json" % base, "inst_%s.hip" % base):
+            with open(os.path.join(GEN, f)) as a, open(os.path.join(str(tmp_path), f)) as b:
+                assert a.read() == b.read(), f
+
+
+def test_opcounts_primordial_cmk():
+    ir = json.load(open(os.path.join(GEN, "PRIMORDIAL_CMK_CSC.json")))
+    oc = ir["opcounts_active"]
+    # one multiply-subtract per elimination update, one division per U entry
+    assert oc["lu"]["div"] == sum(1 for r, c in ir["lu_entries"] if r < c and r < 12 and c < 12)
+    assert oc["solve"]["div"] == 12
+    assert oc["rhs"]["flops"] < 383      # flux sharing beats the naive 383 flops
+
+
+def test_cli_flags(tmp_path):
+    assert cli.main(["--example=ROBER", "--nvector=32", "--outdir", str(tmp_path)]) == 0
+    assert os.path.exists(os.path.join(str(tmp_path), "net_ROBER_NONE_DENSE.h"))
+    with pytest.raises(SystemExit):
+        cli.main(["--example=ROBER", "--LUmethod=2", "--outdir", str(tmp_path)])
+    with pytest.raises(SystemExit):
+        cli.main(["--example=ROBER", "--packaging=COO", "--outdir", str(tmp_path)])
+    with pytest.raises(SystemExit):   # pre_odevec.py:455-456
+        cli.main(["--example=ROBER", "--equilibrium_H=1", "--packaging=CSC", "--outdir", str(tmp_path)])
+
+
+def test_krome_setupfile(tmp_path):
+    f = tmp_path / "net.py"
+    f.write_text("neq = 2\nnrea = 1\ny = symbols('y(\\\\:\\\\,1:3)')\nk = symbols('k(\\\\:\\\\,1:2)')\n"
+                 "rhs = Matrix([-k[0]*y[0]*y[1], k[0]*y[0]*y[1]])\n")
+    net = N.load("KROME", str(f))
+    assert net.neq == 2 and net.nparam == 1 and net.rate_kind == "param"
+    st = S.analyse(net, "None", "DENSE")
+    assert len(st.terms) == 2
+
+
+def test_synth64_seeded():
+    a = N.synth_reactions(64, 300, 6)
+    b = N.synth_reactions(64, 300, 6)
+    assert a == b
+    reac, A, Ea = a
+    assert len(reac) == 300 and all(len(set(r + p)) == 4 for r, p, _ in reac)
+    assert all(1e-12 <= v <= 1e-8 for v in A) and all(0 <= v <= 5e4 for v in Ea)
