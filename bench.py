@@ -62,9 +62,12 @@ class Runner:
         self.ode.AbsoluteTolerance = w["atol"]
         self.ode.MinimalTimestep = w["dt_min"]
 
-    def step(self, count_flops=False):
+    def restore(self):
         self.y.copy_(self.y0)
+
+    def solve(self, count_flops=False):
         stream = self.torch.cuda.current_stream().cuda_stream
+        assert stream, "a created stream is required (the null handle selects the solver's own stream)"
         self.ode.solve_cells_async(self.n, self.ts.data_ptr(), self.y.data_ptr(), dt0_ptr=self.dt0.data_ptr(),
                                    status_ptr=self.st.data_ptr(),
                                    flops_ptr=self.flops.data_ptr() if count_flops else None, stream=stream)
@@ -72,13 +75,17 @@ class Runner:
     def timed(self, steps, warmup):
         torch = self.torch
         for _ in range(warmup):
-            self.step()
+            self.restore()
+            self.solve()
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
         t0 = time.perf_counter()
         for a, b in ev:
+            # a step: restore the initial state, then the solve (the events
+            # bracket the solver's launches on this stream)
+            self.restore()
             a.record()
-            self.step()
+            self.solve()
             b.record()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -87,7 +94,8 @@ class Runner:
 
     def count_flops(self):
         self.flops.zero_()
-        self.step(count_flops=True)
+        self.restore()
+        self.solve(count_flops=True)
         self.torch.cuda.synchronize()
         st = self.st.cpu().numpy()
         return self.flops.cpu().numpy(), st
@@ -148,6 +156,10 @@ def main():
     fl_d, st_d = fp64.count_flops()
     ok_m, ok_d = int((st_m == 0).sum()), int((st_d == 0).sum())
     value = a.steps * a.cells / wall_m
+    for name, wall, ms in (("mixed", wall_m, ms_m), ("fp64", wall_d, ms_d)):
+        if not 0.8 * wall / a.steps <= float(np.mean(ms)) * 1e-3 <= 1.01 * wall / a.steps:
+            print("warning: %s event time %.3f ms/launch disagrees with wall time %.3f ms/step"
+                  % (name, float(np.mean(ms)), wall / a.steps * 1e3), file=sys.stderr)
     value_d = a.steps * a.cells / wall_d
     t_kernel = float(np.mean(ms_m)) * 1e-3
     achieved = (fl_m[0] + fl_m[1] * PEAK_FP64 / PEAK_FP32) / t_kernel

@@ -138,7 +138,6 @@ struct Lane {
   // weight is NaN (the NaN check fires), bit 2 a frozen value is negative
   int fz;
   Counters cnt;
-  double flops64, flops32;
 
   ODV_HD const double* kptr() const { return KS == 0 ? kl : kx; }
   ODV_HD double eo(int j) const { return KS == 0 ? eol[j] : eox[j * KS]; }
@@ -199,14 +198,23 @@ ODV_HD inline void odv_weights(const double* y, double rtol, double atol, double
 
 // sqrt(sum_j x_j^2 w_j / neq) for one cell (src/odevec.F90:669-702); the
 // frozen species' terms are exact zeros unless their weight is not finite
-template <class L>
-ODV_HD inline double odv_wnorm(const L& s, const double* x) {
+template <class L, class X>
+ODV_HD inline double odv_wnorm_f(const L& s, X x) {
   constexpr int N = L::N;
-  double acc = (x[0] * x[0]) * s.w[0];
+  double v = x(0);
+  double acc = (v * v) * s.w[0];
 #pragma unroll
-  for (int j = 1; j < N; ++j) acc = acc + (x[j] * x[j]) * s.w[j];
+  for (int j = 1; j < N; ++j) {
+    v = x(j);
+    acc = acc + (v * v) * s.w[j];
+  }
   if (s.fz & 1) acc = NAN;
   return sqrt(acc / (double)L::NF);
+}
+
+template <class L>
+ODV_HD inline double odv_wnorm(const L& s, const double* x) {
+  return odv_wnorm_f(s, [&](int j) { return x[j]; });
 }
 
 template <int N>
@@ -277,15 +285,13 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
   if (q == 1) {
     down = 0.0;
   } else {
-    double col[N];
+    const double nq = g.max_active(odv_wnorm_f(s, [&](int j) {
+                                     double v = 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double v = 0.0;
-#pragma unroll
-      for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.z[kk][j] : v;
-      col[j] = v;
-    }
-    const double nq = g.max_active(odv_wnorm(s, col), s.active);
+                                     for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.z[kk][j] : v;
+                                     return v;
+                                   }),
+                                   s.active);
     down = 1.0 / (ODV_F32(1.3) * odv::m_pow(nq / odv_tau(q, q - 1, lqq), 1.0 / q) + 1e-6);
   }
   const double ne = g.max_active(odv_wnorm(s, s.en), s.active);
@@ -293,10 +299,7 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
   if (q == MAXORD || s.neglect_up) {
     up = 0.0;
   } else {
-    double d[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) d[j] = s.en[j] - s.eo(j);
-    const double nd = g.max_active(odv_wnorm(s, d), s.active);
+    const double nd = g.max_active(odv_wnorm_f(s, [&](int j) { return s.en[j] - s.eo(j); }), s.active);
     up = 1.0 / (ODV_F32(1.4) * odv::m_pow(nd / odv_tau(q, q + 1, lqq), 1.0 / (q + 2.0)) + 1e-6);
   }
   int loc = 1;
@@ -331,7 +334,6 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
   s.swu = 0;
   s.neglect_up = false;
   s.cnt.stepsel++;
-  s.flops64 += 10.0 * N + 20.0 + q * N;
   return sc;
 }
 
@@ -417,8 +419,6 @@ ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, doubl
   s.t = t0;
   s.tstop = tstop;
   s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
-  s.flops64 = 0.0;
-  s.flops32 = 0.0;
   {
     double kk[Net::NRATE];
     Net::rates(yfull, prm, kk);
@@ -456,7 +456,6 @@ ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, doubl
   const double W0 = fmax(fabs(t0), fabs(tstop));
   double dt = sqrt(tol / (odv::m_pow(W0, -2.0) + acc / (double)L::NF));
   s.dt = fmin(dt, fabs(tstop - t0));
-  s.flops64 += Net::FLOPS_RHS + 12.0 * N + 8.0;
 }
 
 // InterpolateSolution (src/odevec.F90:998-1030): Horner back to t_stop.
@@ -489,6 +488,30 @@ ODV_HD inline void odv_result(const L& s, double* y) {
   }
 }
 
+// Operation-count model of one cell's solve, evaluated once from its event
+// counters: the generated per-network counts for RHS, Jacobian,
+// factorisation and triangular solves, plus the vector work of prediction,
+// correction, error test, history update and order selection at the cell's
+// mean order. The factorisation and solves are FP32 work in the mixed form.
+template <class L>
+ODV_HD inline void odv_flops(const Counters& c, double& f64, double& f32) {
+  using Net = typename L::Net;
+  constexpr double N = L::N;
+  constexpr bool MIXED = sizeof(typename L::M) != sizeof(double);
+  const double at = c.attempts;
+  const double q = c.attempts > 0 ? (double)c.qsum / at : 1.0;
+  const double lu = L::RCP_ ? Net::FLOPS_LU_R : Net::FLOPS_LU;
+  const double sv = L::RCP_ ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
+  double d = Net::FLOPS_RHS + 12.0 * N + 8.0;
+  d += at * 0.5 * q * (q + 1.0) * N;
+  d += c.jac * (double)Net::FLOPS_JAC + c.newton * (Net::FLOPS_RHS + 9.0 * N + 8.0);
+  d += (at - c.convfail) * (7.0 * N + 2.0) + c.steps * 2.0 * (q + 1.0) * N;
+  d += c.stepsel * (10.0 * N + 20.0 + q * N) + c.convfail * (0.5 * q * (q + 1.0) * N + q * N);
+  const double lin = c.jac * lu + c.newton * sv;
+  f64 = MIXED ? d : d + lin;
+  f32 = MIXED ? lin : 0.0;
+}
+
 // One pass of the reference's `predictor:` loop (src/odevec.F90:259-456) plus,
 // on acceptance, the Jacobian-refresh policy (src/odevec.F90:458-464).
 // Returns true when the lane is finished (status set).
@@ -507,7 +530,6 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   }
   s.t = s.t + s.dt;
   odv_predict(s);
-  s.flops64 += 0.5 * s.q * (s.q + 1) * N;
 #pragma unroll
   for (int j = 0; j < N; ++j) s.en[j] = 0.0;
 
@@ -536,9 +558,6 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
     s.rate = 0.7;
     s.jchange = 1.0;
     s.cnt.jac++;
-    s.flops64 += Net::FLOPS_JAC;
-    if (MIXED) s.flops32 += L::RCP_ ? Net::FLOPS_LU_R : Net::FLOPS_LU;
-    else s.flops64 += L::RCP_ ? Net::FLOPS_LU_R : Net::FLOPS_LU;
   }
 
   // corrector (src/odevec.F90:318-394) with CheckConvergence (src/odevec.F90:622-665)
@@ -562,9 +581,6 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
 #pragma unroll
     for (int j = 0; j < N; ++j) s.en[j] = s.en[j] + f[j];
     s.cnt.newton++;
-    s.flops64 += Net::FLOPS_RHS + 9.0 * N + 8.0;
-    if (MIXED) s.flops32 += L::RCP_ ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
-    else s.flops64 += L::RCP_ ? Net::FLOPS_SOLVE_R : Net::FLOPS_SOLVE;
 
     const double e = g.max_active(odv_wnorm(s, f), s.active);
     double rtmp = 0.0;
@@ -607,7 +623,6 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       }
       odv_reset(s);
       odv_set_step(s, 0.25);
-      s.flops64 += 0.5 * s.q * (s.q + 1) * N + s.q * N;
       if (s.dt < c.dt_min) {
         s.status = ST_DT_MIN;
         return true;
@@ -626,8 +641,8 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
     odv_cur_y(s, beta, y);
     odv_weights<N>(y, c.rtol, c.atol, s.w);
   }
-  s.flops64 += 7.0 * N + 2.0;
-  if (err > 1.0) {
+  const bool rejected = err > 1.0;
+  if (rejected) {
     s.cnt.errfail++;
     if (s.fet >= 3) {
       s.t = told;
@@ -638,43 +653,39 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
 #pragma unroll
       for (int j = 0; j < N; ++j) s.z[1][j] = s.dt * f[j];
       s.upd_jac = true;
-      s.flops64 += Net::FLOPS_RHS + N;
     } else {
       s.t = told;
       odv_reset(s);
       s.upd_jac = true;
       s.swu = 0;
       s.neglect_up = true;
-      s.flops64 += 0.5 * s.q * (s.q + 1) * N;
     }
+  } else {
+    s.swu++;
+    s.lju++;
+    // UpdateNordsieck (src/odevec.F90:747-765)
+#pragma unroll
+    for (int kk = 0; kk <= MAXORD; ++kk) {
+      if (kk <= s.q) {
+        const double lk = s.lq[kk];
+#pragma unroll
+        for (int j = 0; j < N; ++j) s.z[kk][j] = s.z[kk][j] + s.en[j] * lk;
+      }
+    }
+  }
+  // CalcStepSizeOrder + SetStepSize after a rejection (src/odevec.F90:434-436)
+  // or after order+1 accepted steps (src/odevec.F90:447-450): one call site
+  if (rejected || s.swu == s.q + 1) {
     const double sc = odv_step_order(s, g);
     if (sc < 0.0) {
       s.status = ST_ORDER;
       return true;
     }
     odv_set_step(s, sc);
+  }
+  if (rejected) {
     s.fet++;
     return false;
-  }
-  s.swu++;
-  s.lju++;
-  // UpdateNordsieck (src/odevec.F90:747-765)
-#pragma unroll
-  for (int kk = 0; kk <= MAXORD; ++kk) {
-    if (kk <= s.q) {
-      const double lk = s.lq[kk];
-#pragma unroll
-      for (int j = 0; j < N; ++j) s.z[kk][j] = s.z[kk][j] + s.en[j] * lk;
-    }
-  }
-  s.flops64 += 2.0 * (s.q + 1) * N;
-  if (s.swu == s.q + 1) {
-    const double sc = odv_step_order(s, g);
-    if (sc < 0.0) {
-      s.status = ST_ORDER;
-      return true;
-    }
-    odv_set_step(s, sc);
   }
   s.fet = 0;
   s.cnt.steps++;

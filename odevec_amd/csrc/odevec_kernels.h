@@ -97,8 +97,6 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
   const double ts = a.tstop[i * a.ts_s];
   const double dh = a.dt0[i * a.dt_s];
   s.active = true;
-  s.flops64 = 0.0;
-  s.flops32 = 0.0;
   s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
   s.dt = dh;
   if ((a.mask && !a.mask[i]) || !(t0 < ts)) {
@@ -139,8 +137,7 @@ __global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
     bool done = odv_load_begin(a, i, s);
     while (!done) done = odv_attempt(s, a.ctl, g);
     odv_finish(a, i, s);
-    fl64 = s.flops64;
-    fl32 = s.flops32;
+    if (a.flops && s.status != ST_SKIPPED) odv_flops<L>(s.cnt, fl64, fl32);
   }
   if (a.flops) {
     fl64 = odv_wave_sum(fl64);
@@ -181,8 +178,12 @@ __global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
     const bool need = !have || done;
     if (have && done) {
       odv_finish(a, i, s);
-      fl64 += s.flops64;
-      fl32 += s.flops32;
+      if (a.flops && s.status != ST_SKIPPED) {
+        double f64, f32;
+        odv_flops<L>(s.cnt, f64, f32);
+        fl64 += f64;
+        fl32 += f32;
+      }
       have = false;
     }
     const unsigned long long ball = __ballot(need);
@@ -274,7 +275,6 @@ __global__ void __launch_bounds__(256) odv_k_lockstep(KArgs a) {
   const double dh = a.dt0[g0 * a.dt_s];
   s.active = valid && (!a.mask || a.mask[ii]);
   s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
-  s.flops64 = s.flops32 = 0.0;
   s.dt = dh;
   s.status = ST_SKIPPED;
   const bool any_act = g.any_of(s.active);
@@ -300,10 +300,13 @@ __global__ void __launch_bounds__(256) odv_k_lockstep(KArgs a) {
     if (a.dt_out) a.dt_out[i] = s.dt;
     if (a.counters) odv_store_counters(a, i, s.cnt);
   }
-  if (a.flops && threadIdx.x == 0) {
+  if (a.flops && threadIdx.x == 0 && s.status != ST_SKIPPED) {
+    // every lane of the group does the group's work
     const double nval = (double)min((long long)a.group, a.ncells - g0);
-    atomicAdd(&a.flops[0], s.flops64 * nval);
-    atomicAdd(&a.flops[1], s.flops32 * nval);
+    double f64, f32;
+    odv_flops<L>(s.cnt, f64, f32);
+    atomicAdd(&a.flops[0], f64 * nval);
+    atomicAdd(&a.flops[1], f32 * nval);
   }
 }
 
