@@ -78,6 +78,8 @@ struct odevec_s {
   int8_t* d_mask = nullptr;
   double* d_scal = nullptr;    // [0..2] uniform t0, t_stop, dt0; [3..] default params; [..] flops
   unsigned int* d_queue = nullptr;   // 64 queue counters, one per in-flight persistent launch
+  double* d_ws = nullptr;            // rate workspace of the per-cell kernels
+  int64_t ws_cap = 0;                // doubles
   int qslot = 0;
   int nscal = 0;
   std::vector<int> h_status;
@@ -162,7 +164,23 @@ int launch(odevec_s* h, const Req& r) {
   if (grp <= 0) grp = (int)(h->nvector < 256 ? h->nvector : 256);
   if (grp < 1) grp = 1;
   a.group = grp;
-  hipError_t e = h->net->launch(a, h->opts.mode, h->opts.precision, h->opts.block, h->opts.grid, r.stream);
+  odv::Plan p{};
+  hipError_t e = h->net->plan(r.ncells, h->opts.mode, h->opts.precision, h->opts.block, h->opts.grid, grp, &p);
+  if (e != hipSuccess) return hip_fail(e, "launch plan");
+  if (p.ws_doubles > h->ws_cap) {
+    // hipFree waits for the device, so no earlier launch still reads the old workspace
+    if (h->d_ws) (void)hipFree(h->d_ws);
+    h->d_ws = nullptr;
+    h->ws_cap = 0;
+    e = hipMalloc((void**)&h->d_ws, (size_t)p.ws_doubles * 8);
+    if (e != hipSuccess) {
+      h->d_ws = nullptr;
+      return fail(ODEVEC_ENOMEM, std::string("rate workspace allocation failed: ") + hipGetErrorString(e));
+    }
+    h->ws_cap = p.ws_doubles;
+  }
+  a.ws = h->d_ws;
+  e = h->net->launch(a, h->opts.mode, h->opts.precision, p, r.stream);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return ODEVEC_OK;
 }
@@ -493,6 +511,7 @@ int32_t odevec_destroy(odevec_t* h) {
   free_bufs(h);
   if (h->d_scal) (void)hipFree(h->d_scal);
   if (h->d_queue) (void)hipFree(h->d_queue);
+  if (h->d_ws) (void)hipFree(h->d_ws);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
   if (h->stream) (void)hipStreamDestroy(h->stream);

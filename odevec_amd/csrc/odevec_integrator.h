@@ -9,10 +9,13 @@
 //
 // All loops over species and Nordsieck columns are fully unrolled against
 // compile-time sizes, and the runtime order q only predicates them, so the
-// hot state (Nordsieck history, correction, weights, LU factors) stays in
-// registers.  The rate coefficients and the previous correction, read once
-// per RHS / once per step-size selection, can live outside the lane (LDS
-// columns with a stride, `Cfg::KS > 0`).  The current iterate y is not kept:
+// state the corrector iterates on (y_NS(:,:,0..1), correction, LU factors)
+// stays in registers.  With `Cfg::KS > 0` the Nordsieck columns 2..MAXORD
+// (touched by the predictor and the update, once per attempt), the error
+// weights (read once per corrector norm), the previous correction and the
+// rate coefficients live outside the lane, in columns with a stride (the
+// kernels put the first three in LDS and the rates in a device workspace).
+// The current iterate y is not kept:
 // in the reference it always equals y_NS(:,:,0) + beta*en for the cells it
 // integrates (src/odevec.F90:293-299, 345-352); only a lockstep group keeps the
 // caller's y for cells outside the mask (`Cfg::KEEPY`).
@@ -61,8 +64,9 @@ struct Counters {
 //  Net   -- generated network struct
 //  M     -- element type of P and its LU factors (double, or float for the mixed form)
 //  RCP   -- reciprocal pivots stored in the LU diagonal (mixed form)
-//  KS    -- 0: rates and previous correction in the lane; > 0: in external
-//           columns, element r of this lane at ptr[r*KS]
+//  KS    -- 0: the whole state in the lane; > 0: the rates, the Nordsieck
+//           columns 2..MAXORD, the error weights and the previous correction
+//           in external columns, element r of this lane at ptr[r*KS]
 //  KEEPY -- keep the caller's y for lanes outside the mask (lockstep)
 template <class Net_, class M_, bool RCP_, int KS_, bool KEEPY_>
 struct Cfg {
@@ -118,15 +122,18 @@ struct Lane {
   static constexpr int KS = C::KS;
   static constexpr bool RCP_ = C::RCP;
   static constexpr bool KEEPY_ = C::KEEPY;
-  double z[NCOL][N];     // Nordsieck history y_NS(:,:,0..MAXORD)
+  static constexpr int ZR = KS == 0 ? NCOL : 2;   // Nordsieck columns held in the lane
+  double zl[ZR][N];      // Nordsieck history y_NS(:,:,0..ZR-1)
   double en[N];          // accumulated correction
-  double w[N];           // inverse squared error weights
+  double wl[KS == 0 ? N : 1];            // inverse squared error weights (KS == 0)
   M a[Net::NNZ];         // P = I - dt*beta*J, then its LU factors
   double kl[KS == 0 ? Net::NRATE : 1];   // rate coefficients (KS == 0)
   double eol[KS == 0 ? N : 1];           // previous accepted correction (KS == 0)
   double yin[C::KEEPY ? N : 1];          // caller's y (KEEPY)
   double* kx;            // external rate column (KS > 0)
   double* eox;           // external previous-correction column (KS > 0)
+  double* zx;            // external Nordsieck columns 2..MAXORD: (c, j) at zx[((c-2)*N + j)*KS]
+  double* wx;            // external error-weight column (KS > 0)
   double lq[7];          // l-vector of the current order (constant indices only)
   double lqq;            // l_q(q)
   double t, dt, tstop, rate, jchange, cerr;
@@ -144,6 +151,20 @@ struct Lane {
   ODV_HD void set_eo(int j, double v) {
     if (KS == 0) eol[j] = v;
     else eox[j * KS] = v;
+  }
+  // Nordsieck element (column c, species j); c is a compile-time constant at
+  // every call site (unrolled loops), so the branch folds away
+  ODV_HD double z(int c, int j) const {
+    return c < ZR ? zl[c < ZR ? c : 0][j] : zx[((c >= 2 ? c - 2 : 0) * N + j) * (KS > 0 ? KS : 1)];
+  }
+  ODV_HD void set_z(int c, int j, double v) {
+    if (c < ZR) zl[c < ZR ? c : 0][j] = v;
+    else zx[((c >= 2 ? c - 2 : 0) * N + j) * (KS > 0 ? KS : 1)] = v;
+  }
+  ODV_HD double w(int j) const { return KS == 0 ? wl[j] : wx[j * KS]; }
+  ODV_HD void set_w(int j, double v) {
+    if (KS == 0) wl[j] = v;
+    else wx[j * KS] = v;
   }
 };
 
@@ -163,7 +184,7 @@ template <class L>
 ODV_HD inline void odv_cur_y(const L& s, double beta, double* y) {
   constexpr int N = L::N;
 #pragma unroll
-  for (int j = 0; j < N; ++j) y[j] = s.z[0][j] + beta * s.en[j];
+  for (int j = 0; j < N; ++j) y[j] = s.z(0, j) + beta * s.en[j];
   if constexpr (L::KEEPY_) {
     if (!s.active) {
 #pragma unroll
@@ -177,7 +198,7 @@ template <class L>
 ODV_HD inline void odv_pred_y(const L& s, double* y) {
   constexpr int N = L::N;
 #pragma unroll
-  for (int j = 0; j < N; ++j) y[j] = s.z[0][j];
+  for (int j = 0; j < N; ++j) y[j] = s.z(0, j);
   if constexpr (L::KEEPY_) {
     if (!s.active) {
 #pragma unroll
@@ -187,12 +208,12 @@ ODV_HD inline void odv_pred_y(const L& s, double* y) {
 }
 
 // 1/(rtol*|y| + atol)^2 (src/odevec.F90:706-716)
-template <int N>
-ODV_HD inline void odv_weights(const double* y, double rtol, double atol, double* w) {
+template <class L>
+ODV_HD inline void odv_weights(L& s, const double* y, double rtol, double atol) {
 #pragma unroll
-  for (int j = 0; j < N; ++j) {
+  for (int j = 0; j < L::N; ++j) {
     const double v = 1.0 / (rtol * fabs(y[j]) + atol);
-    w[j] = v * v;
+    s.set_w(j, v * v);
   }
 }
 
@@ -202,11 +223,11 @@ template <class L, class X>
 ODV_HD inline double odv_wnorm_f(const L& s, X x) {
   constexpr int N = L::N;
   double v = x(0);
-  double acc = (v * v) * s.w[0];
+  double acc = (v * v) * s.w(0);
 #pragma unroll
   for (int j = 1; j < N; ++j) {
     v = x(j);
-    acc = acc + (v * v) * s.w[j];
+    acc = acc + (v * v) * s.w(j);
   }
   if (s.fz & 1) acc = NAN;
   return sqrt(acc / (double)L::NF);
@@ -225,6 +246,14 @@ ODV_HD inline bool odv_has_nan(const double* x) {
   return b;
 }
 
+template <class L>
+ODV_HD inline bool odv_w_has_nan(const L& s) {
+  bool b = false;
+#pragma unroll
+  for (int j = 0; j < L::N; ++j) b = b || (s.w(j) != s.w(j));
+  return b;
+}
+
 // PredictSolution: Pascal triangle with additions only (src/odevec.F90:769-793)
 template <class L>
 ODV_HD inline void odv_predict(L& s) {
@@ -235,7 +264,7 @@ ODV_HD inline void odv_predict(L& s) {
     for (int kk = MAXORD; kk > l; --kk) {
       if (l < s.q && kk <= s.q) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) s.z[kk - 1][j] = s.z[kk][j] + s.z[kk - 1][j];
+        for (int j = 0; j < N; ++j) s.set_z(kk - 1, j, s.z(kk, j) + s.z(kk - 1, j));
       }
     }
   }
@@ -251,7 +280,7 @@ ODV_HD inline void odv_reset(L& s) {
     for (int jj = MAXORD; jj > kk; --jj) {
       if (kk < s.q && jj <= s.q) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) s.z[jj - 1][j] = s.z[jj - 1][j] - s.z[jj][j];
+        for (int j = 0; j < N; ++j) s.set_z(jj - 1, j, s.z(jj - 1, j) - s.z(jj, j));
       }
     }
   }
@@ -267,7 +296,7 @@ ODV_HD inline void odv_set_step(L& s, double scale) {
     if (kk <= s.q) {
       d = scale * d;
 #pragma unroll
-      for (int j = 0; j < N; ++j) s.z[kk][j] = s.z[kk][j] * d;
+      for (int j = 0; j < N; ++j) s.set_z(kk, j, s.z(kk, j) * d);
     }
   }
   s.dt = s.dt * scale;
@@ -288,7 +317,7 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
     const double nq = g.max_active(odv_wnorm_f(s, [&](int j) {
                                      double v = 0.0;
 #pragma unroll
-                                     for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.z[kk][j] : v;
+                                     for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.z(kk, j) : v;
                                      return v;
                                    }),
                                    s.active);
@@ -319,7 +348,7 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
       for (int kk = 2; kk <= MAXORD; ++kk) {
         if (kk == q + 1) {
 #pragma unroll
-          for (int j = 0; j < N; ++j) s.z[kk][j] = (lqq * s.en[j]) / (q + 1.0);
+          for (int j = 0; j < N; ++j) s.set_z(kk, j, (lqq * s.en[j]) / (q + 1.0));
         }
       }
       odv_set_order(s, q + 1);
@@ -437,20 +466,20 @@ ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, doubl
 #pragma unroll
     for (int j = 0; j < N; ++j) s.yin[j] = y[j];
   }
-  odv_weights<N>(y, c.rtol, c.atol, s.w);
+  odv_weights(s, y, c.rtol, c.atol);
 #pragma unroll
   for (int kk = 0; kk < NCOL; ++kk)
 #pragma unroll
-    for (int j = 0; j < N; ++j) s.z[kk][j] = (kk == 0) ? y[j] : 0.0;
+    for (int j = 0; j < N; ++j) s.set_z(kk, j, (kk == 0) ? y[j] : 0.0);
   double f[N];
   odv_rhs(s, y, f);
 #pragma unroll
-  for (int j = 0; j < N; ++j) s.z[1][j] = dt_hint * f[j];
+  for (int j = 0; j < N; ++j) s.set_z(1, j, dt_hint * f[j]);
   // CalcInitialStepSize (src/odevec.F90:216-238)
   const double tol = c.rtol;
-  double acc = (f[0] * f[0]) * ((s.w[0] * tol) * tol);
+  double acc = (f[0] * f[0]) * ((s.w(0) * tol) * tol);
 #pragma unroll
-  for (int j = 1; j < N; ++j) acc = acc + (f[j] * f[j]) * ((s.w[j] * tol) * tol);
+  for (int j = 1; j < N; ++j) acc = acc + (f[j] * f[j]) * ((s.w(j) * tol) * tol);
   if (s.fz & 1) acc = NAN;
   acc = g.max_active(acc, s.active);
   const double W0 = fmax(fabs(t0), fabs(tstop));
@@ -467,10 +496,10 @@ ODV_HD inline void odv_interpolate(const L& s, double* y) {
   for (int j = 0; j < N; ++j) {
     double v = 0.0;
 #pragma unroll
-    for (int kk = 1; kk <= MAXORD; ++kk) v = (kk == s.q) ? s.z[kk][j] : v;
+    for (int kk = 1; kk <= MAXORD; ++kk) v = (kk == s.q) ? s.z(kk, j) : v;
 #pragma unroll
     for (int kk = MAXORD - 1; kk >= 0; --kk)
-      if (kk < s.q) v = s.z[kk][j] + r * v;
+      if (kk < s.q) v = s.z(kk, j) + r * v;
     y[j] = v;
   }
 }
@@ -484,7 +513,7 @@ ODV_HD inline void odv_result(const L& s, double* y) {
     odv_interpolate(s, y);
   } else {
 #pragma unroll
-    for (int j = 0; j < N; ++j) y[j] = s.z[0][j];
+    for (int j = 0; j < N; ++j) y[j] = s.z(0, j);
   }
 }
 
@@ -536,7 +565,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   if (c.check_negatives) {
     bool neg = (s.fz & 4) != 0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) neg = neg || (s.z[0][j] < 0.0);
+    for (int j = 0; j < N; ++j) neg = neg || (s.z(0, j) < 0.0);
     if (g.any_active(neg && s.active)) {
       s.t = told;
       odv_reset(s);
@@ -576,7 +605,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
     }
     first_it = false;
 #pragma unroll
-    for (int j = 0; j < N; ++j) f[j] = s.dt * f[j] - (s.z[1][j] + s.en[j]);
+    for (int j = 0; j < N; ++j) f[j] = s.dt * f[j] - (s.z(1, j) + s.en[j]);
     odv_lin_solve(s, f);
 #pragma unroll
     for (int j = 0; j < N; ++j) s.en[j] = s.en[j] + f[j];
@@ -589,7 +618,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       s.rate = fmax(0.2 * s.rate, rtmp);
     }
     const double crit = e * fmin(1.0, 1.5 * s.rate) / (tauq / (2.0 * (s.q + 2.0)));
-    if (g.any_valid(odv_has_nan<N>(f) || odv_has_nan<N>(s.w) || (s.fz & 2))) {
+    if (g.any_valid(odv_has_nan<N>(f) || odv_w_has_nan(s) || (s.fz & 2))) {
       failed = true;
     } else if (crit < 1.0) {
       converged = true;
@@ -639,7 +668,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   {
     double y[N];
     odv_cur_y(s, beta, y);
-    odv_weights<N>(y, c.rtol, c.atol, s.w);
+    odv_weights(s, y, c.rtol, c.atol);
   }
   const bool rejected = err > 1.0;
   if (rejected) {
@@ -651,7 +680,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       odv_pred_y(s, y);   // y = y_NS(:,:,0), still the predicted value
       odv_rhs(s, y, f);
 #pragma unroll
-      for (int j = 0; j < N; ++j) s.z[1][j] = s.dt * f[j];
+      for (int j = 0; j < N; ++j) s.set_z(1, j, s.dt * f[j]);
       s.upd_jac = true;
     } else {
       s.t = told;
@@ -669,7 +698,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       if (kk <= s.q) {
         const double lk = s.lq[kk];
 #pragma unroll
-        for (int j = 0; j < N; ++j) s.z[kk][j] = s.z[kk][j] + s.en[j] * lk;
+        for (int j = 0; j < N; ++j) s.set_z(kk, j, s.z(kk, j) + s.en[j] * lk);
       }
     }
   }

@@ -13,10 +13,15 @@
 //
 // Data layout in HBM (struct of arrays, cells fastest, as the reference's
 // Fortran y(nvector,neq)): y[j*ncells + i], params[p*ncells + i].
-// For networks with neq >= 8 the per-lane rate coefficients and previous
-// correction live in LDS as columns of the workgroup (element r of lane t at
-// [r*BLOCK + t]: consecutive lanes hit consecutive banks), which keeps the
-// register file for the Nordsieck history and the LU factors.
+// For networks with neq >= 8 (the per-cell kernels) the lane keeps only what
+// the corrector iterates on in registers: y_NS(:,:,0..1), the correction and
+// the LU factors. The Nordsieck columns 2..MAXORD, the error weights and the
+// previous correction live in LDS as columns of the workgroup (element r of
+// lane t at [r*BLOCK + t]: consecutive lanes hit consecutive banks), and the
+// rate coefficients, written once per cell and read by every RHS and
+// Jacobian, in a device workspace with the same layout per workgroup slot
+// (ws[(slot*NRATE + r)*BLOCK + t]; 2 KiB contiguous per rate and slot, so
+// one slot's rates stay in the L2 of the XCD that runs it).
 #pragma once
 #include "odevec_integrator.h"
 
@@ -38,19 +43,38 @@ struct KArgs {
   int* counters;          // optional [8][ncells]
   double* flops;          // optional [2]: FP64, FP32 algorithmic flops (atomic)
   unsigned int* queue;    // persistent: next cell (set to the number of launched lanes)
+  double* ws;             // rate workspace, LdsPolicy<Net>::WS_PER_SLOT doubles per workgroup slot
   Controls ctl;
   int group;              // lockstep group size
 };
 
 constexpr int BLOCK = 256;   // lanes per workgroup of the per-cell kernels
 
+constexpr int LDS_BYTES = 160 * 1024;   // LDS per CU (and per workgroup) on gfx950
+
 template <class Net>
 struct LdsPolicy {
-  static constexpr bool USE = Net::NEQ >= 8 && (Net::NRATE + Net::NEQ) * 8 * BLOCK <= 144 * 1024;
+  // Nordsieck columns 2..MAXORD (4N), weights (N), previous correction (N)
+  static constexpr int COLS = (MAXORD - 1) * Net::NACT + 2 * Net::NACT;
+  static constexpr bool USE = Net::NEQ >= 8 && COLS * 8 * BLOCK <= LDS_BYTES;
   static constexpr int KS = USE ? BLOCK : 0;
-  static constexpr int KWORDS = USE ? Net::NRATE * BLOCK : 1;
-  static constexpr int EWORDS = USE ? Net::NEQ * BLOCK : 1;
+  static constexpr int ZWORDS = USE ? (MAXORD - 1) * Net::NACT * BLOCK : 1;
+  static constexpr int WWORDS = USE ? Net::NACT * BLOCK : 1;
+  static constexpr int EWORDS = USE ? Net::NACT * BLOCK : 1;
+  static constexpr long long WS_PER_SLOT = USE ? (long long)Net::NRATE * BLOCK : 0;
 };
+
+// Point a lane's external columns at LDS and at its workgroup slot of the
+// rate workspace.
+template <class P, class L>
+__device__ inline void odv_bind_columns(L& s, const KArgs& a, long long slot, double* lz, double* lw, double* le) {
+  if constexpr (P::USE) {
+    s.kx = a.ws + slot * P::WS_PER_SLOT + threadIdx.x;
+    s.zx = lz + threadIdx.x;
+    s.wx = lw + threadIdx.x;
+    s.eox = le + threadIdx.x;
+  }
+}
 
 __device__ inline double odv_wave_sum(double v) {
 #pragma unroll
@@ -125,14 +149,14 @@ template <class Net, class M, bool RCP>
 __global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
   using P = LdsPolicy<Net>;
   using L = Lane<Cfg<Net, M, RCP, P::KS, false>>;
-  __shared__ double lds_k[P::KWORDS];
+  __shared__ double lds_z[P::ZWORDS];
+  __shared__ double lds_w[P::WWORDS];
   __shared__ double lds_e[P::EWORDS];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double fl64 = 0.0, fl32 = 0.0;
   if (i < a.ncells) {
     L s;
-    s.kx = lds_k + threadIdx.x;
-    s.eox = lds_e + threadIdx.x;
+    odv_bind_columns<P>(s, a, blockIdx.x, lds_z, lds_w, lds_e);
     LaneGroup g;
     bool done = odv_load_begin(a, i, s);
     while (!done) done = odv_attempt(s, a.ctl, g);
@@ -160,14 +184,14 @@ template <class Net, class M, bool RCP>
 __global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
   using P = LdsPolicy<Net>;
   using L = Lane<Cfg<Net, M, RCP, P::KS, false>>;
-  __shared__ double lds_k[P::KWORDS];
+  __shared__ double lds_z[P::ZWORDS];
+  __shared__ double lds_w[P::WWORDS];
   __shared__ double lds_e[P::EWORDS];
   const long long nlanes = (long long)gridDim.x * blockDim.x;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   L s;
-  s.kx = lds_k + threadIdx.x;
-  s.eox = lds_e + threadIdx.x;
+  odv_bind_columns<P>(s, a, blockIdx.x, lds_z, lds_w, lds_e);
   LaneGroup g;
   double fl64 = 0.0, fl32 = 0.0;
   bool have = i < a.ncells;

@@ -19,12 +19,21 @@ struct KernelInfo {
   int block;
 };
 
+// Launch shape of one solve: threads per workgroup, workgroups, and the rate
+// workspace it needs (doubles).
+struct Plan {
+  int block;
+  long long grid;
+  long long ws_doubles;
+};
+
 struct NetEntry {
   const char* id;
   int repro;   // built with -ffp-contract=off (bit-identical to the oracle)
   int neq, nnz, nrate, nparam, rate_kind, tgas;
   int flops_rhs, flops_jac, flops_lu, flops_solve, flops_lu_r, flops_solve_r;
-  hipError_t (*launch)(const KArgs& a, int mode, int prec, int block, int grid, hipStream_t st);
+  hipError_t (*plan)(long long ncells, int mode, int prec, int block, int grid, int group, Plan* out);
+  hipError_t (*launch)(const KArgs& a, int mode, int prec, const Plan& p, hipStream_t st);
   hipError_t (*info)(int mode, int prec, int block, KernelInfo* out);
   void (*param_defaults)(double* p);
 };
@@ -55,16 +64,19 @@ inline hipError_t kinfo(K kern, int block, KernelInfo* out) {
   return hipSuccess;
 }
 
+// block: threads per workgroup of the per-cell kernels (<= BLOCK, the LDS
+// column stride); grid: workgroups of the persistent kernel (0: one wave of
+// resident workgroups, CUs x occupancy); group: lockstep group size.
 template <class Net, class M, bool RCP>
-inline hipError_t launch_t(const KArgs& a, int mode, int block, int grid, hipStream_t st) {
-  if (a.ncells <= 0) return hipSuccess;
+inline hipError_t plan_t(long long ncells, int mode, int block, int grid, int group, Plan* p) {
+  using P = LdsPolicy<Net>;
   if (mode == MODE_PERCELL) {
-    const int b = block > 0 ? block : 256;
-    const long long g = (a.ncells + b - 1) / b;
-    hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)g), dim3(b), 0, st, a);
+    const int b = block > 0 ? block : BLOCK;
+    p->block = b;
+    p->grid = (ncells + b - 1) / b;
   } else if (mode == MODE_PERSISTENT) {
-    const int b = block > 0 ? block : 256;
-    int g = grid;
+    const int b = block > 0 ? block : BLOCK;
+    long long g = grid;
     if (g <= 0) {
       int dev = 0, ncu = 0, nb = 0;
       hipError_t e = hipGetDevice(&dev);
@@ -73,25 +85,48 @@ inline hipError_t launch_t(const KArgs& a, int mode, int block, int grid, hipStr
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &nb, reinterpret_cast<const void*>(odv_k_persistent<Net, M, RCP>), b, 0);
       if (e != hipSuccess) return e;
-      g = ncu * (nb > 0 ? nb : 1);
+      g = (long long)ncu * (nb > 0 ? nb : 1);
     }
-    long long need = (a.ncells + b - 1) / b;
-    if (g > need) g = (int)need;
-    hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned)((long long)g * b));
-    hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3(g), dim3(b), 0, st, a);
+    const long long need = (ncells + b - 1) / b;
+    if (g > need) g = need;
+    p->block = b;
+    p->grid = g;
   } else {
-    const int grp = a.group;
-    const int b = ((grp + 63) / 64) * 64;
-    const long long g = (a.ncells + grp - 1) / grp;
-    hipLaunchKernelGGL((odv_k_lockstep<Net, M, RCP>), dim3((unsigned)g), dim3(b), 0, st, a);
+    const int grp = group > 0 ? group : 1;
+    p->block = ((grp + 63) / 64) * 64;
+    p->grid = (ncells + grp - 1) / grp;
+  }
+  p->ws_doubles = (mode == MODE_LOCKSTEP) ? 0 : p->grid * P::WS_PER_SLOT;
+  return hipSuccess;
+}
+
+template <class Net, class M, bool RCP>
+inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t st) {
+  if (a.ncells <= 0) return hipSuccess;
+  if (p.block <= 0 || p.grid <= 0 || p.grid > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (mode != MODE_LOCKSTEP && p.block > BLOCK) return hipErrorInvalidValue;
+  if (p.ws_doubles > 0 && !a.ws) return hipErrorInvalidValue;
+  if (mode == MODE_PERCELL) {
+    hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
+  } else if (mode == MODE_PERSISTENT) {
+    hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned)(p.grid * p.block));
+    hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((odv_k_lockstep<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
   }
   return hipGetLastError();
 }
 
 template <class Net>
-hipError_t launch_net(const KArgs& a, int mode, int prec, int block, int grid, hipStream_t st) {
-  if (prec == PREC_MIXED) return launch_t<Net, float, true>(a, mode, block, grid, st);
-  return launch_t<Net, double, false>(a, mode, block, grid, st);
+hipError_t plan_net(long long ncells, int mode, int prec, int block, int grid, int group, Plan* p) {
+  if (prec == PREC_MIXED) return plan_t<Net, float, true>(ncells, mode, block, grid, group, p);
+  return plan_t<Net, double, false>(ncells, mode, block, grid, group, p);
+}
+
+template <class Net>
+hipError_t launch_net(const KArgs& a, int mode, int prec, const Plan& p, hipStream_t st) {
+  if (prec == PREC_MIXED) return launch_t<Net, float, true>(a, mode, p, st);
+  return launch_t<Net, double, false>(a, mode, p, st);
 }
 
 template <class Net, class M, bool RCP>
@@ -114,5 +149,5 @@ hipError_t info_net(int mode, int prec, int block, KernelInfo* out) {
   static const odv::NetEntry odv_entry_##NET = {  /* NOLINT */                                                \
       NET::ID, ODV_REPRO_FLAG, NET::NEQ, NET::NNZ, NET::NRATE, NET::NPARAM, NET::RATE_KIND, NET::TGAS,          \
       NET::FLOPS_RHS, NET::FLOPS_JAC, NET::FLOPS_LU, NET::FLOPS_SOLVE, NET::FLOPS_LU_R,         \
-      NET::FLOPS_SOLVE_R, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults};                          \
+      NET::FLOPS_SOLVE_R, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults};                          \
   static odv::Registrar odv_reg_##NET(&odv_entry_##NET);

@@ -15,20 +15,28 @@
 
 namespace {
 
-// KS > 0 runs the lane with its rates and previous correction in external
-// strided columns, as the device kernels keep them in LDS.
+// KS > 0 runs the lane with its rates, Nordsieck columns 2..MAXORD, error
+// weights and previous correction in external strided columns, as the device
+// kernels keep them in LDS and in the rate workspace.
 template <class Net, class M, bool RCP, int KS>
 void run(long n, const double* t0, const double* ts, const double* dt0, double* y, const double* prm,
          const odv::Controls& c, int* status, double* dtout, int* cnt) {
   constexpr int N = Net::NEQ;
+  constexpr int NA = Net::NACT;
   constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
+  constexpr int S = KS > 0 ? KS : 1;
   using L = odv::Lane<odv::Cfg<Net, M, RCP, KS, false>>;
-  static double kcol[Net::NRATE * (KS > 0 ? KS : 1)];
-  static double ecol[Net::NEQ * (KS > 0 ? KS : 1)];
+  static double kcol[Net::NRATE * S];
+  static double ecol[NA * S];
+  static double zcol[(odv::MAXORD - 1) * NA * S];
+  static double wcol[NA * S];
   for (long i = 0; i < n; ++i) {
     L s;
-    s.kx = kcol + (KS > 0 ? (int)(i % KS) : 0);
-    s.eox = ecol + (KS > 0 ? (int)(i % KS) : 0);
+    const int off = KS > 0 ? (int)(i % KS) : 0;
+    s.kx = kcol + off;
+    s.eox = ecol + off;
+    s.zx = zcol + off;
+    s.wx = wcol + off;
     odv::LaneGroup g;
     double yy[N], p[NP];
     for (int j = 0; j < N; ++j) yy[j] = y[j * n + i];
