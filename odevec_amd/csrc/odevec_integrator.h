@@ -10,11 +10,12 @@
 // All loops over species and Nordsieck columns are fully unrolled against
 // compile-time sizes, and the runtime order q only predicates them, so the
 // state the corrector iterates on (y_NS(:,:,0..1), correction, LU factors)
-// stays in registers.  With `Cfg::KS > 0` the Nordsieck columns 2..MAXORD
-// (touched by the predictor and the update, once per attempt), the error
-// weights (read once per corrector norm), the previous correction and the
-// rate coefficients live outside the lane, in columns with a stride (the
-// kernels put the first three in LDS and the rates in a device workspace).
+// stays in registers.  With `Cfg::KS > 0` the previous correction and the
+// rate coefficients live outside the lane, in columns with a stride, with
+// `Cfg::WX` the error weights (read once per corrector norm) and with
+// `Cfg::ZX` the Nordsieck columns 2..MAXORD (touched by the predictor and the
+// update, once per attempt) too; the kernels choose LDS or a device
+// workspace per column.
 // The current iterate y is not kept:
 // in the reference it always equals y_NS(:,:,0) + beta*en for the cells it
 // integrates (src/odevec.F90:293-299, 345-352); only a lockstep group keeps the
@@ -64,17 +65,20 @@ struct Counters {
 //  Net   -- generated network struct
 //  M     -- element type of P and its LU factors (double, or float for the mixed form)
 //  RCP   -- reciprocal pivots stored in the LU diagonal (mixed form)
-//  KS    -- 0: the whole state in the lane; > 0: the rates, the Nordsieck
-//           columns 2..MAXORD, the error weights and the previous correction
-//           in external columns, element r of this lane at ptr[r*KS]
+//  KS    -- 0: the whole state in the lane; > 0: the rates and the previous
+//           correction in external columns, element r of this lane at ptr[r*KS]
 //  KEEPY -- keep the caller's y for lanes outside the mask (lockstep)
-template <class Net_, class M_, bool RCP_, int KS_, bool KEEPY_>
+//  ZX    -- (KS > 0) the Nordsieck columns 2..MAXORD in external columns too
+//  WX    -- (KS > 0) the error weights in an external column too
+template <class Net_, class M_, bool RCP_, int KS_, bool KEEPY_, bool ZX_ = false, bool WX_ = false>
 struct Cfg {
   using Net = Net_;
   using M = M_;
   static constexpr bool RCP = RCP_;
   static constexpr int KS = KS_;
   static constexpr bool KEEPY = KEEPY_;
+  static constexpr bool ZX = KS_ > 0 && ZX_;
+  static constexpr bool WX = KS_ > 0 && WX_;
 };
 
 // l-vector of the order-q BDF method in Nordsieck form (src/odevec.F90:127-134);
@@ -122,18 +126,18 @@ struct Lane {
   static constexpr int KS = C::KS;
   static constexpr bool RCP_ = C::RCP;
   static constexpr bool KEEPY_ = C::KEEPY;
-  static constexpr int ZR = KS == 0 ? NCOL : 2;   // Nordsieck columns held in the lane
+  static constexpr int ZR = C::ZX ? 2 : NCOL;   // Nordsieck columns held in the lane
   double zl[ZR][N];      // Nordsieck history y_NS(:,:,0..ZR-1)
   double en[N];          // accumulated correction
-  double wl[KS == 0 ? N : 1];            // inverse squared error weights (KS == 0)
+  double wl[C::WX ? 1 : N];              // inverse squared error weights (!WX)
   M a[Net::NNZ];         // P = I - dt*beta*J, then its LU factors
   double kl[KS == 0 ? Net::NRATE : 1];   // rate coefficients (KS == 0)
   double eol[KS == 0 ? N : 1];           // previous accepted correction (KS == 0)
   double yin[C::KEEPY ? N : 1];          // caller's y (KEEPY)
   double* kx;            // external rate column (KS > 0)
   double* eox;           // external previous-correction column (KS > 0)
-  double* zx;            // external Nordsieck columns 2..MAXORD: (c, j) at zx[((c-2)*N + j)*KS]
-  double* wx;            // external error-weight column (KS > 0)
+  double* zx;            // external Nordsieck columns 2..MAXORD (ZX): (c, j) at zx[((c-2)*N + j)*KS]
+  double* wx;            // external error-weight column (WX)
   double lq[7];          // l-vector of the current order (constant indices only)
   double lqq;            // l_q(q)
   double t, dt, tstop, rate, jchange, cerr;
@@ -161,10 +165,10 @@ struct Lane {
     if (c < ZR) zl[c < ZR ? c : 0][j] = v;
     else zx[((c >= 2 ? c - 2 : 0) * N + j) * (KS > 0 ? KS : 1)] = v;
   }
-  ODV_HD double w(int j) const { return KS == 0 ? wl[j] : wx[j * KS]; }
+  ODV_HD double w(int j) const { return C::WX ? wx[j * (KS > 0 ? KS : 1)] : wl[j]; }
   ODV_HD void set_w(int j, double v) {
-    if (KS == 0) wl[j] = v;
-    else wx[j * KS] = v;
+    if (C::WX) wx[j * (KS > 0 ? KS : 1)] = v;
+    else wl[j] = v;
   }
 };
 

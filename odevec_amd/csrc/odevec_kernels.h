@@ -13,15 +13,20 @@
 //
 // Data layout in HBM (struct of arrays, cells fastest, as the reference's
 // Fortran y(nvector,neq)): y[j*ncells + i], params[p*ncells + i].
-// For networks with neq >= 8 (the per-cell kernels) the lane keeps only what
-// the corrector iterates on in registers: y_NS(:,:,0..1), the correction and
-// the LU factors. The Nordsieck columns 2..MAXORD, the error weights and the
-// previous correction live in LDS as columns of the workgroup (element r of
-// lane t at [r*BLOCK + t]: consecutive lanes hit consecutive banks), and the
-// rate coefficients, written once per cell and read by every RHS and
-// Jacobian, in a device workspace with the same layout per workgroup slot
-// (ws[(slot*NRATE + r)*BLOCK + t]; 2 KiB contiguous per rate and slot, so
-// one slot's rates stay in the L2 of the XCD that runs it).
+// For networks with neq >= 8 (the per-cell kernels) part of a lane's state
+// lives outside its registers, as columns of the workgroup (element r of
+// lane t at [r*BLOCK + t]: consecutive lanes hit consecutive banks or bytes),
+// placed by LdsPolicy per precision. The register file (512 VGPR+AGPR per
+// lane at one wavefront per SIMD) and LDS (640 B per lane at 256 lanes per
+// CU) hold ~2.6 KiB per lane together:
+//  * mixed form (LU factors FP32, 4 B each): the Nordsieck history and the
+//    error weights stay in registers; LDS holds the rate coefficients (read
+//    by every RHS and Jacobian) and the previous correction;
+//  * FP64 form (LU factors 8 B each): LDS holds the Nordsieck columns
+//    2..MAXORD, the error weights and the previous correction; the rate
+//    coefficients go to a device workspace with the same layout per
+//    workgroup slot (ws[(slot*NRATE + r)*BLOCK + t]; 2 KiB contiguous per
+//    rate and slot, so a slot's rates stay in the L2 of the XCD running it).
 #pragma once
 #include "odevec_integrator.h"
 
@@ -43,7 +48,7 @@ struct KArgs {
   int* counters;          // optional [8][ncells]
   double* flops;          // optional [2]: FP64, FP32 algorithmic flops (atomic)
   unsigned int* queue;    // persistent: next cell (set to the number of launched lanes)
-  double* ws;             // rate workspace, LdsPolicy<Net>::WS_PER_SLOT doubles per workgroup slot
+  double* ws;             // rate workspace, LdsPolicy::WS_PER_SLOT doubles per workgroup slot
   Controls ctl;
   int group;              // lockstep group size
 };
@@ -52,29 +57,41 @@ constexpr int BLOCK = 256;   // lanes per workgroup of the per-cell kernels
 
 constexpr int LDS_BYTES = 160 * 1024;   // LDS per CU (and per workgroup) on gfx950
 
-template <class Net>
+template <class Net, class M>
 struct LdsPolicy {
-  // Nordsieck columns 2..MAXORD (4N), weights (N), previous correction (N)
-  static constexpr int COLS = (MAXORD - 1) * Net::NACT + 2 * Net::NACT;
+  static constexpr int N = Net::NACT;
+  static constexpr bool FP64 = sizeof(M) == sizeof(double);
+  // mixed: rates, previous correction; FP64: columns 2..MAXORD, weights,
+  // previous correction (doubles per lane)
+  static constexpr int COLS = FP64 ? (MAXORD - 1) * N + 2 * N : Net::NRATE + N;
   static constexpr bool USE = Net::NEQ >= 8 && COLS * 8 * BLOCK <= LDS_BYTES;
   static constexpr int KS = USE ? BLOCK : 0;
-  static constexpr int ZWORDS = USE ? (MAXORD - 1) * Net::NACT * BLOCK : 1;
-  static constexpr int WWORDS = USE ? Net::NACT * BLOCK : 1;
-  static constexpr int EWORDS = USE ? Net::NACT * BLOCK : 1;
-  static constexpr long long WS_PER_SLOT = USE ? (long long)Net::NRATE * BLOCK : 0;
+  static constexpr bool ZX = USE && FP64;          // Nordsieck columns 2..MAXORD in LDS
+  static constexpr bool KLDS = USE && !FP64;       // rates in LDS (else in the workspace)
+  static constexpr bool WX = USE && FP64;          // error weights in LDS
+  static constexpr int KWORDS = KLDS ? Net::NRATE * BLOCK : 1;
+  static constexpr int ZWORDS = ZX ? (MAXORD - 1) * N * BLOCK : 1;
+  static constexpr int WWORDS = WX ? N * BLOCK : 1;
+  static constexpr int EWORDS = USE ? N * BLOCK : 1;
+  static constexpr long long WS_PER_SLOT = (USE && !KLDS) ? (long long)Net::NRATE * BLOCK : 0;
 };
 
 // Point a lane's external columns at LDS and at its workgroup slot of the
 // rate workspace.
 template <class P, class L>
-__device__ inline void odv_bind_columns(L& s, const KArgs& a, long long slot, double* lz, double* lw, double* le) {
+__device__ inline void odv_bind_columns(L& s, const KArgs& a, long long slot, double* lk, double* lz, double* lw,
+                                        double* le) {
   if constexpr (P::USE) {
-    s.kx = a.ws + slot * P::WS_PER_SLOT + threadIdx.x;
+    s.kx = P::KLDS ? lk + threadIdx.x : a.ws + slot * P::WS_PER_SLOT + threadIdx.x;
     s.zx = lz + threadIdx.x;
     s.wx = lw + threadIdx.x;
     s.eox = le + threadIdx.x;
   }
 }
+
+template <class Net, class M, bool RCP>
+using PerCellLane =
+    Lane<Cfg<Net, M, RCP, LdsPolicy<Net, M>::KS, false, LdsPolicy<Net, M>::ZX, LdsPolicy<Net, M>::WX>>;
 
 __device__ inline double odv_wave_sum(double v) {
 #pragma unroll
@@ -147,8 +164,9 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
 
 template <class Net, class M, bool RCP>
 __global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
-  using P = LdsPolicy<Net>;
-  using L = Lane<Cfg<Net, M, RCP, P::KS, false>>;
+  using P = LdsPolicy<Net, M>;
+  using L = PerCellLane<Net, M, RCP>;
+  __shared__ double lds_k[P::KWORDS];
   __shared__ double lds_z[P::ZWORDS];
   __shared__ double lds_w[P::WWORDS];
   __shared__ double lds_e[P::EWORDS];
@@ -156,7 +174,7 @@ __global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
   double fl64 = 0.0, fl32 = 0.0;
   if (i < a.ncells) {
     L s;
-    odv_bind_columns<P>(s, a, blockIdx.x, lds_z, lds_w, lds_e);
+    odv_bind_columns<P>(s, a, blockIdx.x, lds_k, lds_z, lds_w, lds_e);
     LaneGroup g;
     bool done = odv_load_begin(a, i, s);
     while (!done) done = odv_attempt(s, a.ctl, g);
@@ -182,8 +200,9 @@ __global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
 // no cell, leaves.
 template <class Net, class M, bool RCP>
 __global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
-  using P = LdsPolicy<Net>;
-  using L = Lane<Cfg<Net, M, RCP, P::KS, false>>;
+  using P = LdsPolicy<Net, M>;
+  using L = PerCellLane<Net, M, RCP>;
+  __shared__ double lds_k[P::KWORDS];
   __shared__ double lds_z[P::ZWORDS];
   __shared__ double lds_w[P::WWORDS];
   __shared__ double lds_e[P::EWORDS];
@@ -191,7 +210,7 @@ __global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   L s;
-  odv_bind_columns<P>(s, a, blockIdx.x, lds_z, lds_w, lds_e);
+  odv_bind_columns<P>(s, a, blockIdx.x, lds_k, lds_z, lds_w, lds_e);
   LaneGroup g;
   double fl64 = 0.0, fl32 = 0.0;
   bool have = i < a.ncells;

@@ -69,7 +69,7 @@ inline hipError_t kinfo(K kern, int block, KernelInfo* out) {
 // resident workgroups, CUs x occupancy); group: lockstep group size.
 template <class Net, class M, bool RCP>
 inline hipError_t plan_t(long long ncells, int mode, int block, int grid, int group, Plan* p) {
-  using P = LdsPolicy<Net>;
+  using P = LdsPolicy<Net, M>;
   if (mode == MODE_PERCELL) {
     const int b = block > 0 ? block : BLOCK;
     p->block = b;
