@@ -4,6 +4,9 @@
    ``NETWORK/ORDERING/PACKAGING`` ids) into ``csrc/generated``.
 2. Compile each instantiation unit and the C ABI with hipcc (one process per
    unit, in parallel) and link ``odevec_amd/libodevec.so``.
+3. Compile the Fortran module (``fortran/odevec_main.F90``,
+   ``fortran/odevec_commons.F90``) and the ROBER driver program against it
+   (``fortran/_build/rober_driver``), if a Fortran compiler is present.
 
     python -m odevec_amd.build [--extra PRIMORDIAL/INVERT/CSC] [-j 8]
 """
@@ -97,7 +100,52 @@ def build(variants: List[str] = None, jobs: int = 8, verbose: bool = True, repro
     os.replace(tmp, LIB)
     if verbose:
         print("built %s (%d network variants%s)" % (LIB, len(bases), ", each also +REPRO" if repro else ""))
+    if "ROBER/NONE/DENSE" in variants and repro:
+        build_fortran(verbose=verbose)
     return LIB
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN, "_build")
+FORTRAN_DRIVER = os.path.join(FORTRAN_BUILD, "rober_driver")
+
+
+def flang() -> str:
+    for c in (os.environ.get("FLANG"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if c and os.path.exists(c):
+            return c
+    return ""
+
+
+def build_fortran(network: str = "ROBER/NONE/DENSE+REPRO", nvector: int = 32, neq: int = 3,
+                  verbose: bool = True) -> str:
+    """The Fortran module (odevec_main, odevec_commons) and the ROBER driver
+    program linked against libodevec.so (the driver uses the reproducible
+    build, so its output can be checked bit for bit); skipped if no Fortran
+    compiler."""
+    fc = flang()
+    if not fc:
+        if verbose:
+            print("no Fortran compiler: Fortran module not built")
+        return ""
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    defs = ["-DODEVEC_NETWORK=\"%s\"" % network, "-DODEVEC_NVECTOR=%d" % nvector, "-DODEVEC_NEQ=%d" % neq,
+            "-DODEVEC_PRECISION=0", "-DODEVEC_MODE=1"]
+    objs = []
+    for src in ("odevec_main.F90", "odevec_commons.F90", "rober_driver.f90"):
+        obj = os.path.join(FORTRAN_BUILD, src.split(".")[0] + ".o")
+        cmd = [fc, "-cpp", "-O2", "-J", FORTRAN_BUILD] + defs + ["-c", os.path.join(FORTRAN, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("Fortran compile failed: %s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+        objs.append(obj)
+    cmd = [fc, "-o", FORTRAN_DRIVER] + objs + ["-L", HERE, "-lodevec", "-Wl,-rpath,$ORIGIN/../.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran link failed: %s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    if verbose:
+        print("built %s" % FORTRAN_DRIVER)
+    return FORTRAN_DRIVER
 
 
 def main(argv=None) -> int:

@@ -3,6 +3,7 @@ declares, and reports errors through return codes (no compute calls here)."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -59,3 +60,21 @@ def test_create_without_device_fails_cleanly():
     rc = _lib.lib().odevec_create(b"ROBER/NONE/DENSE", 4, None, C.byref(h))
     assert rc == _lib.EDEVICE and not h.value
     assert "device" in _lib.last_error()
+
+
+def test_fortran_module_links_and_fails_cleanly_without_gpu():
+    """The Fortran module (bind(C) interfaces to this ABI) and a driver in the
+    reference's calling pattern compile and link; without a GPU,
+    InitOdevec reports the library's error and stops, as the reference's
+    failures do."""
+    from conftest import gpu_available
+    from odevec_amd import build as B
+    if not B.flang():
+        pytest.skip("no Fortran compiler")
+    if not os.path.exists(B.FORTRAN_DRIVER):
+        B.build_fortran(verbose=False)
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    r = subprocess.run([B.FORTRAN_DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "InitOdevec failed" in r.stdout and "no HIP device" in r.stdout

@@ -133,6 +133,36 @@ def test_reference_rober_driver_pattern(oracle):
     ode.CloseOdevec()
 
 
+def test_fortran_module_driver_equals_oracle(oracle):
+    """The Fortran module (odevec_amd/fortran) driving ROBER in the reference
+    driver's pattern (tests/rober.f90:39-62; nvector 32, lockstep over the
+    vector like the reference, reproducible build): every printed state
+    equals the oracle's at nvector = 32 bit for bit."""
+    import subprocess
+    from odevec_amd import build as B
+    if not os.path.exists(B.FORTRAN_DRIVER):
+        pytest.skip("Fortran driver not built (no Fortran compiler)")
+    say("GPU Fortran driver", B.FORTRAN_DRIVER)
+    r = subprocess.run([B.FORTRAN_DRIVER], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = np.array([[float(v) for v in line.split()] for line in r.stdout.splitlines() if line.strip()])
+    net = oracle.Network.load("ROBER/NONE/DENSE")
+    yref = np.zeros((3, 32))
+    yref[0] = 1.0
+    prm = np.array([0.04, 3e7, 1e4])[:, None] * np.ones((1, 32))
+    t_start, t_stop, k = 0.0, 1e-10, 0
+    while t_stop < 1e11:
+        yref, s, tr, dr, cnt = net.solve_vector(yref, t_start, t_stop, 1e-10, 1e-6, 1e-10, 1e-20, params=prm)
+        t_start = tr
+        blk = rows[k * 32:(k + 1) * 32]
+        assert np.array_equal(blk[:, 0], np.full(32, t_stop))
+        assert np.array_equal(blk[:, 1], np.arange(1, 33))
+        assert np.array_equal(blk[:, 2:5], yref.T), t_stop
+        assert (blk[:, 5] == 0).all()
+        t_stop, k = t_stop * 10.0, k + 1
+    assert k * 32 == len(rows)
+
+
 @pytest.mark.parametrize("n", [1, 63, 257, 1000])
 @pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
 def test_ragged_sizes_and_skips(oracle, n, mode):
