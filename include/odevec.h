@@ -146,6 +146,52 @@ int32_t odevec_solve_cells_async(odevec_t* h, int64_t ncells, const double* d_t0
 /* Replaces CloseOdevec(this) (src/odevec.F90:1058-1074). */
 int32_t odevec_destroy(odevec_t* h);
 
+/* Batches ------------------------------------------------------------------
+ * No reference counterpart: the reference solves one network per build and
+ * call. A bundle is one persistent kernel compiled for several (network,
+ * precision) entries; a batch launch runs a table of segments -- blocks of
+ * cells, each of one entry, with its own arrays, tolerances and queue -- in
+ * ONE launch (work compaction across the whole batch). Every cell keeps its
+ * own step control, so a cell's result does not depend on the other
+ * segments (bitwise, for a given build). Segments are processed in table
+ * order; put the most expensive per cell first. */
+typedef struct odevec_batch_s odevec_batch_t;
+
+typedef struct {
+  const char* network;      /* variant id, one of the bundle's entries, e.g. "PRIMORDIAL/CMK/CSC" */
+  int32_t precision;        /* ODEVEC_FP64 | ODEVEC_MIXED (must be compiled in the bundle) */
+  int32_t check_negatives;
+  int64_t ncells;
+  double* y;                /* device: [neq][ncells], in/out */
+  const double* t0;         /* device: [ncells], NULL = 0 */
+  const double* t_stop;     /* device: [ncells] */
+  const double* dt0;        /* device: [ncells], NULL = 1e-10 */
+  const double* params;     /* device: [nparam][ncells], NULL = the network's defaults */
+  int32_t* status;          /* device: [ncells] or NULL */
+  double* dt_out;           /* device: [ncells] or NULL */
+  int32_t* counters;        /* device: [8][ncells] or NULL */
+  double* flops;            /* device: [2] accumulated (FP64, FP32) or NULL */
+  double rtol, atol, dt_min;
+  int32_t max_attempts;     /* 0 = 1000000 */
+  int32_t reserved;
+} odevec_segment_t;
+
+int32_t odevec_bundle_count(void);
+const char* odevec_bundle_name(int32_t index);   /* e.g. "MIX3", "MIX3+REPRO" */
+/* Entry `index` of a bundle: its network id and precision; returns the
+ * number of entries (or a negative code). */
+int32_t odevec_bundle_entry(const char* bundle, int32_t index, const char** network, int32_t* precision);
+/* block: threads per workgroup (0 = 256); grid: workgroups (0 = fill the
+ * device); use_graph: capture the launch (queue reset + batch kernel) in a
+ * hipGraph once per launch shape and stream, and replay it. */
+int32_t odevec_batch_create(const char* bundle, int32_t block, int32_t grid, int32_t use_graph, odevec_batch_t** out);
+/* Enqueue one launch over nseg (<= 1024) segments on `stream` (a
+ * hipStream_t). Use one stream per batch handle: consecutive calls reuse its
+ * segment table, queue counters and workspace in stream order. */
+int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs, int32_t nseg, void* stream);
+int32_t odevec_batch_kernel_info(const char* bundle, int32_t block, odevec_kernel_info_t* info);
+int32_t odevec_batch_destroy(odevec_batch_t* b);
+
 #ifdef __cplusplus
 }
 #endif

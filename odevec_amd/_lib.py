@@ -47,6 +47,13 @@ class KernelInfo(C.Structure):
     _fields_ = [(n, _I32) for n in ("vgprs", "spill_bytes", "blocks_per_cu", "block")]
 
 
+class Segment(C.Structure):
+    _fields_ = [("network", C.c_char_p), ("precision", _I32), ("check_negatives", _I32), ("ncells", _I64),
+                ("y", _VP), ("t0", _VP), ("t_stop", _VP), ("dt0", _VP), ("params", _VP), ("status", _VP),
+                ("dt_out", _VP), ("counters", _VP), ("flops", _VP), ("rtol", C.c_double), ("atol", C.c_double),
+                ("dt_min", C.c_double), ("max_attempts", _I32), ("reserved", _I32)]
+
+
 class OdevecError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("odevec error %d: %s" % (code, msg))
@@ -87,6 +94,13 @@ def lib():
         "odevec_solve_cells_async": (_I32, [_VP, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _VP, _VP, _I64,
                                             _VP, _VP, _VP, _VP, _VP]),
         "odevec_destroy": (_I32, [_VP]),
+        "odevec_bundle_count": (_I32, []),
+        "odevec_bundle_name": (C.c_char_p, [_I32]),
+        "odevec_bundle_entry": (_I32, [C.c_char_p, _I32, C.POINTER(C.c_char_p), C.POINTER(_I32)]),
+        "odevec_batch_create": (_I32, [C.c_char_p, _I32, _I32, _I32, C.POINTER(_VP)]),
+        "odevec_batch_solve_async": (_I32, [_VP, C.POINTER(Segment), _I32, _VP]),
+        "odevec_batch_kernel_info": (_I32, [C.c_char_p, _I32, C.POINTER(KernelInfo)]),
+        "odevec_batch_destroy": (_I32, [_VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -100,7 +114,9 @@ EXPORTED = [
     "odevec_abi_version", "odevec_last_error", "odevec_network_count", "odevec_network_name",
     "odevec_network_info", "odevec_kernel_info", "odevec_default_opts", "odevec_create",
     "odevec_set_tolerances", "odevec_set_dt_min", "odevec_set_check_negatives", "odevec_solve",
-    "odevec_solve_cells", "odevec_solve_cells_async", "odevec_destroy",
+    "odevec_solve_cells", "odevec_solve_cells_async", "odevec_destroy", "odevec_bundle_count",
+    "odevec_bundle_name", "odevec_bundle_entry", "odevec_batch_create", "odevec_batch_solve_async",
+    "odevec_batch_kernel_info", "odevec_batch_destroy",
 ]
 
 
@@ -129,3 +145,20 @@ def kernel_info(name: str, mode: int = PERCELL, precision: int = FP64, block: in
     info = KernelInfo()
     check(lib().odevec_kernel_info(name.encode(), mode, precision, block, C.byref(info)))
     return {f: getattr(info, f) for f, _ in KernelInfo._fields_}
+
+
+def bundles():
+    L = lib()
+    return [L.odevec_bundle_name(i).decode() for i in range(L.odevec_bundle_count())]
+
+
+def bundle_entries(name: str):
+    L = lib()
+    net, prec = C.c_char_p(), _I32()
+    n = L.odevec_bundle_entry(name.encode(), 0, C.byref(net), C.byref(prec))
+    check(0 if n > 0 else n)
+    out = []
+    for k in range(n):
+        check(0 if L.odevec_bundle_entry(name.encode(), k, C.byref(net), C.byref(prec)) > 0 else -1)
+        out.append((net.value.decode(), prec.value))
+    return out

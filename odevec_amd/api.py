@@ -189,3 +189,71 @@ class Odevec:
             self.CloseOdevec()
         except Exception:
             pass
+
+
+def _ptr(v):
+    """Device pointer of a torch tensor (or an int address, or None)."""
+    if v is None:
+        return None
+    if isinstance(v, int):
+        return v
+    return v.data_ptr()
+
+
+class Batch:
+    """One launch over many blocks of cells of different networks and
+    precisions (a bundle's batch kernel; no reference counterpart).
+
+    ``solve_async(segments, stream)`` takes a list of dicts with keys
+    ``network``, ``precision``, ``y`` (device, shape (neq, ncells)), ``t_stop``
+    and optionally ``t0``, ``dt0``, ``params``, ``status``, ``dt_out``,
+    ``counters``, ``flops``, ``rtol``, ``atol``, ``dt_min``,
+    ``check_negatives``, ``max_attempts``. Array values are torch tensors on
+    the GPU (or raw device addresses); ``ncells`` is taken from ``y``.
+    """
+
+    def __init__(self, bundle: str = "MIX3", block: int = 0, grid: int = 0, use_graph: bool = False):
+        self.bundle = bundle
+        self.entries = L.bundle_entries(bundle)
+        h = C.c_void_p()
+        L.check(L.lib().odevec_batch_create(bundle.encode(), int(block), int(grid), 1 if use_graph else 0,
+                                            C.byref(h)))
+        self._h = h
+        self._keep = None
+
+    def solve_async(self, segments, stream=None):
+        if self._h is None:
+            raise RuntimeError("batch is closed")
+        n = len(segments)
+        arr = (L.Segment * max(n, 1))()
+        names = []
+        for k, sg in enumerate(segments):
+            y = sg["y"]
+            ncells = int(sg.get("ncells", y.shape[-1] if hasattr(y, "shape") else 0))
+            nm = sg["network"].encode()
+            names.append(nm)
+            arr[k] = L.Segment(nm, int(sg.get("precision", L.FP64)), 1 if sg.get("check_negatives") else 0, ncells,
+                               _ptr(y), _ptr(sg.get("t0")), _ptr(sg["t_stop"]), _ptr(sg.get("dt0")),
+                               _ptr(sg.get("params")), _ptr(sg.get("status")), _ptr(sg.get("dt_out")),
+                               _ptr(sg.get("counters")), _ptr(sg.get("flops")), float(sg.get("rtol", 1e-6)),
+                               float(sg.get("atol", 1e-10)), float(sg.get("dt_min", float(np.float32(1e-10)))),
+                               int(sg.get("max_attempts", 0)), 0)
+        self._keep = (arr, names)
+        L.check(L.lib().odevec_batch_solve_async(self._h, arr, n, stream))
+
+    def close(self):
+        if self._h is not None:
+            L.lib().odevec_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

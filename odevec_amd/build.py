@@ -36,6 +36,12 @@ BUILTIN_VARIANTS = [
     "PRIMORDIAL/FF/CSC",
 ]
 
+# bundles: one batch kernel over several networks, each in both precisions
+# (odv_k_batch); every network listed must also be a variant above
+BUNDLES = {
+    "MIX3": ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"],
+}
+
 # the reproducible build: no contraction of a*b+c into FMA, so device results
 # equal the oracle's bit for bit
 REPRO_FLAGS = ["-ffp-contract=off", "-DODV_REPRO=1"]
@@ -71,6 +77,41 @@ def generate(variants: List[str]) -> List[str]:
     return bases
 
 
+def _struct(variant: str) -> str:
+    name, ordering, packaging = variant.split("/")
+    return "odv_net_%s_%s_%s" % (name, ordering.upper(), packaging)
+
+
+def generate_bundle(name: str, nets: List[str]) -> str:
+    """Instantiation unit of bundle `name`: its networks in FP64 and in the
+    mixed form, registered under the bundle's name."""
+    entries = [(n, p) for n in nets for p in (0, 1)]
+    lines = ["// Generated by odevec_amd.build -- do not edit.",
+             "// bundle %s: %s, each FP64 and mixed" % (name, ", ".join(nets)),
+             '#include "odevec_registry.h"']
+    for n in nets:
+        lines.append('#include "generated/net_%s.h"' % _struct(n)[len("odv_net_"):])
+    ent = ",\n    ".join("odv::Entry<%s, %s, %s>" % (_struct(n), "float" if p else "double", "true" if p else "false")
+                          for n, p in entries)
+    lines += ["namespace {",
+              "using Bundle_%s = odv::Bundle<\n    %s>;" % (name, ent),
+              "const char* const nets_%s[] = {%s};" % (name, ", ".join('"%s"' % n for n, _ in entries)),
+              "const int precs_%s[] = {%s};" % (name, ", ".join(str(p) for _, p in entries)),
+              "const int neq_%s[] = {%s};" % (name, ", ".join("%s::NEQ" % _struct(n) for n, _ in entries)),
+              "const int npar_%s[] = {%s};" % (name, ", ".join("%s::NPARAM" % _struct(n) for n, _ in entries)),
+              "void (*const defs_%s[])(double*) = {%s};" % (
+                  name, ", ".join("&%s::param_defaults" % _struct(n) for n, _ in entries)),
+              "}  // namespace",
+              "ODV_REGISTER_BUNDLE(%s, Bundle_%s, nets_%s, precs_%s, neq_%s, npar_%s, defs_%s)" % ((name,) * 7),
+              ""]
+    path = os.path.join(GEN, "inst_bundle_%s.hip" % name)
+    text = "\n".join(lines)
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as f:
+            f.write(text)
+    return path
+
+
 def _compile(src: str, obj: str, extra: List[str]) -> str:
     cmd = [hipcc()] + HIPCC_FLAGS + extra + ["-I", CSRC, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -79,7 +120,8 @@ def _compile(src: str, obj: str, extra: List[str]) -> str:
     return obj
 
 
-def build(variants: List[str] = None, jobs: int = 8, verbose: bool = True, repro: bool = True) -> str:
+def build(variants: List[str] = None, jobs: int = 8, verbose: bool = True, repro: bool = True,
+          bundles: bool = True) -> str:
     variants = list(BUILTIN_VARIANTS if variants is None else variants)
     bases = generate(variants)
     os.makedirs(BUILD, exist_ok=True)
@@ -89,6 +131,13 @@ def build(variants: List[str] = None, jobs: int = 8, verbose: bool = True, repro
         units.append((src, os.path.join(BUILD, "inst_%s.o" % b), []))
         if repro:
             units.append((src, os.path.join(BUILD, "inst_%s_repro.o" % b), REPRO_FLAGS))
+    for name, nets in (BUNDLES.items() if bundles else []):
+        if not all(n in variants for n in nets):
+            continue
+        src = generate_bundle(name, nets)
+        units.append((src, os.path.join(BUILD, "bundle_%s.o" % name), []))
+        if repro:
+            units.append((src, os.path.join(BUILD, "bundle_%s_repro.o" % name), REPRO_FLAGS))
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         futs = [ex.submit(_compile, s, o, f) for s, o, f in units]
         objs = [f.result() for f in futs]

@@ -17,7 +17,12 @@ std::vector<const NetEntry*>& registry() {
   return r;
 }
 
-__global__ void k_setq(unsigned int* q, unsigned int v) { *q = v; }
+std::vector<const BundleEntry*>& bundles() {
+  static std::vector<const BundleEntry*> r;
+  return r;
+}
+
+__global__ void k_setq(unsigned long long* q, unsigned long long v) { *q = v; }
 
 __global__ void k_fill_f64(double* p, double v, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,7 +82,7 @@ struct odevec_s {
   int* d_cnt = nullptr;
   int8_t* d_mask = nullptr;
   double* d_scal = nullptr;    // [0..2] uniform t0, t_stop, dt0; [3..] default params; [..] flops
-  unsigned int* d_queue = nullptr;   // 64 queue counters, one per in-flight persistent launch
+  unsigned long long* d_queue = nullptr;   // 64 queue counters, one per in-flight persistent launch
   double* d_ws = nullptr;            // rate workspace of the per-cell kernels
   int64_t ws_cap = 0;                // doubles
   int qslot = 0;
@@ -291,7 +296,7 @@ int32_t odevec_create(const char* network, int64_t nvector, const odevec_opts_t*
   const int np = e->nparam > 0 ? e->nparam : 1;
   h->nscal = 3 + np + 2;
   if ((he = hipMalloc((void**)&h->d_scal, h->nscal * 8)) != hipSuccess ||
-      (he = hipMalloc((void**)&h->d_queue, 64 * sizeof(unsigned int))) != hipSuccess) {
+      (he = hipMalloc((void**)&h->d_queue, 64 * sizeof(unsigned long long))) != hipSuccess) {
     odevec_destroy(h);
     return fail(ODEVEC_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(he));
   }
@@ -520,3 +525,248 @@ int32_t odevec_destroy(odevec_t* h) {
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------ batches */
+
+namespace {
+
+const odv::BundleEntry* find_bundle(const char* id) {
+  if (!id) return nullptr;
+  std::string s(id);
+  int repro = 0;
+  const std::string suf = "+REPRO";
+  if (s.size() > suf.size() && s.compare(s.size() - suf.size(), suf.size(), suf) == 0) {
+    repro = 1;
+    s.resize(s.size() - suf.size());
+  }
+  for (const odv::BundleEntry* e : odv::bundles())
+    if (e->repro == repro && s == e->id) return e;
+  return nullptr;
+}
+
+constexpr int MAXP = 8;   // parameter-default slots per entry
+
+}  // namespace
+
+struct odevec_batch_s {
+  const odv::BundleEntry* b = nullptr;
+  int block = 0, grid = 0, use_graph = 0;
+  odv::SegDev* h_segs = nullptr;   // pinned staging of the segment table
+  int* h_nseg = nullptr;
+  odv::SegDev* d_segs = nullptr;
+  int* d_nseg = nullptr;
+  unsigned long long* d_queues = nullptr;
+  double* d_scal = nullptr;        // [0] t0 default 0, [1] dt hint default 1e-10, [2..] per-entry param defaults
+  double* d_ws = nullptr;
+  int64_t ws_cap = 0;
+  hipEvent_t ev_copied = nullptr;  // the staging table has been copied (it may be rewritten)
+  bool ev_pending = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  odv::Plan gplan{};
+  hipStream_t gstream = nullptr;
+};
+
+namespace {
+
+void batch_drop_graph(odevec_batch_s* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+}  // namespace
+
+int32_t odevec_bundle_count(void) { return (int32_t)odv::bundles().size(); }
+
+const char* odevec_bundle_name(int32_t i) {
+  static std::vector<std::string> names;
+  if (i < 0 || i >= (int32_t)odv::bundles().size()) return nullptr;
+  if (names.size() != odv::bundles().size()) {
+    names.clear();
+    for (const odv::BundleEntry* e : odv::bundles()) names.push_back(std::string(e->id) + (e->repro ? "+REPRO" : ""));
+  }
+  return names[(size_t)i].c_str();
+}
+
+int32_t odevec_bundle_entry(const char* bundle, int32_t index, const char** network, int32_t* precision) {
+  const odv::BundleEntry* e = find_bundle(bundle);
+  if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown bundle ") + (bundle ? bundle : "(null)"));
+  if (index < 0 || index >= e->n) return fail(ODEVEC_EINVAL, "entry index out of range");
+  if (network) *network = e->nets[index];
+  if (precision) *precision = e->precs[index];
+  return e->n;
+}
+
+int32_t odevec_batch_create(const char* bundle, int32_t block, int32_t grid, int32_t use_graph, odevec_batch_t** out) {
+  if (!out) return fail(ODEVEC_EINVAL, "out is NULL");
+  *out = nullptr;
+  const odv::BundleEntry* e = find_bundle(bundle);
+  if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown bundle ") + (bundle ? bundle : "(null)"));
+  if (block != 0 && (block < 64 || block > 256 || block % 64)) return fail(ODEVEC_EINVAL, "block must be 64..256, a multiple of 64");
+  if (grid < 0) return fail(ODEVEC_EINVAL, "grid must be >= 0");
+  if (e->segdev_bytes != (int)sizeof(odv::SegDev)) return fail(ODEVEC_EINVAL, "bundle built with another segment layout");
+  int ndev = 0;
+  hipError_t he = hipGetDeviceCount(&ndev);
+  if (he != hipSuccess || ndev < 1) return fail(ODEVEC_EDEVICE, "no HIP device available");
+  odevec_batch_s* b = new odevec_batch_s();
+  b->b = e;
+  b->block = block;
+  b->grid = grid;
+  b->use_graph = use_graph ? 1 : 0;
+  const size_t tbytes = sizeof(odv::SegDev) * odv::MAXSEG;
+  const int nsc = 2 + e->n * MAXP;
+  if ((he = hipHostMalloc((void**)&b->h_segs, tbytes, hipHostMallocDefault)) != hipSuccess ||
+      (he = hipHostMalloc((void**)&b->h_nseg, sizeof(int), hipHostMallocDefault)) != hipSuccess ||
+      (he = hipMalloc((void**)&b->d_segs, tbytes)) != hipSuccess ||
+      (he = hipMalloc((void**)&b->d_nseg, sizeof(int))) != hipSuccess ||
+      (he = hipMalloc((void**)&b->d_queues, odv::MAXSEG * sizeof(unsigned long long))) != hipSuccess ||
+      (he = hipMalloc((void**)&b->d_scal, nsc * sizeof(double))) != hipSuccess ||
+      (he = hipEventCreateWithFlags(&b->ev_copied, hipEventDisableTiming)) != hipSuccess) {
+    odevec_batch_destroy(b);
+    return fail(ODEVEC_ENOMEM, std::string("batch allocation failed: ") + hipGetErrorString(he));
+  }
+  std::vector<double> sc(nsc, 0.0);
+  sc[1] = 1e-10;
+  for (int k = 0; k < e->n; ++k) {
+    if (e->nparam[k] > MAXP) {
+      odevec_batch_destroy(b);
+      return fail(ODEVEC_EINVAL, "network with more than 8 parameters in a bundle");
+    }
+    if (e->nparam[k] > 0) e->param_defaults[k](sc.data() + 2 + k * MAXP);
+  }
+  if ((he = hipMemcpy(b->d_scal, sc.data(), nsc * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
+    odevec_batch_destroy(b);
+    return hip_fail(he, "batch defaults upload");
+  }
+  *out = b;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs, int32_t nseg, void* stream) {
+  if (!b || (!segs && nseg > 0)) return fail(ODEVEC_EINVAL, "NULL argument");
+  if (nseg < 0 || nseg > odv::MAXSEG) return fail(ODEVEC_EINVAL, "nseg must be 0..1024");
+  const odv::BundleEntry* e = b->b;
+  hipStream_t st = (hipStream_t)stream;
+  // map and check every segment before touching the staging table
+  std::vector<int> kind(nseg);
+  int64_t total = 0;
+  for (int k = 0; k < nseg; ++k) {
+    const odevec_segment_t& g = segs[k];
+    int kk = -1;
+    for (int j = 0; j < e->n; ++j)
+      if (g.network && !std::strcmp(g.network, e->nets[j]) && g.precision == e->precs[j]) kk = j;
+    if (kk < 0)
+      return fail(ODEVEC_ENOTFOUND, std::string("segment network/precision not in bundle ") + e->id + ": " +
+                                        (g.network ? g.network : "(null)"));
+    if (g.ncells < 0) return fail(ODEVEC_EINVAL, "segment ncells < 0");
+    if (g.ncells > 0 && (!g.y || !g.t_stop)) return fail(ODEVEC_EINVAL, "segment y / t_stop is NULL");
+    if (!(g.rtol > 0.0) || !(g.atol >= 0.0)) return fail(ODEVEC_EINVAL, "segment tolerances");
+    kind[k] = kk;
+    total += g.ncells;
+  }
+  if (total == 0) return ODEVEC_OK;
+  // a graph is captured for one launch shape: in graph mode the grid does not
+  // depend on the batch's size (workgroups without cells leave at once), so
+  // one captured launch serves every batch
+  odv::Plan p{};
+  hipError_t he = e->plan(b->use_graph ? (int64_t)1 << 40 : total, b->block, b->grid, &p);
+  if (he != hipSuccess) return hip_fail(he, "batch plan");
+  if (p.ws_doubles > b->ws_cap) {
+    batch_drop_graph(b);   // the captured launch holds the old workspace
+    if (b->d_ws) (void)hipFree(b->d_ws);
+    b->d_ws = nullptr;
+    b->ws_cap = 0;
+    if ((he = hipMalloc((void**)&b->d_ws, (size_t)p.ws_doubles * 8)) != hipSuccess)
+      return fail(ODEVEC_ENOMEM, std::string("rate workspace allocation failed: ") + hipGetErrorString(he));
+    b->ws_cap = p.ws_doubles;
+  }
+  // the previous call's table copy must have left the staging buffer
+  if (b->ev_pending && (he = hipEventSynchronize(b->ev_copied)) != hipSuccess) return hip_fail(he, "batch staging");
+  for (int k = 0; k < nseg; ++k) {
+    const odevec_segment_t& g = segs[k];
+    odv::SegDev& d = b->h_segs[k];
+    std::memset(&d, 0, sizeof d);
+    odv::KArgs& a = d.a;
+    a.y = g.y;
+    a.ncells = g.ncells;
+    if (g.params && e->nparam[kind[k]] > 0) { a.prm = g.params; a.ps_cell = 1; a.ps_plane = g.ncells; }
+    else { a.prm = b->d_scal + 2 + kind[k] * MAXP; a.ps_cell = 0; a.ps_plane = 1; }
+    if (g.t0) { a.t0 = g.t0; a.t0_s = 1; } else { a.t0 = b->d_scal; a.t0_s = 0; }
+    a.tstop = g.t_stop; a.ts_s = 1;
+    if (g.dt0) { a.dt0 = g.dt0; a.dt_s = 1; } else { a.dt0 = b->d_scal + 1; a.dt_s = 0; }
+    a.mask = nullptr;
+    a.status = g.status;
+    a.dt_out = g.dt_out;
+    a.counters = g.counters;
+    a.flops = g.flops;
+    a.queue = b->d_queues + k;
+    a.ws = b->d_ws;
+    a.ctl.rtol = g.rtol;
+    a.ctl.atol = g.atol;
+    a.ctl.dt_min = g.dt_min;
+    a.ctl.check_negatives = g.check_negatives;
+    a.ctl.max_attempts = g.max_attempts > 0 ? g.max_attempts : 1000000;
+    a.group = 1;
+    d.kind = kind[k];
+  }
+  *b->h_nseg = nseg;
+  if ((he = hipMemcpyAsync(b->d_segs, b->h_segs, sizeof(odv::SegDev) * nseg, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (he = hipMemcpyAsync(b->d_nseg, b->h_nseg, sizeof(int), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (he = hipEventRecord(b->ev_copied, st)) != hipSuccess)
+    return hip_fail(he, "batch table upload");
+  b->ev_pending = true;
+  if (!b->use_graph) {
+    he = e->launch(b->d_segs, b->d_nseg, odv::MAXSEG, p, st);
+    return he == hipSuccess ? ODEVEC_OK : hip_fail(he, "batch launch");
+  }
+  // graph: the queue reset and the batch kernel, captured once per launch
+  // shape and stream and replayed; the table is read from device memory, so
+  // one graph serves every batch of that shape
+  if (b->exec && (b->gplan.grid != p.grid || b->gplan.block != p.block || b->gstream != st)) batch_drop_graph(b);
+  if (!b->exec) {
+    if ((he = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(he, "graph capture");
+    hipError_t le = e->launch(b->d_segs, b->d_nseg, odv::MAXSEG, p, st);
+    he = hipStreamEndCapture(st, &b->graph);
+    if (le != hipSuccess || he != hipSuccess) {
+      batch_drop_graph(b);
+      return hip_fail(le != hipSuccess ? le : he, "graph capture");
+    }
+    if ((he = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) {
+      batch_drop_graph(b);
+      return hip_fail(he, "graph instantiate");
+    }
+    b->gplan = p;
+    b->gstream = st;
+  }
+  if ((he = hipGraphLaunch(b->exec, st)) != hipSuccess) return hip_fail(he, "graph launch");
+  return ODEVEC_OK;
+}
+
+int32_t odevec_batch_kernel_info(const char* bundle, int32_t block, odevec_kernel_info_t* info) {
+  const odv::BundleEntry* e = find_bundle(bundle);
+  if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown bundle ") + (bundle ? bundle : "(null)"));
+  if (!info) return fail(ODEVEC_EINVAL, "info is NULL");
+  odv::KernelInfo k{};
+  hipError_t he = e->info(block, &k);
+  if (he != hipSuccess) return hip_fail(he, "kernel info");
+  info->vgprs = k.regs; info->spill_bytes = k.spill_bytes;
+  info->blocks_per_cu = k.blocks_per_cu; info->block = k.block;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_batch_destroy(odevec_batch_t* b) {
+  if (!b) return ODEVEC_OK;
+  if (b->ev_pending) (void)hipEventSynchronize(b->ev_copied);
+  (void)hipDeviceSynchronize();   // no launch of this batch may still read its buffers
+  batch_drop_graph(b);
+  if (b->h_segs) (void)hipHostFree(b->h_segs);
+  if (b->h_nseg) (void)hipHostFree(b->h_nseg);
+  void* ps[] = {b->d_segs, b->d_nseg, b->d_queues, b->d_scal, b->d_ws};
+  for (void* q : ps)
+    if (q) (void)hipFree(q);
+  if (b->ev_copied) (void)hipEventDestroy(b->ev_copied);
+  delete b;
+  return ODEVEC_OK;
+}

@@ -47,7 +47,7 @@ struct KArgs {
   double* dt_out;         // optional [ncells]: last step size
   int* counters;          // optional [8][ncells]
   double* flops;          // optional [2]: FP64, FP32 algorithmic flops (atomic)
-  unsigned int* queue;    // persistent: next cell (set to the number of launched lanes)
+  unsigned long long* queue;  // persistent: next cell (set to the number of launched lanes)
   double* ws;             // rate workspace, LdsPolicy::WS_PER_SLOT doubles per workgroup slot
   Controls ctl;
   int group;              // lockstep group size
@@ -74,6 +74,7 @@ struct LdsPolicy {
   static constexpr int WWORDS = WX ? N * BLOCK : 1;
   static constexpr int EWORDS = USE ? N * BLOCK : 1;
   static constexpr long long WS_PER_SLOT = (USE && !KLDS) ? (long long)Net::NRATE * BLOCK : 0;
+  static constexpr int WORDS = KWORDS + ZWORDS + WWORDS + EWORDS;   // one workgroup's LDS
 };
 
 // Point a lane's external columns at LDS and at its workgroup slot of the
@@ -197,20 +198,17 @@ __global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
 // single atomic and hands them out in lane order. Every lane's loop ends: each
 // iteration either advances a cell by one attempt (cells are bounded by
 // max_attempts) or, once the wavefront has seen the queue exhausted and holds
-// no cell, leaves.
+// no cell, leaves. `lds` is the workgroup's LdsPolicy<Net, M>::WORDS doubles.
 template <class Net, class M, bool RCP>
-__global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
+__device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
-  __shared__ double lds_k[P::KWORDS];
-  __shared__ double lds_z[P::ZWORDS];
-  __shared__ double lds_w[P::WWORDS];
-  __shared__ double lds_e[P::EWORDS];
   const long long nlanes = (long long)gridDim.x * blockDim.x;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   L s;
-  odv_bind_columns<P>(s, a, blockIdx.x, lds_k, lds_z, lds_w, lds_e);
+  odv_bind_columns<P>(s, a, blockIdx.x, lds, lds + P::KWORDS, lds + P::KWORDS + P::ZWORDS,
+                      lds + P::KWORDS + P::ZWORDS + P::WWORDS);
   LaneGroup g;
   double fl64 = 0.0, fl32 = 0.0;
   bool have = i < a.ncells;
@@ -234,8 +232,8 @@ __global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
       const int cnt = __popcll(ball);
       const int rank = __popcll(ball & ((1ull << lane) - 1ull));
       const int leader = __ffsll((long long)ball) - 1;
-      unsigned int base = 0;
-      if (lane == leader) base = atomicAdd(a.queue, (unsigned int)cnt);
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
       base = __shfl(base, leader, 64);
       if ((long long)base + cnt >= a.ncells) exhausted = true;
       if (need) {
@@ -258,6 +256,77 @@ __global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
       atomicAdd(&a.flops[1], fl32);
     }
   }
+}
+
+template <class Net, class M, bool RCP>
+__global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
+  __shared__ double lds[LdsPolicy<Net, M>::WORDS];
+  odv_persistent_run<Net, M, RCP>(a, lds);
+}
+
+// ---------------------------------------------------------------- batches
+// One launch over a table of segments, each a block of cells of one network
+// in one precision (its own arrays, tolerances and queue counter). Every
+// wavefront works through the segments in table order: for each it runs the
+// persistent loop on that segment's queue, then moves to the next. The
+// segment kind selects one of the bundle's compiled (network, precision)
+// entries; the kind is uniform over the workgroup, so the wavefront executes
+// one network's code at a time. Segments ordered by decreasing cost per cell
+// (the host sorts them) leave cheap cells for the end of the launch.
+template <class Net_, class M_, bool RCP_>
+struct Entry {
+  using Net = Net_;
+  using M = M_;
+  static constexpr bool RCP = RCP_;
+  static constexpr int WORDS = LdsPolicy<Net_, M_>::WORDS;
+  static constexpr long long WS_PER_SLOT = LdsPolicy<Net_, M_>::WS_PER_SLOT;
+};
+
+struct SegDev {
+  KArgs a;      // the segment's cells (queue: its own counter)
+  int kind;     // index into the bundle's entries
+  int pad;
+};
+
+template <class... E>
+struct MaxWords;
+template <class E>
+struct MaxWords<E> {
+  static constexpr int WORDS = E::WORDS;
+  static constexpr long long WS = E::WS_PER_SLOT;
+};
+template <class E, class F, class... R>
+struct MaxWords<E, F, R...> {
+  static constexpr int WORDS = E::WORDS > MaxWords<F, R...>::WORDS ? E::WORDS : MaxWords<F, R...>::WORDS;
+  static constexpr long long WS = E::WS_PER_SLOT > MaxWords<F, R...>::WS ? E::WS_PER_SLOT : MaxWords<F, R...>::WS;
+};
+
+template <int K, class E, class... R>
+__device__ inline void odv_batch_dispatch(int kind, const KArgs& a, double* lds) {
+  if (kind == K) {
+    odv_persistent_run<typename E::Net, typename E::M, E::RCP>(a, lds);
+  } else {
+    if constexpr (sizeof...(R) > 0) odv_batch_dispatch<K + 1, R...>(kind, a, lds);
+  }
+}
+
+template <class... E>
+__global__ void __launch_bounds__(BLOCK) odv_k_batch(const SegDev* segs, const int* nseg) {
+  __shared__ double lds[MaxWords<E...>::WORDS];
+  const int n = *nseg;
+  for (int k = 0; k < n; ++k) {
+    const KArgs a = segs[k].a;
+    odv_batch_dispatch<0, E...>(segs[k].kind, a, lds);
+    __syncthreads();   // the next segment's kind may carve LDS differently
+  }
+}
+
+// Queue counters of a batch: each segment's starts at the launched lane count
+// (a kernel, not a memset, so that the launch can be captured in a graph).
+template <int UNUSED = 0>
+__global__ void k_batch_reset(SegDev* segs, const int* nseg, unsigned long long nlanes) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < *nseg) *segs[k].a.queue = nlanes;
 }
 
 // Lockstep group reductions over one workgroup (group <= blockDim.x <= 256).

@@ -40,11 +40,37 @@ struct NetEntry {
 
 std::vector<const NetEntry*>& registry();
 
+// A bundle: one batch kernel compiled for several (network, precision)
+// entries, launched over a table of segments (odv_k_batch).
+constexpr int MAXSEG = 1024;   // segments per batch launch
+
+struct BundleEntry {
+  const char* id;              // e.g. "MIX3"
+  int repro;
+  int n;                       // entries
+  const char* const* nets;     // network id of each entry
+  const int* precs;            // precision of each entry (Prec)
+  const int* neq;              // species of each entry's network
+  const int* nparam;
+  void (*const* param_defaults)(double*);
+  long long ws_per_slot;       // rate workspace doubles per workgroup slot (max over entries)
+  int segdev_bytes;            // sizeof(SegDev) of the build
+  hipError_t (*plan)(long long ncells, int block, int grid, Plan* out);
+  hipError_t (*launch)(void* segs, const int* nseg, int maxseg, const Plan& p, hipStream_t st);
+  hipError_t (*info)(int block, KernelInfo* out);
+};
+
+std::vector<const BundleEntry*>& bundles();
+
+struct BundleRegistrar {
+  explicit BundleRegistrar(const BundleEntry* e) { bundles().push_back(e); }
+};
+
 struct Registrar {
   explicit Registrar(const NetEntry* e) { registry().push_back(e); }
 };
 
-__global__ void k_setq(unsigned int* q, unsigned int v);
+__global__ void k_setq(unsigned long long* q, unsigned long long v);
 
 inline namespace ODV_VARIANT_NS {
 
@@ -109,7 +135,7 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
   if (mode == MODE_PERCELL) {
     hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
   } else if (mode == MODE_PERSISTENT) {
-    hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned)(p.grid * p.block));
+    hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * p.block));
     hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
   } else {
     hipLaunchKernelGGL((odv_k_lockstep<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
@@ -142,8 +168,57 @@ hipError_t info_net(int mode, int prec, int block, KernelInfo* out) {
   return info_t<Net, double, false>(mode, block, out);
 }
 
+// Plan, launch and kernel info of one bundle's batch kernel.
+template <class... E>
+struct Bundle {
+  static constexpr long long WS = MaxWords<E...>::WS;
+
+  static hipError_t plan(long long ncells, int block, int grid, Plan* p) {
+    const int b = block > 0 ? block : BLOCK;
+    if (b > BLOCK || b % 64 != 0) return hipErrorInvalidValue;
+    long long g = grid;
+    if (g <= 0) {
+      int dev = 0, ncu = 0, nb = 0;
+      hipError_t e = hipGetDevice(&dev);
+      if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(odv_k_batch<E...>), b, 0);
+      if (e != hipSuccess) return e;
+      g = (long long)ncu * (nb > 0 ? nb : 1);
+    }
+    const long long need = (ncells + b - 1) / b;
+    if (g > need) g = need > 0 ? need : 1;
+    p->block = b;
+    p->grid = g;
+    p->ws_doubles = g * WS;
+    return hipSuccess;
+  }
+
+  static hipError_t launch(void* segs, const int* nseg, int maxseg, const Plan& p, hipStream_t st) {
+    if (p.block <= 0 || p.block > BLOCK || p.grid <= 0 || p.grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    SegDev* sd = static_cast<SegDev*>(segs);
+    hipLaunchKernelGGL(k_batch_reset<0>, dim3((unsigned)((maxseg + 255) / 256)), dim3(256), 0, st, sd, nseg,
+                       (unsigned long long)(p.grid * p.block));
+    hipLaunchKernelGGL((odv_k_batch<E...>), dim3((unsigned)p.grid), dim3(p.block), 0, st,
+                       static_cast<const SegDev*>(sd), nseg);
+    return hipGetLastError();
+  }
+
+  static hipError_t info(int block, KernelInfo* out) { return kinfo(odv_k_batch<E...>, block > 0 ? block : BLOCK, out); }
+};
+
 }  // namespace ODV_VARIANT_NS
 }  // namespace odv
+
+// Register a bundle of (network, M, RCP) entries; NETS/PRECS/NEQ/NPARAM/DEFAULTS
+// are arrays of the entries' network ids, precisions, species counts,
+// parameter counts and parameter-default functions, in entry order.
+#define ODV_REGISTER_BUNDLE(NAME, BUNDLE, NETS, PRECS, NEQ, NPARAM, DEFAULTS)                               \
+  static const odv::BundleEntry odv_bundle_##NAME = {#NAME, ODV_REPRO_FLAG, (int)(sizeof(PRECS) / sizeof(int)), \
+                                                     NETS, PRECS, NEQ, NPARAM, DEFAULTS, BUNDLE::WS,            \
+                                                     (int)sizeof(odv::SegDev), &BUNDLE::plan, &BUNDLE::launch,   \
+                                                     &BUNDLE::info};                                             \
+  static odv::BundleRegistrar odv_bundle_reg_##NAME(&odv_bundle_##NAME);
 
 #define ODV_REGISTER(NET)                                                                       \
   static const odv::NetEntry odv_entry_##NET = {  /* NOLINT */                                                \

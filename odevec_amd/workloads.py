@@ -20,6 +20,10 @@ Each generator returns a dict with the network id, the state ``y`` (shape
   MinimalTimestep 1e-20 (tests/primordial.f90:44-45).
 * SYNTH64 (serving): abundances logU[1e-6, 1] of n, n ~ logU[1, 1e6],
   T ~ logU[100, 1e4] K.
+* Batching mix (seed 4): categorical 40/30/30 ROBER/OREGO/PRIMORDIAL cells,
+  t_stop log-uniform over 6 decades per network (ROBER [1e-1, 1e5], OREGO
+  [3.6e-4, 360], PRIMORDIAL [1e8, 1e14]), each network's own inputs and
+  tolerances as above.
 """
 from __future__ import annotations
 
@@ -103,3 +107,25 @@ def subset(w: dict, idx) -> dict:
     if w.get("params") is not None:
         out["params"] = np.ascontiguousarray(w["params"][:, idx])
     return out
+
+
+BATCH_SPANS = {"ROBER/NONE/DENSE": (1e-1, 1e5), "OREGO/NONE/DENSE": (3.6e-4, 360.0),
+               "PRIMORDIAL/CMK/CSC": (1e8, 1e14)}
+
+
+def batch_mix(n: int, seed: int = 4, shares=(0.4, 0.3, 0.3)):
+    """The batching workload: one segment per network (dicts as above), cell
+    counts drawn categorically with `shares`, t_stop log-uniform over each
+    network's 6-decade span."""
+    rng = np.random.default_rng(seed)
+    nets = list(BATCH_SPANS)
+    counts = np.bincount(rng.choice(len(nets), size=n, p=np.asarray(shares) / np.sum(shares)), minlength=len(nets))
+    gens = {"ROBER/NONE/DENSE": lambda m, sd: rober(m, seed=sd), "OREGO/NONE/DENSE": lambda m, sd: orego(m, seed=sd),
+            "PRIMORDIAL/CMK/CSC": lambda m, sd: primordial(m, seed=sd)}
+    segs = []
+    for k, (net, m) in enumerate(zip(nets, counts)):
+        w = gens[net](int(m), 100 * seed + k)
+        lo, hi = BATCH_SPANS[net]
+        w["t_stop"] = _logu(rng, lo, hi, int(m))
+        segs.append(w)
+    return segs

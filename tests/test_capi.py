@@ -78,3 +78,15 @@ def test_fortran_module_links_and_fails_cleanly_without_gpu():
     r = subprocess.run([B.FORTRAN_DRIVER], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "InitOdevec failed" in r.stdout and "no HIP device" in r.stdout
+
+
+def test_bundles_registered_and_batch_argument_checks():
+    assert "MIX3" in _lib.bundles() and "MIX3+REPRO" in _lib.bundles()
+    ent = _lib.bundle_entries("MIX3")
+    assert ent == [(n, p) for n in ("ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC") for p in (0, 1)]
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.odevec_batch_create(b"NOPE", 0, 0, 0, C.byref(h)) == _lib.ENOTFOUND
+    assert L.odevec_batch_create(b"MIX3", 100, 0, 0, C.byref(h)) == _lib.EINVAL
+    assert L.odevec_batch_solve_async(None, None, 0, None) == _lib.EINVAL
+    assert L.odevec_batch_destroy(None) == _lib.OK

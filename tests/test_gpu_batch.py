@@ -1,0 +1,147 @@
+"""Batching form on the GPU: one launch over segments of different networks,
+precisions and sizes (odevec_batch_*).
+
+* +REPRO bundle: every segment's cells equal the oracle bit for bit (states,
+  status, last step, counters), uneven and empty segments included.
+* default bundle: a segment's results are the same alone and inside the mix
+  (bitwise: per-cell step control, no data shared between cells).
+* graph mode: replays of the captured launch, with eager work and
+  synchronizations between them, each equal the eager launch.
+"""
+import numpy as np
+import pytest
+
+from odevec_amd import _lib
+from odevec_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def to_dev(w, precision):
+    import torch
+    dev = torch.device("cuda:0")
+    n = w["y"].shape[1]
+    seg = {"network": w["network"], "precision": precision,
+           "y": torch.tensor(w["y"], dtype=torch.float64, device=dev),
+           "t0": torch.tensor(np.broadcast_to(w["t0"], (n,)).copy(), dtype=torch.float64, device=dev),
+           "t_stop": torch.tensor(np.broadcast_to(w["t_stop"], (n,)).copy(), dtype=torch.float64, device=dev),
+           "dt0": torch.tensor(np.broadcast_to(w["dt0"], (n,)).copy(), dtype=torch.float64, device=dev),
+           "status": torch.full((n,), -9, dtype=torch.int32, device=dev),
+           "dt_out": torch.zeros(n, dtype=torch.float64, device=dev),
+           "counters": torch.zeros((8, n), dtype=torch.int32, device=dev),
+           "rtol": w["rtol"], "atol": w["atol"], "dt_min": w["dt_min"]}
+    if w.get("params") is not None:
+        seg["params"] = torch.tensor(w["params"], dtype=torch.float64, device=dev)
+    return seg
+
+
+def host(seg):
+    return (seg["y"].cpu().numpy(), seg["status"].cpu().numpy(), seg["dt_out"].cpu().numpy(),
+            seg["counters"].cpu().numpy())
+
+
+_STREAM = []
+
+
+def stream():
+    """A created stream (graph capture is not possible on the null stream)."""
+    import torch
+    if not _STREAM:
+        _STREAM.append(torch.cuda.Stream())
+    return _STREAM[0]
+
+
+def run(bundle, segs, use_graph=False, batch=None):
+    import torch
+    from odevec_amd.api import Batch
+    b = batch or Batch(bundle, use_graph=use_graph)
+    torch.cuda.synchronize()            # the segments' inputs were written on the current stream
+    b.solve_async(segs, stream().cuda_stream)
+    torch.cuda.synchronize()
+    if batch is None:
+        b.close()
+    return [host(s) for s in segs]
+
+
+def mixed_workload():
+    rob = W.rober(100, seed=11)
+    rob["t_stop"] = W._logu(np.random.default_rng(1), 1e-1, 1e5, 100)
+    ore = W.orego(63, seed=12)
+    ore["t_stop"] = W._logu(np.random.default_rng(2), 3.6e-4, 360.0, 63)
+    pri = W.primordial(300, seed=13)
+    pri["t_stop"][::17] = 0.0            # skipped cells
+    empty = W.rober(0, seed=14)
+    one = W.primordial(1, seed=15)
+    return [pri, rob, ore, empty, one]
+
+
+def test_batch_repro_equals_oracle(oracle):
+    ws = mixed_workload()
+    say("GPU batch MIX3+REPRO, segments", [w["y"].shape[1] for w in ws])
+    segs = [to_dev(w, _lib.FP64) for w in ws]
+    out = run("MIX3+REPRO", segs)
+    for w, (y, st, dto, cnt) in zip(ws, out):
+        n = w["y"].shape[1]
+        if n == 0:
+            continue
+        yr, sr, dr, cr = oracle.Network.load(w["network"]).solve_cells(
+            w["y"], w["t_stop"], t0=w["t0"], dt0=w["dt0"], params=w["params"], rtol=w["rtol"], atol=w["atol"],
+            dt_min=w["dt_min"])
+        assert np.array_equal(st, sr), w["network"]
+        assert np.array_equal(y, yr), w["network"]
+        assert np.array_equal(dto, dr), w["network"]
+        assert np.array_equal(cnt, cr), w["network"]
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
+def test_batch_segment_same_alone_and_in_mix(precision):
+    ws = mixed_workload()
+    say("GPU batch mix invariance, precision", precision)
+    together = run("MIX3", [to_dev(w, precision) for w in ws])
+    for k, w in enumerate(ws):
+        alone = run("MIX3", [to_dev(w, precision)])[0]
+        for a, b in zip(alone, together[k]):
+            assert np.array_equal(a, b), (w["network"], precision)
+    pri = together[0]
+    assert (pri[1][::17] == 5).all() and (np.delete(pri[1], np.s_[::17]) == 0).all()
+
+
+def test_batch_graph_replays_equal_eager():
+    import torch
+    from odevec_amd.api import Batch
+    ws = mixed_workload()
+    say("GPU batch graph replays")
+    eager = run("MIX3", [to_dev(w, _lib.MIXED) for w in ws])
+    b = Batch("MIX3", use_graph=True)
+    for rep in range(3):
+        segs = [to_dev(w, _lib.MIXED) for w in ws]     # fresh state: eager host->device work between replays
+        got = run("MIX3", segs, batch=b)
+        torch.cuda.synchronize()
+        for e, g in zip(eager, got):
+            for a, c in zip(e, g):
+                assert np.array_equal(a, c), rep
+    # a different table through the same captured graph (the table is read
+    # from device memory at run time; in graph mode the launch shape does not
+    # depend on the batch)
+    ws2 = [W.rober(50, seed=21), W.orego(40, seed=22)]
+    got = run("MIX3", [to_dev(w, _lib.FP64) for w in ws2], batch=b)
+    ref = run("MIX3", [to_dev(w, _lib.FP64) for w in ws2])
+    for e, g in zip(ref, got):
+        for a, c in zip(e, g):
+            assert np.array_equal(a, c)
+    b.close()
+
+
+def test_batch_rejects_entries_not_in_bundle():
+    import torch
+    from odevec_amd.api import Batch
+    w = W.primordial(4)
+    seg = to_dev(dict(w, network="PRIMORDIAL/FF/CSC"), _lib.FP64)
+    with Batch("MIX3") as b:
+        with pytest.raises(_lib.OdevecError) as e:
+            b.solve_async([seg], stream().cuda_stream)
+        assert e.value.code == _lib.ENOTFOUND
