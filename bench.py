@@ -38,6 +38,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--cells", type=int, default=NCELLS)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-batch", action="store_true")
     return p.parse_args()
 
 
@@ -99,6 +100,31 @@ class Runner:
         self.torch.cuda.synchronize()
         st = self.st.cpu().numpy()
         return self.flops.cpu().numpy(), st
+
+
+def batching(steps, warmup):
+    """Batching form: one persistent launch (hipGraph replay) over the
+    2,097,152-cell 40/30/30 ROBER/OREGO/PRIMORDIAL batch (seed 4), with the
+    bundle, precisions and block from the autotuner's cached result for this
+    device (odevec_amd/tuned.json), else all-FP64 at 256 threads."""
+    import torch
+    from odevec_amd import autotune as A
+    from odevec_amd import workloads as W
+    segs = W.batch_mix(2097152, seed=4)
+    nets = [w["network"] for w in segs]
+    t = A.lookup(nets)
+    if t is None:
+        t = {"bundle": "MIX3_FP64", "block": 256, "precision": {n: 0 for n in nets}, "per_cell_seconds": {},
+             "order": nets}
+    d = A.cost_order(A._device_segments(segs, [t["precision"][n] for n in nets]),
+                     {n: -k for k, n in enumerate(t["order"])})
+    sec = A.time_batch(t["bundle"], d, t["block"], use_graph=True, reps=max(steps, 1))
+    ok = sum(int((x["status"] == 0).sum()) for x in d)
+    return {"value": 2097152 / sec, "unit": "cell-solves/s", "ms_per_launch": sec * 1e3,
+            "workload": "2,097,152 cells, 40/30/30 ROBER/OREGO/PRIMORDIAL (seed 4), t_stop logU over 6 decades "
+                        "per network, one persistent launch per step, hipGraph replay",
+            "bundle": t["bundle"], "block": t["block"], "precision": t["precision"], "cells_ok": ok,
+            "tuned": "per_cell_seconds" in t and bool(t["per_cell_seconds"])}
 
 
 def cpu_baseline(w, seconds=12.0):
@@ -192,6 +218,8 @@ def main():
         "flops_per_launch": {"fp64": float(fl_m[0]), "fp32": float(fl_m[1])},
         "kernel_ms": {"mixed": ms_m, "fp64": ms_d},
     }
+    if not a.no_batch:
+        res["batching"] = batching(a.steps, a.warmup)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(res))

@@ -36,10 +36,16 @@ BUILTIN_VARIANTS = [
     "PRIMORDIAL/FF/CSC",
 ]
 
-# bundles: one batch kernel over several networks, each in both precisions
-# (odv_k_batch); every network listed must also be a variant above
+# bundles: one batch kernel (odv_k_batch) over several (network, precision)
+# entries; every network listed must also be a variant above. The kernel's
+# register allocation covers all its entries, so a bundle with one
+# precision per network spills less than one with both: the autotuner picks
+# the bundle (the register/LDS split) and the block size per batch.
+_MIX3 = ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"]
 BUNDLES = {
-    "MIX3": ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"],
+    "MIX3": [(n, p) for n in _MIX3 for p in (0, 1)],                          # every split
+    "MIX3_FP64": [(n, 0) for n in _MIX3],                                     # all FP64
+    "MIX3_MIXED": [(n, 1 if n.startswith("PRIMORDIAL") else 0) for n in _MIX3],  # FP32 LU for PRIMORDIAL
 }
 
 # the reproducible build: no contraction of a*b+c into FMA, so device results
@@ -82,12 +88,12 @@ def _struct(variant: str) -> str:
     return "odv_net_%s_%s_%s" % (name, ordering.upper(), packaging)
 
 
-def generate_bundle(name: str, nets: List[str]) -> str:
-    """Instantiation unit of bundle `name`: its networks in FP64 and in the
-    mixed form, registered under the bundle's name."""
-    entries = [(n, p) for n in nets for p in (0, 1)]
+def generate_bundle(name: str, entries) -> str:
+    """Instantiation unit of bundle `name`: its (network, precision) entries
+    (precision 0 FP64, 1 mixed), registered under the bundle's name."""
+    nets = sorted(set(n for n, _ in entries), key=[n for n, _ in entries].index)
     lines = ["// Generated by odevec_amd.build -- do not edit.",
-             "// bundle %s: %s, each FP64 and mixed" % (name, ", ".join(nets)),
+             "// bundle %s: %s" % (name, ", ".join("%s %s" % (n, "mixed" if p else "FP64") for n, p in entries)),
              '#include "odevec_registry.h"']
     for n in nets:
         lines.append('#include "generated/net_%s.h"' % _struct(n)[len("odv_net_"):])
@@ -131,10 +137,10 @@ def build(variants: List[str] = None, jobs: int = 8, verbose: bool = True, repro
         units.append((src, os.path.join(BUILD, "inst_%s.o" % b), []))
         if repro:
             units.append((src, os.path.join(BUILD, "inst_%s_repro.o" % b), REPRO_FLAGS))
-    for name, nets in (BUNDLES.items() if bundles else []):
-        if not all(n in variants for n in nets):
+    for name, entries in (BUNDLES.items() if bundles else []):
+        if not all(n in variants for n, _ in entries):
             continue
-        src = generate_bundle(name, nets)
+        src = generate_bundle(name, entries)
         units.append((src, os.path.join(BUILD, "bundle_%s.o" % name), []))
         if repro:
             units.append((src, os.path.join(BUILD, "bundle_%s_repro.o" % name), REPRO_FLAGS))

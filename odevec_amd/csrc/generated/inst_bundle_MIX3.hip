@@ -1,5 +1,5 @@
 // Generated by odevec_amd.build -- do not edit.
-// bundle MIX3: ROBER/NONE/DENSE, OREGO/NONE/DENSE, PRIMORDIAL/CMK/CSC, each FP64 and mixed
+// bundle MIX3: ROBER/NONE/DENSE FP64, ROBER/NONE/DENSE mixed, OREGO/NONE/DENSE FP64, OREGO/NONE/DENSE mixed, PRIMORDIAL/CMK/CSC FP64, PRIMORDIAL/CMK/CSC mixed
 #include "odevec_registry.h"
 #include "generated/net_ROBER_NONE_DENSE.h"
 #include "generated/net_OREGO_NONE_DENSE.h"
