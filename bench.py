@@ -11,7 +11,13 @@ FP64 residual, history, norms and control) with the persistent
 work-compacting kernel is the headline value; the plain FP64 kernel (one
 lane per cell, no compaction) runs the same steps for the ratio.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+Also on the line (skipped with --no-batch): the batching form (one launch
+over a 2,097,152-cell ROBER/OREGO/PRIMORDIAL batch) and the serving form
+(continuous batching under Poisson arrivals: p50/p99 latency and
+cell-solves/s at 40 requests/s), and (skipped with --no-cpu) the CPU
+baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batch] [--no-cpu]
 """
 import argparse
 import json
@@ -127,6 +133,20 @@ def batching(steps, warmup):
             "tuned": "per_cell_seconds" in t and bool(t["per_cell_seconds"])}
 
 
+def serving(rate=40.0, seconds=8.0):
+    """Serving form at sustained load: Poisson arrivals (seed 5) of blocks of
+    logU[1e3, 2.56e5] cells of ROBER / OREGO / PRIMORDIAL, one dt target per
+    block drawn logU over the network's span, rtol 1e-4 or 1e-6; two launch
+    slots; latency = arrival to results on the host."""
+    from odevec_amd import autotune as A
+    from odevec_amd import serve as S
+    tuned = A.lookup(list(S.SERVE_SPANS)) or {}
+    srv = S.Server(tuned.get("bundle", "MIX3"), slots=2, precision=tuned.get("precision", {}))
+    res = S.run_trace(srv, S.make_trace(rate, seconds, seed=5))
+    srv.close()
+    return dict(rate_per_s=rate, seconds=seconds, **res)
+
+
 def cpu_baseline(w, seconds=12.0):
     """The oracle (a C restatement of the reference solver) on the host cores,
     per-cell mode, on a seeded sample of the same workload."""
@@ -220,6 +240,9 @@ def main():
     }
     if not a.no_batch:
         res["batching"] = batching(a.steps, a.warmup)
+        sv = serving()
+        res["serving"] = sv
+        res["serving_p50_ms"], res["serving_p99_ms"] = sv["p50_ms"], sv["p99_ms"]
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(res))

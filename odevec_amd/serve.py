@@ -1,0 +1,262 @@
+"""Serving form: a continuous-batching request loop over the batch kernel.
+
+A request is a block of cells of one network with its own end time (dt
+target) and tolerances. The server
+
+  * admits waiting requests while their device footprint fits the HBM budget
+    (state, end times, parameters, status: the bytes the request holds on the
+    device until its results are returned);
+  * keeps up to `slots` launches in flight, each on its own HIP stream with
+    its own batch handle (segment table, queue counters, rate workspace) and
+    a share of the device's workgroups, so a request that arrives while
+    others run starts on a free slot instead of waiting for them: their work
+    overlaps on the device;
+  * forms each launch on a free slot from the waiting requests that fit
+    (continuous batching: requests join at the next free slot, not when the
+    previous batch drains), ordered by cost per cell;
+  * uploads a request's arrays on admission and keeps them resident until
+    its launch completes, then copies the results back and reports
+    latency = completion - arrival.
+
+Every cell keeps its own step control, so a request's result does not
+depend on what it was batched with (bitwise, for one bundle).
+
+    python -m odevec_amd.serve --rates 4,8,16,32 --seconds 10
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib as L
+from . import workloads as W
+
+SERVE_SPANS = {"ROBER/NONE/DENSE": (1e-1, 1e5), "OREGO/NONE/DENSE": (3.6e-4, 360.0),
+               "PRIMORDIAL/CMK/CSC": (1e8, 1e14)}
+COST_ORDER = ["PRIMORDIAL/CMK/CSC", "OREGO/NONE/DENSE", "ROBER/NONE/DENSE"]
+
+
+@dataclass
+class Request:
+    rid: int
+    network: str
+    y: np.ndarray                   # (neq, ncells), float64
+    t_stop: np.ndarray              # (ncells,)
+    rtol: float
+    atol: float
+    dt_min: float
+    dt0: np.ndarray                 # (ncells,)
+    params: Optional[np.ndarray] = None
+    precision: int = L.FP64
+    arrival: float = 0.0            # seconds (server clock)
+    done: float = 0.0
+    status: Optional[np.ndarray] = None
+    result: Optional[np.ndarray] = None
+    dev: dict = field(default_factory=dict)
+
+    @property
+    def ncells(self) -> int:
+        return self.y.shape[1]
+
+    def footprint(self) -> int:
+        """Device bytes held from admission to completion."""
+        n = self.ncells
+        p = 0 if self.params is None else self.params.shape[0]
+        return n * (8 * self.y.shape[0] + 8 + 8 + 8 * p + 4)
+
+
+class Server:
+    def __init__(self, bundle: str = "MIX3", slots: int = 2, hbm_budget: int = 16 << 30, block: int = 256,
+                 precision: Optional[dict] = None):
+        import torch
+        from .api import Batch
+        self.torch = torch
+        self.bundle = bundle
+        self.entries = set(L.bundle_entries(bundle))
+        dev = torch.cuda.get_device_properties(0)
+        kinfo = L.KernelInfo()
+        L.check(L.lib().odevec_batch_kernel_info(bundle.encode(), block, L.C.byref(kinfo)))
+        full = dev.multi_processor_count * max(kinfo.blocks_per_cu, 1)
+        self.grid = max(1, full // slots)
+        self.slots = [{"stream": torch.cuda.Stream(), "batch": Batch(bundle, block=block, grid=self.grid),
+                       "reqs": None, "event": None} for _ in range(slots)]
+        self.budget = hbm_budget
+        self.in_use = 0
+        self.waiting: deque = deque()      # arrived, not yet admitted
+        self.completed: List[Request] = []
+        self.precision = precision or {}
+        self.launches = 0
+        self.max_inflight = 0      # most launches seen in flight at once
+        self.t0 = time.perf_counter()
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def submit(self, r: Request):
+        if r.precision is None or (r.network, r.precision) not in self.entries:
+            r.precision = self.precision.get(r.network, L.FP64)
+        if (r.network, r.precision) not in self.entries:
+            raise ValueError("network %s precision %d not in bundle %s" % (r.network, r.precision, self.bundle))
+        if r.footprint() > self.budget:
+            raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.footprint()))
+        self.waiting.append(r)
+
+    def _admit(self, stream) -> List[Request]:
+        """Upload waiting requests that fit the budget (arrival order, at most
+        one launch's worth) on `stream`; returns them."""
+        torch = self.torch
+        new = []
+        while (self.waiting and len(new) < L_MAXSEG and
+               self.in_use + self.waiting[0].footprint() <= self.budget):
+            r = self.waiting.popleft()
+            dev = torch.device("cuda:0")
+            with torch.cuda.stream(stream):
+                d = {"y": torch.from_numpy(r.y).pin_memory().to(dev, non_blocking=True),
+                     "t_stop": torch.from_numpy(np.ascontiguousarray(r.t_stop)).pin_memory().to(dev, non_blocking=True),
+                     "dt0": torch.from_numpy(np.ascontiguousarray(r.dt0)).pin_memory().to(dev, non_blocking=True),
+                     "status": torch.empty(r.ncells, dtype=torch.int32, device=dev)}
+                if r.params is not None:
+                    d["params"] = torch.from_numpy(np.ascontiguousarray(r.params)).pin_memory().to(dev, non_blocking=True)
+            r.dev = d
+            self.in_use += r.footprint()
+            new.append(r)
+        return new
+
+    def _launch(self, slot, reqs: List[Request]):
+        reqs = sorted(reqs, key=lambda r: COST_ORDER.index(r.network) if r.network in COST_ORDER else len(COST_ORDER))
+        segs = []
+        for r in reqs:
+            s = {"network": r.network, "precision": r.precision, "y": r.dev["y"], "t_stop": r.dev["t_stop"],
+                 "dt0": r.dev["dt0"], "status": r.dev["status"], "rtol": r.rtol, "atol": r.atol, "dt_min": r.dt_min}
+            if r.params is not None:
+                s["params"] = r.dev["params"]
+            segs.append(s)
+        torch = self.torch
+        st = slot["stream"]
+        slot["batch"].solve_async(segs, st.cuda_stream)
+        with torch.cuda.stream(st):
+            for r in reqs:
+                r.dev["y_host"] = torch.empty(r.y.shape, dtype=torch.float64, pin_memory=True)
+                r.dev["st_host"] = torch.empty(r.ncells, dtype=torch.int32, pin_memory=True)
+                r.dev["y_host"].copy_(r.dev["y"], non_blocking=True)
+                r.dev["st_host"].copy_(r.dev["status"], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        slot["reqs"], slot["event"] = reqs, ev
+
+    def poll(self) -> List[Request]:
+        """Complete finished launches, admit waiting requests, launch on free
+        slots. Returns the requests completed in this call."""
+        done = []
+        for slot in self.slots:
+            if slot["reqs"] is not None and slot["event"].query():
+                t = self.now()
+                for r in slot["reqs"]:
+                    r.result = r.dev["y_host"].numpy()
+                    r.status = r.dev["st_host"].numpy()
+                    r.done = t
+                    self.in_use -= r.footprint()
+                    r.dev = {}
+                    done.append(r)
+                slot["reqs"], slot["event"] = None, None
+        for slot in self.slots:
+            if slot["reqs"] is None and self.waiting:
+                reqs = self._admit(slot["stream"])
+                if reqs:
+                    self._launch(slot, reqs)
+                    self.launches += 1
+        self.max_inflight = max(self.max_inflight, sum(s["reqs"] is not None for s in self.slots))
+        self.completed.extend(done)
+        return done
+
+    def busy(self) -> bool:
+        return bool(self.waiting or any(s["reqs"] is not None for s in self.slots))
+
+    def drain(self):
+        while self.busy():
+            self.poll()
+            time.sleep(0.0002)
+
+    def close(self):
+        self.drain()
+        for s in self.slots:
+            s["batch"].close()
+
+
+L_MAXSEG = 1024
+
+
+def make_trace(rate: float, seconds: float, seed: int = 5, networks=None, lo: float = 1e3, hi: float = 2.56e5,
+               rtols=(1e-4, 1e-6)) -> List[Request]:
+    """Poisson arrivals (rate per second) over `seconds`; each request a block
+    of logU[lo, hi] cells of a network drawn uniformly, one dt target drawn
+    logU over the network's span, rtol drawn from `rtols`."""
+    rng = np.random.default_rng(seed)
+    networks = networks or list(SERVE_SPANS)
+    t, out, rid = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        net = networks[rng.integers(len(networks))]
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        sd = int(rng.integers(1 << 30))
+        w = {"ROBER/NONE/DENSE": W.rober, "OREGO/NONE/DENSE": W.orego,
+             "PRIMORDIAL/CMK/CSC": W.primordial}[net](n, seed=sd)
+        lo_t, hi_t = SERVE_SPANS[net]
+        ts = float(np.exp(rng.uniform(np.log(lo_t), np.log(hi_t))))
+        rtol = float(rtols[rng.integers(len(rtols))])
+        out.append(Request(rid, net, np.ascontiguousarray(w["y"]), np.full(n, ts), rtol,
+                           w["atol"], w["dt_min"], np.ascontiguousarray(w["dt0"]),
+                           None if w.get("params") is None else np.ascontiguousarray(w["params"]), None, t))
+        rid += 1
+    return out
+
+
+def run_trace(server: Server, trace: List[Request]) -> dict:
+    """Replay arrivals in real time; returns latency percentiles and throughput."""
+    server.t0 = time.perf_counter()
+    pending = deque(sorted(trace, key=lambda r: r.arrival))
+    while pending or server.busy():
+        now = server.now()
+        while pending and pending[0].arrival <= now:
+            server.submit(pending.popleft())
+        server.poll()
+        time.sleep(0.0002)
+    wall = max(r.done for r in trace) if trace else 0.0
+    lat = np.array([r.done - r.arrival for r in trace]) * 1e3
+    cells = sum(r.ncells for r in trace)
+    ok = sum(int((r.status == 0).sum()) for r in trace)
+    return {"requests": len(trace), "cells": cells, "cells_ok": ok, "wall_s": wall,
+            "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+            "mean_ms": float(lat.mean()) if len(lat) else None,
+            "cell_solves_per_s": cells / wall if wall > 0 else None}
+
+
+def main(argv=None) -> int:
+    p = argparse.ArgumentParser()
+    p.add_argument("--rates", default="4,8,16,32")
+    p.add_argument("--seconds", type=float, default=10.0)
+    p.add_argument("--slots", type=int, default=2)
+    p.add_argument("--bundle", default="MIX3")
+    a = p.parse_args(argv)
+    from . import autotune as A
+    tuned = A.lookup(list(SERVE_SPANS)) or {}
+    prec = tuned.get("precision", {})
+    for rate in [float(x) for x in a.rates.split(",")]:
+        srv = Server(a.bundle, slots=a.slots, precision=prec)
+        res = run_trace(srv, make_trace(rate, a.seconds))
+        srv.close()
+        print(json.dumps(dict(rate_per_s=rate, **res)), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
