@@ -58,6 +58,8 @@ extern "C" {
 #define ODEVEC_PERCELL 0     /* one lane per cell, per-cell step control (reference at nvector=1) */
 #define ODEVEC_LOCKSTEP 1    /* groups of `group` cells share dt/order/verdicts (reference at nvector=group) */
 #define ODEVEC_PERSISTENT 2  /* per-cell control, persistent grid with lane refill (work compaction) */
+#define ODEVEC_COOP 3        /* per-cell control, one cell per group of lanes with one species per lane
+                                (CSC networks; the only mode for networks of more than 32 species) */
 
 typedef struct odevec_s odevec_t;
 

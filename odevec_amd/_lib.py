@@ -15,7 +15,7 @@ OK = 0
 EINVAL, ENOMEM, EDEVICE, ENOTFOUND, ESOLVE = -1, -2, -3, -4, -5
 STATUS = {0: "OK", 1: "DT_MIN", 2: "CONV_FAIL", 3: "ORDER", 4: "MAX_STEPS", 5: "SKIPPED"}
 FP64, MIXED = 0, 1
-PERCELL, LOCKSTEP, PERSISTENT = 0, 1, 2
+PERCELL, LOCKSTEP, PERSISTENT, COOP = 0, 1, 2, 3
 
 _I32 = C.c_int32
 _I64 = C.c_int64

@@ -34,6 +34,7 @@ BUILTIN_VARIANTS = [
     "PRIMORDIAL/CMK/CSC",
     "PRIMORDIAL/NONE/DENSE",
     "PRIMORDIAL/FF/CSC",
+    "SYNTH64/FF/CSC",     # 64 species: cooperative kernels only
 ]
 
 # bundles: one batch kernel (odv_k_batch) over several (network, precision)

@@ -30,6 +30,54 @@ struct odv_net_OREGO_NONE_CSC {
                                  double* __restrict__ k) {
     (void)y0; (void)prm; (void)k;
   }
+  static constexpr bool COOP = true;   // cooperative kernels available
+
+  // cooperative kernels (species on lanes): tables in the oracle's order
+  static constexpr int CO_MAXT = 4;   // most RHS terms of one species
+  static constexpr short RT_OFF[4] = {
+      0, 4, 7, 9,
+  };
+  static constexpr short RT_RATE[9] = {
+      -1, -1, -1, -1, -1, -1, -1, -1, -1,
+  };
+  static constexpr double RT_COEF[9] = {
+      77.27, -0.00064713625, -77.27, 77.27, -0.012941633234114146, -0.012941633234114146, 0.012941633234114146, 0.161,
+      -0.161,
+  };
+  static constexpr unsigned char RT_NF[9] = {
+      1, 2, 2, 1, 2, 1, 1, 1, 1,
+  };
+  static constexpr unsigned char RT_F[36] = {
+      0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0,
+      2, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0,
+  };
+  static constexpr short JT_OFF[10] = {
+      0, 3, 5, 5, 6, 8, 9, 10, 10, 11,
+  };
+  static constexpr short JT_RATE[11] = {
+      -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+  };
+  static constexpr double JT_CE[11] = {
+      77.27, -0.0012942725, -77.27, -77.27, 77.27, -0.012941633234114146, -0.012941633234114146, -0.012941633234114146,
+      0.012941633234114146, 0.161, -0.161,
+  };
+  static constexpr unsigned char JT_NR[11] = {
+      0, 1, 1, 1, 0, 1, 1, 0, 0, 0, 0,
+  };
+  static constexpr unsigned char JT_R[33] = {
+      0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0,
+      0, 0, 0, 0, 0, 0, 0, 0, 0,
+  };
+  static constexpr unsigned char PERM[3] = {
+      0, 1, 2,
+  };
+  static constexpr unsigned char IPERM[3] = {
+      0, 1, 2,
+  };
+  static constexpr unsigned long long LU_MASK[3] = {
+      0x3ull, 0x7ull, 0x7ull,
+  };
+  static constexpr bool LANE = true;   // per-lane kernels available
 
   // k[r * S]: rate r of this cell (S = stride between a cell's rates)
   template <int S = 1>

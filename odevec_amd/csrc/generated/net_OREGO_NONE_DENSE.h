@@ -30,6 +30,8 @@ struct odv_net_OREGO_NONE_DENSE {
                                  double* __restrict__ k) {
     (void)y0; (void)prm; (void)k;
   }
+  static constexpr bool COOP = false;
+  static constexpr bool LANE = true;   // per-lane kernels available
 
   // k[r * S]: rate r of this cell (S = stride between a cell's rates)
   template <int S = 1>

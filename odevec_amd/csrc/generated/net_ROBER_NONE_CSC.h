@@ -35,6 +35,50 @@ struct odv_net_ROBER_NONE_CSC {
     k[1] = prm[1];
     k[2] = prm[2];
   }
+  static constexpr bool COOP = true;   // cooperative kernels available
+
+  // cooperative kernels (species on lanes): tables in the oracle's order
+  static constexpr int CO_MAXT = 3;   // most RHS terms of one species
+  static constexpr short RT_OFF[4] = {
+      0, 2, 5, 6,
+  };
+  static constexpr short RT_RATE[6] = {
+      0, 2, 0, 1, 2, 1,
+  };
+  static constexpr double RT_COEF[6] = {
+      -1.0, 1.0, 1.0, -1.0, -1.0, 1.0,
+  };
+  static constexpr unsigned char RT_NF[6] = {
+      1, 2, 1, 2, 2, 2,
+  };
+  static constexpr unsigned char RT_F[24] = {
+      0, 0, 0, 0, 1, 2, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 1, 2, 0, 0, 1, 1, 0, 0,
+  };
+  static constexpr short JT_OFF[10] = {
+      0, 1, 2, 3, 4, 6, 7, 7, 8, 8,
+  };
+  static constexpr short JT_RATE[8] = {
+      0, 2, 2, 0, 1, 2, 2, 1,
+  };
+  static constexpr double JT_CE[8] = {
+      -1.0, 1.0, 1.0, 1.0, -2.0, -1.0, -1.0, 2.0,
+  };
+  static constexpr unsigned char JT_NR[8] = {
+      0, 1, 1, 0, 1, 1, 1, 1,
+  };
+  static constexpr unsigned char JT_R[24] = {
+      0, 0, 0, 2, 0, 0, 1, 0, 0, 0, 0, 0, 1, 0, 0, 2, 0, 0, 1, 0, 0, 1, 0, 0,
+  };
+  static constexpr unsigned char PERM[3] = {
+      0, 1, 2,
+  };
+  static constexpr unsigned char IPERM[3] = {
+      0, 1, 2,
+  };
+  static constexpr unsigned long long LU_MASK[3] = {
+      0x7ull, 0x7ull, 0x6ull,
+  };
+  static constexpr bool LANE = true;   // per-lane kernels available
 
   // k[r * S]: rate r of this cell (S = stride between a cell's rates)
   template <int S = 1>

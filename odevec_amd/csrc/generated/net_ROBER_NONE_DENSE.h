@@ -35,6 +35,8 @@ struct odv_net_ROBER_NONE_DENSE {
     k[1] = prm[1];
     k[2] = prm[2];
   }
+  static constexpr bool COOP = false;
+  static constexpr bool LANE = true;   // per-lane kernels available
 
   // k[r * S]: rate r of this cell (S = stride between a cell's rates)
   template <int S = 1>

@@ -277,7 +277,11 @@ int32_t odevec_create(const char* network, int64_t nvector, const odevec_opts_t*
   odevec_default_opts(&o);
   if (opts) o = *opts;
   if (o.precision != ODEVEC_FP64 && o.precision != ODEVEC_MIXED) return fail(ODEVEC_EINVAL, "bad precision");
-  if (o.mode < ODEVEC_PERCELL || o.mode > ODEVEC_PERSISTENT) return fail(ODEVEC_EINVAL, "bad mode");
+  if (o.mode < ODEVEC_PERCELL || o.mode > ODEVEC_COOP) return fail(ODEVEC_EINVAL, "bad mode");
+  if (o.mode == ODEVEC_COOP && !e->coop_ok)
+    return fail(ODEVEC_EINVAL, "the cooperative mode needs a CSC network of at most 64 integrated species");
+  if (o.mode != ODEVEC_COOP && !e->lane_ok)
+    return fail(ODEVEC_EINVAL, "more than 32 integrated species: only the cooperative mode (ODEVEC_COOP)");
   if (o.group < 0 || o.group > 256) return fail(ODEVEC_EINVAL, "group must be in 0..256");
   if (o.block != 0 && (o.block < 64 || o.block > 256 || o.block % 64)) return fail(ODEVEC_EINVAL, "block must be 64..256, a multiple of 64");
   int ndev = 0;

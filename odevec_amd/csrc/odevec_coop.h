@@ -1,0 +1,315 @@
+// Cooperative kernels: one cell per group of G lanes, one species per lane.
+//
+// The per-lane kernels keep a whole cell in one lane's registers, which
+// limits them to networks of ~16 species and to one wavefront per SIMD. Here
+// a cell's species are spread over G lanes (G = the power of two >= its
+// integrated species, <= 64; groups never straddle a wavefront):
+//
+//  * lane l holds species l's Nordsieck history, correction, weight and
+//    previous correction, and row l of the permuted iteration matrix P and
+//    its factors (dense row, structural zeros kept as exact zeros);
+//  * the cell's rates, full state, parameters, the active species' current
+//    values and one row of U (the broadcast row of the factorization) sit in
+//    the group's LDS block;
+//  * the integrator is the same odv_attempt as the per-lane kernels (the
+//    reference's SolveLinearSystem), instantiated for a one-species lane
+//    (CoopNet adapter) with a group whose species_sum adds the lanes'
+//    partial norms in species order and whose votes are over the group;
+//  * the RHS and Jacobian are table-driven gathers (each lane evaluates the
+//    terms of its own species / P row, in the oracle's order), the Crout
+//    factorization is right-looking over rows with U rows broadcast through
+//    LDS, and the triangular solves pass the pivot value by shuffle.
+//
+// Every floating-point operation a cell sees is the oracle's, in the same
+// order (only exact zeros of the dense storage are skipped), so the
+// reproducible build equals the oracle bit for bit here too.
+#pragma once
+#include "odevec_kernels.h"
+
+namespace odv {
+inline namespace ODV_VARIANT_NS {
+
+extern __shared__ double odv_coop_lds[];   // groups' LDS blocks (dynamic size)
+
+template <int G>
+__device__ inline int coop_l() { return (int)(threadIdx.x & (G - 1)); }
+template <int G>
+__device__ inline int coop_base() { return (int)((threadIdx.x & 63) & ~(G - 1)); }
+template <int G>
+__device__ inline int coop_grp() { return (int)(threadIdx.x / G); }
+
+// orders the LDS accesses of one wavefront's lanes (a group lives in one
+// wavefront, whose LDS operations complete in program order)
+__device__ inline void odv_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int N>
+struct CoopWidth {
+  static constexpr int G = N <= 1 ? 1 : N <= 2 ? 2 : N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
+};
+
+// a group's LDS block (doubles)
+template <class Net>
+struct CoopLayout {
+  static constexpr int K = 0;                                         // rates [NRATE]
+  static constexpr int YF = K + Net::NRATE;                           // the cell's full state [NEQ]
+  static constexpr int PR = YF + Net::NEQ;                            // parameters
+  static constexpr int YA = PR + (Net::NPARAM > 0 ? Net::NPARAM : 1);  // integrated species' values [NACT]
+  static constexpr int UR = YA + Net::NACT;                           // broadcast row [NACT]
+  static constexpr int WORDS = UR + Net::NACT;
+};
+
+// The generated network seen by a one-species lane.
+template <class Net_, int G_>
+struct CoopNet {
+  using Base = Net_;
+  using Y = CoopLayout<Net_>;
+  static constexpr int G = G_;
+  static constexpr int NB = Net_::NACT;    // species of the cell
+  static constexpr const char* ID = Net_::ID;
+  static constexpr int NEQ = Net_::NEQ;     // the norms divide by the caller's species count
+  static constexpr int NACT = 1;            // species per lane
+  static constexpr int ACT[1] = {0};
+  static constexpr int NNZ = NB;            // a lane's row of P / the factors
+  static constexpr int NRATE = 1;           // the rates live in the group's LDS block
+  static constexpr int NPARAM = Net_::NPARAM;
+  static constexpr int RATE_KIND = Net_::RATE_KIND;
+  static constexpr int TGAS = Net_::TGAS;
+  static constexpr int FLOPS_RHS = Net_::FLOPS_RHS, FLOPS_JAC = Net_::FLOPS_JAC;
+  static constexpr int FLOPS_LU = Net_::FLOPS_LU, FLOPS_SOLVE = Net_::FLOPS_SOLVE;
+  static constexpr int FLOPS_LU_R = Net_::FLOPS_LU, FLOPS_SOLVE_R = Net_::FLOPS_SOLVE;
+  static_assert(NB <= G && G <= 64, "a cell must fit one wavefront");
+
+  __device__ static double* blk() { return odv_coop_lds + coop_grp<G>() * Y::WORDS; }
+
+  // once per cell: lane 0 evaluates the rates into the LDS block
+  __device__ static void rates(const double* y0, const double* prm, double* k) {
+    k[0] = 0.0;
+    if (coop_l<G>() == 0) Base::rates(y0, prm, blk() + Y::K);
+    odv_wave_sync();
+  }
+
+  // publish the lanes' species values, then each lane gathers its species' terms
+  template <int S = 1>
+  __device__ static void rhs(const double* y, const double* k, double* f) {
+    (void)k;
+    double* b = blk();
+    const int l = coop_l<G>();
+    if (l < NB) b[Y::YA + l] = y[0];
+    odv_wave_sync();
+    const double* kk = b + Y::K;
+    const double* ya = b + Y::YA;
+    double acc = 0.0;
+    if (l < NB) {
+      const int t0 = Base::RT_OFF[l], t1 = Base::RT_OFF[l + 1];
+      for (int t = t0; t < t1; ++t) {
+        const int r = Base::RT_RATE[t];
+        const int nf = Base::RT_NF[t];
+        const unsigned char* fc = Base::RT_F + 4 * t;
+        double m;
+        int q = 0;
+        if (r >= 0) m = kk[r];
+        else if (nf > 0) { m = ya[fc[0]]; q = 1; }
+        else m = 1.0;
+        for (; q < nf; ++q) m = m * ya[fc[q]];
+        const double v = Base::RT_COEF[t] * m;
+        acc = (t == t0) ? v : acc + v;
+      }
+    }
+    f[0] = acc;
+    odv_wave_sync();
+  }
+
+  // row l of P = I - hb*J (permuted), from the Jacobian contributions of its entries
+  template <class M, int S = 1>
+  __device__ static void jac(const double* y, const double* k, double hb, M* a) {
+    (void)k;
+    double* b = blk();
+    const int l = coop_l<G>();
+    if (l < NB) b[Y::YA + l] = y[0];
+    odv_wave_sync();
+    const double* kk = b + Y::K;
+    const double* ya = b + Y::YA;
+    const int row = l < NB ? l : 0;
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+      double acc = 0.0;
+      bool has = false;
+      const int c0 = Base::JT_OFF[row * NB + jj], c1 = Base::JT_OFF[row * NB + jj + 1];
+      for (int c = c0; c < c1; ++c) {
+        const int r = Base::JT_RATE[c];
+        const int nr = Base::JT_NR[c];
+        const unsigned char* fr = Base::JT_R + 3 * c;
+        double m;
+        int q = 0;
+        if (r >= 0) m = kk[r];
+        else if (nr > 0) { m = ya[fr[0]]; q = 1; }
+        else m = 1.0;
+        for (; q < nr; ++q) m = m * ya[fr[q]];
+        const double v = Base::JT_CE[c] * m;
+        acc = has ? acc + v : v;
+        has = true;
+      }
+      const bool diag = row == jj;
+      const double v = has ? (diag ? 1.0 - hb * acc : -(hb * acc)) : (diag ? 1.0 : 0.0);
+      a[jj] = (M)v;
+    }
+    odv_wave_sync();
+  }
+
+  // Crout (unit U, L keeps the diagonal: src/odevec.F90:929-971), rows on
+  // lanes: at step k, lane k publishes row k, lane j > k forms
+  // U(k,j) = A(k,j) / L(k,k), lane k keeps row k of U, and every lane i > k
+  // with L(i,k) in the pattern updates A(i,j) -= L(i,k) U(k,j) over row k's
+  // pattern (per entry, in increasing k: the oracle's sequence).
+  template <class M>
+  __device__ static void lu(M* a) {
+    double* b = blk();
+    const int l = coop_l<G>();
+    const int base = coop_base<G>();
+    const unsigned long long mine = l < NB ? Base::LU_MASK[l] : 0ull;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+      const unsigned long long rowk = Base::LU_MASK[k];
+      if (l == k) {
+#pragma unroll
+        for (int j = k + 1; j < NB; ++j)
+          if ((rowk >> j) & 1ull) b[Y::UR + j] = (double)a[j];
+      }
+      odv_wave_sync();
+      const M lkk = (M)__shfl((double)a[k], base + k, 64);
+      if (l > k && l < NB && ((rowk >> l) & 1ull)) b[Y::UR + l] = (double)((M)b[Y::UR + l] / lkk);
+      odv_wave_sync();
+      const bool lik = l > k && ((mine >> k) & 1ull);
+#pragma unroll
+      for (int j = k + 1; j < NB; ++j) {
+        if ((rowk >> j) & 1ull) {
+          const M ukj = (M)b[Y::UR + j];
+          if (l == k) a[j] = ukj;
+          else if (lik) a[j] = a[j] - a[k] * ukj;
+        }
+      }
+      odv_wave_sync();
+    }
+  }
+  template <class M>
+  __device__ static void lu_r(M* a) { lu<M>(a); }
+
+  // x (species l's residual) -> the correction; permutation in and out by shuffle
+  template <class M, class V>
+  __device__ static void solve(const M* a, V* x) {
+    const int l = coop_l<G>();
+    const int base = coop_base<G>();
+    const unsigned long long mine = l < NB ? Base::LU_MASK[l] : 0ull;
+    V xp = (V)__shfl((double)x[0], base + (l < NB ? (int)Base::PERM[l] : 0), 64);
+#pragma unroll
+    for (int kk = 0; kk < NB; ++kk) {
+      if (l == kk) xp = xp / (V)a[kk];
+      const V mult = (V)__shfl((double)xp, base + kk, 64);
+      if (l > kk && ((mine >> kk) & 1ull)) xp = xp - mult * (V)a[kk];
+    }
+#pragma unroll
+    for (int kk = NB - 1; kk >= 0; --kk) {
+      const V mult = (V)__shfl((double)xp, base + kk, 64);
+      if (l < kk && ((mine >> kk) & 1ull)) xp = xp - mult * (V)a[kk];
+    }
+    x[0] = (V)__shfl((double)xp, base + (l < NB ? (int)Base::IPERM[l] : 0), 64);
+  }
+  template <class M, class V>
+  __device__ static void solve_r(const M* a, V* x) { solve<M, V>(a, x); }
+};
+
+template <class B, int G>
+struct SpeciesMap<CoopNet<B, G>> {
+  static constexpr int NACT_ALL = B::NACT;
+  static ODV_HD int all(int a) { return B::ACT[a]; }
+  __device__ static int lane(int) {
+    const int l = coop_l<G>();
+    return B::ACT[l < B::NACT ? l : 0];
+  }
+};
+
+// Votes and sums over a group's species lanes (per-cell semantics).
+template <int G, int NB>
+struct CoopGroup {
+  __device__ double max_active(double v, bool) const { return v; }
+  __device__ bool any_of(bool b) const {
+    const unsigned long long ball = __ballot(b && coop_l<G>() < NB);
+    const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << coop_base<G>());
+    return (ball & gm) != 0ull;
+  }
+  __device__ bool any_valid(bool b) const { return any_of(b); }
+  __device__ bool any_active(bool b) const { return any_of(b); }
+  // the lanes' partial sums added in species order: s = v_0; s = s + v_1; ...
+  __device__ double species_sum(double v) const {
+    const int base = coop_base<G>();
+    double s = __shfl(v, base, 64);
+#pragma unroll
+    for (int j = 1; j < NB; ++j) s = s + __shfl(v, base + j, 64);
+    return s;
+  }
+};
+
+template <class Net, class M, bool RCP>
+using CoopLane = Lane<Cfg<CoopNet<Net, CoopWidth<Net::NACT>::G>, M, RCP, 0, false>>;
+
+// One cell per group; groups of a workgroup are independent (no workgroup
+// barrier), every group-wide decision is a group vote, so a group's lanes
+// follow one control path.
+template <class Net, class M, bool RCP>
+__global__ void __launch_bounds__(BLOCK) odv_k_coop(KArgs a) {
+  constexpr int G = CoopWidth<Net::NACT>::G;
+  using CN = CoopNet<Net, G>;
+  using L = CoopLane<Net, M, RCP>;
+  using Y = CoopLayout<Net>;
+  const int l = coop_l<G>();
+  const long long i = (long long)blockIdx.x * (blockDim.x / G) + coop_grp<G>();
+  if (i >= a.ncells) return;   // the whole group
+  double* b = CN::blk();
+  for (int j = l; j < Net::NEQ; j += G) b[Y::YF + j] = a.y[(long long)j * a.ncells + i];
+  for (int p = l; p < Net::NPARAM; p += G) b[Y::PR + p] = a.prm[p * a.ps_plane + i * a.ps_cell];
+  odv_wave_sync();
+  const double t0 = a.t0[i * a.t0_s];
+  const double ts = a.tstop[i * a.ts_s];
+  const double dh = a.dt0[i * a.dt_s];
+  L s;
+  s.active = l < Net::NACT;
+  s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
+  s.dt = dh;
+  CoopGroup<G, Net::NACT> g;
+  if ((a.mask && !a.mask[i]) || !(t0 < ts)) {
+    if (l == 0) {
+      if (a.status) a.status[i] = ST_SKIPPED;
+      if (a.dt_out) a.dt_out[i] = dh;
+      if (a.counters) odv_store_counters(a, i, s.cnt);
+    }
+    return;
+  }
+  odv_begin(s, b + Y::YF, b + Y::PR, t0, ts, dh, a.ctl, g);
+  while (!odv_attempt(s, a.ctl, g)) {
+  }
+  double yv[1];
+  odv_result(s, yv);
+  if (l < Net::NACT) a.y[(long long)Net::ACT[l] * a.ncells + i] = yv[0];
+  if (l == 0) {
+    if (a.status) a.status[i] = s.status;
+    if (a.dt_out) a.dt_out[i] = s.dt;
+    if (a.counters) odv_store_counters(a, i, s.cnt);
+    if (a.flops) {
+      double f64, f32;
+      odv_flops<Lane<Cfg<Net, M, RCP, 0, false>>>(s.cnt, f64, f32);
+      atomicAdd(&a.flops[0], f64);
+      atomicAdd(&a.flops[1], f32);
+    }
+  }
+}
+
+template <class Net>
+constexpr int coop_lds_bytes() {
+  return (BLOCK / CoopWidth<Net::NACT>::G) * CoopLayout<Net>::WORDS * 8;
+}
+
+}  // namespace ODV_VARIANT_NS
+}  // namespace odv

@@ -68,16 +68,17 @@ struct Counters {
 //  KS    -- 0: the whole state in the lane; > 0: the rates and the previous
 //           correction in external columns, element r of this lane at ptr[r*KS]
 //  KEEPY -- keep the caller's y for lanes outside the mask (lockstep)
-//  ZX    -- (KS > 0) the Nordsieck columns 2..MAXORD in external columns too
+//  ZX    -- (KS > 0) how many Nordsieck columns from column 2 on live in
+//           external columns too (0..MAXORD-1); the others stay in the lane
 //  WX    -- (KS > 0) the error weights in an external column too
-template <class Net_, class M_, bool RCP_, int KS_, bool KEEPY_, bool ZX_ = false, bool WX_ = false>
+template <class Net_, class M_, bool RCP_, int KS_, bool KEEPY_, int ZX_ = 0, bool WX_ = false>
 struct Cfg {
   using Net = Net_;
   using M = M_;
   static constexpr bool RCP = RCP_;
   static constexpr int KS = KS_;
   static constexpr bool KEEPY = KEEPY_;
-  static constexpr bool ZX = KS_ > 0 && ZX_;
+  static constexpr int ZX = KS_ > 0 ? ZX_ : 0;
   static constexpr bool WX = KS_ > 0 && WX_;
 };
 
@@ -111,10 +112,23 @@ ODV_HD inline double odv_tau(int q, int q2, double lqq) {
 }
 
 // Lane-local reductions (per-cell semantics: the reference at nvector = 1).
+// species_sum: the sum over a cell's species of per-lane partial sums, in
+// species order (the identity when a lane holds all of its cell's species).
 struct LaneGroup {
   ODV_HD double max_active(double v, bool) const { return v; }
   ODV_HD bool any_valid(bool b) const { return b; }
   ODV_HD bool any_active(bool b) const { return b; }
+  ODV_HD double species_sum(double v) const { return v; }
+};
+
+// Which species a lane integrates. For the generated networks a lane holds
+// all integrated species (ACT); an adapter that spreads a cell over several
+// lanes (one species per lane) specializes this.
+template <class Net>
+struct SpeciesMap {
+  static constexpr int NACT_ALL = Net::NACT;                        // integrated species of the cell
+  static ODV_HD int all(int a) { return Net::ACT[a]; }              // a-th integrated species
+  static ODV_HD int lane(int j) { return Net::ACT[j]; }             // species of the lane's j-th slot
 };
 
 template <class C>
@@ -126,8 +140,9 @@ struct Lane {
   static constexpr int KS = C::KS;
   static constexpr bool RCP_ = C::RCP;
   static constexpr bool KEEPY_ = C::KEEPY;
-  static constexpr int ZR = C::ZX ? 2 : NCOL;   // Nordsieck columns held in the lane
-  double zl[ZR][N];      // Nordsieck history y_NS(:,:,0..ZR-1)
+  static constexpr int ZE = C::ZX;         // external Nordsieck columns: 2 .. 1+ZE
+  static constexpr int ZR = NCOL - ZE;     // Nordsieck columns held in the lane
+  double zl[ZR][N];      // Nordsieck history: columns 0, 1 and 2+ZE..MAXORD
   double en[N];          // accumulated correction
   double wl[C::WX ? 1 : N];              // inverse squared error weights (!WX)
   M a[Net::NNZ];         // P = I - dt*beta*J, then its LU factors
@@ -136,7 +151,7 @@ struct Lane {
   double yin[C::KEEPY ? N : 1];          // caller's y (KEEPY)
   double* kx;            // external rate column (KS > 0)
   double* eox;           // external previous-correction column (KS > 0)
-  double* zx;            // external Nordsieck columns 2..MAXORD (ZX): (c, j) at zx[((c-2)*N + j)*KS]
+  double* zx;            // external Nordsieck columns 2..1+ZE: (c, j) at zx[((c-2)*N + j)*KS]
   double* wx;            // external error-weight column (WX)
   double lq[7];          // l-vector of the current order (constant indices only)
   double lqq;            // l_q(q)
@@ -157,13 +172,15 @@ struct Lane {
     else eox[j * KS] = v;
   }
   // Nordsieck element (column c, species j); c is a compile-time constant at
-  // every call site (unrolled loops), so the branch folds away
+  // every call site (unrolled loops), so the branches fold away
+  static ODV_HD constexpr bool ext(int c) { return c >= 2 && c < 2 + ZE; }
+  static ODV_HD constexpr int lane_col(int c) { return c < 2 ? c : (c - ZE < ZR ? c - ZE : 0); }
   ODV_HD double z(int c, int j) const {
-    return c < ZR ? zl[c < ZR ? c : 0][j] : zx[((c >= 2 ? c - 2 : 0) * N + j) * (KS > 0 ? KS : 1)];
+    return ext(c) ? zx[((c - 2) * N + j) * (KS > 0 ? KS : 1)] : zl[lane_col(c)][j];
   }
   ODV_HD void set_z(int c, int j, double v) {
-    if (c < ZR) zl[c < ZR ? c : 0][j] = v;
-    else zx[((c >= 2 ? c - 2 : 0) * N + j) * (KS > 0 ? KS : 1)] = v;
+    if (ext(c)) zx[((c - 2) * N + j) * (KS > 0 ? KS : 1)] = v;
+    else zl[lane_col(c)][j] = v;
   }
   ODV_HD double w(int j) const { return C::WX ? wx[j * (KS > 0 ? KS : 1)] : wl[j]; }
   ODV_HD void set_w(int j, double v) {
@@ -223,8 +240,8 @@ ODV_HD inline void odv_weights(L& s, const double* y, double rtol, double atol) 
 
 // sqrt(sum_j x_j^2 w_j / neq) for one cell (src/odevec.F90:669-702); the
 // frozen species' terms are exact zeros unless their weight is not finite
-template <class L, class X>
-ODV_HD inline double odv_wnorm_f(const L& s, X x) {
+template <class L, class G, class X>
+ODV_HD inline double odv_wnorm_f(const L& s, const G& g, X x) {
   constexpr int N = L::N;
   double v = x(0);
   double acc = (v * v) * s.w(0);
@@ -233,13 +250,14 @@ ODV_HD inline double odv_wnorm_f(const L& s, X x) {
     v = x(j);
     acc = acc + (v * v) * s.w(j);
   }
+  acc = g.species_sum(acc);
   if (s.fz & 1) acc = NAN;
   return sqrt(acc / (double)L::NF);
 }
 
-template <class L>
-ODV_HD inline double odv_wnorm(const L& s, const double* x) {
-  return odv_wnorm_f(s, [&](int j) { return x[j]; });
+template <class L, class G>
+ODV_HD inline double odv_wnorm(const L& s, const G& g, const double* x) {
+  return odv_wnorm_f(s, g, [&](int j) { return x[j]; });
 }
 
 template <int N>
@@ -318,7 +336,7 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
   if (q == 1) {
     down = 0.0;
   } else {
-    const double nq = g.max_active(odv_wnorm_f(s, [&](int j) {
+    const double nq = g.max_active(odv_wnorm_f(s, g, [&](int j) {
                                      double v = 0.0;
 #pragma unroll
                                      for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.z(kk, j) : v;
@@ -327,12 +345,12 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
                                    s.active);
     down = 1.0 / (ODV_F32(1.3) * odv::m_pow(nq / odv_tau(q, q - 1, lqq), 1.0 / q) + 1e-6);
   }
-  const double ne = g.max_active(odv_wnorm(s, s.en), s.active);
+  const double ne = g.max_active(odv_wnorm(s, g, s.en), s.active);
   same = 1.0 / (ODV_F32(1.2) * odv::m_pow(ne / odv_tau(q, q, lqq), 1.0 / (q + 1.0)) + 1e-6);
   if (q == MAXORD || s.neglect_up) {
     up = 0.0;
   } else {
-    const double nd = g.max_active(odv_wnorm_f(s, [&](int j) { return s.en[j] - s.eo(j); }), s.active);
+    const double nd = g.max_active(odv_wnorm_f(s, g, [&](int j) { return s.en[j] - s.eo(j); }), s.active);
     up = 1.0 / (ODV_F32(1.4) * odv::m_pow(nd / odv_tau(q, q + 1, lqq), 1.0 / (q + 2.0)) + 1e-6);
   }
   int loc = 1;
@@ -417,14 +435,14 @@ ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, doubl
   constexpr int N = L::N;
   double y[N];
 #pragma unroll
-  for (int j = 0; j < N; ++j) y[j] = yfull[Net::ACT[j]];
+  for (int j = 0; j < N; ++j) y[j] = yfull[SpeciesMap<Net>::lane(j)];
   // frozen species: constant values, constant weights, zero corrections
   s.fz = 0;
   {
     int a = 0;
 #pragma unroll
     for (int j = 0; j < L::NF; ++j) {
-      if (a < N && Net::ACT[a] == j) {
+      if (a < SpeciesMap<Net>::NACT_ALL && SpeciesMap<Net>::all(a) == j) {
         ++a;
         continue;
       }
@@ -484,6 +502,7 @@ ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, doubl
   double acc = (f[0] * f[0]) * ((s.w(0) * tol) * tol);
 #pragma unroll
   for (int j = 1; j < N; ++j) acc = acc + (f[j] * f[j]) * ((s.w(j) * tol) * tol);
+  acc = g.species_sum(acc);
   if (s.fz & 1) acc = NAN;
   acc = g.max_active(acc, s.active);
   const double W0 = fmax(fabs(t0), fabs(tstop));
@@ -615,7 +634,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
     for (int j = 0; j < N; ++j) s.en[j] = s.en[j] + f[j];
     s.cnt.newton++;
 
-    const double e = g.max_active(odv_wnorm(s, f), s.active);
+    const double e = g.max_active(odv_wnorm(s, g, f), s.active);
     double rtmp = 0.0;
     if (it != 0) {
       rtmp = e / s.cerr;
@@ -668,7 +687,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   s.fc = 0;
 
   // local truncation error test (src/odevec.F90:400-455)
-  const double err = g.max_active(odv_wnorm(s, s.en), s.active) / tauq;
+  const double err = g.max_active(odv_wnorm(s, g, s.en), s.active) / tauq;
   {
     double y[N];
     odv_cur_y(s, beta, y);

@@ -19,9 +19,11 @@
 // placed by LdsPolicy per precision. The register file (512 VGPR+AGPR per
 // lane at one wavefront per SIMD) and LDS (640 B per lane at 256 lanes per
 // CU) hold ~2.6 KiB per lane together:
-//  * mixed form (LU factors FP32, 4 B each): the Nordsieck history and the
-//    error weights stay in registers; LDS holds the rate coefficients (read
-//    by every RHS and Jacobian) and the previous correction;
+//  * mixed form (LU factors FP32, 4 B each): Nordsieck columns 0, 1, 5 and
+//    the error weights stay in registers; LDS holds the rate coefficients
+//    (read by every RHS and Jacobian) and Nordsieck columns 2..4; the
+//    previous correction (read once per step-size selection) goes to the
+//    device workspace;
 //  * FP64 form (LU factors 8 B each): LDS holds the Nordsieck columns
 //    2..MAXORD, the error weights and the previous correction; the rate
 //    coefficients go to a device workspace with the same layout per
@@ -61,19 +63,21 @@ template <class Net, class M>
 struct LdsPolicy {
   static constexpr int N = Net::NACT;
   static constexpr bool FP64 = sizeof(M) == sizeof(double);
-  // mixed: rates, previous correction; FP64: columns 2..MAXORD, weights,
-  // previous correction (doubles per lane)
-  static constexpr int COLS = FP64 ? (MAXORD - 1) * N + 2 * N : Net::NRATE + N;
+  // LDS doubles per lane -- mixed: rates, Nordsieck columns 2..4; FP64:
+  // Nordsieck columns 2..MAXORD, weights, previous correction
+  static constexpr int COLS = FP64 ? (MAXORD - 1) * N + 2 * N : Net::NRATE + 3 * N;
   static constexpr bool USE = Net::NEQ >= 8 && COLS * 8 * BLOCK <= LDS_BYTES;
   static constexpr int KS = USE ? BLOCK : 0;
-  static constexpr bool ZX = USE && FP64;          // Nordsieck columns 2..MAXORD in LDS
+  static constexpr int ZX = USE ? (FP64 ? MAXORD - 1 : 3) : 0;   // Nordsieck columns from 2 on in LDS
   static constexpr bool KLDS = USE && !FP64;       // rates in LDS (else in the workspace)
   static constexpr bool WX = USE && FP64;          // error weights in LDS
+  static constexpr bool ELDS = USE && FP64;        // previous correction in LDS (else in the workspace)
   static constexpr int KWORDS = KLDS ? Net::NRATE * BLOCK : 1;
-  static constexpr int ZWORDS = ZX ? (MAXORD - 1) * N * BLOCK : 1;
+  static constexpr int ZWORDS = ZX > 0 ? ZX * N * BLOCK : 1;
   static constexpr int WWORDS = WX ? N * BLOCK : 1;
-  static constexpr int EWORDS = USE ? N * BLOCK : 1;
-  static constexpr long long WS_PER_SLOT = (USE && !KLDS) ? (long long)Net::NRATE * BLOCK : 0;
+  static constexpr int EWORDS = ELDS ? N * BLOCK : 1;
+  static constexpr long long WS_RATES = (USE && !KLDS) ? (long long)Net::NRATE * BLOCK : 0;
+  static constexpr long long WS_PER_SLOT = WS_RATES + ((USE && !ELDS) ? (long long)N * BLOCK : 0);
   static constexpr int WORDS = KWORDS + ZWORDS + WWORDS + EWORDS;   // one workgroup's LDS
 };
 
@@ -83,10 +87,11 @@ template <class P, class L>
 __device__ inline void odv_bind_columns(L& s, const KArgs& a, long long slot, double* lk, double* lz, double* lw,
                                         double* le) {
   if constexpr (P::USE) {
-    s.kx = P::KLDS ? lk + threadIdx.x : a.ws + slot * P::WS_PER_SLOT + threadIdx.x;
+    double* ws = a.ws + slot * P::WS_PER_SLOT + threadIdx.x;
+    s.kx = P::KLDS ? lk + threadIdx.x : ws;
     s.zx = lz + threadIdx.x;
     s.wx = lw + threadIdx.x;
-    s.eox = le + threadIdx.x;
+    s.eox = P::ELDS ? le + threadIdx.x : ws + P::WS_RATES;
   }
 }
 
@@ -358,6 +363,7 @@ struct BlockGroup {
   }
   __device__ bool any_valid(bool b) const { return any_of(b && valid); }
   __device__ bool any_active(bool b) const { return any_of(b); }
+  __device__ double species_sum(double v) const { return v; }
 };
 
 // Lockstep form. Every decision is taken from group-reduced values, so all

@@ -4,11 +4,11 @@
 #pragma once
 #include <vector>
 
-#include "odevec_kernels.h"
+#include "odevec_coop.h"
 
 namespace odv {
 
-enum Mode : int { MODE_PERCELL = 0, MODE_LOCKSTEP = 1, MODE_PERSISTENT = 2 };
+enum Mode : int { MODE_PERCELL = 0, MODE_LOCKSTEP = 1, MODE_PERSISTENT = 2, MODE_COOP = 3 };
 enum Prec : int { PREC_FP64 = 0, PREC_MIXED = 1 };
 
 struct KernelInfo {
@@ -32,6 +32,7 @@ struct NetEntry {
   int repro;   // built with -ffp-contract=off (bit-identical to the oracle)
   int neq, nnz, nrate, nparam, rate_kind, tgas;
   int flops_rhs, flops_jac, flops_lu, flops_solve, flops_lu_r, flops_solve_r;
+  int lane_ok, coop_ok;   // per-lane modes (percell, lockstep, persistent) / cooperative mode compiled
   hipError_t (*plan)(long long ncells, int mode, int prec, int block, int grid, int group, Plan* out);
   hipError_t (*launch)(const KArgs& a, int mode, int prec, const Plan& p, hipStream_t st);
   hipError_t (*info)(int mode, int prec, int block, KernelInfo* out);
@@ -75,12 +76,12 @@ __global__ void k_setq(unsigned long long* q, unsigned long long v);
 inline namespace ODV_VARIANT_NS {
 
 template <class K>
-inline hipError_t kinfo(K kern, int block, KernelInfo* out) {
+inline hipError_t kinfo(K kern, int block, KernelInfo* out, size_t dyn_lds = 0) {
   hipFuncAttributes at;
   hipError_t e = hipFuncGetAttributes(&at, reinterpret_cast<const void*>(kern));
   if (e != hipSuccess) return e;
   int nb = 0;
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kern), block, 0);
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kern), block, dyn_lds);
   if (e != hipSuccess) return e;
   out->regs = at.numRegs;
   out->sgprs = 0;
@@ -96,34 +97,45 @@ inline hipError_t kinfo(K kern, int block, KernelInfo* out) {
 template <class Net, class M, bool RCP>
 inline hipError_t plan_t(long long ncells, int mode, int block, int grid, int group, Plan* p) {
   using P = LdsPolicy<Net, M>;
-  if (mode == MODE_PERCELL) {
-    const int b = block > 0 ? block : BLOCK;
-    p->block = b;
-    p->grid = (ncells + b - 1) / b;
-  } else if (mode == MODE_PERSISTENT) {
-    const int b = block > 0 ? block : BLOCK;
-    long long g = grid;
-    if (g <= 0) {
-      int dev = 0, ncu = 0, nb = 0;
-      hipError_t e = hipGetDevice(&dev);
-      if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-      if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void*>(odv_k_persistent<Net, M, RCP>), b, 0);
-      if (e != hipSuccess) return e;
-      g = (long long)ncu * (nb > 0 ? nb : 1);
-    }
-    const long long need = (ncells + b - 1) / b;
-    if (g > need) g = need;
-    p->block = b;
-    p->grid = g;
-  } else {
-    const int grp = group > 0 ? group : 1;
-    p->block = ((grp + 63) / 64) * 64;
-    p->grid = (ncells + grp - 1) / grp;
+  if (mode == MODE_COOP) {
+    if constexpr (!Net::COOP) return hipErrorInvalidValue;
+    p->block = BLOCK;
+    p->grid = (ncells + BLOCK / CoopWidth<Net::NACT>::G - 1) / (BLOCK / CoopWidth<Net::NACT>::G);
+    p->ws_doubles = 0;
+    return hipSuccess;
   }
-  p->ws_doubles = (mode == MODE_LOCKSTEP) ? 0 : p->grid * P::WS_PER_SLOT;
-  return hipSuccess;
+  if constexpr (!Net::LANE) {
+    return hipErrorInvalidValue;
+  } else {
+    if (mode == MODE_PERCELL) {
+      const int b = block > 0 ? block : BLOCK;
+      p->block = b;
+      p->grid = (ncells + b - 1) / b;
+    } else if (mode == MODE_PERSISTENT) {
+      const int b = block > 0 ? block : BLOCK;
+      long long g = grid;
+      if (g <= 0) {
+        int dev = 0, ncu = 0, nb = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+          e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, reinterpret_cast<const void*>(odv_k_persistent<Net, M, RCP>), b, 0);
+        if (e != hipSuccess) return e;
+        g = (long long)ncu * (nb > 0 ? nb : 1);
+      }
+      const long long need = (ncells + b - 1) / b;
+      if (g > need) g = need;
+      p->block = b;
+      p->grid = g;
+    } else {
+      const int grp = group > 0 ? group : 1;
+      p->block = ((grp + 63) / 64) * 64;
+      p->grid = (ncells + grp - 1) / grp;
+    }
+    p->ws_doubles = (mode == MODE_LOCKSTEP) ? 0 : p->grid * P::WS_PER_SLOT;
+    return hipSuccess;
+  }
 }
 
 template <class Net, class M, bool RCP>
@@ -132,15 +144,27 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
   if (p.block <= 0 || p.grid <= 0 || p.grid > 0x7fffffffLL) return hipErrorInvalidValue;
   if (mode != MODE_LOCKSTEP && p.block > BLOCK) return hipErrorInvalidValue;
   if (p.ws_doubles > 0 && !a.ws) return hipErrorInvalidValue;
-  if (mode == MODE_PERCELL) {
-    hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
-  } else if (mode == MODE_PERSISTENT) {
-    hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * p.block));
-    hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
-  } else {
-    hipLaunchKernelGGL((odv_k_lockstep<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
+  if (mode == MODE_COOP) {
+    if constexpr (Net::COOP) {
+      if (p.block != BLOCK) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((odv_k_coop<Net, M, RCP>), dim3((unsigned)p.grid), dim3(BLOCK), coop_lds_bytes<Net>(), st, a);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
   }
-  return hipGetLastError();
+  if constexpr (!Net::LANE) {
+    return hipErrorInvalidValue;
+  } else {
+    if (mode == MODE_PERCELL) {
+      hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
+    } else if (mode == MODE_PERSISTENT) {
+      hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * p.block));
+      hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
+    } else {
+      hipLaunchKernelGGL((odv_k_lockstep<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
+    }
+    return hipGetLastError();
+  }
 }
 
 template <class Net>
@@ -157,9 +181,17 @@ hipError_t launch_net(const KArgs& a, int mode, int prec, const Plan& p, hipStre
 
 template <class Net, class M, bool RCP>
 inline hipError_t info_t(int mode, int block, KernelInfo* out) {
-  if (mode == MODE_PERCELL) return kinfo(odv_k_percell<Net, M, RCP>, block > 0 ? block : 256, out);
-  if (mode == MODE_PERSISTENT) return kinfo(odv_k_persistent<Net, M, RCP>, block > 0 ? block : 256, out);
-  return kinfo(odv_k_lockstep<Net, M, RCP>, block > 0 ? block : 64, out);
+  if (mode == MODE_COOP) {
+    if constexpr (Net::COOP) return kinfo(odv_k_coop<Net, M, RCP>, BLOCK, out, coop_lds_bytes<Net>());
+    return hipErrorInvalidValue;
+  }
+  if constexpr (!Net::LANE) {
+    return hipErrorInvalidValue;
+  } else {
+    if (mode == MODE_PERCELL) return kinfo(odv_k_percell<Net, M, RCP>, block > 0 ? block : 256, out);
+    if (mode == MODE_PERSISTENT) return kinfo(odv_k_persistent<Net, M, RCP>, block > 0 ? block : 256, out);
+    return kinfo(odv_k_lockstep<Net, M, RCP>, block > 0 ? block : 64, out);
+  }
 }
 
 template <class Net>
@@ -224,5 +256,5 @@ struct Bundle {
   static const odv::NetEntry odv_entry_##NET = {  /* NOLINT */                                                \
       NET::ID, ODV_REPRO_FLAG, NET::NEQ, NET::NNZ, NET::NRATE, NET::NPARAM, NET::RATE_KIND, NET::TGAS,          \
       NET::FLOPS_RHS, NET::FLOPS_JAC, NET::FLOPS_LU, NET::FLOPS_SOLVE, NET::FLOPS_LU_R,         \
-      NET::FLOPS_SOLVE_R, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults};                          \
+      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults};                          \
   static odv::Registrar odv_reg_##NET(&odv_entry_##NET);

@@ -28,6 +28,8 @@ from __future__ import annotations
 
 import json
 import math
+
+import numpy as np
 from typing import Dict, List, Tuple
 
 from .structure import Structure, Term, reduce_frozen
@@ -314,6 +316,93 @@ def opcounts(st: Structure) -> Dict[str, Dict[str, int]]:
     return out
 
 
+COOP_MAX = 64   # species per cell of the cooperative kernels (one lane per species)
+LANE_MAX = 32   # species per cell of the per-lane kernels (state in one lane's registers)
+
+
+def _arr(ctype: str, name: str, vals, per_line: int = 24) -> List[str]:
+    vals = list(vals) or [0]
+    out = ["  static constexpr %s %s[%d] = {" % (ctype, name, len(vals))]
+    for i in range(0, len(vals), per_line):
+        out.append("      " + ", ".join(str(v) for v in vals[i:i + per_line]) + ",")
+    out.append("  };")
+    return out
+
+
+def _emit_coop(st: Structure) -> List[str]:
+    """Tables of the cooperative kernels (one lane per species, odevec_coop.h)
+    in the oracle's evaluation order:
+      RHS, per species row c: its terms in canonical order; f_c = v_first,
+        then f_c + v for the rest, v = coef * ((k_rate * y_f0) * y_f1 ...);
+      Jacobian, per permuted P entry (i, jj) (row-major, offsets JT_OFF): the
+        contributions in term order, v = (coef*e) * (k_rate * reduced monomial);
+      PERM / IPERM: permuted position <-> species; LU_MASK: bit jj of row i set
+        when (i, jj) is in the factors' pattern (with fill)."""
+    n = st.neq
+    rows = [[] for _ in range(n)]
+    for t in st.terms:
+        rows[t.row].append(t)
+    rt_off, rt_rate, rt_coef, rt_nf, rt_f = [0], [], [], [], []
+    for c in range(n):
+        for t in rows[c]:
+            rt_rate.append(-1 if t.rate is None or t.rate < 0 else int(t.rate))
+            rt_coef.append(_lit(float(t.coef)))
+            rt_nf.append(len(t.factors))
+            f = list(t.factors) + [0] * (4 - len(t.factors))
+            rt_f += f[:4]
+        rt_off.append(len(rt_rate))
+    perm = [int(p) for p in st.perm]
+    ip = [0] * n
+    for i, p in enumerate(perm):
+        ip[p] = i
+    ent = [[[] for _ in range(n)] for _ in range(n)]
+    for t in st.terms:
+        fac = list(t.factors)
+        for q, j in enumerate(fac):
+            if q > 0 and fac[q - 1] == j:
+                continue
+            e = fac.count(j)
+            red = list(fac)
+            red.remove(j)
+            ent[ip[t.row]][ip[j]].append((-1 if t.rate is None or t.rate < 0 else int(t.rate),
+                                          float(t.coef) * e, red))
+    jt_off, jt_rate, jt_ce, jt_nr, jt_r = [0], [], [], [], []
+    for i in range(n):
+        for jj in range(n):
+            for rate, ce, red in ent[i][jj]:
+                jt_rate.append(rate)
+                jt_ce.append(_lit(ce))
+                jt_nr.append(len(red))
+                rr = red + [0] * (3 - len(red))
+                jt_r += rr[:3]
+            jt_off.append(len(jt_rate))
+    lu = np.asarray(st.LU, dtype=bool)
+    mask = []
+    for i in range(n):
+        m = 0
+        for jj in range(n):
+            if lu[i, jj]:
+                m |= 1 << jj
+        mask.append("0x%xull" % m)
+    maxr = max((len(r) for r in rows), default=0)
+    out = ["", "  // cooperative kernels (species on lanes): tables in the oracle's order",
+           "  static constexpr int CO_MAXT = %d;   // most RHS terms of one species" % maxr]
+    out += _arr("short", "RT_OFF", rt_off)
+    out += _arr("short", "RT_RATE", rt_rate)
+    out += _arr("double", "RT_COEF", rt_coef, 8)
+    out += _arr("unsigned char", "RT_NF", rt_nf)
+    out += _arr("unsigned char", "RT_F", rt_f)
+    out += _arr("short", "JT_OFF", jt_off)
+    out += _arr("short", "JT_RATE", jt_rate)
+    out += _arr("double", "JT_CE", jt_ce, 8)
+    out += _arr("unsigned char", "JT_NR", jt_nr)
+    out += _arr("unsigned char", "JT_R", jt_r)
+    out += _arr("unsigned char", "PERM", perm)
+    out += _arr("unsigned char", "IPERM", ip)
+    out += _arr("unsigned long long", "LU_MASK", mask, 6)
+    return out
+
+
 def emit_header(full: Structure) -> str:
     """Header for the structure's active species (frozen species -- zero RHS,
     read by no term -- are carried unchanged by the caller's state)."""
@@ -373,6 +462,17 @@ def emit_header(full: Structure) -> str:
     hdr.append("                                 double* __restrict__ k) {")
     hdr += rates
     hdr.append("  }")
+    if st.packaging == "CSC" and st.neq <= COOP_MAX:
+        hdr.append("  static constexpr bool COOP = true;   // cooperative kernels available")
+        hdr += _emit_coop(st)
+    else:
+        hdr.append("  static constexpr bool COOP = false;")
+    hdr.append("  static constexpr bool LANE = %s;   // per-lane kernels available" % (
+        "true" if st.neq <= LANE_MAX else "false"))
+    if st.neq > LANE_MAX:
+        hdr.append("};")
+        hdr.append("")
+        return "\n".join(hdr)
     hdr.append("")
     hdr.append("  // k[r * S]: rate r of this cell (S = stride between a cell's rates)")
     hdr.append("  template <int S = 1>")

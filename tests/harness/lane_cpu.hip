@@ -18,14 +18,14 @@ namespace {
 // KS > 0 runs the lane with its rates, error weights, previous correction
 // and (ZX) Nordsieck columns 2..MAXORD in external strided columns, as the
 // device kernels keep them in LDS and in the rate workspace.
-template <class Net, class M, bool RCP, int KS, bool ZX = false>
+template <class Net, class M, bool RCP, int KS, int ZX = 0, bool WX = false>
 void run(long n, const double* t0, const double* ts, const double* dt0, double* y, const double* prm,
          const odv::Controls& c, int* status, double* dtout, int* cnt) {
   constexpr int N = Net::NEQ;
   constexpr int NA = Net::NACT;
   constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
   constexpr int S = KS > 0 ? KS : 1;
-  using L = odv::Lane<odv::Cfg<Net, M, RCP, KS, false, ZX, ZX>>;
+  using L = odv::Lane<odv::Cfg<Net, M, RCP, KS, false, ZX, WX>>;
   static double kcol[Net::NRATE * S];
   static double ecol[NA * S];
   static double zcol[(odv::MAXORD - 1) * NA * S];
@@ -71,10 +71,10 @@ template <class Net>
 void run_prec(int mixed, long n, const double* t0, const double* ts, const double* dt0, double* y,
               const double* prm, const odv::Controls& c, int* status, double* dtout, int* cnt) {
   if (mixed & 2) {
-    // the per-precision placement of the device kernels: FP64 also moves the
-    // Nordsieck columns 2..MAXORD and the error weights out of the lane
-    if (mixed & 1) run<Net, float, true, 7, false>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
-    else run<Net, double, false, 7, true>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+    // the per-precision placement of the device kernels: mixed moves Nordsieck
+    // columns 2..4 out of the lane, FP64 columns 2..MAXORD and the weights
+    if (mixed & 1) run<Net, float, true, 7, 3, false>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
+    else run<Net, double, false, 7, 4, true>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
   } else {
     if (mixed & 1) run<Net, float, true, 0>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);
     else run<Net, double, false, 0>(n, t0, ts, dt0, y, prm, c, status, dtout, cnt);

@@ -90,3 +90,15 @@ def test_bundles_registered_and_batch_argument_checks():
     assert L.odevec_batch_create(b"MIX3", 100, 0, 0, C.byref(h)) == _lib.EINVAL
     assert L.odevec_batch_solve_async(None, None, 0, None) == _lib.EINVAL
     assert L.odevec_batch_destroy(None) == _lib.OK
+
+
+def test_cooperative_mode_registry_and_checks():
+    info = _lib.network_info("SYNTH64/FF/CSC")
+    assert info["neq"] == 64 and info["nparam"] == 1 and info["nrate"] == 300
+    L = _lib.lib()
+    h = C.c_void_p()
+    lane = _lib.Opts(0, _lib.PERCELL, 0, 0, 0, 0, 0, 0)
+    coop = _lib.Opts(0, _lib.COOP, 0, 0, 0, 0, 0, 0)
+    assert L.odevec_create(b"SYNTH64/FF/CSC", 4, C.byref(lane), C.byref(h)) == _lib.EINVAL
+    assert "cooperative" in _lib.last_error()
+    assert L.odevec_create(b"ROBER/NONE/DENSE", 4, C.byref(coop), C.byref(h)) == _lib.EINVAL
