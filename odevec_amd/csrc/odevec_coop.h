@@ -45,6 +45,20 @@ __device__ inline void odv_wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// value of lane `base + j` of this lane's group: a scalar read of that lane
+// when the group is the whole wavefront (j is then uniform), else a shuffle
+template <int G>
+__device__ inline double coop_bcast(double v, int base, int j) {
+  if constexpr (G == 64) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)(u & 0xffffffffull), j);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  } else {
+    return __shfl(v, base + j, 64);
+  }
+}
+
 template <int N>
 struct CoopWidth {
   static constexpr int G = N <= 1 ? 1 : N <= 2 ? 2 : N <= 4 ? 4 : N <= 8 ? 8 : N <= 16 ? 16 : N <= 32 ? 32 : 64;
@@ -179,7 +193,7 @@ struct CoopNet {
           if ((rowk >> j) & 1ull) b[Y::UR + j] = (double)a[j];
       }
       odv_wave_sync();
-      const M lkk = (M)__shfl((double)a[k], base + k, 64);
+      const M lkk = (M)coop_bcast<G>((double)a[k], base, k);
       if (l > k && l < NB && ((rowk >> l) & 1ull)) b[Y::UR + l] = (double)((M)b[Y::UR + l] / lkk);
       odv_wave_sync();
       const bool lik = l > k && ((mine >> k) & 1ull);
@@ -207,12 +221,12 @@ struct CoopNet {
 #pragma unroll
     for (int kk = 0; kk < NB; ++kk) {
       if (l == kk) xp = xp / (V)a[kk];
-      const V mult = (V)__shfl((double)xp, base + kk, 64);
+      const V mult = (V)coop_bcast<G>((double)xp, base, kk);
       if (l > kk && ((mine >> kk) & 1ull)) xp = xp - mult * (V)a[kk];
     }
 #pragma unroll
     for (int kk = NB - 1; kk >= 0; --kk) {
-      const V mult = (V)__shfl((double)xp, base + kk, 64);
+      const V mult = (V)coop_bcast<G>((double)xp, base, kk);
       if (l < kk && ((mine >> kk) & 1ull)) xp = xp - mult * (V)a[kk];
     }
     x[0] = (V)__shfl((double)xp, base + (l < NB ? (int)Base::IPERM[l] : 0), 64);
@@ -245,9 +259,9 @@ struct CoopGroup {
   // the lanes' partial sums added in species order: s = v_0; s = s + v_1; ...
   __device__ double species_sum(double v) const {
     const int base = coop_base<G>();
-    double s = __shfl(v, base, 64);
+    double s = coop_bcast<G>(v, base, 0);
 #pragma unroll
-    for (int j = 1; j < NB; ++j) s = s + __shfl(v, base + j, 64);
+    for (int j = 1; j < NB; ++j) s = s + coop_bcast<G>(v, base, j);
     return s;
   }
 };

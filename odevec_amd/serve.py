@@ -10,7 +10,9 @@ target) and tolerances. The server
     its own batch handle (segment table, queue counters, rate workspace) and
     a share of the device's workgroups, so a request that arrives while
     others run starts on a free slot instead of waiting for them: their work
-    overlaps on the device;
+    overlaps on the device; requests of networks too large for one lane
+    (SYNTH64, 64 species) go through the cooperative kernel on the same
+    slot's stream;
   * forms each launch on a free slot from the waiting requests that fit
     (continuous batching: requests join at the next free slot, not when the
     previous batch drains), ordered by cost per cell;
@@ -38,8 +40,11 @@ from . import _lib as L
 from . import workloads as W
 
 SERVE_SPANS = {"ROBER/NONE/DENSE": (1e-1, 1e5), "OREGO/NONE/DENSE": (3.6e-4, 360.0),
-               "PRIMORDIAL/CMK/CSC": (1e8, 1e14)}
+               "PRIMORDIAL/CMK/CSC": (1e8, 1e14), "SYNTH64/FF/CSC": (1e4, 1e10)}
 COST_ORDER = ["PRIMORDIAL/CMK/CSC", "OREGO/NONE/DENSE", "ROBER/NONE/DENSE"]
+# networks served by the cooperative kernel (one launch per request, on the
+# same slot stream as the slot's batch launch) rather than the bundle
+COOP_NETWORKS = {"SYNTH64/FF/CSC"}
 
 
 @dataclass
@@ -85,7 +90,7 @@ class Server:
         full = dev.multi_processor_count * max(kinfo.blocks_per_cu, 1)
         self.grid = max(1, full // slots)
         self.slots = [{"stream": torch.cuda.Stream(), "batch": Batch(bundle, block=block, grid=self.grid),
-                       "reqs": None, "event": None} for _ in range(slots)]
+                       "coop": {}, "reqs": None, "event": None} for _ in range(slots)]
         self.budget = hbm_budget
         self.in_use = 0
         self.waiting: deque = deque()      # arrived, not yet admitted
@@ -99,10 +104,14 @@ class Server:
         return time.perf_counter() - self.t0
 
     def submit(self, r: Request):
-        if r.precision is None or (r.network, r.precision) not in self.entries:
-            r.precision = self.precision.get(r.network, L.FP64)
-        if (r.network, r.precision) not in self.entries:
-            raise ValueError("network %s precision %d not in bundle %s" % (r.network, r.precision, self.bundle))
+        if r.network in COOP_NETWORKS:
+            if r.precision is None:
+                r.precision = self.precision.get(r.network, L.MIXED)
+        else:
+            if r.precision is None or (r.network, r.precision) not in self.entries:
+                r.precision = self.precision.get(r.network, L.FP64)
+            if (r.network, r.precision) not in self.entries:
+                raise ValueError("network %s precision %d not in bundle %s" % (r.network, r.precision, self.bundle))
         if r.footprint() > self.budget:
             raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.footprint()))
         self.waiting.append(r)
@@ -128,18 +137,36 @@ class Server:
             new.append(r)
         return new
 
+    def _coop(self, slot, network, precision):
+        from .api import Odevec
+        key = (network, precision)
+        h = slot["coop"].get(key)
+        if h is None:
+            h = Odevec(network, 1, precision=precision, mode=L.COOP)
+            slot["coop"][key] = h
+        return h
+
     def _launch(self, slot, reqs: List[Request]):
-        reqs = sorted(reqs, key=lambda r: COST_ORDER.index(r.network) if r.network in COST_ORDER else len(COST_ORDER))
+        reqs = sorted(reqs, key=lambda r: COST_ORDER.index(r.network) if r.network in COST_ORDER else -1)
+        st = slot["stream"]
         segs = []
         for r in reqs:
+            if r.network in COOP_NETWORKS:
+                h = self._coop(slot, r.network, r.precision)
+                h.RelativeTolerance, h.AbsoluteTolerance, h.MinimalTimestep = r.rtol, r.atol, r.dt_min
+                h.solve_cells_async(r.ncells, r.dev["t_stop"].data_ptr(), r.dev["y"].data_ptr(),
+                                    dt0_ptr=r.dev["dt0"].data_ptr(),
+                                    params_ptr=r.dev["params"].data_ptr() if r.params is not None else None,
+                                    status_ptr=r.dev["status"].data_ptr(), stream=st.cuda_stream)
+                continue
             s = {"network": r.network, "precision": r.precision, "y": r.dev["y"], "t_stop": r.dev["t_stop"],
                  "dt0": r.dev["dt0"], "status": r.dev["status"], "rtol": r.rtol, "atol": r.atol, "dt_min": r.dt_min}
             if r.params is not None:
                 s["params"] = r.dev["params"]
             segs.append(s)
         torch = self.torch
-        st = slot["stream"]
-        slot["batch"].solve_async(segs, st.cuda_stream)
+        if segs:
+            slot["batch"].solve_async(segs, st.cuda_stream)
         with torch.cuda.stream(st):
             for r in reqs:
                 r.dev["y_host"] = torch.empty(r.y.shape, dtype=torch.float64, pin_memory=True)
@@ -187,6 +214,8 @@ class Server:
         self.drain()
         for s in self.slots:
             s["batch"].close()
+            for h in s["coop"].values():
+                h.CloseOdevec()
 
 
 L_MAXSEG = 1024
@@ -207,8 +236,8 @@ def make_trace(rate: float, seconds: float, seed: int = 5, networks=None, lo: fl
         net = networks[rng.integers(len(networks))]
         n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         sd = int(rng.integers(1 << 30))
-        w = {"ROBER/NONE/DENSE": W.rober, "OREGO/NONE/DENSE": W.orego,
-             "PRIMORDIAL/CMK/CSC": W.primordial}[net](n, seed=sd)
+        w = {"ROBER/NONE/DENSE": W.rober, "OREGO/NONE/DENSE": W.orego, "PRIMORDIAL/CMK/CSC": W.primordial,
+             "SYNTH64/FF/CSC": W.synth64}[net](n, seed=sd)
         lo_t, hi_t = SERVE_SPANS[net]
         ts = float(np.exp(rng.uniform(np.log(lo_t), np.log(hi_t))))
         rtol = float(rtols[rng.integers(len(rtols))])
