@@ -31,6 +31,8 @@ inline namespace ODV_VARIANT_NS {
 
 extern __shared__ double odv_coop_lds[];   // groups' LDS blocks (dynamic size)
 
+constexpr int COOP_BLOCK = 256;   // lanes per workgroup of the cooperative kernel
+
 template <int G>
 __device__ inline int coop_l() { return (int)(threadIdx.x & (G - 1)); }
 template <int G>
@@ -269,60 +271,86 @@ struct CoopGroup {
 template <class Net, class M, bool RCP>
 using CoopLane = Lane<Cfg<CoopNet<Net, CoopWidth<Net::NACT>::G>, M, RCP, 0, false>>;
 
-// One cell per group; groups of a workgroup are independent (no workgroup
-// barrier), every group-wide decision is a group vote, so a group's lanes
-// follow one control path.
+// Persistent: each group takes cells from a queue (its first cell is its
+// global group id; the counter starts at the number of groups launched)
+// until none is left, so a long cell does not hold a finished group idle.
+// Groups of a workgroup are independent (no workgroup barrier) and every
+// group-wide decision is a group vote, so a group's lanes follow one control
+// path. Each group's loop ends: every cell is bounded by max_attempts.
+// Compiled for COOP_WAVES wavefronts per SIMD (512 / COOP_WAVES registers
+// per lane). A lane's dense P row alone is 64 registers (FP32) or 128
+// (FP64); the spills a small budget costs are cheaper than the latency a low
+// occupancy leaves exposed. Measured, SYNTH64, 65,536 cells, ms per launch
+// at 1 / 2 / 4 / 6 / 8 wavefronts per SIMD: FP64 825 / 614 / 522 / 495 /
+// 510, mixed 376 / 377 / 289 / 270 / 288. Narrower groups (<= 32 species)
+// keep the compiler's own budget: their rows are short, and a forced
+// budget only adds spills (PRIMORDIAL, 16 lanes: 515 -> 633 ms at 6).
+constexpr int COOP_WAVES = 6;
+
 template <class Net, class M, bool RCP>
-__global__ void __launch_bounds__(BLOCK) odv_k_coop(KArgs a) {
+__global__ void __launch_bounds__(COOP_BLOCK)
+__attribute__((amdgpu_waves_per_eu(CoopWidth<Net::NACT>::G == 64 ? COOP_WAVES : 1)))
+odv_k_coop(KArgs a) {
   constexpr int G = CoopWidth<Net::NACT>::G;
   using CN = CoopNet<Net, G>;
   using L = CoopLane<Net, M, RCP>;
   using Y = CoopLayout<Net>;
   const int l = coop_l<G>();
-  const long long i = (long long)blockIdx.x * (blockDim.x / G) + coop_grp<G>();
-  if (i >= a.ncells) return;   // the whole group
+  const int base = coop_base<G>();
   double* b = CN::blk();
-  for (int j = l; j < Net::NEQ; j += G) b[Y::YF + j] = a.y[(long long)j * a.ncells + i];
-  for (int p = l; p < Net::NPARAM; p += G) b[Y::PR + p] = a.prm[p * a.ps_plane + i * a.ps_cell];
-  odv_wave_sync();
-  const double t0 = a.t0[i * a.t0_s];
-  const double ts = a.tstop[i * a.ts_s];
-  const double dh = a.dt0[i * a.dt_s];
-  L s;
-  s.active = l < Net::NACT;
-  s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
-  s.dt = dh;
+  long long i = (long long)blockIdx.x * (blockDim.x / G) + coop_grp<G>();
   CoopGroup<G, Net::NACT> g;
-  if ((a.mask && !a.mask[i]) || !(t0 < ts)) {
-    if (l == 0) {
-      if (a.status) a.status[i] = ST_SKIPPED;
-      if (a.dt_out) a.dt_out[i] = dh;
-      if (a.counters) odv_store_counters(a, i, s.cnt);
+  double fl64 = 0.0, fl32 = 0.0;
+  while (i < a.ncells) {
+    for (int j = l; j < Net::NEQ; j += G) b[Y::YF + j] = a.y[(long long)j * a.ncells + i];
+    for (int p = l; p < Net::NPARAM; p += G) b[Y::PR + p] = a.prm[p * a.ps_plane + i * a.ps_cell];
+    odv_wave_sync();
+    const double t0 = a.t0[i * a.t0_s];
+    const double ts = a.tstop[i * a.ts_s];
+    const double dh = a.dt0[i * a.dt_s];
+    L s;
+    s.active = l < Net::NACT;
+    s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
+    s.dt = dh;
+    if ((a.mask && !a.mask[i]) || !(t0 < ts)) {
+      if (l == 0) {
+        if (a.status) a.status[i] = ST_SKIPPED;
+        if (a.dt_out) a.dt_out[i] = dh;
+        if (a.counters) odv_store_counters(a, i, s.cnt);
+      }
+    } else {
+      odv_begin(s, b + Y::YF, b + Y::PR, t0, ts, dh, a.ctl, g);
+      while (!odv_attempt(s, a.ctl, g)) {
+      }
+      double yv[1];
+      odv_result(s, yv);
+      if (l < Net::NACT) a.y[(long long)Net::ACT[l] * a.ncells + i] = yv[0];
+      if (l == 0) {
+        if (a.status) a.status[i] = s.status;
+        if (a.dt_out) a.dt_out[i] = s.dt;
+        if (a.counters) odv_store_counters(a, i, s.cnt);
+        if (a.flops) {
+          double f64, f32;
+          odv_flops<Lane<Cfg<Net, M, RCP, 0, false>>>(s.cnt, f64, f32);
+          fl64 += f64;
+          fl32 += f32;
+        }
+      }
     }
-    return;
+    odv_wave_sync();   // the group's LDS block is reloaded for the next cell
+    unsigned long long nxt = 0;
+    if (l == 0) nxt = atomicAdd(a.queue, 1ull);
+    i = (long long)(unsigned long long)__shfl((long long)nxt, base, 64);
   }
-  odv_begin(s, b + Y::YF, b + Y::PR, t0, ts, dh, a.ctl, g);
-  while (!odv_attempt(s, a.ctl, g)) {
-  }
-  double yv[1];
-  odv_result(s, yv);
-  if (l < Net::NACT) a.y[(long long)Net::ACT[l] * a.ncells + i] = yv[0];
-  if (l == 0) {
-    if (a.status) a.status[i] = s.status;
-    if (a.dt_out) a.dt_out[i] = s.dt;
-    if (a.counters) odv_store_counters(a, i, s.cnt);
-    if (a.flops) {
-      double f64, f32;
-      odv_flops<Lane<Cfg<Net, M, RCP, 0, false>>>(s.cnt, f64, f32);
-      atomicAdd(&a.flops[0], f64);
-      atomicAdd(&a.flops[1], f32);
-    }
+  if (a.flops && l == 0 && (fl64 != 0.0 || fl32 != 0.0)) {
+    atomicAdd(&a.flops[0], fl64);
+    atomicAdd(&a.flops[1], fl32);
   }
 }
 
 template <class Net>
 constexpr int coop_lds_bytes() {
-  return (BLOCK / CoopWidth<Net::NACT>::G) * CoopLayout<Net>::WORDS * 8;
+  return (COOP_BLOCK / CoopWidth<Net::NACT>::G) * CoopLayout<Net>::WORDS * 8;
 }
 
 }  // namespace ODV_VARIANT_NS

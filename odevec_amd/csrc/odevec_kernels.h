@@ -14,11 +14,14 @@
 // Data layout in HBM (struct of arrays, cells fastest, as the reference's
 // Fortran y(nvector,neq)): y[j*ncells + i], params[p*ncells + i].
 // For networks with neq >= 8 (the per-cell kernels) part of a lane's state
-// lives outside its registers, as columns of the workgroup (element r of
-// lane t at [r*BLOCK + t]: consecutive lanes hit consecutive banks or bytes),
-// placed by LdsPolicy per precision. The register file (512 VGPR+AGPR per
-// lane at one wavefront per SIMD) and LDS (640 B per lane at 256 lanes per
-// CU) hold ~2.6 KiB per lane together:
+// lives outside its registers, as columns of its wavefront (element r of
+// lane t at [r*64 + t] in the wavefront's LDS slice: consecutive lanes hit
+// consecutive banks), placed by LdsPolicy per precision. Each wavefront owns
+// its slice, so a workgroup of any size (64..256 lanes) is a set of
+// independent wavefronts and a CU refills one wavefront's slot as soon as
+// that wavefront ends. The register file (512 VGPR+AGPR per lane at one
+// wavefront per SIMD) and LDS (640 B per lane at 256 lanes per CU) hold
+// ~2.6 KiB per lane together:
 //  * mixed form (LU factors FP32, 4 B each): Nordsieck columns 0, 1, 5 and
 //    the error weights stay in registers; LDS holds the rate coefficients
 //    (read by every RHS and Jacobian) and Nordsieck columns 2..4; the
@@ -27,7 +30,7 @@
 //  * FP64 form (LU factors 8 B each): LDS holds the Nordsieck columns
 //    2..MAXORD, the error weights and the previous correction; the rate
 //    coefficients go to a device workspace with the same layout per
-//    workgroup slot (ws[(slot*NRATE + r)*BLOCK + t]; 2 KiB contiguous per
+//    wavefront slot (ws[slot*WS_PER_SLOT + r*64 + t]; 512 B contiguous per
 //    rate and slot, so a slot's rates stay in the L2 of the XCD running it).
 #pragma once
 #include "odevec_integrator.h"
@@ -50,14 +53,17 @@ struct KArgs {
   int* counters;          // optional [8][ncells]
   double* flops;          // optional [2]: FP64, FP32 algorithmic flops (atomic)
   unsigned long long* queue;  // persistent: next cell (set to the number of launched lanes)
-  double* ws;             // rate workspace, LdsPolicy::WS_PER_SLOT doubles per workgroup slot
+  double* ws;             // rate workspace, LdsPolicy::WS_PER_SLOT doubles per wavefront slot
   Controls ctl;
   int group;              // lockstep group size
 };
 
-constexpr int BLOCK = 256;   // lanes per workgroup of the per-cell kernels
+constexpr int WAVE = 64;         // lanes per wavefront: the column stride of a lane's external state
+constexpr int BLOCK = 64;        // default lanes per workgroup of the per-cell kernels
+constexpr int MAX_BLOCK = 256;   // largest workgroup of the per-cell kernels
 
-constexpr int LDS_BYTES = 160 * 1024;   // LDS per CU (and per workgroup) on gfx950
+constexpr int LDS_BYTES = 160 * 1024;   // LDS per CU on gfx950
+constexpr int LDS_WAVES = 4;            // resident wavefronts per CU the LDS placement is sized for
 
 template <class Net, class M>
 struct LdsPolicy {
@@ -66,32 +72,41 @@ struct LdsPolicy {
   // LDS doubles per lane -- mixed: rates, Nordsieck columns 2..4; FP64:
   // Nordsieck columns 2..MAXORD, weights, previous correction
   static constexpr int COLS = FP64 ? (MAXORD - 1) * N + 2 * N : Net::NRATE + 3 * N;
-  static constexpr bool USE = Net::NEQ >= 8 && COLS * 8 * BLOCK <= LDS_BYTES;
-  static constexpr int KS = USE ? BLOCK : 0;
+  static constexpr bool USE = Net::NEQ >= 8 && COLS * 8 * WAVE * LDS_WAVES <= LDS_BYTES;
+  static constexpr int KS = USE ? WAVE : 0;
   static constexpr int ZX = USE ? (FP64 ? MAXORD - 1 : 3) : 0;   // Nordsieck columns from 2 on in LDS
   static constexpr bool KLDS = USE && !FP64;       // rates in LDS (else in the workspace)
   static constexpr bool WX = USE && FP64;          // error weights in LDS
   static constexpr bool ELDS = USE && FP64;        // previous correction in LDS (else in the workspace)
-  static constexpr int KWORDS = KLDS ? Net::NRATE * BLOCK : 1;
-  static constexpr int ZWORDS = ZX > 0 ? ZX * N * BLOCK : 1;
-  static constexpr int WWORDS = WX ? N * BLOCK : 1;
-  static constexpr int EWORDS = ELDS ? N * BLOCK : 1;
-  static constexpr long long WS_RATES = (USE && !KLDS) ? (long long)Net::NRATE * BLOCK : 0;
-  static constexpr long long WS_PER_SLOT = WS_RATES + ((USE && !ELDS) ? (long long)N * BLOCK : 0);
-  static constexpr int WORDS = KWORDS + ZWORDS + WWORDS + EWORDS;   // one workgroup's LDS
+  static constexpr int KWORDS = KLDS ? Net::NRATE * WAVE : 0;
+  static constexpr int ZWORDS = ZX > 0 ? ZX * N * WAVE : 0;
+  static constexpr int WWORDS = WX ? N * WAVE : 0;
+  static constexpr int EWORDS = ELDS ? N * WAVE : 0;
+  static constexpr long long WS_RATES = (USE && !KLDS) ? (long long)Net::NRATE * WAVE : 0;
+  static constexpr long long WS_PER_SLOT = WS_RATES + ((USE && !ELDS) ? (long long)N * WAVE : 0);   // per wavefront
+  static constexpr int WORDS = KWORDS + ZWORDS + WWORDS + EWORDS;   // one wavefront's LDS slice
 };
 
-// Point a lane's external columns at LDS and at its workgroup slot of the
-// rate workspace.
+extern __shared__ double odv_lane_lds[];   // the workgroup's wavefront slices (dynamic size)
+
+// LDS bytes of a workgroup of `block` lanes whose wavefronts take `words` doubles each
+inline constexpr int lane_lds_bytes(int words, int block) { return words * 8 * (block / WAVE); }
+
+// Point a lane's external columns at its wavefront's LDS slice `lds` and at
+// its wavefront's slot of the rate workspace (slot = global wavefront id).
 template <class P, class L>
-__device__ inline void odv_bind_columns(L& s, const KArgs& a, long long slot, double* lk, double* lz, double* lw,
-                                        double* le) {
+__device__ inline void odv_bind_columns(L& s, const KArgs& a, double* lds) {
   if constexpr (P::USE) {
-    double* ws = a.ws + slot * P::WS_PER_SLOT + threadIdx.x;
-    s.kx = P::KLDS ? lk + threadIdx.x : ws;
-    s.zx = lz + threadIdx.x;
-    s.wx = lw + threadIdx.x;
-    s.eox = P::ELDS ? le + threadIdx.x : ws + P::WS_RATES;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const long long slot = (long long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    double* w = lds + (threadIdx.x / WAVE) * P::WORDS + lane;
+    double* ws = a.ws + slot * P::WS_PER_SLOT + lane;
+    s.kx = P::KLDS ? w : ws;
+    s.zx = w + P::KWORDS;
+    s.wx = w + P::KWORDS + P::ZWORDS;
+    s.eox = P::ELDS ? w + P::KWORDS + P::ZWORDS + P::WWORDS : ws + P::WS_RATES;
+  } else {
+    (void)s; (void)a; (void)lds;
   }
 }
 
@@ -169,18 +184,14 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
 }
 
 template <class Net, class M, bool RCP>
-__global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
+__global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
-  __shared__ double lds_k[P::KWORDS];
-  __shared__ double lds_z[P::ZWORDS];
-  __shared__ double lds_w[P::WWORDS];
-  __shared__ double lds_e[P::EWORDS];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double fl64 = 0.0, fl32 = 0.0;
   if (i < a.ncells) {
     L s;
-    odv_bind_columns<P>(s, a, blockIdx.x, lds_k, lds_z, lds_w, lds_e);
+    odv_bind_columns<P>(s, a, odv_lane_lds);
     LaneGroup g;
     bool done = odv_load_begin(a, i, s);
     while (!done) done = odv_attempt(s, a.ctl, g);
@@ -203,7 +214,8 @@ __global__ void __launch_bounds__(BLOCK) odv_k_percell(KArgs a) {
 // single atomic and hands them out in lane order. Every lane's loop ends: each
 // iteration either advances a cell by one attempt (cells are bounded by
 // max_attempts) or, once the wavefront has seen the queue exhausted and holds
-// no cell, leaves. `lds` is the workgroup's LdsPolicy<Net, M>::WORDS doubles.
+// no cell, leaves. `lds` holds the workgroup's wavefront slices,
+// LdsPolicy<Net, M>::WORDS doubles each.
 template <class Net, class M, bool RCP>
 __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   using P = LdsPolicy<Net, M>;
@@ -212,8 +224,7 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   L s;
-  odv_bind_columns<P>(s, a, blockIdx.x, lds, lds + P::KWORDS, lds + P::KWORDS + P::ZWORDS,
-                      lds + P::KWORDS + P::ZWORDS + P::WWORDS);
+  odv_bind_columns<P>(s, a, lds);
   LaneGroup g;
   double fl64 = 0.0, fl32 = 0.0;
   bool have = i < a.ncells;
@@ -264,9 +275,8 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
 }
 
 template <class Net, class M, bool RCP>
-__global__ void __launch_bounds__(BLOCK) odv_k_persistent(KArgs a) {
-  __shared__ double lds[LdsPolicy<Net, M>::WORDS];
-  odv_persistent_run<Net, M, RCP>(a, lds);
+__global__ void __launch_bounds__(MAX_BLOCK) odv_k_persistent(KArgs a) {
+  odv_persistent_run<Net, M, RCP>(a, odv_lane_lds);
 }
 
 // ---------------------------------------------------------------- batches
@@ -316,12 +326,12 @@ __device__ inline void odv_batch_dispatch(int kind, const KArgs& a, double* lds)
 }
 
 template <class... E>
-__global__ void __launch_bounds__(BLOCK) odv_k_batch(const SegDev* segs, const int* nseg) {
-  __shared__ double lds[MaxWords<E...>::WORDS];
+__global__ void __launch_bounds__(MAX_BLOCK) odv_k_batch(const SegDev* segs, const int* nseg) {
   const int n = *nseg;
   for (int k = 0; k < n; ++k) {
     const KArgs a = segs[k].a;
-    odv_batch_dispatch<0, E...>(segs[k].kind, a, lds);
+    // the entry carves its own wavefront slices (at most MaxWords doubles each)
+    odv_batch_dispatch<0, E...>(segs[k].kind, a, odv_lane_lds);
     __syncthreads();   // the next segment's kind may carve LDS differently
   }
 }

@@ -91,28 +91,18 @@ inline hipError_t kinfo(K kern, int block, KernelInfo* out, size_t dyn_lds = 0) 
   return hipSuccess;
 }
 
-// block: threads per workgroup of the per-cell kernels (<= BLOCK, the LDS
-// column stride); grid: workgroups of the persistent kernel (0: one wave of
-// resident workgroups, CUs x occupancy); group: lockstep group size.
+// block: threads per workgroup of the per-cell kernels (a multiple of 64 up
+// to MAX_BLOCK; 0: BLOCK); grid: workgroups of the persistent kernel (0: one
+// wave of resident workgroups, CUs x occupancy); group: lockstep group size.
 template <class Net, class M, bool RCP>
 inline hipError_t plan_t(long long ncells, int mode, int block, int grid, int group, Plan* p) {
   using P = LdsPolicy<Net, M>;
   if (mode == MODE_COOP) {
-    if constexpr (!Net::COOP) return hipErrorInvalidValue;
-    p->block = BLOCK;
-    p->grid = (ncells + BLOCK / CoopWidth<Net::NACT>::G - 1) / (BLOCK / CoopWidth<Net::NACT>::G);
-    p->ws_doubles = 0;
-    return hipSuccess;
-  }
-  if constexpr (!Net::LANE) {
-    return hipErrorInvalidValue;
-  } else {
-    if (mode == MODE_PERCELL) {
-      const int b = block > 0 ? block : BLOCK;
-      p->block = b;
-      p->grid = (ncells + b - 1) / b;
-    } else if (mode == MODE_PERSISTENT) {
-      const int b = block > 0 ? block : BLOCK;
+    if constexpr (!Net::COOP) {
+      return hipErrorInvalidValue;
+    } else {
+      // persistent groups: one wave of resident workgroups (CUs x occupancy)
+      constexpr int GPB = COOP_BLOCK / CoopWidth<Net::NACT>::G;
       long long g = grid;
       if (g <= 0) {
         int dev = 0, ncu = 0, nb = 0;
@@ -120,7 +110,35 @@ inline hipError_t plan_t(long long ncells, int mode, int block, int grid, int gr
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e == hipSuccess)
           e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &nb, reinterpret_cast<const void*>(odv_k_persistent<Net, M, RCP>), b, 0);
+              &nb, reinterpret_cast<const void*>(odv_k_coop<Net, M, RCP>), COOP_BLOCK, coop_lds_bytes<Net>());
+        if (e != hipSuccess) return e;
+        g = (long long)ncu * (nb > 0 ? nb : 1);
+      }
+      const long long need = (ncells + GPB - 1) / GPB;
+      if (g > need) g = need;
+      p->block = COOP_BLOCK;
+      p->grid = g;
+      p->ws_doubles = 0;
+      return hipSuccess;
+    }
+  }
+  if constexpr (!Net::LANE) {
+    return hipErrorInvalidValue;
+  } else {
+    const int b = block > 0 ? block : BLOCK;
+    if (mode != MODE_LOCKSTEP && (b > MAX_BLOCK || b % WAVE != 0)) return hipErrorInvalidValue;
+    if (mode == MODE_PERCELL) {
+      p->block = b;
+      p->grid = (ncells + b - 1) / b;
+    } else if (mode == MODE_PERSISTENT) {
+      long long g = grid;
+      if (g <= 0) {
+        int dev = 0, ncu = 0, nb = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+          e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &nb, reinterpret_cast<const void*>(odv_k_persistent<Net, M, RCP>), b, lane_lds_bytes(P::WORDS, b));
         if (e != hipSuccess) return e;
         g = (long long)ncu * (nb > 0 ? nb : 1);
       }
@@ -133,7 +151,7 @@ inline hipError_t plan_t(long long ncells, int mode, int block, int grid, int gr
       p->block = ((grp + 63) / 64) * 64;
       p->grid = (ncells + grp - 1) / grp;
     }
-    p->ws_doubles = (mode == MODE_LOCKSTEP) ? 0 : p->grid * P::WS_PER_SLOT;
+    p->ws_doubles = (mode == MODE_LOCKSTEP) ? 0 : p->grid * (p->block / WAVE) * P::WS_PER_SLOT;
     return hipSuccess;
   }
 }
@@ -142,12 +160,14 @@ template <class Net, class M, bool RCP>
 inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t st) {
   if (a.ncells <= 0) return hipSuccess;
   if (p.block <= 0 || p.grid <= 0 || p.grid > 0x7fffffffLL) return hipErrorInvalidValue;
-  if (mode != MODE_LOCKSTEP && p.block > BLOCK) return hipErrorInvalidValue;
+  if (mode != MODE_LOCKSTEP && (p.block > MAX_BLOCK || p.block % WAVE != 0)) return hipErrorInvalidValue;
   if (p.ws_doubles > 0 && !a.ws) return hipErrorInvalidValue;
   if (mode == MODE_COOP) {
     if constexpr (Net::COOP) {
-      if (p.block != BLOCK) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((odv_k_coop<Net, M, RCP>), dim3((unsigned)p.grid), dim3(BLOCK), coop_lds_bytes<Net>(), st, a);
+      if (p.block != COOP_BLOCK) return hipErrorInvalidValue;
+      constexpr long long GPB = COOP_BLOCK / CoopWidth<Net::NACT>::G;
+      hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * GPB));
+      hipLaunchKernelGGL((odv_k_coop<Net, M, RCP>), dim3((unsigned)p.grid), dim3(COOP_BLOCK), coop_lds_bytes<Net>(), st, a);
       return hipGetLastError();
     }
     return hipErrorInvalidValue;
@@ -155,11 +175,12 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
   if constexpr (!Net::LANE) {
     return hipErrorInvalidValue;
   } else {
+    const int lds = lane_lds_bytes(LdsPolicy<Net, M>::WORDS, p.block);
     if (mode == MODE_PERCELL) {
-      hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
+      hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
     } else if (mode == MODE_PERSISTENT) {
       hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * p.block));
-      hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
+      hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
     } else {
       hipLaunchKernelGGL((odv_k_lockstep<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
     }
@@ -182,14 +203,17 @@ hipError_t launch_net(const KArgs& a, int mode, int prec, const Plan& p, hipStre
 template <class Net, class M, bool RCP>
 inline hipError_t info_t(int mode, int block, KernelInfo* out) {
   if (mode == MODE_COOP) {
-    if constexpr (Net::COOP) return kinfo(odv_k_coop<Net, M, RCP>, BLOCK, out, coop_lds_bytes<Net>());
+    if constexpr (Net::COOP) return kinfo(odv_k_coop<Net, M, RCP>, COOP_BLOCK, out, coop_lds_bytes<Net>());
     return hipErrorInvalidValue;
   }
   if constexpr (!Net::LANE) {
     return hipErrorInvalidValue;
   } else {
-    if (mode == MODE_PERCELL) return kinfo(odv_k_percell<Net, M, RCP>, block > 0 ? block : 256, out);
-    if (mode == MODE_PERSISTENT) return kinfo(odv_k_persistent<Net, M, RCP>, block > 0 ? block : 256, out);
+    const int b = block > 0 ? block : BLOCK;
+    if (mode != MODE_LOCKSTEP && (b > MAX_BLOCK || b % WAVE != 0)) return hipErrorInvalidValue;
+    const int lds = lane_lds_bytes(LdsPolicy<Net, M>::WORDS, b);
+    if (mode == MODE_PERCELL) return kinfo(odv_k_percell<Net, M, RCP>, b, out, lds);
+    if (mode == MODE_PERSISTENT) return kinfo(odv_k_persistent<Net, M, RCP>, b, out, lds);
     return kinfo(odv_k_lockstep<Net, M, RCP>, block > 0 ? block : 64, out);
   }
 }
@@ -207,14 +231,15 @@ struct Bundle {
 
   static hipError_t plan(long long ncells, int block, int grid, Plan* p) {
     const int b = block > 0 ? block : BLOCK;
-    if (b > BLOCK || b % 64 != 0) return hipErrorInvalidValue;
+    if (b > MAX_BLOCK || b % WAVE != 0) return hipErrorInvalidValue;
     long long g = grid;
     if (g <= 0) {
       int dev = 0, ncu = 0, nb = 0;
       hipError_t e = hipGetDevice(&dev);
       if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
       if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(odv_k_batch<E...>), b, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(odv_k_batch<E...>), b,
+                                                         lane_lds_bytes(MaxWords<E...>::WORDS, b));
       if (e != hipSuccess) return e;
       g = (long long)ncu * (nb > 0 ? nb : 1);
     }
@@ -222,21 +247,27 @@ struct Bundle {
     if (g > need) g = need > 0 ? need : 1;
     p->block = b;
     p->grid = g;
-    p->ws_doubles = g * WS;
+    p->ws_doubles = g * (b / WAVE) * WS;
     return hipSuccess;
   }
 
   static hipError_t launch(void* segs, const int* nseg, int maxseg, const Plan& p, hipStream_t st) {
-    if (p.block <= 0 || p.block > BLOCK || p.grid <= 0 || p.grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (p.block <= 0 || p.block > MAX_BLOCK || p.block % WAVE != 0 || p.grid <= 0 || p.grid > 0x7fffffffLL)
+      return hipErrorInvalidValue;
     SegDev* sd = static_cast<SegDev*>(segs);
     hipLaunchKernelGGL(k_batch_reset<0>, dim3((unsigned)((maxseg + 255) / 256)), dim3(256), 0, st, sd, nseg,
                        (unsigned long long)(p.grid * p.block));
-    hipLaunchKernelGGL((odv_k_batch<E...>), dim3((unsigned)p.grid), dim3(p.block), 0, st,
+    hipLaunchKernelGGL((odv_k_batch<E...>), dim3((unsigned)p.grid), dim3(p.block),
+                       lane_lds_bytes(MaxWords<E...>::WORDS, p.block), st,
                        static_cast<const SegDev*>(sd), nseg);
     return hipGetLastError();
   }
 
-  static hipError_t info(int block, KernelInfo* out) { return kinfo(odv_k_batch<E...>, block > 0 ? block : BLOCK, out); }
+  static hipError_t info(int block, KernelInfo* out) {
+    const int b = block > 0 ? block : BLOCK;
+    if (b > MAX_BLOCK || b % WAVE != 0) return hipErrorInvalidValue;
+    return kinfo(odv_k_batch<E...>, b, out, lane_lds_bytes(MaxWords<E...>::WORDS, b));
+  }
 };
 
 }  // namespace ODV_VARIANT_NS

@@ -163,6 +163,24 @@ def test_fortran_module_driver_equals_oracle(oracle):
     assert k * 32 == len(rows)
 
 
+@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_workgroup_sizes_bitwise(oracle, block, precision, mode):
+    """Each wavefront owns an LDS slice and a workspace slot: any workgroup
+    size gives the same cells' results (FP64: history, weights and previous
+    correction in LDS, rates in the workspace; mixed: rates and history in
+    LDS, previous correction in the workspace)."""
+    w = W.primordial(1000, seed=11)
+    nid = "PRIMORDIAL/CMK/CSC+REPRO"
+    y, st, dto, stats = gpu_solve(nid, w, precision=precision, mode=mode, block=block, grid=3 if mode else 0)
+    y0, st0, dto0, _ = gpu_solve(nid, w, precision=precision, mode=_lib.PERCELL, block=64)
+    assert np.array_equal(st, st0) and np.array_equal(y, y0) and np.array_equal(dto, dto0)
+    if precision == 0:
+        yr, sr, dr, cr = oracle_cells(oracle, nid, w)
+        assert np.array_equal(st, sr) and np.array_equal(y, yr) and np.array_equal(dto, dr)
+
+
 @pytest.mark.parametrize("n", [1, 63, 257, 1000])
 @pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
 def test_ragged_sizes_and_skips(oracle, n, mode):
