@@ -12,12 +12,14 @@ work-compacting kernel is the headline value; the plain FP64 kernel (one
 lane per cell, no compaction) runs the same steps for the ratio.
 
 Also on the line (skipped with --no-batch): the batching form (one launch
-over a 2,097,152-cell ROBER/OREGO/PRIMORDIAL batch) and the serving form
-(continuous batching under Poisson arrivals: p50/p99 latency and
-cell-solves/s, all four networks at 1 request/s and the three small ones at
-40/s), and (skipped with --no-cpu) the CPU baseline.
+over a 2,097,152-cell ROBER/OREGO/PRIMORDIAL batch), SYNTH64 (64 species)
+through the cooperative kernel, and the serving form (continuous batching
+under Poisson arrivals: p50/p99 latency and cell-solves/s, all four networks
+at 1 request/s and the three small ones at 40/s), and (skipped with
+--no-cpu) the CPU baseline. `--only batching|synth64` runs that part alone
+and prints its object (for profiling one kernel).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batch] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batch] [--no-cpu] [--only PART]
 """
 import argparse
 import json
@@ -45,6 +47,7 @@ def parse():
     p.add_argument("--cells", type=int, default=NCELLS)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--only", choices=["batching", "synth64"], default=None)
     return p.parse_args()
 
 
@@ -62,6 +65,7 @@ class Runner:
         self.y = torch.empty_like(self.y0)
         self.ts = torch.tensor(w["t_stop"], dtype=torch.float64, device=dev)
         self.dt0 = torch.tensor(w["dt0"], dtype=torch.float64, device=dev)
+        self.prm = torch.tensor(w["params"], dtype=torch.float64, device=dev) if w.get("params") is not None else None
         self.st = torch.zeros(n, dtype=torch.int32, device=dev)
         self.flops = torch.zeros(2, dtype=torch.float64, device=dev)
         self.ode = Odevec(w["network"], n, precision=precision, mode=mode)
@@ -76,6 +80,7 @@ class Runner:
         stream = self.torch.cuda.current_stream().cuda_stream
         assert stream, "a created stream is required (the null handle selects the solver's own stream)"
         self.ode.solve_cells_async(self.n, self.ts.data_ptr(), self.y.data_ptr(), dt0_ptr=self.dt0.data_ptr(),
+                                   params_ptr=self.prm.data_ptr() if self.prm is not None else None,
                                    status_ptr=self.st.data_ptr(),
                                    flops_ptr=self.flops.data_ptr() if count_flops else None, stream=stream)
 
@@ -120,6 +125,8 @@ def batching(steps, warmup):
     nets = [w["network"] for w in segs]
     t = A.lookup(nets)
     if t is None:
+        print("batching: no tuned entry for %s in odevec_amd/tuned.json; all-FP64 at 256 threads"
+              % A.cache_key(nets), file=sys.stderr)
         t = {"bundle": "MIX3_FP64", "block": 256, "precision": {n: 0 for n in nets}, "per_cell_seconds": {},
              "order": nets}
     d = A.cost_order(A._device_segments(segs, [t["precision"][n] for n in nets]),
@@ -130,7 +137,25 @@ def batching(steps, warmup):
             "workload": "2,097,152 cells, 40/30/30 ROBER/OREGO/PRIMORDIAL (seed 4), t_stop logU over 6 decades "
                         "per network, one persistent launch per step, hipGraph replay",
             "bundle": t["bundle"], "block": t["block"], "precision": t["precision"], "cells_ok": ok,
-            "tuned": "per_cell_seconds" in t and bool(t["per_cell_seconds"])}
+            "tuned": "per_cell_seconds" in t and bool(t["per_cell_seconds"]), "device": A.device_key()}
+
+
+def synth64(steps, warmup, cells=65536):
+    """SYNTH64 (64 species, 300 reactions, FF ordering, CSC; seed 7) through
+    the cooperative kernel (one wavefront per cell, species on lanes), mixed
+    and FP64: n_tot logU[1, 1e6], T logU[100, 1e4] K, abundances
+    logU[1e-6, 1] x n_tot, integrated from 0 to 1e6 s at rtol 1e-6."""
+    from odevec_amd import _lib as L
+    from odevec_amd import workloads as W
+    w = W.synth64(cells, seed=7)
+    out = {"workload": "SYNTH64 %d cells (seed 7), t_stop 1e6 s, rtol %g, cooperative kernel" % (cells, w["rtol"])}
+    for name, prec in (("mixed", 1), ("fp64", 0)):
+        r = Runner(w, precision=prec, mode=L.COOP)
+        wall, ms = r.timed(steps, warmup)
+        out[name] = {"value": steps * cells / wall, "unit": "cell-solves/s", "kernel_ms": ms,
+                     "cells_ok": int((r.st == 0).sum())}
+        r.ode.CloseOdevec()
+    return out
 
 
 def serving(rate, seconds, networks=None):
@@ -195,6 +220,12 @@ def main():
     # reads as "the handle's own stream"
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
+    if a.only == "batching":
+        print(json.dumps({"batching": batching(a.steps, a.warmup)}))
+        return
+    if a.only == "synth64":
+        print(json.dumps({"synth64": synth64(a.steps, a.warmup)}))
+        return
     w = W.primordial(a.cells, seed=3)
     mixed = Runner(w, precision=1, mode=2)
     fp64 = Runner(w, precision=0, mode=0)
@@ -242,6 +273,7 @@ def main():
     }
     if not a.no_batch:
         res["batching"] = batching(a.steps, a.warmup)
+        res["synth64"] = synth64(a.steps, a.warmup)
         # all four networks (SYNTH64's 64-species cells dominate the cost), at
         # a load the device sustains; and the three small networks at 40/s
         sv = serving(1.0, 15.0)

@@ -104,6 +104,18 @@ def candidate_bundles(networks: List[str]) -> List[str]:
     return out
 
 
+def device_key() -> str:
+    """The device a tuning result belongs to: its ISA and CU count (the
+    marketing name varies with the driver's name tables)."""
+    import torch
+    p = torch.cuda.get_device_properties(0)
+    return "%s/%dCU" % (str(getattr(p, "gcnArchName", "gpu")).split(":")[0], p.multi_processor_count)
+
+
+def cache_key(networks: List[str]) -> str:
+    return "%s|%s" % (device_key(), ",".join(sorted(set(networks))))
+
+
 def tune(segs: List[dict], sample: float = 0.25, seed: int = 0, verbose: bool = True, cache: str = CACHE,
          bundles: List[str] = None) -> dict:
     """Fastest (bundle, precision per network, block) for the batch `segs`.
@@ -144,9 +156,9 @@ def tune(segs: List[dict], sample: float = 0.25, seed: int = 0, verbose: bool = 
                           flush=True)
     best = min(results, key=lambda r: r["seconds"])
     best = dict(best, order=sorted(per_cell, key=lambda n: -per_cell[n]), per_cell_seconds=per_cell,
-                sample=sample, device=torch.cuda.get_device_name(0), all=results)
+                sample=sample, device=device_key(), device_name=torch.cuda.get_device_name(0), all=results)
     if cache:
-        key = "%s|%s" % (best["device"], ",".join(sorted(set(nets))))
+        key = cache_key(nets)
         db = {}
         if os.path.exists(cache):
             try:
@@ -161,14 +173,13 @@ def tune(segs: List[dict], sample: float = 0.25, seed: int = 0, verbose: bool = 
 
 def lookup(networks: List[str], cache: str = CACHE):
     """A cached tuning result for this device and set of networks, or None."""
-    import torch
     if not os.path.exists(cache):
         return None
     try:
         db = json.load(open(cache))
     except (OSError, ValueError):
         return None
-    return db.get("%s|%s" % (torch.cuda.get_device_name(0), ",".join(sorted(set(networks)))))
+    return db.get(cache_key(networks))
 
 
 def main(argv=None) -> int:

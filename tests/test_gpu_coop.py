@@ -3,7 +3,8 @@ one species per lane).
 
 * the +REPRO build equals the oracle bit for bit (states, status, last step,
   counters) for the CSC networks: ROBER, OREGO (groups of 4 lanes),
-  PRIMORDIAL CMK and FF (16), SYNTH64 (64 species, one wavefront per cell);
+  PRIMORDIAL CMK and FF (16), SYNTH64 (64 species, one wavefront per cell),
+  with a group per cell and with one workgroup working through the queue;
 * the default build and the mixed form: every cell finishes and the states
   agree with the oracle's FP64 solution to a bound set by the tolerance.
 """
@@ -20,11 +21,11 @@ def say(*a):
     print(*a, flush=True)
 
 
-def coop_solve(nid, w, precision=_lib.FP64):
+def coop_solve(nid, w, precision=_lib.FP64, grid=0):
     from odevec_amd.api import Odevec
     n = w["y"].shape[1]
-    say("GPU coop solve", nid, "cells", n, "precision", precision)
-    ode = Odevec(nid, max(n, 1), precision=precision, mode=_lib.COOP)
+    say("GPU coop solve", nid, "cells", n, "precision", precision, "grid", grid)
+    ode = Odevec(nid, max(n, 1), precision=precision, mode=_lib.COOP, grid=grid)
     ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
     y = np.array(w["y"], order="C", copy=True)
     st, dto, stats = ode.solve_cells(y, w["t_stop"], t0=w["t0"], dt0=w["dt0"], params=w["params"], stats=True,
@@ -60,12 +61,15 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("grid", [0, 1])
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_coop_repro_bitwise(oracle, case):
+def test_coop_repro_bitwise(oracle, case, grid):
+    """grid 0: a group per cell where the device holds them all; grid 1: one
+    workgroup whose groups take every cell from the queue."""
     w = CASES[case]()
     w["t_stop"] = np.array(np.broadcast_to(w["t_stop"], (w["y"].shape[1],)))
     w["t_stop"][::9] = 0.0                      # skipped cells
-    y, st, dto, stats = coop_solve(w["network"] + "+REPRO", w)
+    y, st, dto, stats = coop_solve(w["network"] + "+REPRO", w, grid=grid)
     yr, sr, dr, cr = oracle_cells(oracle, w["network"], w)
     assert np.array_equal(st, sr), case
     assert np.array_equal(y, yr), case
