@@ -148,6 +148,26 @@ int32_t odevec_solve_cells_async(odevec_t* h, int64_t ncells, const double* d_t0
 /* Replaces CloseOdevec(this) (src/odevec.F90:1058-1074). */
 int32_t odevec_destroy(odevec_t* h);
 
+/* Warm restart --------------------------------------------------------------
+ * No reference counterpart: the reference's SolveODE cold-starts every call
+ * (src/odevec.F90:170-182: order 1, fresh Jacobian, initial step size). With
+ * a history attached, every per-cell solve on the handle (sync or async,
+ * per-cell or persistent mode) saves each successfully solved cell's
+ * integrator state (Nordsieck history, order, step size, LU factors, error
+ * weights, rates, Jacobian-refresh counters) into the history, and a later
+ * call continues a cell from it when the call starts at the saved t_stop on
+ * the state it returned (t0 and every species and parameter bitwise equal;
+ * any other cell starts cold). A solve to t1 continued to t2 then takes the
+ * same steps as one solve to t2 and returns the same result bit for bit.
+ *
+ * odevec_history_planes: doubles per cell of a history for `network`.
+ * odevec_set_history: attach d_hist, a device array of planes x ncells
+ * doubles (plane p of cell i at d_hist[p*ncells + i]) that the caller
+ * allocates zero-filled and keeps; every solve must then cover exactly
+ * ncells cells. NULL detaches. Calls sharing one history must not overlap. */
+int32_t odevec_history_planes(const char* network, int64_t* planes);
+int32_t odevec_set_history(odevec_t* h, double* d_hist, int64_t ncells);
+
 /* Batches ------------------------------------------------------------------
  * No reference counterpart: the reference solves one network per build and
  * call. A bundle is one persistent kernel compiled for several (network,

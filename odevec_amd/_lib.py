@@ -84,6 +84,8 @@ def lib():
         "odevec_network_name": (C.c_char_p, [_I32]),
         "odevec_network_info": (_I32, [C.c_char_p, C.POINTER(NetInfo)]),
         "odevec_kernel_info": (_I32, [C.c_char_p, _I32, _I32, _I32, C.POINTER(KernelInfo)]),
+        "odevec_history_planes": (_I32, [C.c_char_p, C.POINTER(_I64)]),
+        "odevec_set_history": (_I32, [_VP, _VP, _I64]),
         "odevec_default_opts": (None, [C.POINTER(Opts)]),
         "odevec_create": (_I32, [C.c_char_p, _I64, C.POINTER(Opts), C.POINTER(_VP)]),
         "odevec_set_tolerances": (_I32, [_VP, C.c_double, C.c_double]),
@@ -114,7 +116,8 @@ EXPORTED = [
     "odevec_abi_version", "odevec_last_error", "odevec_network_count", "odevec_network_name",
     "odevec_network_info", "odevec_kernel_info", "odevec_default_opts", "odevec_create",
     "odevec_set_tolerances", "odevec_set_dt_min", "odevec_set_check_negatives", "odevec_solve",
-    "odevec_solve_cells", "odevec_solve_cells_async", "odevec_destroy", "odevec_bundle_count",
+    "odevec_solve_cells", "odevec_solve_cells_async", "odevec_destroy", "odevec_history_planes",
+    "odevec_set_history", "odevec_bundle_count",
     "odevec_bundle_name", "odevec_bundle_entry", "odevec_batch_create", "odevec_batch_solve_async",
     "odevec_batch_kernel_info", "odevec_batch_destroy",
 ]
@@ -133,6 +136,13 @@ def check(rc: int) -> int:
 def networks():
     L = lib()
     return [L.odevec_network_name(i).decode() for i in range(L.odevec_network_count())]
+
+
+def history_planes(name: str) -> int:
+    """Doubles per cell of a warm-restart history for `name`."""
+    v = _I64(0)
+    check(lib().odevec_history_planes(name.encode(), C.byref(v)))
+    return int(v.value)
 
 
 def network_info(name: str) -> dict:

@@ -39,6 +39,7 @@ class Odevec:
         self.nvector = int(nvector)
         self._opts = L.Opts(precision, mode, group, block, grid, max_attempts, 0, 1 if abort_on_error else 0)
         self._h = None
+        self._hist = None
         info = L.network_info(network)
         self.neq = info["neq"]
         self.nnz = info["nnz"]
@@ -65,6 +66,24 @@ class Odevec:
         if self._h is not None:
             L.lib().odevec_destroy(self._h)
             self._h = None
+        self._hist = None
+
+    # -- warm restart (no reference counterpart: its SolveODE cold-starts every call)
+    def enable_history(self, ncells: int):
+        """Attach a zero-filled device history for `ncells` cells (a torch
+        tensor this object keeps): later per-cell solves over exactly those
+        cells continue each cell from where the previous call left it, when
+        the call starts at that t_stop on the state it returned."""
+        import torch
+        planes = L.history_planes(self.network)
+        self._hist = torch.zeros(planes * int(ncells), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()   # the fill runs on torch's stream, the solves on others
+        L.check(L.lib().odevec_set_history(self._h, C.c_void_p(self._hist.data_ptr()), int(ncells)))
+        return self._hist
+
+    def disable_history(self):
+        L.check(L.lib().odevec_set_history(self._h, None, 0))
+        self._hist = None
 
     def SolveODE(self, time: float, dt: float, t_stop: float, y: Optional[np.ndarray] = None,
                  GetRHS=None, GetJac=None, GetLU=None, Mask: Optional[np.ndarray] = None):

@@ -87,6 +87,8 @@ struct odevec_s {
   int64_t ws_cap = 0;                // doubles
   int qslot = 0;
   int nscal = 0;
+  double* d_hist = nullptr;          // caller's warm-restart history (odevec_set_history), or null
+  int64_t hist_cells = 0;
   std::vector<int> h_status;
   std::vector<int> h_cnt;
 };
@@ -158,6 +160,12 @@ int launch(odevec_s* h, const Req& r) {
   a.dt_out = r.dt_out;
   a.counters = r.counters;
   a.flops = r.flops;
+  if (h->d_hist) {
+    if (r.ncells != h->hist_cells)
+      return fail(ODEVEC_EINVAL, "the solve covers " + std::to_string(r.ncells) + " cells; the history holds " +
+                                     std::to_string(h->hist_cells));
+    a.hist = h->d_hist;
+  }
   h->qslot = (h->qslot + 1) % 64;
   a.queue = h->d_queue + h->qslot;
   a.ctl.rtol = h->rtol;
@@ -244,6 +252,30 @@ int32_t odevec_network_info(const char* network, odevec_network_info_t* info) {
   info->rate_kind = e->rate_kind; info->tgas_index = e->tgas;
   info->flops_rhs = e->flops_rhs; info->flops_jac = e->flops_jac; info->flops_lu = e->flops_lu;
   info->flops_solve = e->flops_solve; info->flops_lu_r = e->flops_lu_r; info->flops_solve_r = e->flops_solve_r;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_history_planes(const char* network, int64_t* planes) {
+  const odv::NetEntry* e = find_net(network);
+  if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown network ") + (network ? network : "(null)"));
+  if (!planes) return fail(ODEVEC_EINVAL, "planes is NULL");
+  if (!e->lane_ok) return fail(ODEVEC_EINVAL, std::string(e->id) + ": warm restart needs the per-lane kernels");
+  *planes = e->hist_planes;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_set_history(odevec_t* h, double* d_hist, int64_t ncells) {
+  if (!h) return fail(ODEVEC_EINVAL, "NULL handle");
+  if (!d_hist) {
+    h->d_hist = nullptr;
+    h->hist_cells = 0;
+    return ODEVEC_OK;
+  }
+  if (ncells <= 0) return fail(ODEVEC_EINVAL, "ncells must be > 0");
+  if (h->opts.mode != ODEVEC_PERCELL && h->opts.mode != ODEVEC_PERSISTENT)
+    return fail(ODEVEC_EINVAL, "warm restart needs the per-cell or persistent mode (one step control per cell)");
+  h->d_hist = d_hist;
+  h->hist_cells = ncells;
   return ODEVEC_OK;
 }
 

@@ -54,6 +54,7 @@ struct KArgs {
   double* flops;          // optional [2]: FP64, FP32 algorithmic flops (atomic)
   unsigned long long* queue;  // persistent: next cell (set to the number of launched lanes)
   double* ws;             // rate workspace, LdsPolicy::WS_PER_SLOT doubles per wavefront slot
+  double* hist;           // optional warm-restart history, Hist<Net>::PLANES planes of ncells doubles
   Controls ctl;
   int group;              // lockstep group size
 };
@@ -143,9 +144,132 @@ __device__ inline void odv_store(const KArgs& a, long long i, const L& s, const 
   if (a.counters) odv_store_counters(a, i, s.cnt);
 }
 
+// Warm restart (no counterpart in the reference, whose SolveODE cold-starts
+// every call: src/odevec.F90:170-182). After a successful solve a cell's
+// whole integrator state goes to `hist` (plane p of cell i at
+// hist[p*ncells + i]); the next call on that cell continues the integration
+// from it when the call starts where the last one stopped (t0 equal to the
+// saved t_stop) on the state it returned (y and parameters bitwise equal to
+// the saved ones), else the cell starts cold. Steps are never clamped to
+// t_stop (the last step overshoots and the result is interpolated), so a
+// solve to t1 continued to t2 takes exactly the steps of one solve to t2.
+template <class Net>
+struct Hist {
+  static constexpr int N = Net::NACT, NF = Net::NEQ, NP = Net::NPARAM;
+  static constexpr int VALID = 0, TSTOP = 1, T = 2, DT = 3, RATE = 4, JCH = 5, CERR = 6;
+  static constexpr int INTS = 7, NINT = 7;   // q, steps since Jacobian, lju, failed tests, conv fails, flags, fz
+  static constexpr int Y = INTS + NINT;      // the state returned (all NEQ species)
+  static constexpr int P = Y + NF;           // the parameters the rates came from
+  static constexpr int Z = P + NP;           // Nordsieck history, column-major (c*N + j)
+  static constexpr int W = Z + NCOL * N;     // error weights
+  static constexpr int EO = W + N;           // previous accepted correction
+  static constexpr int A = EO + N;           // P's LU factors
+  static constexpr int K = A + Net::NNZ;     // rate coefficients
+  static constexpr int PLANES = K + Net::NRATE;
+};
+
+template <class L>
+__device__ inline bool odv_hist_resume(const KArgs& a, long long i, L& s, const double* y, const double* prm,
+                                       double t0, double ts) {
+  using H = Hist<typename L::Net>;
+  using M = typename L::M;
+  const long long n = a.ncells;
+  const double* h = a.hist + i;
+  if (h[H::VALID * n] != 1.0 || h[H::TSTOP * n] != t0) return false;
+  bool same = true;
+#pragma unroll
+  for (int j = 0; j < H::NF; ++j) same = same && (h[(H::Y + j) * n] == y[j]);
+#pragma unroll
+  for (int p = 0; p < H::NP; ++p) same = same && (h[(H::P + p) * n] == prm[p]);
+  if (!same) return false;
+  s.t = h[H::T * n];
+  s.dt = h[H::DT * n];
+  s.rate = h[H::RATE * n];
+  s.jchange = h[H::JCH * n];
+  s.cerr = h[H::CERR * n];
+  odv_set_order(s, (int)h[H::INTS * n]);
+  s.swu = (int)h[(H::INTS + 1) * n];
+  s.lju = (int)h[(H::INTS + 2) * n];
+  s.fet = (int)h[(H::INTS + 3) * n];
+  s.fc = (int)h[(H::INTS + 4) * n];
+  const int fl = (int)h[(H::INTS + 5) * n];
+  s.first = (fl & 1) != 0;
+  s.upd_jac = (fl & 2) != 0;
+  s.neglect_up = (fl & 4) != 0;
+  s.fz = (int)h[(H::INTS + 6) * n];
+  s.status = ST_RUNNING;
+  s.tstop = ts;
+#pragma unroll
+  for (int c = 0; c < NCOL; ++c)
+#pragma unroll
+    for (int j = 0; j < H::N; ++j) s.set_z(c, j, h[(H::Z + c * H::N + j) * n]);
+#pragma unroll
+  for (int j = 0; j < H::N; ++j) {
+    s.set_w(j, h[(H::W + j) * n]);
+    s.set_eo(j, h[(H::EO + j) * n]);
+    s.en[j] = 0.0;
+  }
+#pragma unroll
+  for (int e = 0; e < L::Net::NNZ; ++e) s.a[e] = (M)h[(H::A + e) * n];
+#pragma unroll
+  for (int r = 0; r < L::Net::NRATE; ++r) {
+    const double v = h[(H::K + r) * n];
+    if (L::KS == 0) s.kl[r] = v;
+    else s.kx[r * L::KS] = v;
+  }
+  return true;
+}
+
+// after odv_store: a.y holds the state returned to the caller
+template <class L>
+__device__ inline void odv_hist_save(const KArgs& a, long long i, const L& s) {
+  using H = Hist<typename L::Net>;
+  const long long n = a.ncells;
+  double* h = a.hist + i;
+  if (s.status == ST_SKIPPED) return;   // nothing integrated: the saved state stays usable
+  if (s.status != ST_OK) {
+    h[H::VALID * n] = 0.0;
+    return;
+  }
+  h[H::TSTOP * n] = s.tstop;
+  h[H::T * n] = s.t;
+  h[H::DT * n] = s.dt;
+  h[H::RATE * n] = s.rate;
+  h[H::JCH * n] = s.jchange;
+  h[H::CERR * n] = s.cerr;
+  h[H::INTS * n] = (double)s.q;
+  h[(H::INTS + 1) * n] = (double)s.swu;
+  h[(H::INTS + 2) * n] = (double)s.lju;
+  h[(H::INTS + 3) * n] = (double)s.fet;
+  h[(H::INTS + 4) * n] = (double)s.fc;
+  h[(H::INTS + 5) * n] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0));
+  h[(H::INTS + 6) * n] = (double)s.fz;
+#pragma unroll
+  for (int j = 0; j < H::NF; ++j) h[(H::Y + j) * n] = a.y[(long long)j * n + i];
+#pragma unroll
+  for (int p = 0; p < H::NP; ++p) h[(H::P + p) * n] = a.prm[p * a.ps_plane + i * a.ps_cell];
+#pragma unroll
+  for (int c = 0; c < NCOL; ++c)
+#pragma unroll
+    for (int j = 0; j < H::N; ++j) h[(H::Z + c * H::N + j) * n] = s.z(c, j);
+#pragma unroll
+  for (int j = 0; j < H::N; ++j) {
+    h[(H::W + j) * n] = s.w(j);
+    h[(H::EO + j) * n] = s.eo(j);
+  }
+#pragma unroll
+  for (int e = 0; e < L::Net::NNZ; ++e) h[(H::A + e) * n] = (double)s.a[e];
+#pragma unroll
+  for (int r = 0; r < L::Net::NRATE; ++r) h[(H::K + r) * n] = s.kptr()[r * (L::KS > 0 ? L::KS : 1)];
+  h[H::VALID * n] = 1.0;
+}
+
 // Load cell i into the lane and start its solve; returns true if there is
 // nothing to integrate (masked out or t_stop <= t0; y is then left as is).
-template <class L>
+// HIST: the kernel variant that reads and writes a warm-restart history (a
+// separate instantiation: the history code raises the register pressure of
+// the refill path, which the kernels without it should not pay).
+template <class L, bool HIST = false>
 __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
   using Net = typename L::Net;
   constexpr int NF = Net::NEQ;
@@ -165,12 +289,19 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
     s.status = ST_SKIPPED;
     return true;
   }
+  if constexpr (HIST) {
+    if (odv_hist_resume(a, i, s, y, prm, t0, ts)) {
+      if (s.t < s.tstop) return false;
+      s.status = ST_OK;   // the last step already passed the new t_stop: interpolate
+      return true;
+    }
+  }
   LaneGroup g;
   odv_begin(s, y, prm, t0, ts, dh, a.ctl, g);
   return false;
 }
 
-template <class L>
+template <class L, bool HIST = false>
 __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   if (s.status == ST_SKIPPED) {
     if (a.status) a.status[i] = s.status;
@@ -181,9 +312,10 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   double y[L::N];
   odv_result(s, y);
   odv_store(a, i, s, y);
+  if constexpr (HIST) odv_hist_save(a, i, s);
 }
 
-template <class Net, class M, bool RCP>
+template <class Net, class M, bool RCP, bool HIST = false>
 __global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
@@ -193,9 +325,9 @@ __global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
     L s;
     odv_bind_columns<P>(s, a, odv_lane_lds);
     LaneGroup g;
-    bool done = odv_load_begin(a, i, s);
+    bool done = odv_load_begin<L, HIST>(a, i, s);
     while (!done) done = odv_attempt(s, a.ctl, g);
-    odv_finish(a, i, s);
+    odv_finish<L, HIST>(a, i, s);
     if (a.flops && s.status != ST_SKIPPED) odv_flops<L>(s.cnt, fl64, fl32);
   }
   if (a.flops) {
@@ -216,7 +348,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
 // max_attempts) or, once the wavefront has seen the queue exhausted and holds
 // no cell, leaves. `lds` holds the workgroup's wavefront slices,
 // LdsPolicy<Net, M>::WORDS doubles each.
-template <class Net, class M, bool RCP>
+template <class Net, class M, bool RCP, bool HIST = false>
 __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
@@ -230,11 +362,11 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   bool have = i < a.ncells;
   bool done = true;
   bool exhausted = nlanes >= a.ncells;   // wave-uniform: no cell left to hand out
-  if (have) done = odv_load_begin(a, i, s);
+  if (have) done = odv_load_begin<L, HIST>(a, i, s);
   while (true) {
     const bool need = !have || done;
     if (have && done) {
-      odv_finish(a, i, s);
+      odv_finish<L, HIST>(a, i, s);
       if (a.flops && s.status != ST_SKIPPED) {
         double f64, f32;
         odv_flops<L>(s.cnt, f64, f32);
@@ -257,7 +389,7 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
         if (ni < a.ncells) {
           i = ni;
           have = true;
-          done = odv_load_begin(a, i, s);
+          done = odv_load_begin<L, HIST>(a, i, s);
         }
       }
     }
@@ -274,9 +406,9 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   }
 }
 
-template <class Net, class M, bool RCP>
+template <class Net, class M, bool RCP, bool HIST = false>
 __global__ void __launch_bounds__(MAX_BLOCK) odv_k_persistent(KArgs a) {
-  odv_persistent_run<Net, M, RCP>(a, odv_lane_lds);
+  odv_persistent_run<Net, M, RCP, HIST>(a, odv_lane_lds);
 }
 
 // ---------------------------------------------------------------- batches

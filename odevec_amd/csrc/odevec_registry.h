@@ -33,6 +33,7 @@ struct NetEntry {
   int neq, nnz, nrate, nparam, rate_kind, tgas;
   int flops_rhs, flops_jac, flops_lu, flops_solve, flops_lu_r, flops_solve_r;
   int lane_ok, coop_ok;   // per-lane modes (percell, lockstep, persistent) / cooperative mode compiled
+  int hist_planes;        // warm-restart history planes per cell (per-lane modes)
   hipError_t (*plan)(long long ncells, int mode, int prec, int block, int grid, int group, Plan* out);
   hipError_t (*launch)(const KArgs& a, int mode, int prec, const Plan& p, hipStream_t st);
   hipError_t (*info)(int mode, int prec, int block, KernelInfo* out);
@@ -176,11 +177,19 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
     return hipErrorInvalidValue;
   } else {
     const int lds = lane_lds_bytes(LdsPolicy<Net, M>::WORDS, p.block);
+    if (a.hist && mode != MODE_PERCELL && mode != MODE_PERSISTENT) return hipErrorInvalidValue;
     if (mode == MODE_PERCELL) {
-      hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
+      if (a.hist)
+        hipLaunchKernelGGL((odv_k_percell<Net, M, RCP, true>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
+      else
+        hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
     } else if (mode == MODE_PERSISTENT) {
       hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * p.block));
-      hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
+      if (a.hist)
+        hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP, true>), dim3((unsigned)p.grid), dim3(p.block), lds, st,
+                           a);
+      else
+        hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
     } else {
       hipLaunchKernelGGL((odv_k_lockstep<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), 0, st, a);
     }
@@ -287,5 +296,5 @@ struct Bundle {
   static const odv::NetEntry odv_entry_##NET = {  /* NOLINT */                                                \
       NET::ID, ODV_REPRO_FLAG, NET::NEQ, NET::NNZ, NET::NRATE, NET::NPARAM, NET::RATE_KIND, NET::TGAS,          \
       NET::FLOPS_RHS, NET::FLOPS_JAC, NET::FLOPS_LU, NET::FLOPS_SOLVE, NET::FLOPS_LU_R,         \
-      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults};                          \
+      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::Hist<NET>::PLANES, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults};                          \
   static odv::Registrar odv_reg_##NET(&odv_entry_##NET);

@@ -1,0 +1,147 @@
+"""Warm restart through the C ABI (odevec_set_history): no reference
+counterpart, so the bar is the property that defines it. Steps are never
+clamped to t_stop, so a solve to t1 continued to t2 takes exactly the steps
+of one solve to t2 -- provided the first step is the same, and the initial
+step size depends on t_stop through W0 = max(|t0|, |t_stop|)
+(src/odevec.F90:228-237). The networks are autonomous, so the tests run the
+span [-S, 0] instead of [0, S]: W0 = S for every t_stop in it. Then states,
+last step sizes and summed event counters of the continued solve equal the
+single solve's bit for bit, per-cell and persistent, FP64 and mixed, and
+splitting the continuation further changes nothing. A cell whose state,
+parameters or start time changed between calls starts cold, and then
+equals a fresh solve from that state."""
+import numpy as np
+import pytest
+
+from odevec_amd import _lib
+from odevec_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make(nid, n, precision, mode, hist=True):
+    from odevec_amd.api import Odevec
+    ode = Odevec(nid, n, precision=precision, mode=mode, grid=2 if mode == _lib.PERSISTENT else 0)
+    if hist:
+        ode.enable_history(n)
+    return ode
+
+
+def solve(ode, w, y, t0, ts, params=None):
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    st, dto, stats = ode.solve_cells(y, ts, t0=t0, dt0=w["dt0"], params=params, stats=True, raise_on_fail=False)
+    return st, dto, stats
+
+
+def split_times(w, frac=0.37):
+    """[t0, t_stop] shifted to [-S, 0] (S = t_stop - t0), and a point inside."""
+    ts = np.broadcast_to(w["t_stop"], (w["y"].shape[1],)).astype(np.float64)
+    t0 = np.broadcast_to(w["t0"], ts.shape).astype(np.float64)
+    span = ts - t0
+    return -span, -span * (1.0 - frac), np.zeros_like(span)
+
+
+CASES = {
+    "PRIMORDIAL": lambda: W.primordial(700, seed=41),
+    "ROBER": lambda: W.rober(300, seed=42),          # per-cell rate parameters
+    "OREGO": lambda: W.orego(300, seed=43),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_continue_equals_one_solve(case, precision, mode):
+    w = CASES[case]()
+    nid = w["network"] + "+REPRO"
+    n = w["y"].shape[1]
+    t0, t1, t2 = split_times(w)
+    say("history", nid, "precision", precision, "mode", mode, "cells", n)
+    one = make(nid, n, precision, mode, hist=False)
+    y_one = np.array(w["y"], order="C", copy=True)
+    st_one, dt_one, s_one = solve(one, w, y_one, t0, t2, w["params"])
+    one.CloseOdevec()
+    ode = make(nid, n, precision, mode)
+    y = np.array(w["y"], order="C", copy=True)
+    st_a, _, s_a = solve(ode, w, y, t0, t1, w["params"])
+    st_b, dt_b, s_b = solve(ode, w, y, t1, t2, w["params"])
+    ode.CloseOdevec()
+    assert (st_one == 0).all() and (st_a == 0).all() and (st_b == 0).all()
+    assert np.array_equal(y, y_one)
+    assert np.array_equal(dt_b, dt_one)
+    for k in ("attempts", "steps", "newton", "jac"):
+        assert s_a[k] + s_b[k] == s_one[k], k
+
+
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_split_continuation(mode):
+    """From [0, S] (t_stop-dependent first step): continuing through one or
+    through two intermediate end times gives the same result."""
+    w = W.primordial(500, seed=46)
+    nid = w["network"] + "+REPRO"
+    n = w["y"].shape[1]
+    ts = np.asarray(w["t_stop"], dtype=np.float64)
+    t1, t15 = 0.3 * ts, 0.55 * ts
+    runs = []
+    for cuts in ([t1], [t1, t15]):
+        ode = make(nid, n, _lib.MIXED, mode)
+        y = np.array(w["y"], order="C", copy=True)
+        t = np.zeros(n)
+        for c in cuts + [ts]:
+            st, dto, _ = solve(ode, w, y, t, c)
+            assert (st == 0).all()
+            t = c
+        ode.CloseOdevec()
+        runs.append((y, dto))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_changed_cells_start_cold():
+    w = W.rober(256, seed=44)
+    nid = w["network"] + "+REPRO"
+    n = w["y"].shape[1]
+    t0, t1, t2 = split_times(w, 0.2)
+    t1 = np.asarray(t1)
+    ode = make(nid, n, _lib.FP64, _lib.PERSISTENT)
+    y = np.array(w["y"], order="C", copy=True)
+    st, _, _ = solve(ode, w, y, t0, t1, w["params"])
+    assert (st == 0).all()
+    # cells 0::4: the host changes the state; 1::4: a parameter; 2::4: the
+    # start time; 3::4 continue
+    y[0, 0::4] *= 1.0 + 1e-12
+    prm = np.array(w["params"], copy=True)
+    prm[1, 1::4] *= 1.0 + 1e-12
+    t1b = t1.copy()
+    t1b[2::4] = t1[2::4] * (1.0 - 1e-9)
+    y_mid = y.copy()
+    st2, dt2, _ = solve(ode, w, y, t1b, t2, prm)
+    ode.CloseOdevec()
+    assert (st2 == 0).all()
+    cold = make(nid, n, _lib.FP64, _lib.PERSISTENT, hist=False)
+    y_cold = y_mid.copy()
+    stc, dtc, _ = solve(cold, w, y_cold, t1b, t2, prm)
+    cold.CloseOdevec()
+    changed = np.zeros(n, dtype=bool)
+    changed[0::4] = changed[1::4] = changed[2::4] = True
+    assert np.array_equal(y[:, changed], y_cold[:, changed]) and np.array_equal(dt2[changed], dtc[changed])
+    # the continued cells differ from a cold restart at t1 (else the test proves nothing)
+    assert not np.array_equal(y[:, ~changed], y_cold[:, ~changed])
+
+
+def test_history_rejects_other_sizes_and_modes():
+    from odevec_amd.api import Odevec
+    w = W.rober(64, seed=45)
+    ode = make(w["network"], 64, _lib.FP64, _lib.PERCELL)
+    y = np.array(w["y"], order="C", copy=True)
+    with pytest.raises(_lib.OdevecError):
+        ode.solve_cells(y[:, :32].copy(), np.asarray(w["t_stop"])[:32].copy(), params=w["params"][:, :32].copy(),
+                        raise_on_fail=False)
+    ode.CloseOdevec()
+    lock = Odevec(w["network"], 64, mode=_lib.LOCKSTEP, group=32)
+    with pytest.raises(_lib.OdevecError):
+        lock.enable_history(64)
+    lock.CloseOdevec()
