@@ -16,8 +16,10 @@ over a 2,097,152-cell ROBER/OREGO/PRIMORDIAL batch), SYNTH64 (64 species)
 through the cooperative kernel, and the serving form (continuous batching
 under Poisson arrivals: p50/p99 latency and cell-solves/s, all four networks
 at 1 request/s and the three small ones at 40/s), and (skipped with
---no-cpu) the CPU baseline. `--only batching|synth64` runs that part alone
-and prints its object (for profiling one kernel).
+--no-cpu) the CPU baseline; and serving with warm restart (sessions whose
+blocks stay on the device and are advanced in closed loop) against the same
+advances cold-started. `--only batching|synth64|sessions` runs that part
+alone and prints its object (for profiling one kernel).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batch] [--no-cpu] [--only PART]
 """
@@ -47,7 +49,7 @@ def parse():
     p.add_argument("--cells", type=int, default=NCELLS)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
-    p.add_argument("--only", choices=["batching", "synth64"], default=None)
+    p.add_argument("--only", choices=["batching", "synth64", "sessions"], default=None)
     return p.parse_args()
 
 
@@ -174,6 +176,26 @@ def serving(rate, seconds, networks=None):
     return dict(rate_per_s=rate, seconds=seconds, networks=nets, **res)
 
 
+def sessions(nsess=8, ncells=65536, nadv=5):
+    """Serving with warm restart: closed-loop clients each advance a block of
+    PRIMORDIAL cells through five equal fractions of its span, as sessions
+    (state and integrator history resident on the device, each advance
+    continues the integration) and, for comparison, as ordinary requests
+    that cold-start every advance from the returned state."""
+    from odevec_amd import serve as S
+    out = {}
+    for warm in (True, False):
+        srv = S.Server("MIX3", slots=2)
+        r = S.run_sessions(srv, nsess=nsess, ncells=ncells, nadv=nadv, warm=warm, seed=7)
+        srv.close()
+        r.pop("final")
+        out["warm" if warm else "cold"] = r
+    out["warm_over_cold"] = out["warm"]["cell_solves_per_s"] / out["cold"]["cell_solves_per_s"]
+    out["workload"] = ("%d closed-loop clients x %d PRIMORDIAL cells (seeds 7..), %d advances each through "
+                       "equal fractions of t_stop logU[1e8,1e14] s, FP64, rtol 1e-6" % (nsess, ncells, nadv))
+    return out
+
+
 def cpu_baseline(w, seconds=12.0):
     """The oracle (a C restatement of the reference solver) on the host cores,
     per-cell mode, on a seeded sample of the same workload."""
@@ -225,6 +247,9 @@ def main():
         return
     if a.only == "synth64":
         print(json.dumps({"synth64": synth64(a.steps, a.warmup)}))
+        return
+    if a.only == "sessions":
+        print(json.dumps({"serving_sessions": sessions()}))
         return
     w = W.primordial(a.cells, seed=3)
     mixed = Runner(w, precision=1, mode=2)
@@ -280,6 +305,7 @@ def main():
         res["serving"] = sv
         res["serving_p50_ms"], res["serving_p99_ms"] = sv["p50_ms"], sv["p99_ms"]
         res["serving_3net"] = serving(40.0, 8.0, ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"])
+        res["serving_sessions"] = sessions()
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(res))

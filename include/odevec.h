@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define ODEVEC_ABI_VERSION 1
+#define ODEVEC_ABI_VERSION 2
 
 /* return codes */
 #define ODEVEC_OK 0
@@ -162,7 +162,7 @@ int32_t odevec_destroy(odevec_t* h);
  *
  * odevec_history_planes: doubles per cell of a history for `network`.
  * odevec_set_history: attach d_hist, a device array of planes x ncells
- * doubles (plane p of cell i at d_hist[p*ncells + i]) that the caller
+ * doubles (cell i's record at d_hist[i*planes .. (i+1)*planes)) that the caller
  * allocates zero-filled and keeps; every solve must then cover exactly
  * ncells cells. NULL detaches. Calls sharing one history must not overlap. */
 int32_t odevec_history_planes(const char* network, int64_t* planes);
@@ -196,6 +196,9 @@ typedef struct {
   double rtol, atol, dt_min;
   int32_t max_attempts;     /* 0 = 1000000 */
   int32_t reserved;
+  double* hist;             /* device: warm-restart history of these cells (odevec_history_planes x ncells
+                               doubles, zero-filled before first use) or NULL = cold start; see
+                               odevec_set_history for the rules */
 } odevec_segment_t;
 
 int32_t odevec_bundle_count(void);

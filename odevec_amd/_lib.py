@@ -51,7 +51,7 @@ class Segment(C.Structure):
     _fields_ = [("network", C.c_char_p), ("precision", _I32), ("check_negatives", _I32), ("ncells", _I64),
                 ("y", _VP), ("t0", _VP), ("t_stop", _VP), ("dt0", _VP), ("params", _VP), ("status", _VP),
                 ("dt_out", _VP), ("counters", _VP), ("flops", _VP), ("rtol", C.c_double), ("atol", C.c_double),
-                ("dt_min", C.c_double), ("max_attempts", _I32), ("reserved", _I32)]
+                ("dt_min", C.c_double), ("max_attempts", _I32), ("reserved", _I32), ("hist", _VP)]
 
 
 class OdevecError(RuntimeError):

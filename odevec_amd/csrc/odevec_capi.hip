@@ -601,6 +601,7 @@ struct odevec_batch_s {
   hipGraphExec_t exec = nullptr;
   odv::Plan gplan{};
   hipStream_t gstream = nullptr;
+  int ghist = 0;                   // the captured launch is the history variant
 };
 
 namespace {
@@ -688,6 +689,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
   // map and check every segment before touching the staging table
   std::vector<int> kind(nseg);
   int64_t total = 0;
+  int hist = 0;
   for (int k = 0; k < nseg; ++k) {
     const odevec_segment_t& g = segs[k];
     int kk = -1;
@@ -701,6 +703,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     if (!(g.rtol > 0.0) || !(g.atol >= 0.0)) return fail(ODEVEC_EINVAL, "segment tolerances");
     kind[k] = kk;
     total += g.ncells;
+    if (g.hist) hist = 1;
   }
   if (total == 0) return ODEVEC_OK;
   // a graph is captured for one launch shape: in graph mode the grid does not
@@ -745,6 +748,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     a.ctl.check_negatives = g.check_negatives;
     a.ctl.max_attempts = g.max_attempts > 0 ? g.max_attempts : 1000000;
     a.group = 1;
+    a.hist = g.hist;
     d.kind = kind[k];
   }
   *b->h_nseg = nseg;
@@ -754,16 +758,17 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     return hip_fail(he, "batch table upload");
   b->ev_pending = true;
   if (!b->use_graph) {
-    he = e->launch(b->d_segs, b->d_nseg, odv::MAXSEG, p, st);
+    he = e->launch(b->d_segs, b->d_nseg, odv::MAXSEG, p, hist, st);
     return he == hipSuccess ? ODEVEC_OK : hip_fail(he, "batch launch");
   }
   // graph: the queue reset and the batch kernel, captured once per launch
   // shape and stream and replayed; the table is read from device memory, so
   // one graph serves every batch of that shape
-  if (b->exec && (b->gplan.grid != p.grid || b->gplan.block != p.block || b->gstream != st)) batch_drop_graph(b);
+  if (b->exec && (b->gplan.grid != p.grid || b->gplan.block != p.block || b->gstream != st || b->ghist != hist))
+    batch_drop_graph(b);
   if (!b->exec) {
     if ((he = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return hip_fail(he, "graph capture");
-    hipError_t le = e->launch(b->d_segs, b->d_nseg, odv::MAXSEG, p, st);
+    hipError_t le = e->launch(b->d_segs, b->d_nseg, odv::MAXSEG, p, hist, st);
     he = hipStreamEndCapture(st, &b->graph);
     if (le != hipSuccess || he != hipSuccess) {
       batch_drop_graph(b);
@@ -775,6 +780,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     }
     b->gplan = p;
     b->gstream = st;
+    b->ghist = hist;
   }
   if ((he = hipGraphLaunch(b->exec, st)) != hipSuccess) return hip_fail(he, "graph launch");
   return ODEVEC_OK;

@@ -54,7 +54,7 @@ struct KArgs {
   double* flops;          // optional [2]: FP64, FP32 algorithmic flops (atomic)
   unsigned long long* queue;  // persistent: next cell (set to the number of launched lanes)
   double* ws;             // rate workspace, LdsPolicy::WS_PER_SLOT doubles per wavefront slot
-  double* hist;           // optional warm-restart history, Hist<Net>::PLANES planes of ncells doubles
+  double* hist;           // optional warm-restart history: Hist<Net>::PLANES doubles per cell, cell-major
   Controls ctl;
   int group;              // lockstep group size
 };
@@ -146,8 +146,7 @@ __device__ inline void odv_store(const KArgs& a, long long i, const L& s, const 
 
 // Warm restart (no counterpart in the reference, whose SolveODE cold-starts
 // every call: src/odevec.F90:170-182). After a successful solve a cell's
-// whole integrator state goes to `hist` (plane p of cell i at
-// hist[p*ncells + i]); the next call on that cell continues the integration
+// whole integrator state goes to `hist`; the next call on that cell continues the integration
 // from it when the call starts where the last one stopped (t0 equal to the
 // saved t_stop) on the state it returned (y and parameters bitwise equal to
 // the saved ones), else the cell starts cold. Steps are never clamped to
@@ -173,47 +172,46 @@ __device__ inline bool odv_hist_resume(const KArgs& a, long long i, L& s, const 
                                        double t0, double ts) {
   using H = Hist<typename L::Net>;
   using M = typename L::M;
-  const long long n = a.ncells;
-  const double* h = a.hist + i;
-  if (h[H::VALID * n] != 1.0 || h[H::TSTOP * n] != t0) return false;
+  const double* h = a.hist + i * H::PLANES;
+  if (h[H::VALID] != 1.0 || h[H::TSTOP] != t0) return false;
   bool same = true;
 #pragma unroll
-  for (int j = 0; j < H::NF; ++j) same = same && (h[(H::Y + j) * n] == y[j]);
+  for (int j = 0; j < H::NF; ++j) same = same && (h[H::Y + j] == y[j]);
 #pragma unroll
-  for (int p = 0; p < H::NP; ++p) same = same && (h[(H::P + p) * n] == prm[p]);
+  for (int p = 0; p < H::NP; ++p) same = same && (h[H::P + p] == prm[p]);
   if (!same) return false;
-  s.t = h[H::T * n];
-  s.dt = h[H::DT * n];
-  s.rate = h[H::RATE * n];
-  s.jchange = h[H::JCH * n];
-  s.cerr = h[H::CERR * n];
-  odv_set_order(s, (int)h[H::INTS * n]);
-  s.swu = (int)h[(H::INTS + 1) * n];
-  s.lju = (int)h[(H::INTS + 2) * n];
-  s.fet = (int)h[(H::INTS + 3) * n];
-  s.fc = (int)h[(H::INTS + 4) * n];
-  const int fl = (int)h[(H::INTS + 5) * n];
+  s.t = h[H::T];
+  s.dt = h[H::DT];
+  s.rate = h[H::RATE];
+  s.jchange = h[H::JCH];
+  s.cerr = h[H::CERR];
+  odv_set_order(s, (int)h[H::INTS]);
+  s.swu = (int)h[H::INTS + 1];
+  s.lju = (int)h[H::INTS + 2];
+  s.fet = (int)h[H::INTS + 3];
+  s.fc = (int)h[H::INTS + 4];
+  const int fl = (int)h[H::INTS + 5];
   s.first = (fl & 1) != 0;
   s.upd_jac = (fl & 2) != 0;
   s.neglect_up = (fl & 4) != 0;
-  s.fz = (int)h[(H::INTS + 6) * n];
+  s.fz = (int)h[H::INTS + 6];
   s.status = ST_RUNNING;
   s.tstop = ts;
 #pragma unroll
   for (int c = 0; c < NCOL; ++c)
 #pragma unroll
-    for (int j = 0; j < H::N; ++j) s.set_z(c, j, h[(H::Z + c * H::N + j) * n]);
+    for (int j = 0; j < H::N; ++j) s.set_z(c, j, h[H::Z + c * H::N + j]);
 #pragma unroll
   for (int j = 0; j < H::N; ++j) {
-    s.set_w(j, h[(H::W + j) * n]);
-    s.set_eo(j, h[(H::EO + j) * n]);
+    s.set_w(j, h[H::W + j]);
+    s.set_eo(j, h[H::EO + j]);
     s.en[j] = 0.0;
   }
 #pragma unroll
-  for (int e = 0; e < L::Net::NNZ; ++e) s.a[e] = (M)h[(H::A + e) * n];
+  for (int e = 0; e < L::Net::NNZ; ++e) s.a[e] = (M)h[H::A + e];
 #pragma unroll
   for (int r = 0; r < L::Net::NRATE; ++r) {
-    const double v = h[(H::K + r) * n];
+    const double v = h[H::K + r];
     if (L::KS == 0) s.kl[r] = v;
     else s.kx[r * L::KS] = v;
   }
@@ -225,43 +223,43 @@ template <class L>
 __device__ inline void odv_hist_save(const KArgs& a, long long i, const L& s) {
   using H = Hist<typename L::Net>;
   const long long n = a.ncells;
-  double* h = a.hist + i;
+  double* h = a.hist + i * H::PLANES;
   if (s.status == ST_SKIPPED) return;   // nothing integrated: the saved state stays usable
   if (s.status != ST_OK) {
-    h[H::VALID * n] = 0.0;
+    h[H::VALID] = 0.0;
     return;
   }
-  h[H::TSTOP * n] = s.tstop;
-  h[H::T * n] = s.t;
-  h[H::DT * n] = s.dt;
-  h[H::RATE * n] = s.rate;
-  h[H::JCH * n] = s.jchange;
-  h[H::CERR * n] = s.cerr;
-  h[H::INTS * n] = (double)s.q;
-  h[(H::INTS + 1) * n] = (double)s.swu;
-  h[(H::INTS + 2) * n] = (double)s.lju;
-  h[(H::INTS + 3) * n] = (double)s.fet;
-  h[(H::INTS + 4) * n] = (double)s.fc;
-  h[(H::INTS + 5) * n] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0));
-  h[(H::INTS + 6) * n] = (double)s.fz;
+  h[H::TSTOP] = s.tstop;
+  h[H::T] = s.t;
+  h[H::DT] = s.dt;
+  h[H::RATE] = s.rate;
+  h[H::JCH] = s.jchange;
+  h[H::CERR] = s.cerr;
+  h[H::INTS] = (double)s.q;
+  h[H::INTS + 1] = (double)s.swu;
+  h[H::INTS + 2] = (double)s.lju;
+  h[H::INTS + 3] = (double)s.fet;
+  h[H::INTS + 4] = (double)s.fc;
+  h[H::INTS + 5] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0));
+  h[H::INTS + 6] = (double)s.fz;
 #pragma unroll
-  for (int j = 0; j < H::NF; ++j) h[(H::Y + j) * n] = a.y[(long long)j * n + i];
+  for (int j = 0; j < H::NF; ++j) h[H::Y + j] = a.y[(long long)j * n + i];
 #pragma unroll
-  for (int p = 0; p < H::NP; ++p) h[(H::P + p) * n] = a.prm[p * a.ps_plane + i * a.ps_cell];
+  for (int p = 0; p < H::NP; ++p) h[H::P + p] = a.prm[p * a.ps_plane + i * a.ps_cell];
 #pragma unroll
   for (int c = 0; c < NCOL; ++c)
 #pragma unroll
-    for (int j = 0; j < H::N; ++j) h[(H::Z + c * H::N + j) * n] = s.z(c, j);
+    for (int j = 0; j < H::N; ++j) h[H::Z + c * H::N + j] = s.z(c, j);
 #pragma unroll
   for (int j = 0; j < H::N; ++j) {
-    h[(H::W + j) * n] = s.w(j);
-    h[(H::EO + j) * n] = s.eo(j);
+    h[H::W + j] = s.w(j);
+    h[H::EO + j] = s.eo(j);
   }
 #pragma unroll
-  for (int e = 0; e < L::Net::NNZ; ++e) h[(H::A + e) * n] = (double)s.a[e];
+  for (int e = 0; e < L::Net::NNZ; ++e) h[H::A + e] = (double)s.a[e];
 #pragma unroll
-  for (int r = 0; r < L::Net::NRATE; ++r) h[(H::K + r) * n] = s.kptr()[r * (L::KS > 0 ? L::KS : 1)];
-  h[H::VALID * n] = 1.0;
+  for (int r = 0; r < L::Net::NRATE; ++r) h[H::K + r] = s.kptr()[r * (L::KS > 0 ? L::KS : 1)];
+  h[H::VALID] = 1.0;
 }
 
 // Load cell i into the lane and start its solve; returns true if there is
@@ -290,7 +288,7 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
     return true;
   }
   if constexpr (HIST) {
-    if (odv_hist_resume(a, i, s, y, prm, t0, ts)) {
+    if (a.hist && odv_hist_resume(a, i, s, y, prm, t0, ts)) {
       if (s.t < s.tstop) return false;
       s.status = ST_OK;   // the last step already passed the new t_stop: interpolate
       return true;
@@ -312,7 +310,9 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   double y[L::N];
   odv_result(s, y);
   odv_store(a, i, s, y);
-  if constexpr (HIST) odv_hist_save(a, i, s);
+  if constexpr (HIST) {
+    if (a.hist) odv_hist_save(a, i, s);
+  }
 }
 
 template <class Net, class M, bool RCP, bool HIST = false>
@@ -448,22 +448,24 @@ struct MaxWords<E, F, R...> {
   static constexpr long long WS = E::WS_PER_SLOT > MaxWords<F, R...>::WS ? E::WS_PER_SLOT : MaxWords<F, R...>::WS;
 };
 
-template <int K, class E, class... R>
+template <int K, bool HIST, class E, class... R>
 __device__ inline void odv_batch_dispatch(int kind, const KArgs& a, double* lds) {
   if (kind == K) {
-    odv_persistent_run<typename E::Net, typename E::M, E::RCP>(a, lds);
+    odv_persistent_run<typename E::Net, typename E::M, E::RCP, HIST>(a, lds);
   } else {
-    if constexpr (sizeof...(R) > 0) odv_batch_dispatch<K + 1, R...>(kind, a, lds);
+    if constexpr (sizeof...(R) > 0) odv_batch_dispatch<K + 1, HIST, R...>(kind, a, lds);
   }
 }
 
-template <class... E>
+// HIST: the variant for tables where segments carry warm-restart histories
+// (segments without one start cold in it too)
+template <bool HIST, class... E>
 __global__ void __launch_bounds__(MAX_BLOCK) odv_k_batch(const SegDev* segs, const int* nseg) {
   const int n = *nseg;
   for (int k = 0; k < n; ++k) {
     const KArgs a = segs[k].a;
     // the entry carves its own wavefront slices (at most MaxWords doubles each)
-    odv_batch_dispatch<0, E...>(segs[k].kind, a, odv_lane_lds);
+    odv_batch_dispatch<0, HIST, E...>(segs[k].kind, a, odv_lane_lds);
     __syncthreads();   // the next segment's kind may carve LDS differently
   }
 }

@@ -58,7 +58,7 @@ struct BundleEntry {
   long long ws_per_slot;       // rate workspace doubles per workgroup slot (max over entries)
   int segdev_bytes;            // sizeof(SegDev) of the build
   hipError_t (*plan)(long long ncells, int block, int grid, Plan* out);
-  hipError_t (*launch)(void* segs, const int* nseg, int maxseg, const Plan& p, hipStream_t st);
+  hipError_t (*launch)(void* segs, const int* nseg, int maxseg, const Plan& p, int hist, hipStream_t st);
   hipError_t (*info)(int block, KernelInfo* out);
 };
 
@@ -247,7 +247,7 @@ struct Bundle {
       hipError_t e = hipGetDevice(&dev);
       if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
       if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(odv_k_batch<E...>), b,
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(odv_k_batch<false, E...>), b,
                                                          lane_lds_bytes(MaxWords<E...>::WORDS, b));
       if (e != hipSuccess) return e;
       g = (long long)ncu * (nb > 0 ? nb : 1);
@@ -260,22 +260,26 @@ struct Bundle {
     return hipSuccess;
   }
 
-  static hipError_t launch(void* segs, const int* nseg, int maxseg, const Plan& p, hipStream_t st) {
+  // hist: launch the variant that reads and writes the segments' warm-restart histories
+  static hipError_t launch(void* segs, const int* nseg, int maxseg, const Plan& p, int hist, hipStream_t st) {
     if (p.block <= 0 || p.block > MAX_BLOCK || p.block % WAVE != 0 || p.grid <= 0 || p.grid > 0x7fffffffLL)
       return hipErrorInvalidValue;
     SegDev* sd = static_cast<SegDev*>(segs);
     hipLaunchKernelGGL(k_batch_reset<0>, dim3((unsigned)((maxseg + 255) / 256)), dim3(256), 0, st, sd, nseg,
                        (unsigned long long)(p.grid * p.block));
-    hipLaunchKernelGGL((odv_k_batch<E...>), dim3((unsigned)p.grid), dim3(p.block),
-                       lane_lds_bytes(MaxWords<E...>::WORDS, p.block), st,
-                       static_cast<const SegDev*>(sd), nseg);
+    if (hist)
+      hipLaunchKernelGGL((odv_k_batch<true, E...>), dim3((unsigned)p.grid), dim3(p.block),
+                         lane_lds_bytes(MaxWords<E...>::WORDS, p.block), st, static_cast<const SegDev*>(sd), nseg);
+    else
+      hipLaunchKernelGGL((odv_k_batch<false, E...>), dim3((unsigned)p.grid), dim3(p.block),
+                         lane_lds_bytes(MaxWords<E...>::WORDS, p.block), st, static_cast<const SegDev*>(sd), nseg);
     return hipGetLastError();
   }
 
   static hipError_t info(int block, KernelInfo* out) {
     const int b = block > 0 ? block : BLOCK;
     if (b > MAX_BLOCK || b % WAVE != 0) return hipErrorInvalidValue;
-    return kinfo(odv_k_batch<E...>, b, out, lane_lds_bytes(MaxWords<E...>::WORDS, b));
+    return kinfo(odv_k_batch<false, E...>, b, out, lane_lds_bytes(MaxWords<E...>::WORDS, b));
   }
 };
 

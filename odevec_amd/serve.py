@@ -23,6 +23,16 @@ target) and tolerances. The server
 Every cell keeps its own step control, so a request's result does not
 depend on what it was batched with (bitwise, for one bundle).
 
+Sessions (warm restart; the reference cold-starts every SolveODE call): a
+client opens a session for a block of cells, whose state, parameters and
+integrator history then stay resident on the device, and sends `advance`
+requests, each moving every cell on to a new end time. An advance continues
+each cell's integration where the previous one stopped (Nordsieck history,
+order, step size, LU factors), so it does not pay the cold start's order-1
+steps, fresh Jacobian and initial step-size search again, and nothing but
+the end times travels to the device. One session's advances run in order;
+different sessions' advances overlap like other requests.
+
     python -m odevec_amd.serve --rates 4,8,16,32 --seconds 10
 """
 from __future__ import annotations
@@ -48,10 +58,25 @@ COOP_NETWORKS = {"SYNTH64/FF/CSC"}
 
 
 @dataclass
+class Session:
+    """A block of cells resident on the device with its warm-restart history."""
+    sid: int
+    network: str
+    precision: int
+    ncells: int
+    rtol: float
+    atol: float
+    dt_min: float
+    dev: dict = field(default_factory=dict)   # y, t, dt0, params, hist, status
+    busy: bool = False                        # an advance is admitted or in flight
+    bytes: int = 0
+
+
+@dataclass
 class Request:
     rid: int
     network: str
-    y: np.ndarray                   # (neq, ncells), float64
+    y: Optional[np.ndarray]         # (neq, ncells), float64 (None for a session's advance)
     t_stop: np.ndarray              # (ncells,)
     rtol: float
     atol: float
@@ -64,14 +89,18 @@ class Request:
     status: Optional[np.ndarray] = None
     result: Optional[np.ndarray] = None
     dev: dict = field(default_factory=dict)
+    session: Optional[Session] = None
 
     @property
     def ncells(self) -> int:
-        return self.y.shape[1]
+        return self.session.ncells if self.session is not None else self.y.shape[1]
 
     def footprint(self) -> int:
-        """Device bytes held from admission to completion."""
+        """Device bytes held from admission to completion (a session's
+        resident arrays are counted once, when it opens)."""
         n = self.ncells
+        if self.session is not None:
+            return 8 * n
         p = 0 if self.params is None else self.params.shape[0]
         return n * (8 * self.y.shape[0] + 8 + 8 + 8 * p + 4)
 
@@ -90,7 +119,9 @@ class Server:
         full = dev.multi_processor_count * max(kinfo.blocks_per_cu, 1)
         self.grid = max(1, full // slots)
         self.slots = [{"stream": torch.cuda.Stream(), "batch": Batch(bundle, block=block, grid=self.grid),
-                       "coop": {}, "reqs": None, "event": None} for _ in range(slots)]
+                       "coop": {}, "lane": {}, "reqs": None, "event": None} for _ in range(slots)]
+        self.sessions = {}
+        self._next_sid = 0
         self.budget = hbm_budget
         self.in_use = 0
         self.waiting: deque = deque()      # arrived, not yet admitted
@@ -102,6 +133,54 @@ class Server:
 
     def now(self) -> float:
         return time.perf_counter() - self.t0
+
+    # -- sessions
+    def open_session(self, network: str, y: np.ndarray, params: Optional[np.ndarray] = None, rtol: float = 1e-6,
+                     atol: float = 1e-20, dt_min: float = 1e-20, dt0=1e-10, t0=0.0,
+                     precision: Optional[int] = None) -> Session:
+        """Upload a block of cells (y: (neq, ncells)) that later `advance`
+        requests integrate further; its arrays and history stay on the device
+        until close_session."""
+        torch = self.torch
+        if network in COOP_NETWORKS:
+            raise ValueError("%s runs on the cooperative kernel, which has no warm restart" % network)
+        n = y.shape[1]
+        planes = L.history_planes(network)
+        prec = self.precision.get(network, L.FP64) if precision is None else precision
+        dev = torch.device("cuda:0")
+        d = {"y": torch.tensor(np.ascontiguousarray(y), dtype=torch.float64, device=dev),
+             "t": torch.tensor(np.broadcast_to(np.asarray(t0, dtype=np.float64), (n,)).copy(), device=dev),
+             "dt0": torch.tensor(np.broadcast_to(np.asarray(dt0, dtype=np.float64), (n,)).copy(), device=dev),
+             "hist": torch.zeros(planes * n, dtype=torch.float64, device=dev),
+             "status": torch.zeros(n, dtype=torch.int32, device=dev)}
+        if params is not None:
+            d["params"] = torch.tensor(np.ascontiguousarray(params), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        ss = Session(self._next_sid, network, prec, n, rtol, atol, dt_min, d)
+        ss.bytes = sum(v.numel() * v.element_size() for v in d.values())
+        if self.in_use + ss.bytes > self.budget:
+            raise ValueError("session of %d cells needs %d bytes, more than the HBM budget left" % (n, ss.bytes))
+        self.in_use += ss.bytes
+        self.sessions[ss.sid] = ss
+        self._next_sid += 1
+        return ss
+
+    def advance(self, ss: Session, t_stop, rid: int = -1, arrival: Optional[float] = None) -> Request:
+        """Queue an advance of every cell of the session to t_stop (scalar or
+        per cell); the request's result is the state there."""
+        ts = np.ascontiguousarray(np.broadcast_to(np.asarray(t_stop, dtype=np.float64), (ss.ncells,)))
+        r = Request(rid, ss.network, None, ts, ss.rtol, ss.atol, ss.dt_min, None, precision=ss.precision,
+                    arrival=self.now() if arrival is None else arrival, session=ss)
+        self.waiting.append(r)
+        return r
+
+    def close_session(self, ss: Session):
+        while ss.busy or any(r.session is ss for r in self.waiting):
+            self.poll()
+            time.sleep(0.0002)
+        self.in_use -= ss.bytes
+        ss.dev = {}
+        del self.sessions[ss.sid]
 
     def submit(self, r: Request):
         if r.network in COOP_NETWORKS:
@@ -121,10 +200,21 @@ class Server:
         one launch's worth) on `stream`; returns them."""
         torch = self.torch
         new = []
+        skipped = deque()
         while (self.waiting and len(new) < L_MAXSEG and
                self.in_use + self.waiting[0].footprint() <= self.budget):
             r = self.waiting.popleft()
             dev = torch.device("cuda:0")
+            if r.session is not None:
+                if r.session.busy:           # one advance of a session at a time, in order
+                    skipped.append(r)
+                    continue
+                r.session.busy = True
+                with torch.cuda.stream(stream):
+                    r.dev = {"t_stop": torch.from_numpy(r.t_stop).pin_memory().to(dev, non_blocking=True)}
+                self.in_use += r.footprint()
+                new.append(r)
+                continue
             with torch.cuda.stream(stream):
                 d = {"y": torch.from_numpy(r.y).pin_memory().to(dev, non_blocking=True),
                      "t_stop": torch.from_numpy(np.ascontiguousarray(r.t_stop)).pin_memory().to(dev, non_blocking=True),
@@ -135,6 +225,7 @@ class Server:
             r.dev = d
             self.in_use += r.footprint()
             new.append(r)
+        self.waiting.extendleft(reversed(skipped))
         return new
 
     def _coop(self, slot, network, precision):
@@ -146,11 +237,57 @@ class Server:
             slot["coop"][key] = h
         return h
 
+    def _lane(self, slot, network, precision):
+        """The slot's persistent-mode handle for (network, precision): runs
+        session advances, each with its session's history attached."""
+        from .api import Odevec
+        key = (network, precision)
+        h = slot["lane"].get(key)
+        if h is None:
+            # the slot's share of the device's resident workgroups of this kernel
+            k = L.kernel_info(network, L.PERSISTENT, precision)
+            ncu = self.torch.cuda.get_device_properties(0).multi_processor_count
+            grid = max(1, ncu * max(k["blocks_per_cu"], 1) // len(self.slots))
+            h = Odevec(network, 1, precision=precision, mode=L.PERSISTENT, grid=grid)
+            slot["lane"][key] = h
+        return h
+
+    def _advance(self, slot, r: Request):
+        ss, d = r.session, r.session.dev
+        h = self._lane(slot, ss.network, ss.precision)
+        h.RelativeTolerance, h.AbsoluteTolerance, h.MinimalTimestep = ss.rtol, ss.atol, ss.dt_min
+        L.check(L.lib().odevec_set_history(h._h, L.C.c_void_p(d["hist"].data_ptr()), ss.ncells))
+        st = slot["stream"]
+        h.solve_cells_async(ss.ncells, r.dev["t_stop"].data_ptr(), d["y"].data_ptr(), t0_ptr=d["t"].data_ptr(),
+                            dt0_ptr=d["dt0"].data_ptr(),
+                            params_ptr=d["params"].data_ptr() if "params" in d else None,
+                            status_ptr=d["status"].data_ptr(), stream=st.cuda_stream)
+        L.check(L.lib().odevec_set_history(h._h, None, 0))
+        with self.torch.cuda.stream(st):
+            d["t"].copy_(r.dev["t_stop"], non_blocking=True)   # the next advance starts here
+        r.dev["y"], r.dev["status"] = d["y"], d["status"]
+
     def _launch(self, slot, reqs: List[Request]):
         reqs = sorted(reqs, key=lambda r: COST_ORDER.index(r.network) if r.network in COST_ORDER else -1)
         st = slot["stream"]
         segs = []
+        advanced = []
         for r in reqs:
+            if r.session is not None:
+                ss = r.session
+                if (ss.network, ss.precision) not in self.entries:
+                    self._advance(slot, r)      # not in the bundle: its own launch
+                    continue
+                d = ss.dev
+                s = {"network": ss.network, "precision": ss.precision, "y": d["y"], "t0": d["t"],
+                     "t_stop": r.dev["t_stop"], "dt0": d["dt0"], "status": d["status"], "hist": d["hist"],
+                     "rtol": ss.rtol, "atol": ss.atol, "dt_min": ss.dt_min}
+                if "params" in d:
+                    s["params"] = d["params"]
+                segs.append(s)
+                r.dev["y"], r.dev["status"] = d["y"], d["status"]
+                advanced.append(r)
+                continue
             if r.network in COOP_NETWORKS:
                 h = self._coop(slot, r.network, r.precision)
                 h.RelativeTolerance, h.AbsoluteTolerance, h.MinimalTimestep = r.rtol, r.atol, r.dt_min
@@ -168,8 +305,10 @@ class Server:
         if segs:
             slot["batch"].solve_async(segs, st.cuda_stream)
         with torch.cuda.stream(st):
+            for r in advanced:     # the session's next advance starts where this one ends
+                r.session.dev["t"].copy_(r.dev["t_stop"], non_blocking=True)
             for r in reqs:
-                r.dev["y_host"] = torch.empty(r.y.shape, dtype=torch.float64, pin_memory=True)
+                r.dev["y_host"] = torch.empty(tuple(r.dev["y"].shape), dtype=torch.float64, pin_memory=True)
                 r.dev["st_host"] = torch.empty(r.ncells, dtype=torch.int32, pin_memory=True)
                 r.dev["y_host"].copy_(r.dev["y"], non_blocking=True)
                 r.dev["st_host"].copy_(r.dev["status"], non_blocking=True)
@@ -190,6 +329,8 @@ class Server:
                     r.done = t
                     self.in_use -= r.footprint()
                     r.dev = {}
+                    if r.session is not None:
+                        r.session.busy = False
                     done.append(r)
                 slot["reqs"], slot["event"] = None, None
         for slot in self.slots:
@@ -214,7 +355,7 @@ class Server:
         self.drain()
         for s in self.slots:
             s["batch"].close()
-            for h in s["coop"].values():
+            for h in list(s["coop"].values()) + list(s["lane"].values()):
                 h.CloseOdevec()
 
 
@@ -267,6 +408,71 @@ def run_trace(server: Server, trace: List[Request]) -> dict:
             "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
             "mean_ms": float(lat.mean()) if len(lat) else None,
             "cell_solves_per_s": cells / wall if wall > 0 else None}
+
+
+def run_sessions(server: Server, nsess: int = 8, ncells: int = 65536, nadv: int = 5, warm: bool = True,
+                 seed: int = 7, network: str = "PRIMORDIAL/CMK/CSC", precision: int = L.FP64) -> dict:
+    """Closed-loop clients: each of `nsess` blocks of `ncells` seeded cells
+    is advanced `nadv` times through equal fractions of its span (each
+    client sends its next advance when the previous result is back). warm:
+    sessions (state and history stay on the device, each advance continues
+    the integration); cold: each advance is an ordinary request carrying the
+    state returned by the previous one, integrated from a cold start over
+    the same interval (the networks are autonomous). Returns latency
+    percentiles and cell-solves/s over all advances."""
+    gen = {"ROBER/NONE/DENSE": W.rober, "OREGO/NONE/DENSE": W.orego, "PRIMORDIAL/CMK/CSC": W.primordial}[network]
+    blocks = [gen(ncells, seed=seed + k) for k in range(nsess)]
+    spans = [np.broadcast_to(np.asarray(w["t_stop"], dtype=np.float64), (ncells,)).copy() for w in blocks]
+    server.t0 = time.perf_counter()
+    sess = [server.open_session(network, w["y"], w.get("params"), w["rtol"], w["atol"], w["dt_min"], w["dt0"],
+                                precision=precision) if warm else None for w in blocks]
+    state = [np.ascontiguousarray(w["y"]) for w in blocks]
+    step = [0] * nsess
+    live = {}
+    lat = []
+    rid = 0
+
+    def send(k):
+        nonlocal rid
+        frac0, frac1 = step[k] / nadv, (step[k] + 1) / nadv
+        if warm:
+            r = server.advance(sess[k], spans[k] * frac1, rid=rid)
+        else:
+            w = blocks[k]
+            r = Request(rid, network, state[k], spans[k] * (frac1 - frac0), w["rtol"], w["atol"], w["dt_min"],
+                        np.ascontiguousarray(np.broadcast_to(w["dt0"], (ncells,))),
+                        None if w.get("params") is None else np.ascontiguousarray(w["params"]), precision,
+                        server.now())
+            server.submit(r)
+        live[r.rid] = k
+        rid += 1
+
+    t_begin = server.now()
+    for k in range(nsess):
+        send(k)
+    ok = 0
+    while live:
+        for r in server.poll():
+            k = live.pop(r.rid, None)
+            if k is None:
+                continue
+            lat.append((r.done - r.arrival) * 1e3)
+            ok += int((r.status == 0).sum())
+            state[k] = np.ascontiguousarray(r.result)
+            step[k] += 1
+            if step[k] < nadv:
+                send(k)
+        time.sleep(0.0002)
+    wall = server.now() - t_begin
+    for ss in sess:
+        if ss is not None:
+            server.close_session(ss)
+    lat = np.array(lat)
+    cells = nsess * ncells * nadv
+    return {"mode": "warm" if warm else "cold", "sessions": nsess, "cells": ncells, "advances": nadv,
+            "cells_ok": ok, "wall_s": wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "cell_solves_per_s": cells / wall,
+            "final": state}
 
 
 def main(argv=None) -> int:

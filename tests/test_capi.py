@@ -36,7 +36,7 @@ def test_registry_and_network_info():
     with pytest.raises(_lib.OdevecError) as e:
         _lib.network_info("NOPE/NONE/DENSE")
     assert e.value.code == _lib.ENOTFOUND
-    assert _lib.lib().odevec_abi_version() == 1
+    assert _lib.lib().odevec_abi_version() == 2
 
 
 def test_create_argument_checks():

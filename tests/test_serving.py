@@ -70,3 +70,76 @@ def test_serving_admission_respects_hbm_budget():
     srv.close()
     assert seen and max(seen) <= budget
     assert all(r.status is not None for r in trace)
+
+
+def test_session_advance_footprint_is_end_times_only():
+    ss = S.Session(0, "ROBER/NONE/DENSE", 0, 1000, 1e-6, 1e-10, 1e-10)
+    r = S.Request(1, ss.network, None, np.zeros(1000), 1e-6, 1e-10, 1e-10, None, session=ss)
+    assert r.ncells == 1000 and r.footprint() == 8 * 1000
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bundle", ["MIX3+REPRO", "MIX3"])
+def test_session_advances_continue_the_integration(bundle):
+    """Three advances of a session over [-S, 0] (autonomous network; the
+    first step is then the same for every end time) equal one warm-restart
+    free solve over [-S, 0], bit for bit, and the first advance equals a
+    solve over its own interval. MIX3+REPRO holds the session's network, so
+    its advances run in the batch kernel; MIX3 does not, so they run on the
+    slot's own per-network handle."""
+    from odevec_amd import _lib as L
+    from odevec_amd import workloads as W
+    from odevec_amd.api import Odevec
+    print("GPU session advances", flush=True)
+    w = W.primordial(3000, seed=21)
+    nid = w["network"] + "+REPRO"
+    n = w["y"].shape[1]
+    span = np.asarray(w["t_stop"], dtype=np.float64)
+    srv = S.Server(bundle, slots=2)
+    # a +REPRO bundle names its entries without the suffix; its kernel is the reproducible build
+    sid = w["network"] if bundle == "MIX3+REPRO" else nid
+    ss = srv.open_session(sid, w["y"], None, w["rtol"], w["atol"], w["dt_min"], w["dt0"], t0=-span,
+                          precision=L.MIXED)
+    assert ((sid, L.MIXED) in srv.entries) == (bundle == "MIX3+REPRO")
+    rs = [srv.advance(ss, -span * f) for f in (0.6, 0.2, 0.0)]
+    srv.drain()
+    srv.close_session(ss)
+    srv.close()
+    assert all((r.status == 0).all() for r in rs)
+
+    def one(t_stop):
+        ode = Odevec(nid, n, precision=L.MIXED, mode=L.PERSISTENT)
+        ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+        y = np.array(w["y"], order="C", copy=True)
+        st, _, _ = ode.solve_cells(y, t_stop, t0=-span, dt0=w["dt0"], raise_on_fail=False)
+        ode.CloseOdevec()
+        assert (st == 0).all()
+        return y
+
+    assert np.array_equal(rs[0].result, one(-span * 0.6))
+    assert np.array_equal(rs[2].result, one(np.zeros(n)))
+
+
+@pytest.mark.gpu
+def test_sessions_overlap_with_requests_and_match_cold_within_tolerance():
+    """Two sessions advanced in closed loop next to each other give the same
+    results as each alone (bitwise), and agree with cold-restarted advances
+    over the same intervals to the tolerance."""
+    print("GPU sessions overlap", flush=True)
+    srv = S.Server("MIX3", slots=2)
+    both = S.run_sessions(srv, nsess=2, ncells=2000, nadv=3, warm=True, seed=31)
+    srv.close()
+    alone = []
+    for k in range(2):
+        srv = S.Server("MIX3", slots=2)
+        alone.append(S.run_sessions(srv, nsess=1, ncells=2000, nadv=3, warm=True, seed=31 + k)["final"][0])
+        srv.close()
+    assert all(np.array_equal(a, b) for a, b in zip(both["final"], alone))
+    srv = S.Server("MIX3", slots=2)
+    cold = S.run_sessions(srv, nsess=2, ncells=2000, nadv=3, warm=False, seed=31)
+    srv.close()
+    assert both["cells_ok"] == cold["cells_ok"] == 2 * 2000 * 3
+    for a, b in zip(both["final"], cold["final"]):
+        big = np.abs(b) > 1e-6 * np.abs(b).max(axis=0)
+        rel = np.abs(a - b)[big] / np.abs(b[big])
+        assert np.median(rel) < 1e-4, np.median(rel)
