@@ -35,6 +35,8 @@ BUILTIN_VARIANTS = [
     "PRIMORDIAL/NONE/DENSE",
     "PRIMORDIAL/FF/CSC",
     "SYNTH64/FF/CSC",     # 64 species: cooperative kernels only
+    "ROBER/NONE/DENSE/LU3",        # LUmethod 3: Jacobian by forward differences
+    "PRIMORDIAL/NONE/DENSE/LU3",
 ]
 
 # bundles: one batch kernel (odv_k_batch) over several (network, precision)
@@ -77,16 +79,26 @@ def generate(variants: List[str]) -> List[str]:
     from .pre_odevec import cli
     bases = []
     for v in variants:
-        name, ordering, packaging = v.split("/")
+        name, ordering, packaging, lu = _parts(v)
         ordering = "None" if ordering.upper() == "NONE" else ordering
-        cli.generate(name, ordering, packaging, GEN)
-        bases.append("%s_%s_%s" % (name, ordering.upper(), packaging))
+        cli.generate(name, ordering, packaging, GEN, lu_method=lu)
+        bases.append("%s_%s_%s%s" % (name, ordering.upper(), packaging, "_LU3" if lu == 3 else ""))
     return bases
 
 
+def _parts(variant: str):
+    """NAME/ORDERING/PACKAGING[/LU3] -> (name, ordering, packaging, LUmethod)."""
+    p = variant.split("/")
+    if len(p) == 4 and p[3] == "LU3":
+        return p[0], p[1], p[2], 3
+    if len(p) != 3:
+        raise ValueError("variant %r: expected NAME/ORDERING/PACKAGING[/LU3]" % variant)
+    return p[0], p[1], p[2], 1
+
+
 def _struct(variant: str) -> str:
-    name, ordering, packaging = variant.split("/")
-    return "odv_net_%s_%s_%s" % (name, ordering.upper(), packaging)
+    name, ordering, packaging, lu = _parts(variant)
+    return "odv_net_%s_%s_%s%s" % (name, ordering.upper(), packaging, "_LU3" if lu == 3 else "")
 
 
 def generate_bundle(name: str, entries) -> str:

@@ -17,6 +17,10 @@ struct odv_net_ROBER_NONE_DENSE {
   static constexpr int RATE_KIND = 1;
   static constexpr int TGAS = -1;
   static constexpr bool CROUT = false;
+  // LUmethod 3: P by forward differences of the RHS (CalcJac_dense); AIDX[r*NACT + c]
+  // = storage index of P_order(r, c), -1 outside the LU pattern (there the difference
+  // is exactly zero: no RHS term of row r reads species c)
+  static constexpr bool FD_JAC = false;
   static constexpr int FLOPS_RHS = 8, FLOPS_JAC = 18, FLOPS_LU = 8, FLOPS_SOLVE = 13;
   static constexpr int FLOPS_LU_R = 11, FLOPS_SOLVE_R = 13;
   // permutation Perm (0-based): P_order[i][j] = P[Perm[i]][Perm[j]]

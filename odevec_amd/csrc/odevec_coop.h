@@ -89,6 +89,7 @@ struct CoopNet {
   static constexpr int NACT = 1;            // species per lane
   static constexpr int ACT[1] = {0};
   static constexpr int NNZ = NB;            // a lane's row of P / the factors
+  static constexpr bool FD_JAC = false;     // the symbolic Jacobian (tables below)
   static constexpr int NRATE = 1;           // the rates live in the group's LDS block
   static constexpr int NPARAM = Net_::NPARAM;
   static constexpr int RATE_KIND = Net_::RATE_KIND;

@@ -388,14 +388,57 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
   return sc;
 }
 
-// GetJac + LUDecompose (src/odevec.F90:301-304) at y = y_NS(:,:,0)
-template <class L>
-ODV_HD inline void odv_factor(L& s, const double* y) {
+// CalcJac_dense (src/odevec.F90:469-506), LUmethod 3: P = I - dt*beta*J with
+// column k of J a forward difference of the RHS in species k, step
+// r = max(srur*|y_k|, r0/sqrt(w_k)), r0 = 1e3*|dt|*uround*neq*||f||
+// (weighted norm, group-reduced like the reference's over the vector). The
+// literals uround = 2^-52 and srur = 2^-26 are exact in the reference's
+// default real. Entries are formed in FP64 and stored in the factor type.
+template <class L, class G>
+ODV_HD inline void odv_jac_fd(L& s, const G& g, const double* y0, double beta) {
+  using Net = typename L::Net;
+  using M = typename L::M;
+  constexpr int N = L::N;
+  double y[N], f[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) y[j] = y0[j];
+  odv_rhs(s, y, f);
+  const double nrm = g.max_active(odv_wnorm(s, g, f), s.active);
+  double r0 = 1e3 * fabs(s.dt) * 2.2204460492503131e-16 * (double)L::NF * nrm;
+  if (r0 == 0.0) r0 = 1.0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const double yk = y[k];
+    const double r = fmax(1.4901161193847656e-08 * fabs(yk), r0 / sqrt(s.w(k)));
+    y[k] = yk + r;
+    const double fac = -(s.dt * beta / r);
+    double d[N];
+    odv_rhs(s, y, d);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      // outside the LU pattern d[j] == f[j] exactly (row j does not read species k)
+      if (Net::AIDX[j * N + k] >= 0) {
+        const double v = (d[j] - f[j]) * fac;
+        s.a[Net::AIDX[j * N + k]] = (M)(j == k ? 1.0 + v : v);
+      }
+    }
+    y[k] = yk;
+  }
+}
+
+// GetJac (or CalcJac, LUmethod 3) + LUDecompose (src/odevec.F90:301-309) at
+// y = y_NS(:,:,0)
+template <class L, class G>
+ODV_HD inline void odv_factor(L& s, const G& g, const double* y) {
   using Net = typename L::Net;
   using M = typename L::M;
   constexpr int S = L::KS > 0 ? L::KS : 1;
-  const double hb = s.lq[0] * s.dt;
-  Net::template jac<M, S>(y, s.kptr(), hb, s.a);
+  if constexpr (Net::FD_JAC) {
+    odv_jac_fd(s, g, y, s.lq[0]);
+  } else {
+    const double hb = s.lq[0] * s.dt;
+    Net::template jac<M, S>(y, s.kptr(), hb, s.a);
+  }
   if (L::RCP_) Net::template lu_r<M>(s.a);
   else Net::template lu<M>(s.a);
 }
@@ -604,7 +647,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   if (s.upd_jac) {
     double y[N];
     odv_pred_y(s, y);
-    odv_factor(s, y);
+    odv_factor(s, g, y);
     s.upd_jac = false;
     s.lju = 0;
     s.rate = 0.7;

@@ -12,8 +12,13 @@ library (``python -m odevec_amd.build --extra <id>``) adds the network to it.
 Flag notes:
 * ``--nvector`` is accepted for compatibility; the cell count is a runtime
   size of the library, not a code-generation constant.
-* ``--LUmethod`` 1 is the method this preprocessor generates (symbolic
-  Jacobian + generated sparse LU); 2 and 3 are rejected with a message.
+* ``--LUmethod`` 1 (symbolic Jacobian + generated LU) and 3 (Jacobian by
+  forward differences of the RHS, src/odevec.F90:469-506, then the generated
+  LU; variant id ``NAME/NONE/DENSE/LU3``) are generated. 3 needs DENSE
+  packaging (the reference aborts for CSC, src/odevec.F90:509-523) and no
+  ordering (the reference permutes the residual but not this Jacobian,
+  src/odevec.F90:306-308, 328-342, which is only consistent without one).
+  2 (closed-form LU) is rejected with a message.
 * ``--heatcool 1`` needs KROME's heating/cooling Fortran and is rejected.
 * ``--equilibrium_H`` is validated exactly as the reference validates it
   (pre_odevec.py:455-460), which means it can never be set together with
@@ -33,11 +38,16 @@ from . import networks, structure, emit
 
 def generate(example: str, ordering: str, packaging: str, outdir: str,
              krome_setupfile: Optional[str] = None, dt_min: Optional[float] = None,
-             quiet: bool = True) -> dict:
+             quiet: bool = True, lu_method: int = 1) -> dict:
     net = networks.load(example, krome_setupfile)
     st = structure.analyse(net, ordering, packaging)
+    if lu_method not in (1, 3):
+        raise ValueError("LUmethod %d is not generated (1: symbolic Jacobian, 3: finite differences)" % lu_method)
+    if lu_method == 3 and (st.packaging != "DENSE" or st.ordering != "None"):
+        raise ValueError("LUmethod 3 needs DENSE packaging and no ordering")
+    st.lu_method = lu_method
     os.makedirs(outdir, exist_ok=True)
-    base = "%s_%s_%s" % (net.name, st.ordering.upper(), st.packaging)
+    base = "%s_%s_%s%s" % (net.name, st.ordering.upper(), st.packaging, "_LU3" if lu_method == 3 else "")
     hdr = emit.emit_header(st)
     ir = emit.ir_dict(st)
     if dt_min is not None:
@@ -92,14 +102,17 @@ def main(argv: Optional[List[str]] = None) -> int:
         sys.exit("args.equilibrium_H is only allowed without ordering.")
     if args.equilibrium_H == 1 and args.LUmethod != 3:
         sys.exit("args.equilibrium_H is currently only usable with LUmethod==3.")
-    if args.LUmethod != 1:
-        sys.exit("LUmethod %d is not generated; LUmethod 1 (symbolic Jacobian, generated LU) is." % args.LUmethod)
+    if args.LUmethod not in (1, 3):
+        sys.exit("LUmethod %d is not generated; 1 (symbolic Jacobian) and 3 (finite differences) are."
+                 % args.LUmethod)
+    if args.LUmethod == 3 and (args.packaging != "DENSE" or args.ordering not in ("None", "NONE")):
+        sys.exit("LUmethod 3 needs DENSE packaging and no ordering.")
     if args.heatcool:
         sys.exit("--heatcool needs KROME's heating/cooling routines, which are not part of this preprocessor.")
     if args.packaging not in structure.PACKAGINGS:
         sys.exit("# Error: This format is not known.")
     ir = generate(args.example, args.ordering, args.packaging, args.outdir,
-                  args.krome_setupfile, args.dt_min, quiet=False)
+                  args.krome_setupfile, args.dt_min, quiet=False, lu_method=args.LUmethod)
     if args.sparsity_structure == "True":
         n = ir["neq"]
         lu = {tuple(e) for e in ir["lu_entries"]}

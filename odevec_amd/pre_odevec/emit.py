@@ -48,11 +48,12 @@ def _lit(v: float) -> str:
 
 
 def struct_name(st: Structure) -> str:
-    return "odv_net_%s_%s_%s" % (st.net.name, st.ordering.upper(), st.packaging)
+    return "odv_net_%s_%s_%s%s" % (st.net.name, st.ordering.upper(), st.packaging,
+                                   "_LU3" if st.lu_method == 3 else "")
 
 
 def network_id(st: Structure) -> str:
-    return "%s/%s/%s" % (st.net.name, st.ordering.upper(), st.packaging)
+    return "%s/%s/%s%s" % (st.net.name, st.ordering.upper(), st.packaging, "/LU3" if st.lu_method == 3 else "")
 
 
 class _Counter:
@@ -313,6 +314,14 @@ def opcounts(st: Structure) -> Dict[str, Dict[str, int]]:
         c = _Counter()
         fn(c)
         out[name] = c.as_dict()
+    if st.lu_method == 3:
+        # forward differences: N+1 RHS evaluations; the weighted norm of f
+        # (3N+2), r0 (4), per column 9 (|y|, two products, sqrt, quotient,
+        # max, y+r, dt*beta, its quotient), per entry 2 (difference, product),
+        # per diagonal 1
+        n = st.neq
+        fl = (n + 1) * out["rhs"]["flops"] + 3 * n + 6 + n * (9 + 2 * n) + n
+        out["jac"] = {"add": 0, "mul": 0, "div": 0, "flops": fl}
     return out
 
 
@@ -444,6 +453,14 @@ def emit_header(full: Structure) -> str:
     hdr.append("  static constexpr int RATE_KIND = %d;" % RATE_KIND_ID[net.rate_kind])
     hdr.append("  static constexpr int TGAS = %d;" % net.tgas_index)
     hdr.append("  static constexpr bool CROUT = %s;" % ("true" if crout else "false"))
+    hdr.append("  // LUmethod 3: P by forward differences of the RHS (CalcJac_dense); AIDX[r*NACT + c]")
+    hdr.append("  // = storage index of P_order(r, c), -1 outside the LU pattern (there the difference")
+    hdr.append("  // is exactly zero: no RHS term of row r reads species c)")
+    hdr.append("  static constexpr bool FD_JAC = %s;" % ("true" if st.lu_method == 3 else "false"))
+    if st.lu_method == 3:
+        pos = {tuple(map(int, e)): k for k, e in enumerate(st.csc.entries)}
+        aidx = [pos.get((r, c), -1) for r in range(st.neq) for c in range(st.neq)]
+        hdr.append("  static constexpr short AIDX[%d] = {%s};" % (len(aidx), ", ".join(str(a) for a in aidx)))
     hdr.append("  static constexpr int FLOPS_RHS = %d, FLOPS_JAC = %d, FLOPS_LU = %d, FLOPS_SOLVE = %d;" % (
         ops["rhs"]["flops"], ops["jac"]["flops"], ops["lu"]["flops"], ops["solve"]["flops"]))
     hdr.append("  static constexpr int FLOPS_LU_R = %d, FLOPS_SOLVE_R = %d;" % (
@@ -536,6 +553,7 @@ def ir_dict(st: Structure) -> dict:
         "u_col_start": st.csc.u_col_start,
         "l_col_start": st.csc.l_col_start,
         "nnz": st.csc.nnz,
+        "lu_method": st.lu_method,
         # the generated code integrates the active species only
         "active": reduce_frozen(analyse_copy(st)).active,
         "opcounts": opcounts(st),

@@ -222,6 +222,11 @@ class Structure:
     # frozen: their RHS is identically zero and no RHS term reads them
     active: Optional[List[int]] = None
 
+    # the reference's LUmethod: 1 symbolic Jacobian (GetJac) + LU; 3 the
+    # Jacobian by forward differences of the RHS (CalcJac_dense,
+    # src/odevec.F90:469-506) + LU
+    lu_method: int = 1
+
     @property
     def neq(self) -> int:
         """Number of species in this structure's index space."""
@@ -270,6 +275,7 @@ def reduce_frozen(st: Structure) -> Structure:
     LU = st.LU[np.ix_(keep_pos, keep_pos)]
     red = Structure(st.net, st.ordering, st.packaging, terms, J, P, perm, Po, LU, csc_structure(LU))
     red.active = active
+    red.lu_method = st.lu_method
     return red
 
 

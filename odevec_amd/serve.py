@@ -168,7 +168,7 @@ class Server:
     def advance(self, ss: Session, t_stop, rid: int = -1, arrival: Optional[float] = None) -> Request:
         """Queue an advance of every cell of the session to t_stop (scalar or
         per cell); the request's result is the state there."""
-        ts = np.ascontiguousarray(np.broadcast_to(np.asarray(t_stop, dtype=np.float64), (ss.ncells,)))
+        ts = np.array(np.broadcast_to(np.asarray(t_stop, dtype=np.float64), (ss.ncells,)))
         r = Request(rid, ss.network, None, ts, ss.rtol, ss.atol, ss.dt_min, None, precision=ss.precision,
                     arrival=self.now() if arrival is None else arrival, session=ss)
         self.waiting.append(r)

@@ -12,14 +12,17 @@
  * share one step size, order and verdict. G = 1 gives the per-cell
  * semantics. Networks are evaluated from the preprocessor's term table
  * (one monomial per term, canonical left-to-right order), the Jacobian by
- * differentiating those monomials, the factorizations with the CSC index
+ * differentiating those monomials (or, LUmethod 3, by CalcJac_dense's
+ * forward differences of the RHS, src/odevec.F90:469-506), the factorizations with the CSC index
  * arrays exactly as the reference loops over them (src/odevec.F90:853-971)
  * or dense (src/odevec.F90:797-849, 898-922).
  *
  * Parity pins (see tests/): BDF l-vectors against their closed form,
  * LU factors against L*U = P, RHS against the reference network
  * definitions' own values, rates against the reference formulas, whole
- * solves against an independent Radau integrator at rtol 1e-12.
+ * solves against an independent Radau integrator at rtol 1e-12, the
+ * forward-difference Jacobian against the analytic one within its error
+ * model (truncation + rounding).
  *
  * Build: make -C oracle (gcc, -O2 -ffp-contract=off).
  */
@@ -52,6 +55,7 @@ typedef struct {
   const int* l_col_start;/* [neq] */
   const double* arr_A;   /* arrhenius */
   const double* arr_Ea;
+  int lu_method;         /* 1: GetJac (symbolic P); 3: CalcJac_dense (forward differences) */
 } onet_t;
 
 
@@ -478,16 +482,59 @@ static int step_order(ovec_t* v, double* scale) {
   return 0;
 }
 
-/* GetJac + LUDecompose for all cells */
+/* CalcJac_dense (src/odevec.F90:469-506), LUmethod 3: P = I - dt*beta*J with
+ * column k of J by a forward difference of the RHS in species k, step
+ * r = max(srur*|y_k|, r0/sqrt(w_k)), r0 = 1e3*|dt|*uround*neq*||f|| (the
+ * weighted norm over the vector, masked cells excluded). Writes P of every
+ * cell, row-major, to Pall[i*N*N ...]. The literals uround = 2^-52 and srur =
+ * 2^-26 are exact in the reference's default real. */
+static void fd_p_all(ovec_t* v, const double* y, double dt, double beta, double* Pall) {
+  const int G = v->G, N = v->N;
+  const double uround = 2.2204460492503131e-16, srur = 1.4901161193847656e-08;
+  double* yy = (double*)malloc(sizeof(double) * (size_t)N * G);
+  double* f0 = (double*)malloc(sizeof(double) * (size_t)N * G);
+  double* D = (double*)malloc(sizeof(double) * (size_t)N * G);
+  double* r = (double*)malloc(sizeof(double) * (size_t)G);
+  double* fac = (double*)malloc(sizeof(double) * (size_t)G);
+  memcpy(yy, y, sizeof(double) * (size_t)N * G);
+  rhs_all(v, yy, f0);
+  const double nrm = wnorm(v, f0);
+  double r0 = 1e3 * fabs(dt) * uround * (double)N * nrm;
+  if (r0 == 0.0) r0 = 1.0;
+  for (int k = 0; k < N; ++k) {
+    for (int i = 0; i < G; ++i) {
+      const double ytmp = A2(yy, k, i);
+      r[i] = fmax(srur * fabs(ytmp), r0 / sqrt(A2(v->w, k, i)));
+      A2(yy, k, i) = ytmp + r[i];
+      fac[i] = -(dt * beta / r[i]);
+    }
+    rhs_all(v, yy, D);
+    for (int i = 0; i < G; ++i) {
+      for (int j = 0; j < N; ++j) Pall[(size_t)i * N * N + (size_t)j * N + k] = (A2(D, j, i) - A2(f0, j, i)) * fac[i];
+      A2(yy, k, i) = A2(y, k, i);
+    }
+  }
+  for (int i = 0; i < G; ++i)
+    for (int j = 0; j < N; ++j) Pall[(size_t)i * N * N + (size_t)j * N + j] = 1.0 + Pall[(size_t)i * N * N + (size_t)j * N + j];
+  free(yy); free(f0); free(D); free(r); free(fac);
+}
+
+/* GetJac (or CalcJac, LUmethod 3) + LUDecompose for all cells */
 static void factor_all(ovec_t* v, const double* y, double dt) {
   const onet_t* n = v->n;
   const int G = v->G, N = v->N;
   const double hb = v->coeff[v->order][0] * dt;
   double* yc = v->tmp;
   double* P = v->tmp + 2 * N;
+  double* Pfd = NULL;
+  if (n->lu_method == 3) {
+    Pfd = (double*)malloc(sizeof(double) * (size_t)N * N * G);
+    fd_p_all(v, y, dt, v->coeff[v->order][0], Pfd);
+  }
   for (int i = 0; i < G; ++i) {
     for (int j = 0; j < N; ++j) yc[j] = A2(y, j, i);
-    oracle_p_dense(n, yc, v->k + (size_t)i * (n->nrate > 0 ? n->nrate : 1), hb, P);
+    if (Pfd) memcpy(P, Pfd + (size_t)i * N * N, sizeof(double) * (size_t)N * N);
+    else oracle_p_dense(n, yc, v->k + (size_t)i * (n->nrate > 0 ? n->nrate : 1), hb, P);
     double* A = v->LU + (size_t)i * (n->csc ? n->nnz : N * N);
     if (n->csc) {
       /* jac%sdata in CSC order (fill entries are structural zeros of P) */
@@ -517,6 +564,7 @@ static void factor_all(ovec_t* v, const double* y, double dt) {
       }
     }
   }
+  free(Pfd);
 }
 
 /* permute residual, SolveLU, unpermute (src/odevec.F90:328-342, 797-891) */
@@ -850,6 +898,23 @@ void oracle_factor_one(const onet_t* n, const double* y, const double* k, double
   v.LU = out;
   /* factor_all uses coeff[order][0]*dt as hb: order 1 has beta = 1 */
   factor_all(&v, y, hb);
+  free(tmp);
+}
+
+/* CalcJac_dense's P for one cell with weights w, step dt and beta, for tests */
+void oracle_fd_p_one(const onet_t* n, const double* y, const double* k, const double* w, double dt, double beta,
+                     double* P) {
+  ovec_t v;
+  memset(&v, 0, sizeof v);
+  const int N = n->neq;
+  v.n = n;
+  v.G = 1;
+  v.N = N;
+  double* tmp = (double*)calloc((size_t)4 * N + 16, sizeof(double));
+  v.tmp = tmp;
+  v.k = (double*)k;
+  v.w = (double*)w;
+  fd_p_all(&v, y, dt, beta, P);
   free(tmp);
 }
 

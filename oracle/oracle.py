@@ -29,7 +29,7 @@ class ONet(C.Structure):
                 ("t_row", _IP), ("t_coef", _DP), ("t_rate", _IP), ("t_nfac", _IP), ("t_fac", _IP),
                 ("perm", _IP), ("csc", C.c_int), ("nnz", C.c_int),
                 ("row_index", _IP), ("u_col_start", _IP), ("l_col_start", _IP),
-                ("arr_A", _DP), ("arr_Ea", _DP)]
+                ("arr_A", _DP), ("arr_Ea", _DP), ("lu_method", C.c_int)]
 
 
 class OCtl(C.Structure):
@@ -67,6 +67,7 @@ def lib():
         L.oracle_p_dense.argtypes = [C.POINTER(ONet), _DP, _DP, C.c_double, _DP]
         L.oracle_factor_one.argtypes = [C.POINTER(ONet), _DP, _DP, C.c_double, _DP]
         L.oracle_solve_one.argtypes = [C.POINTER(ONet), _DP, _DP, _DP]
+        L.oracle_fd_p_one.argtypes = [C.POINTER(ONet), _DP, _DP, _DP, C.c_double, C.c_double, _DP]
         L.oracle_rates_one.argtypes = [C.POINTER(ONet), _DP, _DP, _DP]
         L.oracle_primordial_rates.argtypes = [_DP, C.c_int, C.c_int, _DP]
         L.oracle_coeff_table.argtypes = [_DP]
@@ -114,7 +115,7 @@ class Network:
         self.s = ONet(self.neq, self.nrate, self.nparam, RATE_KIND[ir["rate_kind"]], ir["tgas_index"], nt,
                       _ip(self._row), _dp(self._coef), _ip(self._rate), _ip(self._nfac), _ip(self._fac),
                       _ip(self._perm), 1 if self.csc else 0, self.nnz, _ip(self._ri), _ip(self._ucs),
-                      _ip(self._lcs), _dp(self._A), _dp(self._Ea))
+                      _ip(self._lcs), _dp(self._A), _dp(self._Ea), int(ir.get("lu_method", 1)))
 
     @classmethod
     def load(cls, net_id: str) -> "Network":
@@ -153,6 +154,17 @@ class Network:
             k = np.zeros(1)
         P = np.zeros(self.neq * self.neq)
         lib().oracle_p_dense(C.byref(self.s), _dp(y), _dp(k), float(hb), _dp(P))
+        return P.reshape(self.neq, self.neq)
+
+    def fd_p(self, y, k, w, dt, beta) -> np.ndarray:
+        """CalcJac_dense's P = I - dt*beta*J (forward differences), row-major."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        if k.size == 0:
+            k = np.zeros(1)
+        P = np.zeros(self.neq * self.neq)
+        lib().oracle_fd_p_one(C.byref(self.s), _dp(y), _dp(k), _dp(w), float(dt), float(beta), _dp(P))
         return P.reshape(self.neq, self.neq)
 
     def factor(self, y, k, hb) -> np.ndarray:

@@ -12,6 +12,8 @@
 #include "generated/net_PRIMORDIAL_CMK_CSC.h"
 #include "generated/net_PRIMORDIAL_NONE_DENSE.h"
 #include "generated/net_PRIMORDIAL_FF_CSC.h"
+#include "generated/net_ROBER_NONE_DENSE_LU3.h"
+#include "generated/net_PRIMORDIAL_NONE_DENSE_LU3.h"
 
 namespace {
 
@@ -97,6 +99,8 @@ extern "C" int lane_solve_cells(const char* id, int mixed, long n, const double*
   ODV_CASE(odv_net_PRIMORDIAL_CMK_CSC)
   ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE)
   ODV_CASE(odv_net_PRIMORDIAL_FF_CSC)
+  ODV_CASE(odv_net_ROBER_NONE_DENSE_LU3)
+  ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE_LU3)
 #undef ODV_CASE
   return -1;
 }
@@ -121,6 +125,8 @@ extern "C" int lane_netfun(const char* id, int what, const double* y, const doub
   ODV_CASE(odv_net_PRIMORDIAL_CMK_CSC)
   ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE)
   ODV_CASE(odv_net_PRIMORDIAL_FF_CSC)
+  ODV_CASE(odv_net_ROBER_NONE_DENSE_LU3)
+  ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE_LU3)
 #undef ODV_CASE
   return -1;
 }

@@ -15,6 +15,9 @@ NETS = {
     "PRIMORDIAL/CMK/CSC": lambda n: W.primordial(n),
     "PRIMORDIAL/NONE/DENSE": lambda n: W.primordial(n, network="PRIMORDIAL/NONE/DENSE"),
     "PRIMORDIAL/FF/CSC": lambda n: W.primordial(n, network="PRIMORDIAL/FF/CSC"),
+    # LUmethod 3: Jacobian by forward differences of the RHS
+    "ROBER/NONE/DENSE/LU3": lambda n: dict(W.rober(n), network="ROBER/NONE/DENSE/LU3"),
+    "PRIMORDIAL/NONE/DENSE/LU3": lambda n: W.primordial(n, network="PRIMORDIAL/NONE/DENSE/LU3"),
 }
 
 

@@ -52,6 +52,9 @@ CASES = {
     "PRIMORDIAL": lambda: W.primordial(512, seed=3),
     "PRIMORDIAL_FF": lambda: W.primordial(256, seed=3, network="PRIMORDIAL/FF/CSC"),
     "PRIMORDIAL_DENSE": lambda: W.primordial(256, seed=3, network="PRIMORDIAL/NONE/DENSE"),
+    # LUmethod 3: the Jacobian by forward differences of the RHS
+    "ROBER_LU3": lambda: dict(W.rober(32, seed=1), network="ROBER/NONE/DENSE/LU3"),
+    "PRIMORDIAL_LU3": lambda: W.primordial(256, seed=3, network="PRIMORDIAL/NONE/DENSE/LU3"),
 }
 
 
@@ -68,7 +71,8 @@ def test_repro_bitwise_percell(oracle, case, mode):
     assert stats["attempts"] == int(cr[0].sum()) and stats["newton"] == int(cr[2].sum())
 
 
-@pytest.mark.parametrize("case,G", [("ROBER", 32), ("OREGO", 64), ("PRIMORDIAL", 64)])
+@pytest.mark.parametrize("case,G", [("ROBER", 32), ("OREGO", 64), ("PRIMORDIAL", 64), ("ROBER_LU3", 32),
+                                    ("PRIMORDIAL_LU3", 64)])
 def test_repro_bitwise_lockstep(oracle, case, G):
     """Lockstep groups reproduce the reference at nvector = G (shared t_stop)."""
     w = CASES[case]()

@@ -115,6 +115,59 @@ def test_jacobian_finite_differences(oracle, nid):
     assert np.allclose(P, Pfd, rtol=1e-6, atol=1e-8)
 
 
+@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE/LU3", "PRIMORDIAL/NONE/DENSE/LU3"])
+def test_calcjac_forward_differences(oracle, nid):
+    """LUmethod 3 (CalcJac_dense): the forward-difference P against the
+    analytic P of the same network. With steps r_k = 2^-26 |y_k| (the
+    r0/sqrt(w_k) branch is far smaller at these weights) an entry's error
+    is at most |dt*beta| * (truncation r_k |d2f_j/dy_k2| + rounding
+    eps*T_j/r_k), T_j the sum of |terms| of row j; both are below
+    |dt*beta| * T_j/|y_k| * 1e-6, about 1e-6 of the entry's natural scale,
+    so a wrong step, sign or column fails."""
+    net = _net(oracle, nid)
+    sym = _net(oracle, nid[:-4])
+    rng = np.random.default_rng(7)
+    n = net.neq
+    for trial in range(5):
+        y = np.exp(rng.uniform(-3, 1, n))
+        k = np.exp(rng.uniform(-3, 0, max(net.nrate, 1))) if net.nrate else np.zeros(1)
+        if "PRIMORDIAL" in nid:
+            y[net.ir["tgas_index"]] = 10.0 ** rng.uniform(1, 4)
+            k = oracle.primordial_rates(y)
+        w = 1.0 / (1e-6 * y + 1e-20) ** 2
+        dt, beta = 10.0 ** rng.uniform(-8, -2), 6.0 / 11.0
+        Pfd = net.fd_p(y, k, w, dt, beta)
+        Pan = sym.p_dense(y, k, dt * beta)
+        assert np.array_equal(np.diag(Pfd) != 0, np.ones(n, dtype=bool))
+        # the analytic structural zeros are exact zeros of the differences too
+        assert (Pfd[(Pan == 0)] == 0).all()
+        T = np.zeros(n)
+        for row, coef, rate, fac in net.ir["terms"]:
+            T[row] += abs(coef * (k[rate] if rate >= 0 else 1.0) * np.prod([y[f] for f in fac]))
+        bound = abs(dt * beta) * T[:, None] / y[None, :] * 1e-6
+        err = np.abs(Pfd - Pan)
+        assert (err <= bound + 4e-16 * np.abs(Pan)).all(), np.max(err / (bound + 1e-300))
+        # the bound is tight enough to see the Jacobian: most off-diagonal
+        # entries exceed it by orders of magnitude
+        off = (Pan != 0) & ~np.eye(n, dtype=bool)
+        assert np.median(np.abs(Pan[off]) / bound[off]) > 1e3
+
+
+def test_lumethod3_solution_agrees_with_symbolic(oracle):
+    """A ROBER solve with the difference Jacobian stays within the
+    tolerance of the symbolic-Jacobian solve (only the Newton iteration
+    matrix differs; converged steps satisfy the same equations)."""
+    from odevec_amd import workloads as W
+    w = W.rober(16, seed=3)
+    a = _net(oracle, "ROBER/NONE/DENSE").solve_cells(w["y"], w["t_stop"], params=w["params"], rtol=1e-6,
+                                                     atol=1e-10)
+    b = _net(oracle, "ROBER/NONE/DENSE/LU3").solve_cells(w["y"], w["t_stop"], params=w["params"], rtol=1e-6,
+                                                         atol=1e-10)
+    assert (a[1] == 0).all() and (b[1] == 0).all()
+    assert np.all(np.abs(a[0] - b[0]) <= 10 * 1e-6 * np.abs(a[0]) + 1e-9)
+    assert not np.array_equal(a[0], b[0])
+
+
 def _unpack(net, lu):
     """L and U from the factor storage (Crout: unit U; Doolittle: unit L)."""
     n = net.neq
