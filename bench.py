@@ -196,6 +196,68 @@ def sessions(nsess=8, ncells=65536, nadv=5):
     return out
 
 
+def timeline(w):
+    """One more headline solve with the per-cell timeline attached: when the
+    cells end, relative to the first cell's start (ms), and the attempts of
+    the slowest cells -- the launch's tail."""
+    import torch
+    from odevec_amd.api import Odevec
+    n = w["y"].shape[1]
+    dev = torch.device("cuda:0")
+    ode = Odevec(w["network"], n, precision=1, mode=2)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    times, hz = ode.enable_timeline(n)
+    y = torch.tensor(w["y"], device=dev)
+    ts = torch.tensor(np.asarray(w["t_stop"], dtype=np.float64), device=dev)
+    dt0 = torch.tensor(np.broadcast_to(w["dt0"], (n,)).copy(), device=dev)
+    cnt = torch.zeros(8 * n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    ode.solve_cells_async(n, ts.data_ptr(), y.data_ptr(), dt0_ptr=dt0.data_ptr(), counters_ptr=cnt.data_ptr(),
+                          stream=stream)
+    torch.cuda.synchronize()
+    t = times.cpu().numpy().reshape(2, n).astype(np.float64)
+    att = cnt.cpu().numpy().reshape(8, n)[0]
+    ode.CloseOdevec()
+    t0 = t[0].min()
+    end = (t[1] - t0) / hz * 1e3
+    start = (t[0] - t0) / hz * 1e3
+    slow = np.argsort(-att)[:5]
+    return {"cells_end_ms": {"p50": float(np.percentile(end, 50)), "p99": float(np.percentile(end, 99)),
+                             "p99.9": float(np.percentile(end, 99.9)), "max": float(end.max())},
+            "last_cell_start_ms": float(start.max()),
+            "attempts": {"median": float(np.median(att)), "p99.9": float(np.percentile(att, 99.9)),
+                         "max": int(att.max())},
+            "slowest_cells": [{"attempts": int(att[k]), "start_ms": float(start[k]), "end_ms": float(end[k])}
+                              for k in slow],
+            "note": "mixed persistent, counters on; ticks of the device wall clock"}
+
+
+def end_to_end(w, steps, kernel_ms):
+    """The headline solve through the host-buffer entry point
+    (odevec_solve_cells: the state, end times and dt hints go host -> device
+    over PCIe, the state and status come back), mixed persistent form:
+    cell-solves/s of the whole call, beside the device-resident kernel time
+    measured for the headline (`kernel_ms`)."""
+    from odevec_amd.api import Odevec
+    n = w["y"].shape[1]
+    ode = Odevec(w["network"], n, precision=1, mode=2)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    ts = np.ascontiguousarray(w["t_stop"])
+    dt0 = np.ascontiguousarray(np.broadcast_to(w["dt0"], (n,)))
+    walls = []
+    for k in range(steps + 1):
+        y = np.array(w["y"], order="C", copy=True)
+        t = time.perf_counter()
+        ode.solve_cells(y, ts, dt0=dt0, raise_on_fail=False)
+        if k > 0:          # the first call sizes the handle's buffers
+            walls.append(time.perf_counter() - t)
+    ode.CloseOdevec()
+    wall = float(np.mean(walls))
+    return {"value": n / wall, "unit": "cell-solves/s", "ms_per_call": wall * 1e3,
+            "device_resident_kernel_ms": kernel_ms, "copies_and_host_ms": wall * 1e3 - kernel_ms,
+            "entry": "odevec_solve_cells (host arrays), mixed persistent"}
+
+
 def cpu_baseline(w, seconds=12.0):
     """The oracle (a C restatement of the reference solver) on the host cores,
     per-cell mode, on a seeded sample of the same workload."""
@@ -296,6 +358,8 @@ def main():
         "flops_per_launch": {"fp64": float(fl_m[0]), "fp32": float(fl_m[1])},
         "kernel_ms": {"mixed": ms_m, "fp64": ms_d},
     }
+    res["end_to_end"] = end_to_end(w, a.steps, float(np.mean(ms_m)))
+    res["timeline"] = timeline(w)
     if not a.no_batch:
         res["batching"] = batching(a.steps, a.warmup)
         res["synth64"] = synth64(a.steps, a.warmup)

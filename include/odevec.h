@@ -67,11 +67,13 @@ typedef struct {
   int32_t precision;      /* ODEVEC_FP64 | ODEVEC_MIXED */
   int32_t mode;           /* ODEVEC_PERCELL | ODEVEC_LOCKSTEP | ODEVEC_PERSISTENT */
   int32_t group;          /* lockstep group size, 1..256; 0 = min(nvector, 256) */
-  int32_t block;          /* threads per workgroup (percell/persistent), 0 = 256 */
+  int32_t block;          /* threads per workgroup (percell/persistent), 0 = 64 */
   int32_t grid;           /* persistent: workgroups, 0 = fill the device */
   int32_t max_attempts;   /* per-cell step-attempt budget, 0 = 1000000 */
   int32_t device_ptrs;    /* 1: array arguments are device pointers (no copies) */
   int32_t abort_on_error; /* 1: print and abort on a failed cell, like the reference's STOP */
+  int32_t schedule;       /* persistent: 0 = costliest cells first (ranked by log2(span / initial
+                             step) before the launch), 1 = index order; results are the same */
 } odevec_opts_t;
 
 typedef struct {
@@ -167,6 +169,14 @@ int32_t odevec_destroy(odevec_t* h);
  * ncells cells. NULL detaches. Calls sharing one history must not overlap. */
 int32_t odevec_history_planes(const char* network, int64_t* planes);
 int32_t odevec_set_history(odevec_t* h, double* d_hist, int64_t ncells);
+
+/* Timeline (diagnostics; no reference counterpart): with d_times attached
+ * (device, 2 x ncells int64), every solve on the handle writes the device
+ * wall-clock tick at which each cell was loaded (d_times[i]) and finished
+ * (d_times[ncells + i]); ticks_per_second (optional) receives the clock's
+ * rate. It shows a launch's tail: how long the last cells run after the rest
+ * are done. Every solve must cover exactly ncells cells; NULL detaches. */
+int32_t odevec_set_timeline(odevec_t* h, int64_t* d_times, int64_t ncells, int64_t* ticks_per_second);
 
 /* Batches ------------------------------------------------------------------
  * No reference counterpart: the reference solves one network per build and

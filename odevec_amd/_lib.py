@@ -26,7 +26,7 @@ _VP = C.c_void_p
 
 class Opts(C.Structure):
     _fields_ = [("precision", _I32), ("mode", _I32), ("group", _I32), ("block", _I32), ("grid", _I32),
-                ("max_attempts", _I32), ("device_ptrs", _I32), ("abort_on_error", _I32)]
+                ("max_attempts", _I32), ("device_ptrs", _I32), ("abort_on_error", _I32), ("schedule", _I32)]
 
 
 class Stats(C.Structure):
@@ -86,6 +86,7 @@ def lib():
         "odevec_kernel_info": (_I32, [C.c_char_p, _I32, _I32, _I32, C.POINTER(KernelInfo)]),
         "odevec_history_planes": (_I32, [C.c_char_p, C.POINTER(_I64)]),
         "odevec_set_history": (_I32, [_VP, _VP, _I64]),
+        "odevec_set_timeline": (_I32, [_VP, _VP, _I64, C.POINTER(_I64)]),
         "odevec_default_opts": (None, [C.POINTER(Opts)]),
         "odevec_create": (_I32, [C.c_char_p, _I64, C.POINTER(Opts), C.POINTER(_VP)]),
         "odevec_set_tolerances": (_I32, [_VP, C.c_double, C.c_double]),
@@ -117,7 +118,7 @@ EXPORTED = [
     "odevec_network_info", "odevec_kernel_info", "odevec_default_opts", "odevec_create",
     "odevec_set_tolerances", "odevec_set_dt_min", "odevec_set_check_negatives", "odevec_solve",
     "odevec_solve_cells", "odevec_solve_cells_async", "odevec_destroy", "odevec_history_planes",
-    "odevec_set_history", "odevec_bundle_count",
+    "odevec_set_history", "odevec_set_timeline", "odevec_bundle_count",
     "odevec_bundle_name", "odevec_bundle_entry", "odevec_batch_create", "odevec_batch_solve_async",
     "odevec_batch_kernel_info", "odevec_batch_destroy",
 ]

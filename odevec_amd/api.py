@@ -34,12 +34,14 @@ from . import _lib as L
 class Odevec:
     def __init__(self, network: str, nvector: int, precision: int = L.FP64, mode: int = L.PERCELL,
                  group: int = 0, block: int = 0, grid: int = 0, max_attempts: int = 0,
-                 abort_on_error: bool = False, init: bool = True):
+                 abort_on_error: bool = False, init: bool = True, schedule: int = 0):
         self.network = network
         self.nvector = int(nvector)
-        self._opts = L.Opts(precision, mode, group, block, grid, max_attempts, 0, 1 if abort_on_error else 0)
+        self._opts = L.Opts(precision, mode, group, block, grid, max_attempts, 0, 1 if abort_on_error else 0,
+                            int(schedule))
         self._h = None
         self._hist = None
+        self._times = None
         info = L.network_info(network)
         self.neq = info["neq"]
         self.nnz = info["nnz"]
@@ -67,6 +69,7 @@ class Odevec:
             L.lib().odevec_destroy(self._h)
             self._h = None
         self._hist = None
+        self._times = None
 
     # -- warm restart (no reference counterpart: its SolveODE cold-starts every call)
     def enable_history(self, ncells: int):
@@ -80,6 +83,17 @@ class Odevec:
         torch.cuda.synchronize()   # the fill runs on torch's stream, the solves on others
         L.check(L.lib().odevec_set_history(self._h, C.c_void_p(self._hist.data_ptr()), int(ncells)))
         return self._hist
+
+    def enable_timeline(self, ncells: int):
+        """Attach a device array of per-cell start / end ticks (2 x ncells);
+        returns (tensor, ticks per second). Later solves over exactly those
+        cells fill it."""
+        import torch
+        self._times = torch.zeros(2 * int(ncells), dtype=torch.int64, device="cuda")
+        torch.cuda.synchronize()
+        hz = C.c_int64(0)
+        L.check(L.lib().odevec_set_timeline(self._h, C.c_void_p(self._times.data_ptr()), int(ncells), C.byref(hz)))
+        return self._times, int(hz.value)
 
     def disable_history(self):
         L.check(L.lib().odevec_set_history(self._h, None, 0))

@@ -89,6 +89,12 @@ struct odevec_s {
   int nscal = 0;
   double* d_hist = nullptr;          // caller's warm-restart history (odevec_set_history), or null
   int64_t hist_cells = 0;
+  long long* d_times = nullptr;      // caller's per-cell start/end ticks (odevec_set_timeline), or null
+  int64_t times_cells = 0;
+  int* d_order = nullptr;            // persistent schedule: queue position -> cell
+  unsigned char* d_bin = nullptr;    // per-cell cost bin
+  unsigned* d_count = nullptr;       // COST_BINS counters / offsets
+  int64_t order_cap = 0;
   std::vector<int> h_status;
   std::vector<int> h_cnt;
 };
@@ -160,6 +166,12 @@ int launch(odevec_s* h, const Req& r) {
   a.dt_out = r.dt_out;
   a.counters = r.counters;
   a.flops = r.flops;
+  if (h->d_times) {
+    if (r.ncells != h->times_cells)
+      return fail(ODEVEC_EINVAL, "the solve covers " + std::to_string(r.ncells) + " cells; the timeline holds " +
+                                     std::to_string(h->times_cells));
+    a.times = h->d_times;
+  }
   if (h->d_hist) {
     if (r.ncells != h->hist_cells)
       return fail(ODEVEC_EINVAL, "the solve covers " + std::to_string(r.ncells) + " cells; the history holds " +
@@ -193,6 +205,25 @@ int launch(odevec_s* h, const Req& r) {
     h->ws_cap = p.ws_doubles;
   }
   a.ws = h->d_ws;
+  // cost-ordered queue when the cells outnumber the launched lanes
+  if (h->opts.mode == ODEVEC_PERSISTENT && h->opts.schedule == 0 && r.ncells > p.grid * p.block) {
+    if (r.ncells > h->order_cap) {
+      if (h->d_order) (void)hipFree(h->d_order);
+      if (h->d_bin) (void)hipFree(h->d_bin);
+      h->d_order = nullptr;
+      h->d_bin = nullptr;
+      h->order_cap = 0;
+      if ((e = hipMalloc((void**)&h->d_order, (size_t)r.ncells * sizeof(int))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_bin, (size_t)r.ncells)) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("schedule allocation failed: ") + hipGetErrorString(e));
+      h->order_cap = r.ncells;
+    }
+    if (!h->d_count && (e = hipMalloc((void**)&h->d_count, odv::COST_BINS * sizeof(unsigned))) != hipSuccess)
+      return fail(ODEVEC_ENOMEM, std::string("schedule allocation failed: ") + hipGetErrorString(e));
+    if ((e = h->net->order(a, h->d_bin, h->d_count, h->d_order, r.stream)) != hipSuccess)
+      return hip_fail(e, "cost order");
+    a.order = h->d_order;
+  }
   e = h->net->launch(a, h->opts.mode, h->opts.precision, p, r.stream);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return ODEVEC_OK;
@@ -279,6 +310,25 @@ int32_t odevec_set_history(odevec_t* h, double* d_hist, int64_t ncells) {
   return ODEVEC_OK;
 }
 
+int32_t odevec_set_timeline(odevec_t* h, int64_t* d_times, int64_t ncells, int64_t* ticks_per_second) {
+  if (!h) return fail(ODEVEC_EINVAL, "NULL handle");
+  if (ticks_per_second) {
+    int khz = 0;
+    hipError_t e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device);
+    if (e != hipSuccess) return hip_fail(e, "wall clock rate");
+    *ticks_per_second = (int64_t)khz * 1000;
+  }
+  if (!d_times) {
+    h->d_times = nullptr;
+    h->times_cells = 0;
+    return ODEVEC_OK;
+  }
+  if (ncells <= 0) return fail(ODEVEC_EINVAL, "ncells must be > 0");
+  h->d_times = reinterpret_cast<long long*>(d_times);
+  h->times_cells = ncells;
+  return ODEVEC_OK;
+}
+
 int32_t odevec_kernel_info(const char* network, int32_t mode, int32_t precision, int32_t block,
                            odevec_kernel_info_t* info) {
   const odv::NetEntry* e = find_net(network);
@@ -310,6 +360,7 @@ int32_t odevec_create(const char* network, int64_t nvector, const odevec_opts_t*
   if (opts) o = *opts;
   if (o.precision != ODEVEC_FP64 && o.precision != ODEVEC_MIXED) return fail(ODEVEC_EINVAL, "bad precision");
   if (o.mode < ODEVEC_PERCELL || o.mode > ODEVEC_COOP) return fail(ODEVEC_EINVAL, "bad mode");
+  if (o.schedule < 0 || o.schedule > 1) return fail(ODEVEC_EINVAL, "schedule must be 0 (costliest first) or 1");
   if (o.mode == ODEVEC_COOP && !e->coop_ok)
     return fail(ODEVEC_EINVAL, "the cooperative mode needs a CSC network of at most 64 integrated species");
   if (o.mode != ODEVEC_COOP && !e->lane_ok)
@@ -553,6 +604,9 @@ int32_t odevec_destroy(odevec_t* h) {
   if (h->d_scal) (void)hipFree(h->d_scal);
   if (h->d_queue) (void)hipFree(h->d_queue);
   if (h->d_ws) (void)hipFree(h->d_ws);
+  if (h->d_order) (void)hipFree(h->d_order);
+  if (h->d_bin) (void)hipFree(h->d_bin);
+  if (h->d_count) (void)hipFree(h->d_count);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
   if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -306,6 +306,7 @@ odv_k_coop(KArgs a) {
     for (int j = l; j < Net::NEQ; j += G) b[Y::YF + j] = a.y[(long long)j * a.ncells + i];
     for (int p = l; p < Net::NPARAM; p += G) b[Y::PR + p] = a.prm[p * a.ps_plane + i * a.ps_cell];
     odv_wave_sync();
+    if (a.times && l == 0) a.times[i] = (long long)wall_clock64();
     const double t0 = a.t0[i * a.t0_s];
     const double ts = a.tstop[i * a.ts_s];
     const double dh = a.dt0[i * a.dt_s];
@@ -338,6 +339,7 @@ odv_k_coop(KArgs a) {
         }
       }
     }
+    if (a.times && l == 0) a.times[a.ncells + i] = (long long)wall_clock64();
     odv_wave_sync();   // the group's LDS block is reloaded for the next cell
     unsigned long long nxt = 0;
     if (l == 0) nxt = atomicAdd(a.queue, 1ull);

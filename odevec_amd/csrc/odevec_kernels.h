@@ -55,6 +55,8 @@ struct KArgs {
   unsigned long long* queue;  // persistent: next cell (set to the number of launched lanes)
   double* ws;             // rate workspace, LdsPolicy::WS_PER_SLOT doubles per wavefront slot
   double* hist;           // optional warm-restart history: Hist<Net>::PLANES doubles per cell, cell-major
+  const int* order;       // persistent: queue position -> cell (costliest first), or null = index order
+  long long* times;       // optional [2][ncells]: device wall-clock ticks when each cell started / ended
   Controls ctl;
   int group;              // lockstep group size
 };
@@ -283,6 +285,7 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
   s.active = true;
   s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
   s.dt = dh;
+  if (a.times) a.times[i] = (long long)wall_clock64();
   if ((a.mask && !a.mask[i]) || !(t0 < ts)) {
     s.status = ST_SKIPPED;
     return true;
@@ -301,6 +304,7 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
 
 template <class L, bool HIST = false>
 __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
+  if (a.times) a.times[a.ncells + i] = (long long)wall_clock64();
   if (s.status == ST_SKIPPED) {
     if (a.status) a.status[i] = s.status;
     if (a.dt_out) a.dt_out[i] = s.dt;
@@ -340,26 +344,33 @@ __global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
   }
 }
 
-// Persistent form. Every lane starts on cell (global lane id); the queue
+// Persistent form. Every lane starts on one queue position; the queue
 // counter starts at the number of launched lanes. Whenever at least one lane
-// of a wavefront has finished, the wavefront reserves that many cells with a
-// single atomic and hands them out in lane order. Every lane's loop ends: each
-// iteration either advances a cell by one attempt (cells are bounded by
-// max_attempts) or, once the wavefront has seen the queue exhausted and holds
-// no cell, leaves. `lds` holds the workgroup's wavefront slices,
+// of a wavefront has finished, the wavefront reserves that many positions
+// with a single atomic and hands them out in lane order. Position p holds
+// cell a.order[p] (costliest first) or cell p. With an order, lane l of
+// wavefront w starts on position l*W + w (W wavefronts): the costliest cells
+// start at once, one per wavefront, rather than together in the first
+// wavefronts (cells that run side by side in one wavefront run at the pace
+// of the union of their phases). Every lane's loop ends: each iteration
+// either advances a cell by one attempt (cells are bounded by max_attempts)
+// or, once the wavefront has seen the queue exhausted and holds no cell,
+// leaves. `lds` holds the workgroup's wavefront slices,
 // LdsPolicy<Net, M>::WORDS doubles each.
 template <class Net, class M, bool RCP, bool HIST = false>
 __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
   const long long nlanes = (long long)gridDim.x * blockDim.x;
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
+  const long long p0 = a.order ? (long long)lane * (nlanes >> 6) + (gl >> 6) : gl;
+  long long i = p0 < a.ncells ? (a.order ? (long long)a.order[p0] : p0) : a.ncells;
   L s;
   odv_bind_columns<P>(s, a, lds);
   LaneGroup g;
   double fl64 = 0.0, fl32 = 0.0;
-  bool have = i < a.ncells;
+  bool have = p0 < a.ncells;
   bool done = true;
   bool exhausted = nlanes >= a.ncells;   // wave-uniform: no cell left to hand out
   if (have) done = odv_load_begin<L, HIST>(a, i, s);
@@ -387,7 +398,7 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
       if (need) {
         const long long ni = (long long)base + rank;
         if (ni < a.ncells) {
-          i = ni;
+          i = a.order ? (long long)a.order[ni] : ni;
           have = true;
           done = odv_load_begin<L, HIST>(a, i, s);
         }
@@ -409,6 +420,83 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
 template <class Net, class M, bool RCP, bool HIST = false>
 __global__ void __launch_bounds__(MAX_BLOCK) odv_k_persistent(KArgs a) {
   odv_persistent_run<Net, M, RCP, HIST>(a, odv_lane_lds);
+}
+
+// ------------------------------------------------------------- cost order
+// Persistent-mode scheduling. A launch lasts at least as long as its slowest
+// cell (PRIMORDIAL, 1M cells: one cell takes 41,008 step attempts, ~320 ms
+// alone, the median 264), so the queue hands out the costliest cells first.
+// Proxy for a cell's cost: log2((t_stop - t0)/h0), h0 the reference's
+// initial step size (CalcInitialStepSize, src/odevec.F90:216-238): a stiff
+// cell over a long span needs many steps (correlation 0.89 with the log of
+// the attempts on that workload). The cells are binned by the proxy (4 bins
+// per octave, costliest bin first) and counting-sorted into `order`; within
+// a bin the order is arbitrary. Only the order in which cells start
+// changes: every cell's arithmetic is the same.
+constexpr int COST_BINS = 256;
+
+template <int UNUSED = 0>
+__global__ void k_cost_zero(unsigned* count) {
+  if (threadIdx.x < COST_BINS) count[threadIdx.x] = 0u;
+}
+
+template <class Net>
+__global__ void k_cost_bin(KArgs a, unsigned char* bin, unsigned* count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncells) return;
+  constexpr int NF = Net::NEQ, N = Net::NACT;
+  constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
+  const double t0 = a.t0[i * a.t0_s], ts = a.tstop[i * a.ts_s];
+  int b = COST_BINS - 1;   // skipped cells last
+  if (!(a.mask && !a.mask[i]) && t0 < ts) {
+    double y[NF], prm[NP], k[Net::NRATE], ya[N], f[N];
+#pragma unroll
+    for (int j = 0; j < NF; ++j) y[j] = a.y[(long long)j * a.ncells + i];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) prm[p] = (Net::NPARAM > 0) ? a.prm[p * a.ps_plane + i * a.ps_cell] : 0.0;
+    Net::rates(y, prm, k);
+#pragma unroll
+    for (int j = 0; j < N; ++j) ya[j] = y[Net::ACT[j]];
+    Net::template rhs<1>(ya, k, f);
+    const double tol = a.ctl.rtol;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const double v = 1.0 / (tol * fabs(ya[j]) + a.ctl.atol);
+      acc += f[j] * f[j] * (v * v) * tol * tol;
+    }
+    const double W0 = fmax(fabs(t0), fabs(ts));
+    const double h0 = fmin(sqrt(tol / (1.0 / (W0 * W0) + acc / NF)), ts - t0);
+    const double q = log2((ts - t0) / h0);
+    const int qb = (q == q) ? (int)fmin(fmax(q * 4.0, 0.0), (double)(COST_BINS - 2)) : 0;
+    b = COST_BINS - 2 - qb;
+  }
+  bin[i] = (unsigned char)b;
+  atomicAdd(&count[b], 1u);
+}
+
+// exclusive prefix sum of the bin counts (one workgroup of COST_BINS lanes)
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(COST_BINS) k_cost_scan(unsigned* count) {
+  __shared__ unsigned s[COST_BINS];
+  const int t = threadIdx.x;
+  s[t] = count[t];
+  __syncthreads();
+  for (int off = 1; off < COST_BINS; off <<= 1) {
+    const unsigned v = t >= off ? s[t - off] : 0u;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  count[t] = s[t] - count[t];
+}
+
+template <int UNUSED = 0>
+__global__ void k_cost_scatter(long long ncells, const unsigned char* bin, unsigned* offset, int* order) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells) return;
+  const unsigned p = atomicAdd(&offset[bin[i]], 1u);
+  order[p] = (int)i;
 }
 
 // ---------------------------------------------------------------- batches

@@ -38,6 +38,8 @@ struct NetEntry {
   hipError_t (*launch)(const KArgs& a, int mode, int prec, const Plan& p, hipStream_t st);
   hipError_t (*info)(int mode, int prec, int block, KernelInfo* out);
   void (*param_defaults)(double* p);
+  // persistent mode: rank the cells by expected cost into `order` (scratch: bin[ncells], count[COST_BINS])
+  hipError_t (*order)(const KArgs& a, unsigned char* bin, unsigned* count, int* order, hipStream_t st);
 };
 
 std::vector<const NetEntry*>& registry();
@@ -198,6 +200,21 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
 }
 
 template <class Net>
+hipError_t order_net(const KArgs& a, unsigned char* bin, unsigned* count, int* order, hipStream_t st) {
+  if constexpr (!Net::LANE) {
+    return hipErrorInvalidValue;
+  } else {
+    const unsigned g = (unsigned)((a.ncells + 255) / 256);
+    hipLaunchKernelGGL(k_cost_zero<0>, dim3(1), dim3(COST_BINS), 0, st, count);
+    hipLaunchKernelGGL(k_cost_bin<Net>, dim3(g), dim3(256), 0, st, a, bin, count);
+    hipLaunchKernelGGL(k_cost_scan<0>, dim3(1), dim3(COST_BINS), 0, st, count);
+    hipLaunchKernelGGL(k_cost_scatter<0>, dim3(g), dim3(256), 0, st, (long long)a.ncells,
+                       static_cast<const unsigned char*>(bin), count, order);
+    return hipGetLastError();
+  }
+}
+
+template <class Net>
 hipError_t plan_net(long long ncells, int mode, int prec, int block, int grid, int group, Plan* p) {
   if (prec == PREC_MIXED) return plan_t<Net, float, true>(ncells, mode, block, grid, group, p);
   return plan_t<Net, double, false>(ncells, mode, block, grid, group, p);
@@ -300,5 +317,5 @@ struct Bundle {
   static const odv::NetEntry odv_entry_##NET = {  /* NOLINT */                                                \
       NET::ID, ODV_REPRO_FLAG, NET::NEQ, NET::NNZ, NET::NRATE, NET::NPARAM, NET::RATE_KIND, NET::TGAS,          \
       NET::FLOPS_RHS, NET::FLOPS_JAC, NET::FLOPS_LU, NET::FLOPS_SOLVE, NET::FLOPS_LU_R,         \
-      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::Hist<NET>::PLANES, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults};                          \
+      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::Hist<NET>::PLANES, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults, &odv::order_net<NET>};                          \
   static odv::Registrar odv_reg_##NET(&odv_entry_##NET);

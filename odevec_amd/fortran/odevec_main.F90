@@ -42,7 +42,7 @@ module odevec_main
   integer, parameter :: dp = kind(1.0d0)
 
   type, bind(C) :: odevec_opts_t
-    integer(c_int32_t) :: precision, mode, group, block, grid, max_attempts, device_ptrs, abort_on_error
+    integer(c_int32_t) :: precision, mode, group, block, grid, max_attempts, device_ptrs, abort_on_error, schedule
   end type
 
   type :: odevec

@@ -48,6 +48,8 @@ def test_create_argument_checks():
     assert L.odevec_create(b"ROBER/NONE/DENSE", 4, C.byref(bad), C.byref(h)) == _lib.EINVAL
     grp = _lib.Opts(0, 1, 999, 0, 0, 0, 0, 0)
     assert L.odevec_create(b"ROBER/NONE/DENSE", 4, C.byref(grp), C.byref(h)) == _lib.EINVAL
+    sched = _lib.Opts(0, _lib.PERSISTENT, 0, 0, 0, 0, 0, 0, 2)
+    assert L.odevec_create(b"ROBER/NONE/DENSE", 4, C.byref(sched), C.byref(h)) == _lib.EINVAL
     assert L.odevec_set_tolerances(None, 1e-6, 1e-10) == _lib.EINVAL
     assert L.odevec_destroy(None) == _lib.OK
 

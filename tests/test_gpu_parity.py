@@ -185,6 +185,24 @@ def test_workgroup_sizes_bitwise(oracle, block, precision, mode):
         assert np.array_equal(st, sr) and np.array_equal(y, yr) and np.array_equal(dto, dr)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cost_ordered_schedule_changes_nothing(oracle, precision):
+    """The persistent queue in cost order (costliest cells first, spread one
+    per wavefront) and in index order give the same cells' results and
+    counters, bit for bit; few lanes (grid 2) so the queue does the work."""
+    w = W.primordial(3000, seed=12)
+    w["t_stop"][::7] = 0.0                        # skipped cells rank last
+    nid = "PRIMORDIAL/CMK/CSC+REPRO"
+    a = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=0)
+    b = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=1)
+    assert all(np.array_equal(x, y) for x, y in zip(a[:3], b[:3]))
+    assert a[3]["attempts"] == b[3]["attempts"] and a[3]["newton"] == b[3]["newton"]
+    assert (a[1][::7] == 5).all()
+    if precision == 0:
+        yr, sr, dr, cr = oracle_cells(oracle, nid, w)
+        assert np.array_equal(a[0], yr) and np.array_equal(a[1], sr)
+
+
 @pytest.mark.parametrize("n", [1, 63, 257, 1000])
 @pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
 def test_ragged_sizes_and_skips(oracle, n, mode):
@@ -307,3 +325,28 @@ def test_primordial_full_size_conservation():
                 bound = 1e-4 if name != "D" else 1e-2
                 assert np.quantile(r, 0.999) <= bound and drift <= 0.2
         assert np.array_equal(y[12:], w["y"][12:])    # frozen species untouched
+
+
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT, _lib.COOP])
+def test_timeline_records_every_cell(mode):
+    """With a timeline attached each cell's start and end ticks are written
+    (end >= start, skipped cells included) and the results do not change."""
+    import torch
+    from odevec_amd.api import Odevec
+    w = W.primordial(700, seed=13)
+    w["t_stop"][::11] = 0.0
+    n = w["y"].shape[1]
+    nid = "PRIMORDIAL/CMK/CSC+REPRO"
+    say("GPU timeline, mode", mode)
+    ref = gpu_solve(nid, w, mode=mode, grid=2 if mode == _lib.PERSISTENT else 0)
+    ode = Odevec(nid, n, mode=mode, grid=2 if mode == _lib.PERSISTENT else 0)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    times, hz = ode.enable_timeline(n)
+    y = np.array(w["y"], order="C", copy=True)
+    st, dto, _ = ode.solve_cells(y, w["t_stop"], t0=w["t0"], dt0=w["dt0"], raise_on_fail=False)
+    t = times.cpu().numpy().reshape(2, n)
+    ode.CloseOdevec()
+    assert hz > 0
+    assert np.array_equal(y, ref[0]) and np.array_equal(st, ref[1]) and np.array_equal(dto, ref[2])
+    assert (t[0] > 0).all() and (t[1] >= t[0]).all()
+    assert (t[1] - t[0]).max() / hz < 60.0
