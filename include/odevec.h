@@ -175,7 +175,8 @@ int32_t odevec_set_history(odevec_t* h, double* d_hist, int64_t ncells);
  * wall-clock tick at which each cell was loaded (d_times[i]) and finished
  * (d_times[ncells + i]); ticks_per_second (optional) receives the clock's
  * rate. It shows a launch's tail: how long the last cells run after the rest
- * are done. Every solve must cover exactly ncells cells; NULL detaches. */
+ * are done. Per-cell, persistent and cooperative modes; every solve must
+ * cover exactly ncells cells; NULL detaches. */
 int32_t odevec_set_timeline(odevec_t* h, int64_t* d_times, int64_t ncells, int64_t* ticks_per_second);
 
 /* Batches ------------------------------------------------------------------

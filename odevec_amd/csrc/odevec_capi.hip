@@ -324,6 +324,8 @@ int32_t odevec_set_timeline(odevec_t* h, int64_t* d_times, int64_t ncells, int64
     return ODEVEC_OK;
   }
   if (ncells <= 0) return fail(ODEVEC_EINVAL, "ncells must be > 0");
+  if (h->opts.mode == ODEVEC_LOCKSTEP)
+    return fail(ODEVEC_EINVAL, "the timeline records per-cell, persistent and cooperative solves");
   h->d_times = reinterpret_cast<long long*>(d_times);
   h->times_cells = ncells;
   return ODEVEC_OK;
