@@ -72,8 +72,8 @@ typedef struct {
   int32_t max_attempts;   /* per-cell step-attempt budget, 0 = 1000000 */
   int32_t device_ptrs;    /* 1: array arguments are device pointers (no copies) */
   int32_t abort_on_error; /* 1: print and abort on a failed cell, like the reference's STOP */
-  int32_t schedule;       /* persistent: 0 = costliest cells first (ranked by log2(span / initial
-                             step) before the launch), 1 = index order; results are the same */
+  int32_t schedule;       /* persistent: 0 = index order, 1 = costliest cells first (ranked by
+                             log2(span / initial step) before the launch); results are the same */
 } odevec_opts_t;
 
 typedef struct {

@@ -206,7 +206,7 @@ int launch(odevec_s* h, const Req& r) {
   }
   a.ws = h->d_ws;
   // cost-ordered queue when the cells outnumber the launched lanes
-  if (h->opts.mode == ODEVEC_PERSISTENT && h->opts.schedule == 0 && r.ncells > p.grid * p.block) {
+  if (h->opts.mode == ODEVEC_PERSISTENT && h->opts.schedule == 1 && r.ncells > p.grid * p.block) {
     if (r.ncells > h->order_cap) {
       if (h->d_order) (void)hipFree(h->d_order);
       if (h->d_bin) (void)hipFree(h->d_bin);
@@ -362,7 +362,7 @@ int32_t odevec_create(const char* network, int64_t nvector, const odevec_opts_t*
   if (opts) o = *opts;
   if (o.precision != ODEVEC_FP64 && o.precision != ODEVEC_MIXED) return fail(ODEVEC_EINVAL, "bad precision");
   if (o.mode < ODEVEC_PERCELL || o.mode > ODEVEC_COOP) return fail(ODEVEC_EINVAL, "bad mode");
-  if (o.schedule < 0 || o.schedule > 1) return fail(ODEVEC_EINVAL, "schedule must be 0 (costliest first) or 1");
+  if (o.schedule < 0 || o.schedule > 1) return fail(ODEVEC_EINVAL, "schedule must be 0 (index order) or 1 (costliest first)");
   if (o.mode == ODEVEC_COOP && !e->coop_ok)
     return fail(ODEVEC_EINVAL, "the cooperative mode needs a CSC network of at most 64 integrated species");
   if (o.mode != ODEVEC_COOP && !e->lane_ok)

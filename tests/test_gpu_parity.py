@@ -193,8 +193,8 @@ def test_cost_ordered_schedule_changes_nothing(oracle, precision):
     w = W.primordial(3000, seed=12)
     w["t_stop"][::7] = 0.0                        # skipped cells rank last
     nid = "PRIMORDIAL/CMK/CSC+REPRO"
-    a = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=0)
-    b = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=1)
+    a = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=1)
+    b = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=0)
     assert all(np.array_equal(x, y) for x, y in zip(a[:3], b[:3]))
     assert a[3]["attempts"] == b[3]["attempts"] and a[3]["newton"] == b[3]["newton"]
     assert (a[1][::7] == 5).all()
