@@ -350,3 +350,51 @@ def test_timeline_records_every_cell(mode):
     assert np.array_equal(y, ref[0]) and np.array_equal(st, ref[1]) and np.array_equal(dto, ref[2])
     assert (t[0] > 0).all() and (t[1] >= t[0]).all()
     assert (t[1] - t[0]).max() / hz < 60.0
+
+
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_check_negatives_per_cell(oracle, mode):
+    """CheckNegatives in the per-cell forms (src/odevec.F90:265-291): cells
+    whose predicted state goes negative are reset with a quarter step, as in
+    the oracle, bit for bit; cells that start negative end at dt_min."""
+    from odevec_amd.api import Odevec
+    w = W.primordial(96, seed=14)
+    w["y"][4, 48:] = -1e-3                      # negative H2 in the second half
+    nid = "PRIMORDIAL/CMK/CSC+REPRO"
+    n = w["y"].shape[1]
+    say("GPU check negatives, mode", mode)
+    ode = Odevec(nid, n, mode=mode)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], 1e-3
+    ode.CheckNegatives = True
+    y = np.array(w["y"], order="C", copy=True)
+    st, dto, stats = ode.solve_cells(y, w["t_stop"], t0=w["t0"], dt0=w["dt0"], stats=True, raise_on_fail=False)
+    ode.CloseOdevec()
+    yr, sr, dr, cr = oracle.Network.load("PRIMORDIAL/CMK/CSC").solve_cells(
+        w["y"], w["t_stop"], t0=w["t0"], dt0=w["dt0"], rtol=w["rtol"], atol=w["atol"], dt_min=1e-3,
+        check_negatives=True)
+    assert np.array_equal(st, sr) and np.array_equal(y, yr) and np.array_equal(dto, dr)
+    assert (st[48:] == 1).all()
+    assert stats["attempts"] == int(cr[0].sum())
+
+
+def test_solveode_mask_per_cell(oracle):
+    """SolveODE's Mask with per-cell step control: masked-out cells come back
+    unchanged, the others equal the oracle's per-cell solves."""
+    from odevec_amd.api import Odevec
+    w = W.rober(48, seed=15)
+    mask = np.zeros(48, np.int8)
+    mask[1::4] = 1
+    mask[2::4] = 1
+    ode = Odevec("ROBER/NONE/DENSE+REPRO", 48, mode=_lib.PERCELL)
+    ode.y[:] = w["y"].T
+    say("GPU SolveODE masked per cell")
+    # per-cell rate constants are the network defaults here (SolveODE has no parameter argument)
+    t, dt = ode.SolveODE(0.0, 1e-10, 1e3, Mask=mask)
+    y = ode.y.T.copy()
+    ode.CloseOdevec()
+    yr, sr, dr, cr = oracle.Network.load("ROBER/NONE/DENSE").solve_cells(w["y"], 1e3, dt0=1e-10, rtol=1e-6,
+                                                                         atol=1e-10)
+    on = mask == 1
+    assert t == 1e3
+    assert np.array_equal(y[:, ~on], w["y"][:, ~on])
+    assert np.array_equal(y[:, on], yr[:, on])
