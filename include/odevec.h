@@ -147,6 +147,23 @@ int32_t odevec_solve_cells_async(odevec_t* h, int64_t ncells, const double* d_t0
                                  int64_t params_cell_stride, int32_t* d_status, double* d_dt_out,
                                  int32_t* d_counters, double* d_flops, void* stream);
 
+/* The generated plugin's network functions, evaluated on the GPU for
+ * ncells cells of the handle's network, in the reference's layouts (arrays
+ * are host arrays, or device arrays with opts.device_ptrs):
+ *   GetRHS(this, y, rhs) (src/odevec_commons.F90:78-95):
+ *     rhs[j*ncells + i] = f_j(y_i) for all neq species (frozen ones 0);
+ *   GetJac(this, beta, y, dt, jac) (src/odevec_commons.F90:32-51,
+ *     pre_odevec.py:144-180): P = I - beta*dt*J (permuted) in the
+ *     packaging's storage, jac[e*ncells + i] for e < odevec_jac_entries:
+ *     DENSE e = c*neq + r holds P_order(r, c) (Fortran jac(:, r, c)); CSC e
+ *     is the e-th entry of the LU pattern (jac%sdata(:, e), fill entries 0).
+ * Rates come from each cell's y and params (NULL = the network defaults).
+ * The reference's GetLU (LUmethod 2) has no counterpart. */
+int32_t odevec_get_rhs(odevec_t* h, int64_t ncells, const double* y, const double* params, double* rhs);
+int32_t odevec_get_jac(odevec_t* h, int64_t ncells, double beta, double dt, const double* y, const double* params,
+                       double* jac);
+int32_t odevec_jac_entries(const char* network, int64_t* entries);
+
 /* Replaces CloseOdevec(this) (src/odevec.F90:1058-1074). */
 int32_t odevec_destroy(odevec_t* h);
 

@@ -87,6 +87,9 @@ def lib():
         "odevec_history_planes": (_I32, [C.c_char_p, C.POINTER(_I64)]),
         "odevec_set_history": (_I32, [_VP, _VP, _I64]),
         "odevec_set_timeline": (_I32, [_VP, _VP, _I64, C.POINTER(_I64)]),
+        "odevec_get_rhs": (_I32, [_VP, _I64, _VP, _VP, _VP]),
+        "odevec_get_jac": (_I32, [_VP, _I64, C.c_double, C.c_double, _VP, _VP, _VP]),
+        "odevec_jac_entries": (_I32, [C.c_char_p, C.POINTER(_I64)]),
         "odevec_default_opts": (None, [C.POINTER(Opts)]),
         "odevec_create": (_I32, [C.c_char_p, _I64, C.POINTER(Opts), C.POINTER(_VP)]),
         "odevec_set_tolerances": (_I32, [_VP, C.c_double, C.c_double]),
@@ -118,7 +121,8 @@ EXPORTED = [
     "odevec_network_info", "odevec_kernel_info", "odevec_default_opts", "odevec_create",
     "odevec_set_tolerances", "odevec_set_dt_min", "odevec_set_check_negatives", "odevec_solve",
     "odevec_solve_cells", "odevec_solve_cells_async", "odevec_destroy", "odevec_history_planes",
-    "odevec_set_history", "odevec_set_timeline", "odevec_bundle_count",
+    "odevec_set_history", "odevec_set_timeline", "odevec_get_rhs", "odevec_get_jac", "odevec_jac_entries",
+    "odevec_bundle_count",
     "odevec_bundle_name", "odevec_bundle_entry", "odevec_batch_create", "odevec_batch_solve_async",
     "odevec_batch_kernel_info", "odevec_batch_destroy",
 ]

@@ -71,6 +71,29 @@ class Odevec:
         self._hist = None
         self._times = None
 
+    # -- the generated plugin's network functions (src/odevec_commons.F90:32-95)
+    def GetRHS(self, y: np.ndarray) -> np.ndarray:
+        """f(y) for y of shape (neq, ncells) (rates from each cell's y and
+        the network's default parameters); returns (neq, ncells)."""
+        self._require()
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros_like(y)
+        L.check(L.lib().odevec_get_rhs(self._h, y.shape[1], y.ctypes.data, None, out.ctypes.data))
+        return out
+
+    def GetJac(self, beta: float, y: np.ndarray, dt: float, params: Optional[np.ndarray] = None) -> np.ndarray:
+        """P = I - beta*dt*J in the packaging's storage: (entries, ncells)
+        (DENSE: entry c*neq + r is P_order(r, c); CSC: the LU pattern)."""
+        self._require()
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        ne = C.c_int64(0)
+        L.check(L.lib().odevec_jac_entries(self.network.encode(), C.byref(ne)))
+        out = np.zeros((ne.value, y.shape[1]))
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        L.check(L.lib().odevec_get_jac(self._h, y.shape[1], float(beta), float(dt), y.ctypes.data,
+                                       None if p is None else p.ctypes.data, out.ctypes.data))
+        return out
+
     # -- warm restart (no reference counterpart: its SolveODE cold-starts every call)
     def enable_history(self, ncells: int):
         """Attach a zero-filled device history for `ncells` cells (a torch

@@ -36,6 +36,11 @@ struct odv_net_OREGO_NONE_DENSE {
   }
   static constexpr bool COOP = false;
   static constexpr bool LANE = true;   // per-lane kernels available
+  // the reference's GetJac layout from this storage (see odevec_get_jac)
+  static constexpr int JMAP_N = 9;
+  static constexpr short JMAP[9] = {
+      0, 1, 2, 3, 4, 5, -1, 6, 7,
+  };
 
   // k[r * S]: rate r of this cell (S = stride between a cell's rates)
   template <int S = 1>

@@ -83,6 +83,11 @@ struct odv_net_ROBER_NONE_CSC {
       0x7ull, 0x7ull, 0x6ull,
   };
   static constexpr bool LANE = true;   // per-lane kernels available
+  // the reference's GetJac layout from this storage (see odevec_get_jac)
+  static constexpr int JMAP_N = 8;
+  static constexpr short JMAP[8] = {
+      0, 1, 2, 3, 4, 5, 6, 7,
+  };
 
   // k[r * S]: rate r of this cell (S = stride between a cell's rates)
   template <int S = 1>

@@ -570,6 +570,74 @@ int32_t odevec_solve_cells(odevec_t* h, int64_t ncells, const double* t0, const 
   return check_status(h, h->h_status.data(), ncells, stats);
 }
 
+namespace {
+
+// GetRHS (what 0) / GetJac (what 1) of ncells cells through the handle's buffers
+int netfun(odevec_t* h, int what, int64_t ncells, double hb, const double* y, const double* params, double* out) {
+  if (!h || !y || !out) return fail(ODEVEC_EINVAL, "NULL argument");
+  if (ncells < 0) return fail(ODEVEC_EINVAL, "ncells < 0");
+  if (!h->net->lane_ok || h->net->jac_entries <= 0)
+    return fail(ODEVEC_EINVAL, std::string(h->net->id) + ": the network functions need the per-lane code");
+  if (ncells == 0) return ODEVEC_OK;
+  const bool dev = h->opts.device_ptrs != 0;
+  int rc = ensure_cap(h, ncells);
+  if (rc != ODEVEC_OK) return rc;
+  const int64_t neq = h->net->neq, np = h->net->nparam;
+  const int64_t per = what == 0 ? neq : h->net->jac_entries;
+  hipStream_t st = h->stream;
+  hipError_t e;
+  odv::KArgs a{};
+  a.ncells = ncells;
+  if (dev) a.y = const_cast<double*>(y);
+  else {
+    if ((e = hipMemcpyAsync(h->d_y, y, (size_t)(neq * ncells) * 8, hipMemcpyHostToDevice, st)) != hipSuccess)
+      return hip_fail(e, "H2D y");
+    a.y = h->d_y;
+  }
+  if (params && np > 0) {
+    if (dev) a.prm = params;
+    else {
+      if ((e = hipMemcpyAsync(h->d_prm, params, (size_t)(np * ncells) * 8, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(e, "H2D params");
+      a.prm = h->d_prm;
+    }
+    a.ps_cell = 1;
+    a.ps_plane = ncells;
+  } else {
+    a.prm = h->d_scal + 3;
+    a.ps_cell = 0;
+    a.ps_plane = 1;
+  }
+  double* d_out = out;
+  if (!dev && (e = hipMalloc((void**)&d_out, (size_t)(per * ncells) * 8)) != hipSuccess)
+    return fail(ODEVEC_ENOMEM, std::string("output allocation failed: ") + hipGetErrorString(e));
+  e = h->net->netfun(what, a, hb, d_out, st);
+  if (e == hipSuccess && !dev) e = hipMemcpyAsync(out, d_out, (size_t)(per * ncells) * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (!dev) (void)hipFree(d_out);
+  return e == hipSuccess ? ODEVEC_OK : hip_fail(e, what == 0 ? "GetRHS" : "GetJac");
+}
+
+}  // namespace
+
+int32_t odevec_get_rhs(odevec_t* h, int64_t ncells, const double* y, const double* params, double* rhs) {
+  return netfun(h, 0, ncells, 0.0, y, params, rhs);
+}
+
+int32_t odevec_get_jac(odevec_t* h, int64_t ncells, double beta, double dt, const double* y, const double* params,
+                       double* jac) {
+  return netfun(h, 1, ncells, beta * dt, y, params, jac);
+}
+
+int32_t odevec_jac_entries(const char* network, int64_t* entries) {
+  const odv::NetEntry* e = find_net(network);
+  if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown network ") + (network ? network : "(null)"));
+  if (!entries) return fail(ODEVEC_EINVAL, "entries is NULL");
+  if (e->jac_entries <= 0) return fail(ODEVEC_EINVAL, std::string(e->id) + ": no per-lane network code");
+  *entries = e->jac_entries;
+  return ODEVEC_OK;
+}
+
 int32_t odevec_solve_cells_async(odevec_t* h, int64_t ncells, const double* d_t0, int64_t t0_stride,
                                  const double* d_t_stop, int64_t t_stop_stride, const double* d_dt0,
                                  int64_t dt0_stride, double* d_y, const double* d_params,

@@ -422,6 +422,56 @@ __global__ void __launch_bounds__(MAX_BLOCK) odv_k_persistent(KArgs a) {
   odv_persistent_run<Net, M, RCP, HIST>(a, odv_lane_lds);
 }
 
+// ------------------------------------------------------- network functions
+// The generated plugin's GetRHS and GetJac (src/odevec_commons.F90:32-95)
+// for a block of cells, in the reference's layouts: rhs[j*ncells + i] over
+// all NEQ species (frozen ones 0); jac[e*ncells + i] over the full system's
+// GetJac storage (Net::JMAP). The rates come from each cell's own state and
+// parameters, as in the solver.
+template <class Net>
+__global__ void k_get_rhs(KArgs a, double* rhs) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncells) return;
+  constexpr int NF = Net::NEQ, N = Net::NACT;
+  constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
+  double y[NF], prm[NP], k[Net::NRATE], ya[N], f[N], out[NF];
+#pragma unroll
+  for (int j = 0; j < NF; ++j) y[j] = a.y[(long long)j * a.ncells + i];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) prm[p] = (Net::NPARAM > 0) ? a.prm[p * a.ps_plane + i * a.ps_cell] : 0.0;
+  Net::rates(y, prm, k);
+#pragma unroll
+  for (int j = 0; j < N; ++j) ya[j] = y[Net::ACT[j]];
+  Net::template rhs<1>(ya, k, f);
+#pragma unroll
+  for (int j = 0; j < NF; ++j) out[j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) out[Net::ACT[j]] = f[j];
+#pragma unroll
+  for (int j = 0; j < NF; ++j) rhs[(long long)j * a.ncells + i] = out[j];
+}
+
+template <class Net>
+__global__ void k_get_jac(KArgs a, double hb, double* jac) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncells) return;
+  constexpr int NF = Net::NEQ, N = Net::NACT;
+  constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
+  double y[NF], prm[NP], k[Net::NRATE], ya[N], P[Net::NNZ];
+#pragma unroll
+  for (int j = 0; j < NF; ++j) y[j] = a.y[(long long)j * a.ncells + i];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) prm[p] = (Net::NPARAM > 0) ? a.prm[p * a.ps_plane + i * a.ps_cell] : 0.0;
+  Net::rates(y, prm, k);
+#pragma unroll
+  for (int j = 0; j < N; ++j) ya[j] = y[Net::ACT[j]];
+  Net::template jac<double, 1>(ya, k, hb, P);
+  for (int e = 0; e < Net::JMAP_N; ++e) {
+    const int m = Net::JMAP[e];
+    jac[(long long)e * a.ncells + i] = m >= 0 ? P[m] : (m == -2 ? 1.0 : 0.0);
+  }
+}
+
 // ------------------------------------------------------------- cost order
 // Persistent-mode scheduling. A launch lasts at least as long as its slowest
 // cell (PRIMORDIAL, 1M cells: one cell takes 41,008 step attempts, ~320 ms

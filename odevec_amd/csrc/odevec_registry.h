@@ -40,6 +40,9 @@ struct NetEntry {
   void (*param_defaults)(double* p);
   // persistent mode: rank the cells by expected cost into `order` (scratch: bin[ncells], count[COST_BINS])
   hipError_t (*order)(const KArgs& a, unsigned char* bin, unsigned* count, int* order, hipStream_t st);
+  // GetRHS (what 0) / GetJac (what 1, P = I - hb*J) for a.ncells cells into `out`
+  hipError_t (*netfun)(int what, const KArgs& a, double hb, double* out, hipStream_t st);
+  int jac_entries;        // entries per cell of the GetJac layout (0: per-lane code not built)
 };
 
 std::vector<const NetEntry*>& registry();
@@ -200,6 +203,24 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
 }
 
 template <class Net>
+hipError_t netfun_net(int what, const KArgs& a, double hb, double* out, hipStream_t st) {
+  if constexpr (!Net::LANE) {
+    return hipErrorInvalidValue;
+  } else {
+    const unsigned g = (unsigned)((a.ncells + 63) / 64);
+    if (what == 0) hipLaunchKernelGGL(k_get_rhs<Net>, dim3(g), dim3(64), 0, st, a, out);
+    else hipLaunchKernelGGL(k_get_jac<Net>, dim3(g), dim3(64), 0, st, a, hb, out);
+    return hipGetLastError();
+  }
+}
+
+template <class Net>
+constexpr int jac_entries_of() {
+  if constexpr (Net::LANE) return Net::JMAP_N;
+  else return 0;
+}
+
+template <class Net>
 hipError_t order_net(const KArgs& a, unsigned char* bin, unsigned* count, int* order, hipStream_t st) {
   if constexpr (!Net::LANE) {
     return hipErrorInvalidValue;
@@ -317,5 +338,5 @@ struct Bundle {
   static const odv::NetEntry odv_entry_##NET = {  /* NOLINT */                                                \
       NET::ID, ODV_REPRO_FLAG, NET::NEQ, NET::NNZ, NET::NRATE, NET::NPARAM, NET::RATE_KIND, NET::TGAS,          \
       NET::FLOPS_RHS, NET::FLOPS_JAC, NET::FLOPS_LU, NET::FLOPS_SOLVE, NET::FLOPS_LU_R,         \
-      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::Hist<NET>::PLANES, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults, &odv::order_net<NET>};                          \
+      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::Hist<NET>::PLANES, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults, &odv::order_net<NET>, &odv::netfun_net<NET>, odv::jac_entries_of<NET>()};                          \
   static odv::Registrar odv_reg_##NET(&odv_entry_##NET);

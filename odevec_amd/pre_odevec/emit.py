@@ -490,6 +490,7 @@ def emit_header(full: Structure) -> str:
         hdr.append("};")
         hdr.append("")
         return "\n".join(hdr)
+    hdr += _emit_jmap(full, st)
     hdr.append("")
     hdr.append("  // k[r * S]: rate r of this cell (S = stride between a cell's rates)")
     hdr.append("  template <int S = 1>")
@@ -520,6 +521,36 @@ def emit_header(full: Structure) -> str:
     hdr.append("};")
     hdr.append("")
     return "\n".join(hdr)
+
+
+def _emit_jmap(full: Structure, st: Structure) -> List[str]:
+    """The reference's GetJac storage (src/odevec_commons.F90:32-51,
+    pre_odevec.py:144-180) from the generated code's: entry e of the full
+    system's layout -- DENSE: P_order(r, c) of all neq x neq at e = c*neq + r
+    (Fortran jac(:, r, c)); CSC: jac%sdata(:, e) over the full LU pattern --
+    takes storage index JMAP[e] >= 0 of the integrated species' P, or is 0
+    (-1: structurally zero, fill, or coupling a frozen species) or 1 (-2: a
+    frozen species' diagonal)."""
+    active = st.active if st.active is not None else list(range(full.neq))
+    amap = {s: a for a, s in enumerate(active)}
+    redpos = {int(a): p for p, a in enumerate(st.perm)}
+    store = st.csc.index()
+    fperm = [int(p) for p in full.perm]
+    if full.packaging == "DENSE":
+        cells = [(r, c) for c in range(full.neq) for r in range(full.neq)]
+    else:
+        cells = [tuple(map(int, e)) for e in full.csc.entries]
+    jm = []
+    for r, c in cells:
+        sr, sc = fperm[r], fperm[c]
+        if sr in amap and sc in amap:
+            jm.append(store.get((redpos[amap[sr]], redpos[amap[sc]]), -1))
+        else:
+            jm.append(-2 if r == c else -1)
+    out = ["  // the reference's GetJac layout from this storage (see odevec_get_jac)",
+           "  static constexpr int JMAP_N = %d;" % len(jm)]
+    out += _arr("short", "JMAP", jm)
+    return out
 
 
 def analyse_copy(st: Structure) -> Structure:

@@ -149,7 +149,10 @@ def test_fortran_module_driver_equals_oracle(oracle):
     say("GPU Fortran driver", B.FORTRAN_DRIVER)
     r = subprocess.run([B.FORTRAN_DRIVER], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    rows = np.array([[float(v) for v in line.split()] for line in r.stdout.splitlines() if line.strip()])
+    lines = [line for line in r.stdout.splitlines() if line.strip()]
+    rows = np.array([[float(v) for v in line.split()] for line in lines if not line.startswith(("RHS", "JAC"))])
+    rhs = np.array([[float(v) for v in line[3:].split()] for line in lines if line.startswith("RHS")])
+    jac = np.array([[float(v) for v in line[3:].split()] for line in lines if line.startswith("JAC")])
     net = oracle.Network.load("ROBER/NONE/DENSE")
     yref = np.zeros((3, 32))
     yref[0] = 1.0
@@ -165,6 +168,12 @@ def test_fortran_module_driver_equals_oracle(oracle):
         assert (blk[:, 5] == 0).all()
         t_stop, k = t_stop * 10.0, k + 1
     assert k * 32 == len(rows)
+    # the plugin's GetRHS / GetJac at the final state (default rate constants)
+    kd = np.array([0.04, 3e7, 1e4])
+    assert np.array_equal(rhs[:, 0], np.arange(1, 33)) and np.array_equal(jac[:, 0], np.arange(1, 33))
+    for i in range(32):
+        assert np.array_equal(rhs[i, 1:], net.rhs(yref[:, i], kd))
+        assert np.array_equal(jac[i, 1:].reshape(3, 3).T, net.p_dense(yref[:, i], kd, 0.5 * 1e-3))
 
 
 @pytest.mark.parametrize("block", [64, 128, 256])
@@ -398,3 +407,32 @@ def test_solveode_mask_per_cell(oracle):
     assert t == 1e3
     assert np.array_equal(y[:, ~on], w["y"][:, ~on])
     assert np.array_equal(y[:, on], yr[:, on])
+
+
+@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "OREGO/NONE/CSC", "PRIMORDIAL/CMK/CSC", "PRIMORDIAL/NONE/DENSE"])
+def test_plugin_getrhs_getjac(oracle, nid):
+    """The generated plugin's GetRHS / GetJac through the C ABI (evaluated
+    on the GPU, +REPRO build) equal the oracle's RHS and permuted P bit for
+    bit, in the reference's layouts (DENSE: all neq x neq entries; CSC: the
+    full LU pattern, fill entries 0; PRIMORDIAL's frozen species included)."""
+    from odevec_amd.api import Odevec
+    w = {"ROBER": lambda: W.rober(40, seed=16), "OREGO": lambda: W.orego(40, seed=17),
+         "PRIMORDIAL": lambda: W.primordial(40, seed=18)}[nid.split("/")[0]]()
+    n = w["y"].shape[1]
+    say("GPU GetRHS/GetJac", nid)
+    ode = Odevec(nid + "+REPRO", n)
+    f = ode.GetRHS(w["y"])
+    beta, dt = 6.0 / 11.0, 3e-4
+    jac = ode.GetJac(beta, w["y"], dt)
+    ode.CloseOdevec()
+    net = oracle.Network.load(nid)
+    neq = net.neq
+    for i in range(n):
+        k = net.rates(w["y"][:, i])
+        assert np.array_equal(f[:, i], net.rhs(w["y"][:, i], k))
+        P = net.p_dense(w["y"][:, i], k, beta * dt)
+        if net.csc:
+            ref = np.array([P[r, c] for r, c in net.ir["lu_entries"]])
+        else:
+            ref = P.T.reshape(-1)          # entry c*neq + r
+        assert len(ref) == jac.shape[0] and np.array_equal(jac[:, i], ref), i
