@@ -440,7 +440,7 @@ def run_sessions(server: Server, nsess: int = 8, ncells: int = 65536, nadv: int 
         else:
             w = blocks[k]
             r = Request(rid, network, state[k], spans[k] * (frac1 - frac0), w["rtol"], w["atol"], w["dt_min"],
-                        np.ascontiguousarray(np.broadcast_to(w["dt0"], (ncells,))),
+                        np.array(np.broadcast_to(w["dt0"], (ncells,))),
                         None if w.get("params") is None else np.ascontiguousarray(w["params"]), precision,
                         server.now())
             server.submit(r)
