@@ -21,8 +21,8 @@ Flag notes:
   2 (closed-form LU) is rejected with a message.
 * ``--heatcool 1`` needs KROME's heating/cooling Fortran and is rejected.
 * ``--equilibrium_H`` is validated exactly as the reference validates it
-  (pre_odevec.py:455-460), which means it can never be set together with
-  LUmethod 1.
+  (pre_odevec.py:455-460: DENSE, no ordering, LUmethod 3) and, as there,
+  changes nothing else in the generated code.
 * ``--dt_min`` is recorded in the IR as the network's default MinimalTimestep.
 """
 from __future__ import annotations

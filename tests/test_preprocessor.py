@@ -118,6 +118,15 @@ def test_cli_flags(tmp_path):
         cli.main(["--example=ROBER", "--packaging=COO", "--outdir", str(tmp_path)])
     with pytest.raises(SystemExit):   # pre_odevec.py:455-456
         cli.main(["--example=ROBER", "--equilibrium_H=1", "--packaging=CSC", "--outdir", str(tmp_path)])
+    with pytest.raises(SystemExit):   # pre_odevec.py:459-460
+        cli.main(["--example=ROBER", "--equilibrium_H=1", "--outdir", str(tmp_path)])
+    with pytest.raises(SystemExit):   # LUmethod 3 needs DENSE (src/odevec.F90:509-523)
+        cli.main(["--example=ROBER", "--LUmethod=3", "--packaging=CSC", "--outdir", str(tmp_path)])
+    with pytest.raises(SystemExit):   # ... and no ordering
+        cli.main(["--example=PRIMORDIAL", "--LUmethod=3", "--ordering=CMK", "--outdir", str(tmp_path)])
+    assert cli.main(["--example=OREGO", "--LUmethod=3", "--equilibrium_H=1", "--outdir", str(tmp_path)]) == 0
+    hdr = open(os.path.join(str(tmp_path), "net_OREGO_NONE_DENSE_LU3.h")).read()
+    assert "FD_JAC = true" in hdr and '"OREGO/NONE/DENSE/LU3"' in hdr
 
 
 def test_krome_setupfile(tmp_path):
