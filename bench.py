@@ -170,7 +170,8 @@ def serving(rate, seconds, networks=None):
     from odevec_amd import serve as S
     nets = networks or list(S.SERVE_SPANS)
     tuned = A.lookup([n for n in nets if n not in S.COOP_NETWORKS]) or {}
-    srv = S.Server(tuned.get("bundle", "MIX3"), slots=2, precision=tuned.get("precision", {}))
+    srv = S.Server(tuned.get("bundle", "MIX3"), slots=2, precision=tuned.get("precision", {}),
+                   block=tuned.get("block", 256))
     res = S.run_trace(srv, S.make_trace(rate, seconds, seed=5, networks=nets))
     srv.close()
     return dict(rate_per_s=rate, seconds=seconds, networks=nets, **res)
