@@ -296,8 +296,9 @@ def read_traffic():
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (cells are independent; "
-              "see DESIGN.md for how the work would shard).", file=sys.stderr)
+        print("bench.py runs on one GPU: there is no multi-GPU path (cells are independent: each GPU "
+              "would take a contiguous range of cells, one process per GPU, with no exchange during "
+              "the solve).", file=sys.stderr)
         sys.exit(2)
     from odevec_amd import workloads as W
     import torch
