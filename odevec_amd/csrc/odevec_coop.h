@@ -110,7 +110,7 @@ struct CoopNet {
 
   // publish the lanes' species values, then each lane gathers its species' terms
   template <int S = 1>
-  __device__ static void rhs(const double* y, const double* k, double* f) {
+  __device__ __forceinline__ static void rhs(const double* y, const double* k, double* f) {
     (void)k;
     double* b = blk();
     const int l = coop_l<G>();
@@ -141,7 +141,7 @@ struct CoopNet {
 
   // row l of P = I - hb*J (permuted), from the Jacobian contributions of its entries
   template <class M, int S = 1>
-  __device__ static void jac(const double* y, const double* k, double hb, M* a) {
+  __device__ __forceinline__ static void jac(const double* y, const double* k, double hb, M* a) {
     (void)k;
     double* b = blk();
     const int l = coop_l<G>();
@@ -182,7 +182,7 @@ struct CoopNet {
   // with L(i,k) in the pattern updates A(i,j) -= L(i,k) U(k,j) over row k's
   // pattern (per entry, in increasing k: the oracle's sequence).
   template <class M>
-  __device__ static void lu(M* a) {
+  __device__ __forceinline__ static void lu(M* a) {
     double* b = blk();
     const int l = coop_l<G>();
     const int base = coop_base<G>();
@@ -212,11 +212,11 @@ struct CoopNet {
     }
   }
   template <class M>
-  __device__ static void lu_r(M* a) { lu<M>(a); }
+  __device__ __forceinline__ static void lu_r(M* a) { lu<M>(a); }
 
   // x (species l's residual) -> the correction; permutation in and out by shuffle
   template <class M, class V>
-  __device__ static void solve(const M* a, V* x) {
+  __device__ __forceinline__ static void solve(const M* a, V* x) {
     const int l = coop_l<G>();
     const int base = coop_base<G>();
     const unsigned long long mine = l < NB ? Base::LU_MASK[l] : 0ull;
@@ -235,7 +235,7 @@ struct CoopNet {
     x[0] = (V)__shfl((double)xp, base + (l < NB ? (int)Base::IPERM[l] : 0), 64);
   }
   template <class M, class V>
-  __device__ static void solve_r(const M* a, V* x) { solve<M, V>(a, x); }
+  __device__ __forceinline__ static void solve_r(const M* a, V* x) { solve<M, V>(a, x); }
 };
 
 template <class B, int G>
@@ -278,19 +278,22 @@ using CoopLane = Lane<Cfg<CoopNet<Net, CoopWidth<Net::NACT>::G>, M, RCP, 0, fals
 // Groups of a workgroup are independent (no workgroup barrier) and every
 // group-wide decision is a group vote, so a group's lanes follow one control
 // path. Each group's loop ends: every cell is bounded by max_attempts.
-// Compiled for COOP_WAVES wavefronts per SIMD (512 / COOP_WAVES registers
-// per lane). A lane's dense P row alone is 64 registers (FP32) or 128
-// (FP64); the spills a small budget costs are cheaper than the latency a low
-// occupancy leaves exposed. Measured, SYNTH64, 65,536 cells, ms per launch
-// at 1 / 2 / 4 / 6 / 8 wavefronts per SIMD: FP64 825 / 614 / 522 / 495 /
-// 510, mixed 376 / 377 / 289 / 270 / 288. Narrower groups (<= 32 species)
-// keep the compiler's own budget: their rows are short, and a forced
-// budget only adds spills (PRIMORDIAL, 16 lanes: 515 -> 633 ms at 6).
-constexpr int COOP_WAVES = 6;
+// The network functions above are inlined (__forceinline__): called out of
+// line, a lane's P row (passed by pointer) lives in scratch memory and every
+// factorization step goes through it (SYNTH64: 1.9e12 B of HBM traffic per
+// FP64 launch). Inlined, the row stays in registers, and the kernel is
+// compiled for COOP_WAVES_F64 / COOP_WAVES_F32 wavefronts per SIMD (512 /
+// waves registers per lane; the FP64 row alone is 128 registers, the FP32
+// one 64). Measured, SYNTH64, 65,536 cells, ms per launch at 1 / 2 / 3 / 4 /
+// 6 wavefronts per SIMD: FP64 312 / 276 / 386 / 464 / 764, mixed 307 / 212 /
+// 200 / 236 / 297 (out of line, the best were 495 and 270). Narrower groups
+// (<= 32 species) keep the compiler's own budget: their rows are short.
+constexpr int COOP_WAVES_F64 = 2;
+constexpr int COOP_WAVES_F32 = 3;
 
 template <class Net, class M, bool RCP>
 __global__ void __launch_bounds__(COOP_BLOCK)
-__attribute__((amdgpu_waves_per_eu(CoopWidth<Net::NACT>::G == 64 ? COOP_WAVES : 1)))
+__attribute__((amdgpu_waves_per_eu(CoopWidth<Net::NACT>::G < 64 ? 1 : (sizeof(M) == 8 ? COOP_WAVES_F64 : COOP_WAVES_F32))))
 odv_k_coop(KArgs a) {
   constexpr int G = CoopWidth<Net::NACT>::G;
   using CN = CoopNet<Net, G>;
