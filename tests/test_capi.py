@@ -104,3 +104,26 @@ def test_cooperative_mode_registry_and_checks():
     assert L.odevec_create(b"SYNTH64/FF/CSC", 4, C.byref(lane), C.byref(h)) == _lib.EINVAL
     assert "cooperative" in _lib.last_error()
     assert L.odevec_create(b"ROBER/NONE/DENSE", 4, C.byref(coop), C.byref(h)) == _lib.EINVAL
+
+
+def test_header_is_c99_and_links(tmp_path):
+    """include/odevec.h is plain C: a C99 host compiles against it and links
+    libodevec.so (every declared entry point referenced)."""
+    import re
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "odevec.h")).read()
+    names = sorted(set(re.findall(r"\b(odevec_[a-z_0-9]+)\s*\(", hdr)))
+    src = tmp_path / "host.c"
+    src.write_text('#include "odevec.h"\n#include <stdio.h>\nint main(void) {\n'
+                   "  void* p[] = {%s};\n  odevec_opts_t o; odevec_default_opts(&o);\n"
+                   '  printf("%%d %%d\\n", odevec_abi_version(), (int)(sizeof p / sizeof p[0]));\n  return 0;\n}\n'
+                   % ", ".join("(void*)&" + n for n in names))
+    exe = tmp_path / "host"
+    libdir = os.path.join(root, "odevec_amd")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-pedantic-errors", "-Wno-pedantic",
+                    "-I", os.path.join(root, "include"), str(src), "-o", str(exe), "-L", libdir, "-lodevec",
+                    "-Wl,-rpath," + libdir], check=True)
