@@ -276,54 +276,145 @@ ODV_HD inline bool odv_w_has_nan(const L& s) {
   return b;
 }
 
+// Nordsieck sweeps. For a lane whose columns 2.. live outside its registers
+// (L::ZE > 0, in LDS) they run species by species with the current order q
+// as a compile-time constant: each species' columns 0..q are read once,
+// combined in registers and written back once, instead of one pass over all
+// species per column pair with a branch on q each (a lone lane waits for
+// LDS once per sweep instead of once per column pair). Lanes that hold every
+// column in registers keep the column sweeps: there the per-order switch
+// makes the compiler keep the lane's state in scratch memory. Both forms do
+// the reference's operations on each element in the reference's order.
+// Op<Q>::run is the sweep at order Q; odv_by_order calls it for the lane's
+// q (1..MAXORD).
+template <template <int> class Op, class L, class... A>
+ODV_HD inline void odv_by_order(L& s, A... a) {
+  switch (s.q) {
+    case 1: Op<1>::run(s, a...); break;
+    case 2: Op<2>::run(s, a...); break;
+    case 3: Op<3>::run(s, a...); break;
+    case 4: Op<4>::run(s, a...); break;
+    default: Op<MAXORD>::run(s, a...); break;
+  }
+}
+
 // PredictSolution: Pascal triangle with additions only (src/odevec.F90:769-793)
+template <int Q>
+struct PredictOp {
+  template <class L>
+  ODV_HD static void run(L& s) {
+#pragma unroll
+    for (int j = 0; j < L::N; ++j) {
+      double z[Q + 1];
+#pragma unroll
+      for (int kk = 0; kk <= Q; ++kk) z[kk] = s.z(kk, j);
+#pragma unroll
+      for (int l = 0; l < Q; ++l)
+#pragma unroll
+        for (int kk = Q; kk > l; --kk) z[kk - 1] = z[kk] + z[kk - 1];
+#pragma unroll
+      for (int kk = 0; kk < Q; ++kk) s.set_z(kk, j, z[kk]);
+    }
+  }
+};
+
 template <class L>
 ODV_HD inline void odv_predict(L& s) {
-  constexpr int N = L::N;
+  if constexpr (L::ZE > 0) {
+    odv_by_order<PredictOp>(s);
+  } else {
 #pragma unroll
-  for (int l = 0; l < MAXORD; ++l) {
+    for (int l = 0; l < MAXORD; ++l)
 #pragma unroll
-    for (int kk = MAXORD; kk > l; --kk) {
-      if (l < s.q && kk <= s.q) {
+      for (int kk = MAXORD; kk > l; --kk)
+        if (l < s.q && kk <= s.q) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) s.set_z(kk - 1, j, s.z(kk, j) + s.z(kk - 1, j));
-      }
-    }
+          for (int j = 0; j < L::N; ++j) s.set_z(kk - 1, j, s.z(kk, j) + s.z(kk - 1, j));
+        }
   }
 }
 
 // ResetSystem: exact inverse sequence of the predictor (src/odevec.F90:526-544)
+template <int Q>
+struct ResetOp {
+  template <class L>
+  ODV_HD static void run(L& s) {
+#pragma unroll
+    for (int j = 0; j < L::N; ++j) {
+      double z[Q + 1];
+#pragma unroll
+      for (int kk = 0; kk <= Q; ++kk) z[kk] = s.z(kk, j);
+#pragma unroll
+      for (int kk = 0; kk < Q; ++kk)
+#pragma unroll
+        for (int jj = Q; jj > kk; --jj) z[jj - 1] = z[jj - 1] - z[jj];
+#pragma unroll
+      for (int kk = 0; kk < Q; ++kk) s.set_z(kk, j, z[kk]);
+    }
+  }
+};
+
 template <class L>
 ODV_HD inline void odv_reset(L& s) {
-  constexpr int N = L::N;
+  if constexpr (L::ZE > 0) {
+    odv_by_order<ResetOp>(s);
+  } else {
 #pragma unroll
-  for (int kk = 0; kk < MAXORD; ++kk) {
+    for (int kk = 0; kk < MAXORD; ++kk)
 #pragma unroll
-    for (int jj = MAXORD; jj > kk; --jj) {
-      if (kk < s.q && jj <= s.q) {
+      for (int jj = MAXORD; jj > kk; --jj)
+        if (kk < s.q && jj <= s.q) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) s.set_z(jj - 1, j, s.z(jj - 1, j) - s.z(jj, j));
-      }
-    }
+          for (int j = 0; j < L::N; ++j) s.set_z(jj - 1, j, s.z(jj - 1, j) - s.z(jj, j));
+        }
   }
 }
 
-// SetStepSize (src/odevec.F90:720-743)
+// SetStepSize (src/odevec.F90:720-743): column k scaled by scale^k
+template <int Q>
+struct ScaleOp {
+  template <class L>
+  ODV_HD static void run(L& s, double scale) {
+    double d[Q + 1];
+    d[0] = 1.0;
+#pragma unroll
+    for (int kk = 1; kk <= Q; ++kk) d[kk] = scale * d[kk - 1];
+#pragma unroll
+    for (int j = 0; j < L::N; ++j)
+#pragma unroll
+      for (int kk = 1; kk <= Q; ++kk) s.set_z(kk, j, s.z(kk, j) * d[kk]);
+  }
+};
+
 template <class L>
 ODV_HD inline void odv_set_step(L& s, double scale) {
-  constexpr int N = L::N;
-  double d = 1.0;
+  if constexpr (L::ZE > 0) {
+    odv_by_order<ScaleOp>(s, scale);
+  } else {
+    double d = 1.0;
 #pragma unroll
-  for (int kk = 1; kk <= MAXORD; ++kk) {
-    if (kk <= s.q) {
-      d = scale * d;
+    for (int kk = 1; kk <= MAXORD; ++kk)
+      if (kk <= s.q) {
+        d = scale * d;
 #pragma unroll
-      for (int j = 0; j < N; ++j) s.set_z(kk, j, s.z(kk, j) * d);
-    }
+        for (int j = 0; j < L::N; ++j) s.set_z(kk, j, s.z(kk, j) * d);
+      }
   }
   s.dt = s.dt * scale;
   s.jchange = s.jchange * scale;
 }
+
+// UpdateNordsieck (src/odevec.F90:747-765): column k += l_k(q) * correction
+template <int Q>
+struct UpdateOp {
+  template <class L>
+  ODV_HD static void run(L& s) {
+#pragma unroll
+    for (int j = 0; j < L::N; ++j)
+#pragma unroll
+      for (int kk = 0; kk <= Q; ++kk) s.set_z(kk, j, s.z(kk, j) + s.en[j] * s.lq[kk]);
+  }
+};
 
 // CalcStepSizeOrder (src/odevec.F90:547-618); returns the step scale or a
 // negative value on the (unreachable in practice) order error.
@@ -759,13 +850,16 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
     s.swu++;
     s.lju++;
     // UpdateNordsieck (src/odevec.F90:747-765)
+    if constexpr (L::ZE > 0) {
+      odv_by_order<UpdateOp>(s);
+    } else {
 #pragma unroll
-    for (int kk = 0; kk <= MAXORD; ++kk) {
-      if (kk <= s.q) {
-        const double lk = s.lq[kk];
+      for (int kk = 0; kk <= MAXORD; ++kk)
+        if (kk <= s.q) {
+          const double lk = s.lq[kk];
 #pragma unroll
-        for (int j = 0; j < N; ++j) s.set_z(kk, j, s.z(kk, j) + s.en[j] * lk);
-      }
+          for (int j = 0; j < N; ++j) s.set_z(kk, j, s.z(kk, j) + s.en[j] * lk);
+        }
     }
   }
   // CalcStepSizeOrder + SetStepSize after a rejection (src/odevec.F90:434-436)
