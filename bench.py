@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--cells", type=int, default=NCELLS)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
-    p.add_argument("--only", choices=["batching", "synth64", "sessions"], default=None)
+    p.add_argument("--only", choices=["batching", "synth64", "sessions", "tail"], default=None)
     return p.parse_args()
 
 
@@ -223,6 +223,7 @@ def timeline(w):
     end = (t[1] - t0) / hz * 1e3
     start = (t[0] - t0) / hz * 1e3
     slow = np.argsort(-att)[:5]
+    alone = _alone_ms(w, int(slow[0]))
     return {"cells_end_ms": {"p50": float(np.percentile(end, 50)), "p99": float(np.percentile(end, 99)),
                              "p99.9": float(np.percentile(end, 99.9)), "max": float(end.max())},
             "last_cell_start_ms": float(start.max()),
@@ -230,7 +231,35 @@ def timeline(w):
                          "max": int(att.max())},
             "slowest_cells": [{"attempts": int(att[k]), "start_ms": float(start[k]), "end_ms": float(end[k])}
                               for k in slow],
-            "note": "mixed persistent, counters on; ticks of the device wall clock"}
+            "slowest_cell_alone_ms": alone,
+            "note": "mixed persistent, counters on; ticks of the device wall clock; slowest_cell_alone_ms: "
+                    "that cell solved by itself (HIP events), the floor of the launch time"}
+
+
+def _alone_ms(w, k, reps=2):
+    """Kernel time of cell k of workload w solved alone (mixed persistent),
+    HIP events on the launch stream, `reps` launches."""
+    import torch
+    from odevec_amd.api import Odevec
+    dev = torch.device("cuda:0")
+    ode = Odevec(w["network"], 1, precision=1, mode=2)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    y0 = torch.tensor(np.ascontiguousarray(w["y"][:, k:k + 1]), device=dev)
+    ts = torch.tensor([float(w["t_stop"][k])], dtype=torch.float64, device=dev)
+    dt0 = torch.tensor([float(np.broadcast_to(w["dt0"], w["t_stop"].shape)[k])], dtype=torch.float64, device=dev)
+    y = torch.empty_like(y0)
+    stream = torch.cuda.current_stream()
+    ms = []
+    for _ in range(reps):
+        y.copy_(y0)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        ode.solve_cells_async(1, ts.data_ptr(), y.data_ptr(), dt0_ptr=dt0.data_ptr(), stream=stream.cuda_stream)
+        b.record()
+        torch.cuda.synchronize()
+        ms.append(a.elapsed_time(b))
+    ode.CloseOdevec()
+    return ms
 
 
 def end_to_end(w, steps, kernel_ms):
@@ -314,6 +343,9 @@ def main():
         return
     if a.only == "sessions":
         print(json.dumps({"serving_sessions": sessions()}))
+        return
+    if a.only == "tail":
+        print(json.dumps({"timeline": timeline(W.primordial(a.cells, seed=3))}))
         return
     w = W.primordial(a.cells, seed=3)
     mixed = Runner(w, precision=1, mode=2)
