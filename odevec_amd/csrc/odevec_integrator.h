@@ -155,6 +155,9 @@ struct Lane {
   double* wx;            // external error-weight column (WX)
   double lq[7];          // l-vector of the current order (constant indices only)
   double lqq;            // l_q(q)
+  // per-order constants of every attempt, set with the order: tau(q, q)
+  // and the convergence test's divisor tau(q, q) / (2 (q + 2))
+  double tau_s, crit_den;
   double t, dt, tstop, rate, jchange, cerr;
   int q, swu, lju, fet, fc, status;
   bool upd_jac, neglect_up, first;
@@ -197,6 +200,8 @@ ODV_HD inline void odv_set_order(L& s, int q) {
 #pragma unroll
   for (int kk = 2; kk <= MAXORD; ++kk) v = (kk == q) ? s.lq[kk] : v;
   s.lqq = v;
+  s.tau_s = odv_tau(q, q, v);
+  s.crit_den = s.tau_s / (2.0 * (q + 2.0));
 }
 
 // the reference's working y after a corrector update: y_NS(:,:,0) + beta*en
@@ -437,7 +442,7 @@ ODV_HD inline double odv_step_order(L& s, const G& g) {
     down = 1.0 / (ODV_F32(1.3) * odv::m_pow(nq / odv_tau(q, q - 1, lqq), 1.0 / q) + 1e-6);
   }
   const double ne = g.max_active(odv_wnorm(s, g, s.en), s.active);
-  same = 1.0 / (ODV_F32(1.2) * odv::m_pow(ne / odv_tau(q, q, lqq), 1.0 / (q + 1.0)) + 1e-6);
+  same = 1.0 / (ODV_F32(1.2) * odv::m_pow(ne / s.tau_s, 1.0 / (q + 1.0)) + 1e-6);
   if (q == MAXORD || s.neglect_up) {
     up = 0.0;
   } else {
@@ -750,7 +755,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
   int it = 0;
   bool converged = false;
   bool failed = false;
-  const double tauq = odv_tau(s.q, s.q, s.lqq);
+  const double tauq = s.tau_s;
   bool first_it = true;
   while (true) {
     double f[N];
@@ -774,7 +779,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       rtmp = e / s.cerr;
       s.rate = fmax(0.2 * s.rate, rtmp);
     }
-    const double crit = e * fmin(1.0, 1.5 * s.rate) / (tauq / (2.0 * (s.q + 2.0)));
+    const double crit = e * fmin(1.0, 1.5 * s.rate) / s.crit_den;
     if (g.any_valid(odv_has_nan<N>(f) || odv_w_has_nan(s) || (s.fz & 2))) {
       failed = true;
     } else if (crit < 1.0) {
