@@ -140,6 +140,7 @@ struct Lane {
   static constexpr int KS = C::KS;
   static constexpr bool RCP_ = C::RCP;
   static constexpr bool KEEPY_ = C::KEEPY;
+  static constexpr bool WX_ = C::WX;       // error weights in an external column
   static constexpr int ZE = C::ZX;         // external Nordsieck columns: 2 .. 1+ZE
   static constexpr int ZR = NCOL - ZE;     // Nordsieck columns held in the lane
   double zl[ZR][N];      // Nordsieck history: columns 0, 1 and 2+ZE..MAXORD
@@ -164,7 +165,9 @@ struct Lane {
   bool active;           // lane takes part in norms / y updates (lockstep mask)
   // what the frozen species contribute to the reference's reductions: bit 0 a
   // frozen weight is inf or NaN (its 0*0*w norm term is NaN), bit 1 a frozen
-  // weight is NaN (the NaN check fires), bit 2 a frozen value is negative
+  // weight is NaN (the NaN check fires), bit 2 a frozen value is negative;
+  // and bit 3: one of the lane's own weights is NaN (set with the weights
+  // when they are held in registers)
   int fz;
   Counters cnt;
 
@@ -233,14 +236,23 @@ ODV_HD inline void odv_pred_y(const L& s, double* y) {
   }
 }
 
-// 1/(rtol*|y| + atol)^2 (src/odevec.F90:706-716)
+// 1/(rtol*|y| + atol)^2 (src/odevec.F90:706-716). For weights held in
+// registers, bit 3 of fz records whether one of them is NaN, for the
+// corrector's NaN check (src/odevec.F90:645-648), which would otherwise
+// rescan the weights at every Newton iteration although they change only
+// here; weights kept in LDS are rescanned (the flag there made the compiler
+// spill more of the FP64 kernels' registers)
 template <class L>
 ODV_HD inline void odv_weights(L& s, const double* y, double rtol, double atol) {
+  bool nan = false;
 #pragma unroll
   for (int j = 0; j < L::N; ++j) {
     const double v = 1.0 / (rtol * fabs(y[j]) + atol);
-    s.set_w(j, v * v);
+    const double wj = v * v;
+    s.set_w(j, wj);
+    nan = nan || (wj != wj);
   }
+  if constexpr (!L::WX_) s.fz = (s.fz & ~8) | (nan ? 8 : 0);
 }
 
 // sqrt(sum_j x_j^2 w_j / neq) for one cell (src/odevec.F90:669-702); the
@@ -265,6 +277,19 @@ ODV_HD inline double odv_wnorm(const L& s, const G& g, const double* x) {
   return odv_wnorm_f(s, g, [&](int j) { return x[j]; });
 }
 
+// one of the lane's weights is NaN
+template <class L>
+ODV_HD inline bool odv_w_nan(const L& s) {
+  if constexpr (L::WX_) {
+    bool b = false;
+#pragma unroll
+    for (int j = 0; j < L::N; ++j) b = b || (s.w(j) != s.w(j));
+    return b;
+  } else {
+    return (s.fz & 8) != 0;
+  }
+}
+
 template <int N>
 ODV_HD inline bool odv_has_nan(const double* x) {
   bool b = false;
@@ -273,13 +298,6 @@ ODV_HD inline bool odv_has_nan(const double* x) {
   return b;
 }
 
-template <class L>
-ODV_HD inline bool odv_w_has_nan(const L& s) {
-  bool b = false;
-#pragma unroll
-  for (int j = 0; j < L::N; ++j) b = b || (s.w(j) != s.w(j));
-  return b;
-}
 
 // Nordsieck sweeps. For a lane whose columns 2.. live outside its registers
 // (L::ZE > 0, in LDS) they run species by species with the current order q
@@ -780,7 +798,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       s.rate = fmax(0.2 * s.rate, rtmp);
     }
     const double crit = e * fmin(1.0, 1.5 * s.rate) / s.crit_den;
-    if (g.any_valid(odv_has_nan<N>(f) || odv_w_has_nan(s) || (s.fz & 2))) {
+    if (g.any_valid(odv_has_nan<N>(f) || odv_w_nan(s) || (s.fz & 2))) {
       failed = true;
     } else if (crit < 1.0) {
       converged = true;

@@ -814,13 +814,18 @@ int oracle_solve_vector(const onet_t* n, int G, double* time, double* dt, double
   /* CalcInitialStepSize (src/odevec.F90:216-238) */
   {
     const double tol = c->rtol;
+    /* maxval over the cells: NaN sums are skipped unless all are NaN (as in wnorm) */
     double mx = -INFINITY;
+    int nonnan = 0, nan = 0;
     for (int i = 0; i < G; ++i) {
       if (!active(&v, i)) continue;
       double s = (A2(v.rhs, 0, i) * A2(v.rhs, 0, i)) * ((A2(v.w, 0, i) * tol) * tol);
       for (int j = 1; j < N; ++j) s = s + (A2(v.rhs, j, i) * A2(v.rhs, j, i)) * ((A2(v.w, j, i) * tol) * tol);
+      if (s != s) { nan = 1; continue; }
+      nonnan = 1;
       if (s > mx) mx = s;
     }
+    if (nan && !nonnan) mx = NAN;
     const double W0 = fmax(fabs(*time), fabs(t_stop));
     double d = sqrt(tol / (o_pow(W0, -2.0) + mx / (double)N));
     *dt = fmin(d, fabs(t_stop - *time));

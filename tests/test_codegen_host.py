@@ -90,3 +90,22 @@ def test_generated_rhs_and_solve_bitwise(harness, oracle):
         f_or = net.rhs(y, k)
         f_pr = harness.netfun(nid, 0, y[:12], k, nout=12)
         assert np.array_equal(f_pr, f_or[:12]) and np.all(f_or[12:] == 0.0)
+
+
+@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "PRIMORDIAL/CMK/CSC"])
+@pytest.mark.parametrize("strided", [False, True])
+def test_lane_nan_state_matches_oracle(harness, oracle, nid, strided):
+    """NaN in a cell's state: the corrector's NaN check (a NaN correction or
+    weight fails the iteration) and the failure path agree with the oracle,
+    with the weights in registers (strided=False) and in external columns."""
+    w = NETS[nid](4)
+    y = w["y"].copy()
+    y[0, 0] = np.nan                 # an integrated species
+    y[y.shape[0] - 1, 1] = np.nan    # PRIMORDIAL: a frozen species (ROBER: integrated)
+    y[1, 3] = np.inf
+    wn = dict(w, y=y)
+    yh, sh, dh, ch = harness.solve_cells(nid, wn, strided=strided)
+    yr, sr, dr, cr = _oracle_solve(oracle, nid, wn)
+    assert np.array_equal(sh, sr) and (sh[[0, 1]] != 0).all() and sh[2] == 0
+    assert np.array_equal(yh, yr, equal_nan=True)
+    assert np.array_equal(dh, dr, equal_nan=True) and np.array_equal(ch, cr)

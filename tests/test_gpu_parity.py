@@ -71,6 +71,29 @@ def test_repro_bitwise_percell(oracle, case, mode):
     assert stats["attempts"] == int(cr[0].sum()) and stats["newton"] == int(cr[2].sum())
 
 
+@pytest.mark.parametrize("case", ["ROBER", "PRIMORDIAL"])
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_repro_nan_state_bitwise(oracle, case, mode):
+    """Cells whose state holds a NaN or an infinity fail through the
+    corrector's NaN check exactly as in the oracle (status, returned state,
+    last step, counters); the other cells are unaffected. PRIMORDIAL's FP64
+    kernels keep the error weights in LDS, ROBER's in registers."""
+    w = CASES[case]()
+    w = W.subset(w, np.arange(16))
+    y = w["y"].copy()
+    y[0, 0] = np.nan                 # an integrated species
+    y[y.shape[0] - 1, 1] = np.nan    # PRIMORDIAL: a frozen species
+    y[1, 3] = np.inf
+    w = dict(w, y=y)
+    nid = w["network"] + "+REPRO"
+    yg, st, dto, stats = gpu_solve(nid, w, mode=mode)
+    yr, sr, dr, cr = oracle_cells(oracle, nid, w)
+    assert np.array_equal(st, sr) and (st[[0, 1, 3]] != 0).all() and (np.delete(st, [0, 1, 3]) == 0).all()
+    assert np.array_equal(yg, yr, equal_nan=True)
+    assert np.array_equal(dto, dr, equal_nan=True)
+    assert stats["attempts"] == int(cr[0].sum())
+
+
 @pytest.mark.parametrize("case,G", [("ROBER", 32), ("OREGO", 64), ("PRIMORDIAL", 64), ("ROBER_LU3", 32),
                                     ("PRIMORDIAL_LU3", 64)])
 def test_repro_bitwise_lockstep(oracle, case, G):
