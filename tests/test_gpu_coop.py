@@ -89,3 +89,23 @@ def test_coop_default_build_accuracy(oracle, case, precision):
     big = np.abs(yr) > 1e-6 * np.abs(yr).max(axis=0)
     rel = (np.abs(y - yr)[big & ok[None, :]] / np.abs(yr[big & ok[None, :]]))
     assert np.median(rel) < 1e-4, np.median(rel)
+
+
+@pytest.mark.parametrize("case", ["ROBER", "SYNTH64"])
+def test_repro_coop_nan_state_bitwise(oracle, case):
+    """A NaN or an infinity in a cell's state: the group's NaN vote fails
+    the corrector exactly as the oracle's per-cell solve does, and the
+    other cells are unaffected."""
+    w = W.subset(CASES[case](), np.arange(8))
+    y = w["y"].copy()
+    y[0, 0] = np.nan
+    y[y.shape[0] - 1, 1] = np.nan
+    y[1, 3] = np.inf
+    w = dict(w, y=y)
+    nid = w["network"] + "+REPRO"
+    yg, st, dto, stats = coop_solve(nid, w)
+    yr, sr, dr, cr = oracle_cells(oracle, nid, w)
+    assert np.array_equal(st, sr) and (st[[0, 1, 3]] != 0).all()
+    assert np.array_equal(yg, yr, equal_nan=True)
+    assert np.array_equal(dto, dr, equal_nan=True)
+    assert stats["attempts"] == int(cr[0].sum())

@@ -116,6 +116,26 @@ def test_repro_bitwise_lockstep(oracle, case, G):
     assert (st == 0).all() and np.array_equal(y, yr)
 
 
+def test_repro_lockstep_nan_group(oracle):
+    """Lockstep (nvector = 32): a NaN in one cell fails its whole group
+    through the vector's NaN check, as in the oracle; the next group is
+    unaffected."""
+    G, n = 32, 64
+    w = W.rober(n, seed=1)
+    w["t_stop"] = np.full(n, w["t_stop"][0])
+    w["y"] = w["y"].copy()
+    w["y"][1, 5] = np.nan
+    y, st, dto, stats = gpu_solve("ROBER/NONE/DENSE+REPRO", w, mode=_lib.LOCKSTEP, group=G)
+    net = oracle.Network.load("ROBER/NONE/DENSE")
+    for g0 in (0, G):
+        sl = slice(g0, g0 + G)
+        prm = None if w["params"] is None else w["params"][:, sl]
+        yv, s, t, d, cnt = net.solve_vector(w["y"][:, sl], 0.0, w["t_stop"][0], w["dt0"][0], w["rtol"], w["atol"],
+                                            w["dt_min"], params=prm)
+        assert (st[sl] == s).all() and np.array_equal(y[:, sl], yv, equal_nan=True)
+    assert (st[:G] != 0).all() and (st[G:] == 0).all()
+
+
 def test_repro_lockstep_mask(oracle):
     w = W.rober(32)
     mask = np.zeros(32, np.int8)
