@@ -115,6 +115,18 @@ int32_t odevec_set_tolerances(odevec_t* h, double rtol, double atol);
 int32_t odevec_set_dt_min(odevec_t* h, double dt_min);
 int32_t odevec_set_check_negatives(odevec_t* h, int32_t flag);
 
+/* FP64 fallback of the mixed form (no reference counterpart). With
+ * convfail_limit > 0, a cell solved in the mixed form (ODEVEC_MIXED, per-cell
+ * or persistent mode, no history attached) that reaches convfail_limit
+ * convergence failures stops, keeps its input state, and is solved again
+ * from its start by the FP64 persistent kernel in a second launch on the same
+ * stream; its state, status, last step and counters are then those of the
+ * FP64 form. A few cells fall into long runs of convergence failures with FP32
+ * factors (tens of thousands of attempts where FP64 needs about a thousand)
+ * and set the whole launch's length; the fallback bounds them. 0 (default):
+ * off. */
+int32_t odevec_set_fp64_fallback(odevec_t* h, int32_t convfail_limit);
+
 /* Replaces SolveODE(this,time,dt,t_stop,y,GetRHS,GetJac,GetLU,Mask)
  * (src/odevec.F90:158-214) for the handle's nvector cells: integrate y from
  * *time to t_stop; *dt is the initial-step hint on entry and the last step

@@ -95,6 +95,7 @@ def lib():
         "odevec_set_tolerances": (_I32, [_VP, C.c_double, C.c_double]),
         "odevec_set_dt_min": (_I32, [_VP, C.c_double]),
         "odevec_set_check_negatives": (_I32, [_VP, _I32]),
+        "odevec_set_fp64_fallback": (_I32, [_VP, _I32]),
         "odevec_solve": (_I32, [_VP, _DP, _DP, C.c_double, _VP, _VP, _VP]),
         "odevec_solve_cells": (_I32, [_VP, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(Stats)]),
         "odevec_solve_cells_async": (_I32, [_VP, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _VP, _VP, _I64,
@@ -119,7 +120,7 @@ def lib():
 EXPORTED = [
     "odevec_abi_version", "odevec_last_error", "odevec_network_count", "odevec_network_name",
     "odevec_network_info", "odevec_kernel_info", "odevec_default_opts", "odevec_create",
-    "odevec_set_tolerances", "odevec_set_dt_min", "odevec_set_check_negatives", "odevec_solve",
+    "odevec_set_tolerances", "odevec_set_dt_min", "odevec_set_check_negatives", "odevec_set_fp64_fallback", "odevec_solve",
     "odevec_solve_cells", "odevec_solve_cells_async", "odevec_destroy", "odevec_history_planes",
     "odevec_set_history", "odevec_set_timeline", "odevec_get_rhs", "odevec_get_jac", "odevec_jac_entries",
     "odevec_bundle_count",

@@ -118,6 +118,12 @@ class Odevec:
         L.check(L.lib().odevec_set_timeline(self._h, C.c_void_p(self._times.data_ptr()), int(ncells), C.byref(hz)))
         return self._times, int(hz.value)
 
+    def set_fp64_fallback(self, convfail_limit: int):
+        """Mixed form: cells that reach `convfail_limit` convergence failures
+        are solved again from their start in FP64 (a second launch); 0 turns
+        it off (the default)."""
+        L.check(L.lib().odevec_set_fp64_fallback(self._h, int(convfail_limit)))
+
     def disable_history(self):
         L.check(L.lib().odevec_set_history(self._h, None, 0))
         self._hist = None

@@ -23,6 +23,7 @@ std::vector<const BundleEntry*>& bundles() {
 }
 
 __global__ void k_setq(unsigned long long* q, unsigned long long v) { *q = v; }
+__global__ void k_setu(unsigned* p, unsigned v) { *p = v; }
 
 __global__ void k_fill_f64(double* p, double v, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,6 +96,10 @@ struct odevec_s {
   unsigned char* d_bin = nullptr;    // per-cell cost bin
   unsigned* d_count = nullptr;       // COST_BINS counters / offsets
   int64_t order_cap = 0;
+  int promote_cf = 0;                // mixed form: FP64 fallback after this many convergence failures (0: off)
+  int* d_promo = nullptr;            // cells handed to the FP64 pass
+  unsigned* d_promo_n = nullptr;     // their count
+  int64_t promo_cap = 0;
   std::vector<int> h_status;
   std::vector<int> h_cnt;
 };
@@ -192,6 +197,27 @@ int launch(odevec_s* h, const Req& r) {
   odv::Plan p{};
   hipError_t e = h->net->plan(r.ncells, h->opts.mode, h->opts.precision, h->opts.block, h->opts.grid, grp, &p);
   if (e != hipSuccess) return hip_fail(e, "launch plan");
+  // mixed form with the FP64 fallback: cells that reach promote_cf convergence
+  // failures stop, keep their input state and are solved again from the start
+  // by the FP64 persistent kernel over the list of such cells (second launch)
+  const bool promote = h->promote_cf > 0 && h->opts.precision == ODEVEC_MIXED && !a.hist &&
+                       (h->opts.mode == ODEVEC_PERCELL || h->opts.mode == ODEVEC_PERSISTENT);
+  odv::Plan p2{};
+  if (promote) {
+    e = h->net->plan(r.ncells, ODEVEC_PERSISTENT, ODEVEC_FP64, h->opts.block, h->opts.grid, grp, &p2);
+    if (e != hipSuccess) return hip_fail(e, "launch plan (FP64 pass)");
+    if (p2.ws_doubles > p.ws_doubles) p.ws_doubles = p2.ws_doubles;
+    if (r.ncells > h->promo_cap) {
+      if (h->d_promo) (void)hipFree(h->d_promo);
+      h->d_promo = nullptr;
+      h->promo_cap = 0;
+      if ((e = hipMalloc((void**)&h->d_promo, (size_t)r.ncells * sizeof(int))) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("fallback list allocation failed: ") + hipGetErrorString(e));
+      h->promo_cap = r.ncells;
+    }
+    if (!h->d_promo_n && (e = hipMalloc((void**)&h->d_promo_n, sizeof(unsigned))) != hipSuccess)
+      return fail(ODEVEC_ENOMEM, std::string("fallback list allocation failed: ") + hipGetErrorString(e));
+  }
   if (p.ws_doubles > h->ws_cap) {
     // hipFree waits for the device, so no earlier launch still reads the old workspace
     if (h->d_ws) (void)hipFree(h->d_ws);
@@ -224,8 +250,26 @@ int launch(odevec_s* h, const Req& r) {
       return hip_fail(e, "cost order");
     a.order = h->d_order;
   }
+  if (promote) {
+    a.ctl.promote_cf = h->promote_cf;
+    a.promo = h->d_promo;
+    a.promo_n = h->d_promo_n;
+    hipLaunchKernelGGL(odv::k_setu, dim3(1), dim3(1), 0, r.stream, h->d_promo_n, 0u);
+  }
   e = h->net->launch(a, h->opts.mode, h->opts.precision, p, r.stream);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  if (promote) {
+    odv::KArgs a2 = a;
+    a2.ctl.promote_cf = 0;
+    a2.promo = nullptr;
+    a2.promo_n = nullptr;
+    a2.order = h->d_promo;
+    a2.qlen = h->d_promo_n;
+    h->qslot = (h->qslot + 1) % 64;
+    a2.queue = h->d_queue + h->qslot;
+    e = h->net->launch(a2, ODEVEC_PERSISTENT, ODEVEC_FP64, p2, r.stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch (FP64 pass)");
+  }
   return ODEVEC_OK;
 }
 
@@ -424,6 +468,13 @@ int32_t odevec_set_dt_min(odevec_t* h, double dt_min) {
 int32_t odevec_set_check_negatives(odevec_t* h, int32_t flag) {
   if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
   h->check_neg = flag ? 1 : 0;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_set_fp64_fallback(odevec_t* h, int32_t convfail_limit) {
+  if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
+  if (convfail_limit < 0) return fail(ODEVEC_EINVAL, "convfail_limit must be >= 0");
+  h->promote_cf = convfail_limit;
   return ODEVEC_OK;
 }
 
@@ -675,6 +726,8 @@ int32_t odevec_destroy(odevec_t* h) {
   if (h->d_queue) (void)hipFree(h->d_queue);
   if (h->d_ws) (void)hipFree(h->d_ws);
   if (h->d_order) (void)hipFree(h->d_order);
+  if (h->d_promo) (void)hipFree(h->d_promo);
+  if (h->d_promo_n) (void)hipFree(h->d_promo_n);
   if (h->d_bin) (void)hipFree(h->d_bin);
   if (h->d_count) (void)hipFree(h->d_count);
   if (h->e0) (void)hipEventDestroy(h->e0);

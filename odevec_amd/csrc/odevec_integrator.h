@@ -38,6 +38,8 @@ enum Status : int {
   ST_ORDER = 3,        // order/step-size mismatch (src/odevec.F90:584-587, 595-598)
   ST_MAX_STEPS = 4,    // attempt budget exhausted (no reference counterpart)
   ST_SKIPPED = 5,      // masked out / empty span: y returned unchanged
+  ST_PROMOTE = 6,      // mixed form: too many convergence failures, the cell is re-solved in FP64
+                       // (never returned to the caller)
 };
 
 constexpr int MAXORD = 5;          // pre_odevec.py:83
@@ -47,6 +49,7 @@ struct Controls {
   double rtol, atol, dt_min;
   int check_negatives;
   int max_attempts;
+  int promote_cf;      // > 0: stop a cell at this many convergence failures (ST_PROMOTE)
 };
 
 // Per-cell event counters (feed the flop model and diagnostics).
@@ -826,6 +829,10 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       s.t = told;
       s.fc++;
       s.cnt.convfail++;
+      if (c.promote_cf > 0 && s.cnt.convfail >= c.promote_cf) {
+        s.status = ST_PROMOTE;
+        return true;
+      }
       if (s.fc > 10) {
         s.status = ST_CONV_FAIL;
         return true;

@@ -57,6 +57,9 @@ struct KArgs {
   double* hist;           // optional warm-restart history: Hist<Net>::PLANES doubles per cell, cell-major
   const int* order;       // persistent: queue position -> cell (costliest first), or null = index order
   long long* times;       // optional [2][ncells]: device wall-clock ticks when each cell started / ended
+  int* promo;             // ctl.promote_cf > 0: cells stopped with ST_PROMOTE are appended here ...
+  unsigned* promo_n;      // ... at promo[atomicAdd(promo_n, 1)]; their y, status, counters are not written
+  const unsigned* qlen;   // persistent: number of queue positions (device count), or null = ncells
   Controls ctl;
   int group;              // lockstep group size
 };
@@ -305,6 +308,11 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
 template <class L, bool HIST = false>
 __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   if (a.times) a.times[a.ncells + i] = (long long)wall_clock64();
+  if (s.status == ST_PROMOTE) {
+    // the caller's state stays untouched for the FP64 pass over promo[]
+    a.promo[atomicAdd(a.promo_n, 1u)] = (int)i;
+    return;
+  }
   if (s.status == ST_SKIPPED) {
     if (a.status) a.status[i] = s.status;
     if (a.dt_out) a.dt_out[i] = s.dt;
@@ -364,15 +372,16 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   const long long nlanes = (long long)gridDim.x * blockDim.x;
   const long long gl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
+  const long long nq = a.qlen ? (long long)*a.qlen : a.ncells;   // queue positions
   const long long p0 = a.order ? (long long)lane * (nlanes >> 6) + (gl >> 6) : gl;
-  long long i = p0 < a.ncells ? (a.order ? (long long)a.order[p0] : p0) : a.ncells;
+  long long i = p0 < nq ? (a.order ? (long long)a.order[p0] : p0) : a.ncells;
   L s;
   odv_bind_columns<P>(s, a, lds);
   LaneGroup g;
   double fl64 = 0.0, fl32 = 0.0;
-  bool have = p0 < a.ncells;
+  bool have = p0 < nq;
   bool done = true;
-  bool exhausted = nlanes >= a.ncells;   // wave-uniform: no cell left to hand out
+  bool exhausted = nlanes >= nq;   // wave-uniform: no cell left to hand out
   if (have) done = odv_load_begin<L, HIST>(a, i, s);
   while (true) {
     const bool need = !have || done;
@@ -394,10 +403,10 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
       unsigned long long base = 0;
       if (lane == leader) base = atomicAdd(a.queue, (unsigned long long)cnt);
       base = __shfl(base, leader, 64);
-      if ((long long)base + cnt >= a.ncells) exhausted = true;
+      if ((long long)base + cnt >= nq) exhausted = true;
       if (need) {
         const long long ni = (long long)base + rank;
-        if (ni < a.ncells) {
+        if (ni < nq) {
           i = a.order ? (long long)a.order[ni] : ni;
           have = true;
           done = odv_load_begin<L, HIST>(a, i, s);
@@ -417,6 +426,8 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
   }
 }
 
+// With a.order and a.qlen the queue is the first *a.qlen entries of a.order
+// (the FP64 pass over the mixed form's promoted cells).
 template <class Net, class M, bool RCP, bool HIST = false>
 __global__ void __launch_bounds__(MAX_BLOCK) odv_k_persistent(KArgs a) {
   odv_persistent_run<Net, M, RCP, HIST>(a, odv_lane_lds);

@@ -479,3 +479,57 @@ def test_plugin_getrhs_getjac(oracle, nid):
         else:
             ref = P.T.reshape(-1)          # entry c*neq + r
         assert len(ref) == jac.shape[0] and np.array_equal(jac[:, i], ref), i
+
+
+def _solve_dev(nid, w, precision, mode, fallback=0):
+    """Device-resident solve with per-cell counters: (y, status, dt_out, counters[8, n])."""
+    import torch
+    from odevec_amd.api import Odevec
+    n = w["y"].shape[1]
+    say("GPU solve (device arrays)", nid, "cells", n, "precision", precision, "mode", mode, "fallback", fallback)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream()
+    ode = Odevec(nid, n, precision=precision, mode=mode)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    if fallback:
+        ode.set_fp64_fallback(fallback)
+    y = torch.tensor(np.ascontiguousarray(w["y"]), device=dev)
+    ts = torch.tensor(np.ascontiguousarray(w["t_stop"], dtype=np.float64), device=dev)
+    dt0 = torch.tensor(np.broadcast_to(w["dt0"], (n,)).astype(np.float64), device=dev)
+    st = torch.zeros(n, dtype=torch.int32, device=dev)
+    dto = torch.zeros(n, dtype=torch.float64, device=dev)
+    cnt = torch.zeros(8 * n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    ode.solve_cells_async(n, ts.data_ptr(), y.data_ptr(), dt0_ptr=dt0.data_ptr(), status_ptr=st.data_ptr(),
+                          dt_out_ptr=dto.data_ptr(), counters_ptr=cnt.data_ptr(), stream=stream.cuda_stream)
+    stream.synchronize()
+    out = (y.cpu().numpy(), st.cpu().numpy(), dto.cpu().numpy(), cnt.cpu().numpy().reshape(8, n))
+    ode.CloseOdevec()
+    return out
+
+
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_mixed_fp64_fallback(mode):
+    """Mixed form with the FP64 fallback: the cells whose mixed solve reaches
+    the limit of convergence failures carry the FP64 form's result bit for
+    bit (state, status, last step, counters), all others the plain mixed
+    result; per-cell and persistent modes agree."""
+    w = W.primordial(3000, seed=13)
+    w["t_stop"][::11] = 0.0                      # skipped cells stay skipped
+    nid = "PRIMORDIAL/CMK/CSC+REPRO"
+    limit = 8
+    ym, sm, dm, cm = _solve_dev(nid, w, 1, mode)
+    yd, sd, dd, cd = _solve_dev(nid, w, 0, _lib.PERCELL)
+    yf, sf, df, cf = _solve_dev(nid, w, 1, mode, fallback=limit)
+    prom = cm[5] >= limit
+    say("promoted cells", int(prom.sum()), "of", len(prom))
+    assert 20 <= prom.sum() <= len(prom) // 2
+    assert np.array_equal(yf[:, prom], yd[:, prom]) and np.array_equal(sf[prom], sd[prom])
+    assert np.array_equal(df[prom], dd[prom]) and np.array_equal(cf[:, prom], cd[:, prom])
+    keep = ~prom
+    assert np.array_equal(yf[:, keep], ym[:, keep]) and np.array_equal(sf[keep], sm[keep])
+    assert np.array_equal(df[keep], dm[keep]) and np.array_equal(cf[:, keep], cm[:, keep])
+    assert (sf[::11] == 5).all()
+    if mode == _lib.PERSISTENT:
+        yp, sp, dp, cp = _solve_dev(nid, w, 1, _lib.PERCELL, fallback=limit)
+        assert np.array_equal(yp, yf) and np.array_equal(sp, sf) and np.array_equal(dp, df)
