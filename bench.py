@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--only", choices=["batching", "synth64", "sessions", "tail"], default=None)
+    p.add_argument("--fallback", type=int, default=FALLBACK,
+                   help="mixed form: FP64 fallback after this many convergence failures (0: off)")
     return p.parse_args()
 
 
@@ -332,7 +334,9 @@ def read_traffic():
 
 
 def main():
+    global FALLBACK
     a = parse()
+    FALLBACK = a.fallback
     if a.gpus != 1:
         print("bench.py runs on one GPU: there is no multi-GPU path (cells are independent: each GPU "
               "would take a contiguous range of cells, one process per GPU, with no exchange during "
