@@ -69,6 +69,10 @@ constexpr int BLOCK = 64;        // default lanes per workgroup of the per-cell 
 constexpr int MAX_BLOCK = 256;   // largest workgroup of the per-cell kernels
 
 constexpr int LDS_BYTES = 160 * 1024;   // LDS per CU on gfx950
+// persistent kernel: lanes waiting for a new cell before a wavefront refills
+// (1M-cell PRIMORDIAL, mixed with the FP64 fallback: 1 -> 221 ms, 4 -> 212 ms,
+// 8 -> 213 ms per solve; results do not depend on it)
+constexpr int REFILL_MIN = 4;
 constexpr int LDS_WAVES = 4;            // resident wavefronts per CU the LDS placement is sized for
 
 template <class Net, class M>
@@ -396,7 +400,10 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
       have = false;
     }
     const unsigned long long ball = __ballot(need);
-    if (ball && !exhausted) {
+    // refill once REFILL_MIN lanes wait, or when every lane does: the queue
+    // atomic and the wavefront's pass through the new cells' start (rates,
+    // initial step) then serve several lanes at once
+    if (ball && !exhausted && (__popcll(ball) >= REFILL_MIN || ball == __ballot(true))) {
       const int cnt = __popcll(ball);
       const int rank = __popcll(ball & ((1ull << lane) - 1ull));
       const int leader = __ffsll((long long)ball) - 1;
