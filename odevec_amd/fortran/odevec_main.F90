@@ -53,6 +53,9 @@ module odevec_main
     real(dp) :: AbsoluteTolerance = 1e-10_dp
     real(dp) :: MinimalTimestep = real(1e-10, dp)
     logical :: CheckNegatives = .false.
+    ! mixed form (ODEVEC_PRECISION=1): cells reaching this many convergence
+    ! failures are solved again in FP64 (0: off); no reference counterpart
+    integer :: FP64Fallback = 0
     real(dp), allocatable :: y(:,:)
     integer(c_int32_t), allocatable :: status(:)
     type(c_ptr) :: handle = c_null_ptr
@@ -87,6 +90,12 @@ module odevec_main
       import :: c_ptr, c_int32_t
       type(c_ptr), value :: h
       integer(c_int32_t), value :: flag
+      integer(c_int32_t) :: rc
+    end function
+    function c_set_fp64_fallback(h, limit) bind(C, name='odevec_set_fp64_fallback') result(rc)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: limit
       integer(c_int32_t) :: rc
     end function
     function c_solve(h, time, dt, t_stop, y, mask, status) bind(C, name='odevec_solve') result(rc)
@@ -172,6 +181,7 @@ contains
     rc = c_set_tolerances(this%handle, this%RelativeTolerance, this%AbsoluteTolerance)
     if (rc == 0) rc = c_set_dt_min(this%handle, this%MinimalTimestep)
     if (rc == 0) rc = c_set_check_negatives(this%handle, merge(1_c_int32_t, 0_c_int32_t, this%CheckNegatives))
+    if (rc == 0) rc = c_set_fp64_fallback(this%handle, int(this%FP64Fallback, c_int32_t))
     if (rc /= 0) call fail(rc, "SolveODE (settings)")
     mp = c_null_ptr
     if (present(Mask)) then
