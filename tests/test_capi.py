@@ -51,6 +51,7 @@ def test_create_argument_checks():
     sched = _lib.Opts(0, _lib.PERSISTENT, 0, 0, 0, 0, 0, 0, 2)
     assert L.odevec_create(b"ROBER/NONE/DENSE", 4, C.byref(sched), C.byref(h)) == _lib.EINVAL
     assert L.odevec_set_tolerances(None, 1e-6, 1e-10) == _lib.EINVAL
+    assert L.odevec_set_fp64_fallback(None, 128) == _lib.EINVAL
     assert L.odevec_destroy(None) == _lib.OK
 
 
