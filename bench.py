@@ -405,7 +405,9 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": frac, "traffic": read_traffic(),
                      "note": "compute (FP64/FP32 VALU) bound; achieved = FP64 + FP32*peak64/peak32 algorithmic "
-                             "flops per launch (op-count model) / mean launch time; no MFMA is used"},
+                             "flops per solve (op-count model; the mixed kernel and the FP64 pass) / mean solve "
+                             "time (HIP events around both launches); traffic: the mixed kernel's HBM bytes per "
+                             "launch; no MFMA is used"},
         "flops_per_launch": {"fp64": float(fl_m[0]), "fp32": float(fl_m[1])},
         "kernel_ms": {"mixed": ms_m, "fp64": ms_d},
         "mixed_without_fallback": {"value": min(a.steps, 2) * a.cells / wall_n, "unit": "cell-solves/s",
