@@ -141,12 +141,13 @@ def batching(steps, warmup):
              "order": nets}
     d = A.cost_order(A._device_segments(segs, [t["precision"][n] for n in nets]),
                      {n: -k for k, n in enumerate(t["order"])})
-    sec = A.time_batch(t["bundle"], d, t["block"], use_graph=True, reps=max(steps, 1))
+    sec = A.time_batch(t["bundle"], d, t["block"], use_graph=True, reps=max(steps, 1),
+                       fp64_fallback=t.get("fp64_fallback", 0))
     ok = sum(int((x["status"] == 0).sum()) for x in d)
     return {"value": 2097152 / sec, "unit": "cell-solves/s", "ms_per_launch": sec * 1e3,
             "workload": "2,097,152 cells, 40/30/30 ROBER/OREGO/PRIMORDIAL (seed 4), t_stop logU over 6 decades "
                         "per network, one persistent launch per step, hipGraph replay",
-            "bundle": t["bundle"], "block": t["block"], "precision": t["precision"], "cells_ok": ok,
+            "bundle": t["bundle"], "block": t["block"], "fp64_fallback": t.get("fp64_fallback", 0), "precision": t["precision"], "cells_ok": ok,
             "tuned": "per_cell_seconds" in t and bool(t["per_cell_seconds"]), "device": A.device_key()}
 
 
@@ -179,7 +180,7 @@ def serving(rate, seconds, networks=None):
     nets = networks or list(S.SERVE_SPANS)
     tuned = A.lookup([n for n in nets if n not in S.COOP_NETWORKS]) or {}
     srv = S.Server(tuned.get("bundle", "MIX3"), slots=2, precision=tuned.get("precision", {}),
-                   block=tuned.get("block", 256))
+                   block=tuned.get("block", 256), fp64_fallback=tuned.get("fp64_fallback", 0))
     res = S.run_trace(srv, S.make_trace(rate, seconds, seed=5, networks=nets))
     srv.close()
     return dict(rate_per_s=rate, seconds=seconds, networks=nets, **res)

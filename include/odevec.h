@@ -254,6 +254,14 @@ int32_t odevec_batch_create(const char* bundle, int32_t block, int32_t grid, int
  * hipStream_t). Use one stream per batch handle: consecutive calls reuse its
  * segment table, queue counters and workspace in stream order. */
 int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs, int32_t nseg, void* stream);
+
+/* FP64 fallback for the batch's mixed segments (no reference counterpart;
+ * see odevec_set_fp64_fallback): with convfail_limit > 0, a cell of a mixed
+ * segment (without history) that reaches convfail_limit convergence failures
+ * keeps its input state, and after the batch launch the network's FP64
+ * persistent kernel solves such cells on the same stream (one launch per
+ * segment, outside the batch's captured graph). 0 (default): off. */
+int32_t odevec_batch_set_fp64_fallback(odevec_batch_t* b, int32_t convfail_limit);
 int32_t odevec_batch_kernel_info(const char* bundle, int32_t block, odevec_kernel_info_t* info);
 int32_t odevec_batch_destroy(odevec_batch_t* b);
 

@@ -96,6 +96,7 @@ def lib():
         "odevec_set_dt_min": (_I32, [_VP, C.c_double]),
         "odevec_set_check_negatives": (_I32, [_VP, _I32]),
         "odevec_set_fp64_fallback": (_I32, [_VP, _I32]),
+        "odevec_batch_set_fp64_fallback": (_I32, [_VP, _I32]),
         "odevec_solve": (_I32, [_VP, _DP, _DP, C.c_double, _VP, _VP, _VP]),
         "odevec_solve_cells": (_I32, [_VP, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(Stats)]),
         "odevec_solve_cells_async": (_I32, [_VP, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _VP, _VP, _I64,
@@ -125,7 +126,7 @@ EXPORTED = [
     "odevec_set_history", "odevec_set_timeline", "odevec_get_rhs", "odevec_get_jac", "odevec_jac_entries",
     "odevec_bundle_count",
     "odevec_bundle_name", "odevec_bundle_entry", "odevec_batch_create", "odevec_batch_solve_async",
-    "odevec_batch_kernel_info", "odevec_batch_destroy",
+    "odevec_batch_kernel_info", "odevec_batch_destroy", "odevec_batch_set_fp64_fallback",
 ]
 
 

@@ -274,7 +274,8 @@ class Batch:
     the GPU (or raw device addresses); ``ncells`` is taken from ``y``.
     """
 
-    def __init__(self, bundle: str = "MIX3", block: int = 0, grid: int = 0, use_graph: bool = False):
+    def __init__(self, bundle: str = "MIX3", block: int = 0, grid: int = 0, use_graph: bool = False,
+                 fp64_fallback: int = 0):
         self.bundle = bundle
         self.entries = L.bundle_entries(bundle)
         h = C.c_void_p()
@@ -282,6 +283,13 @@ class Batch:
                                             C.byref(h)))
         self._h = h
         self._keep = None
+        if fp64_fallback:
+            self.set_fp64_fallback(fp64_fallback)
+
+    def set_fp64_fallback(self, convfail_limit: int):
+        """Mixed segments: cells that reach `convfail_limit` convergence
+        failures are solved again in FP64 after the batch launch (0: off)."""
+        L.check(L.lib().odevec_batch_set_fp64_fallback(self._h, int(convfail_limit)))
 
     def solve_async(self, segments, stream=None):
         if self._h is None:

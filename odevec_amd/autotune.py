@@ -64,13 +64,20 @@ def _device_segments(segs, precisions):
     return out
 
 
-def time_batch(bundle: str, dsegs, block: int, use_graph: bool = False, reps: int = 2, stream=None) -> float:
-    """Median seconds of one batch launch over device segments (state
-    restored by a device copy before each launch, outside the timing)."""
+# mixed segments are timed (and then run) with the FP64 fallback at this many
+# convergence failures: without it a few cells set a mixed launch's length
+FP64_FALLBACK = 128
+
+
+def time_batch(bundle: str, dsegs, block: int, use_graph: bool = False, reps: int = 2, stream=None,
+               fp64_fallback: int = FP64_FALLBACK) -> float:
+    """Median seconds of one batch launch (with its FP64 passes) over device
+    segments (state restored by a device copy before each launch, outside
+    the timing)."""
     import torch
     from .api import Batch
     stream = stream or torch.cuda.Stream()
-    b = Batch(bundle, block=block, use_graph=use_graph)
+    b = Batch(bundle, block=block, use_graph=use_graph, fp64_fallback=fp64_fallback)
     ts = []
     with torch.cuda.stream(stream):
         for r in range(reps + 1):
@@ -150,7 +157,8 @@ def tune(segs: List[dict], sample: float = 0.25, seed: int = 0, verbose: bool = 
             for block in BLOCKS:
                 t = time_batch(bundle, dsegs, block, stream=stream)
                 results.append({"bundle": bundle, "block": block, "seconds": t,
-                                "precision": {n: int(p) for n, p in zip(nets, precs)}})
+                                "precision": {n: int(p) for n, p in zip(nets, precs)},
+                                "fp64_fallback": FP64_FALLBACK})
                 if verbose:
                     print("autotune %-11s block %3d precision %s: %.2f ms" % (bundle, block, precs, t * 1e3),
                           flush=True)

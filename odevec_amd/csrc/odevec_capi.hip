@@ -24,6 +24,10 @@ std::vector<const BundleEntry*>& bundles() {
 
 __global__ void k_setq(unsigned long long* q, unsigned long long v) { *q = v; }
 __global__ void k_setu(unsigned* p, unsigned v) { *p = v; }
+__global__ void k_zero_u(unsigned* p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0u;
+}
 
 __global__ void k_fill_f64(double* p, double v, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -779,6 +783,12 @@ struct odevec_batch_s {
   odv::Plan gplan{};
   hipStream_t gstream = nullptr;
   int ghist = 0;                   // the captured launch is the history variant
+  int promote_cf = 0;              // FP64 fallback of mixed segments (0: off)
+  int* d_promo = nullptr;          // handed-back cells of every mixed segment, one region per segment
+  int64_t promo_cap = 0;
+  unsigned* d_promo_n = nullptr;   // MAXSEG counts
+  double* d_ws2 = nullptr;         // workspace of the FP64 passes
+  int64_t ws2_cap = 0;
 };
 
 namespace {
@@ -788,6 +798,37 @@ void batch_drop_graph(odevec_batch_s* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
+}
+
+// The FP64 passes over the cells the batch launch handed back: one launch
+// of the network's FP64 persistent kernel per mixed segment, its queue the
+// segment's list (order) and device count (qlen), on the batch's stream.
+int batch_fallback_passes(odevec_batch_s* b, const odevec_segment_t* segs, int nseg,
+                          const std::vector<const odv::NetEntry*>& fbnet, hipStream_t st) {
+  for (int k = 0; k < nseg; ++k) {
+    if (!fbnet[k]) continue;
+    odv::Plan p2{};
+    hipError_t he = fbnet[k]->plan(segs[k].ncells, ODEVEC_PERSISTENT, ODEVEC_FP64, b->block, 0, 1, &p2);
+    if (he != hipSuccess) return hip_fail(he, "launch plan (FP64 pass)");
+    if (p2.ws_doubles > b->ws2_cap) {
+      if (b->d_ws2) (void)hipFree(b->d_ws2);
+      b->d_ws2 = nullptr;
+      b->ws2_cap = 0;
+      if ((he = hipMalloc((void**)&b->d_ws2, (size_t)p2.ws_doubles * 8)) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("fallback workspace allocation failed: ") + hipGetErrorString(he));
+      b->ws2_cap = p2.ws_doubles;
+    }
+    odv::KArgs a2 = b->h_segs[k].a;
+    a2.ctl.promote_cf = 0;
+    a2.order = a2.promo;
+    a2.qlen = a2.promo_n;
+    a2.promo = nullptr;
+    a2.promo_n = nullptr;
+    a2.ws = b->d_ws2;
+    he = fbnet[k]->launch(a2, ODEVEC_PERSISTENT, ODEVEC_FP64, p2, st);
+    if (he != hipSuccess) return hip_fail(he, "kernel launch (FP64 pass)");
+  }
+  return ODEVEC_OK;
 }
 
 }  // namespace
@@ -898,6 +939,34 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       return fail(ODEVEC_ENOMEM, std::string("rate workspace allocation failed: ") + hipGetErrorString(he));
     b->ws_cap = p.ws_doubles;
   }
+  // FP64 fallback: mixed segments (no history) hand back the cells that reach
+  // promote_cf convergence failures; after the batch launch, the network's
+  // FP64 persistent kernel solves each segment's handed-back cells
+  std::vector<const odv::NetEntry*> fbnet(nseg, nullptr);
+  std::vector<int64_t> fboff(nseg, 0);
+  int64_t fbtotal = 0;
+  if (b->promote_cf > 0) {
+    for (int k = 0; k < nseg; ++k) {
+      const odevec_segment_t& g = segs[k];
+      if (g.precision != ODEVEC_MIXED || g.hist || g.ncells <= 0) continue;
+      const odv::NetEntry* ne = find_net(g.network);
+      if (!ne || !ne->lane_ok) continue;
+      fbnet[k] = ne;
+      fboff[k] = fbtotal;
+      fbtotal += g.ncells;
+    }
+    if (fbtotal > b->promo_cap) {
+      if (b->d_promo) (void)hipFree(b->d_promo);
+      b->d_promo = nullptr;
+      b->promo_cap = 0;
+      if ((he = hipMalloc((void**)&b->d_promo, (size_t)fbtotal * sizeof(int))) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("fallback list allocation failed: ") + hipGetErrorString(he));
+      b->promo_cap = fbtotal;
+    }
+    if (fbtotal > 0 && !b->d_promo_n &&
+        (he = hipMalloc((void**)&b->d_promo_n, odv::MAXSEG * sizeof(unsigned))) != hipSuccess)
+      return fail(ODEVEC_ENOMEM, std::string("fallback list allocation failed: ") + hipGetErrorString(he));
+  }
   // the previous call's table copy must have left the staging buffer
   if (b->ev_pending && (he = hipEventSynchronize(b->ev_copied)) != hipSuccess) return hip_fail(he, "batch staging");
   for (int k = 0; k < nseg; ++k) {
@@ -926,6 +995,11 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     a.ctl.max_attempts = g.max_attempts > 0 ? g.max_attempts : 1000000;
     a.group = 1;
     a.hist = g.hist;
+    if (fbnet[k]) {
+      a.ctl.promote_cf = b->promote_cf;
+      a.promo = b->d_promo + fboff[k];
+      a.promo_n = b->d_promo_n + k;
+    }
     d.kind = kind[k];
   }
   *b->h_nseg = nseg;
@@ -934,9 +1008,11 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       (he = hipEventRecord(b->ev_copied, st)) != hipSuccess)
     return hip_fail(he, "batch table upload");
   b->ev_pending = true;
+  if (fbtotal > 0) hipLaunchKernelGGL(odv::k_zero_u, dim3((nseg + 63) / 64), dim3(64), 0, st, b->d_promo_n, nseg);
   if (!b->use_graph) {
     he = e->launch(b->d_segs, b->d_nseg, odv::MAXSEG, p, hist, st);
-    return he == hipSuccess ? ODEVEC_OK : hip_fail(he, "batch launch");
+    if (he != hipSuccess) return hip_fail(he, "batch launch");
+    return fbtotal > 0 ? batch_fallback_passes(b, segs, nseg, fbnet, st) : ODEVEC_OK;
   }
   // graph: the queue reset and the batch kernel, captured once per launch
   // shape and stream and replayed; the table is read from device memory, so
@@ -960,6 +1036,13 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     b->ghist = hist;
   }
   if ((he = hipGraphLaunch(b->exec, st)) != hipSuccess) return hip_fail(he, "graph launch");
+  return fbtotal > 0 ? batch_fallback_passes(b, segs, nseg, fbnet, st) : ODEVEC_OK;
+}
+
+int32_t odevec_batch_set_fp64_fallback(odevec_batch_t* b, int32_t convfail_limit) {
+  if (!b) return fail(ODEVEC_EINVAL, "batch is NULL");
+  if (convfail_limit < 0) return fail(ODEVEC_EINVAL, "convfail_limit must be >= 0");
+  b->promote_cf = convfail_limit;
   return ODEVEC_OK;
 }
 
@@ -982,7 +1065,7 @@ int32_t odevec_batch_destroy(odevec_batch_t* b) {
   batch_drop_graph(b);
   if (b->h_segs) (void)hipHostFree(b->h_segs);
   if (b->h_nseg) (void)hipHostFree(b->h_nseg);
-  void* ps[] = {b->d_segs, b->d_nseg, b->d_queues, b->d_scal, b->d_ws};
+  void* ps[] = {b->d_segs, b->d_nseg, b->d_queues, b->d_scal, b->d_ws, b->d_promo, b->d_promo_n, b->d_ws2};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (b->ev_copied) (void)hipEventDestroy(b->ev_copied);

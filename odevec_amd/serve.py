@@ -107,7 +107,7 @@ class Request:
 
 class Server:
     def __init__(self, bundle: str = "MIX3", slots: int = 2, hbm_budget: int = 16 << 30, block: int = 256,
-                 precision: Optional[dict] = None):
+                 precision: Optional[dict] = None, fp64_fallback: int = 0):
         import torch
         from .api import Batch
         self.torch = torch
@@ -118,7 +118,8 @@ class Server:
         L.check(L.lib().odevec_batch_kernel_info(bundle.encode(), block, L.C.byref(kinfo)))
         full = dev.multi_processor_count * max(kinfo.blocks_per_cu, 1)
         self.grid = max(1, full // slots)
-        self.slots = [{"stream": torch.cuda.Stream(), "batch": Batch(bundle, block=block, grid=self.grid),
+        self.slots = [{"stream": torch.cuda.Stream(),
+                       "batch": Batch(bundle, block=block, grid=self.grid, fp64_fallback=fp64_fallback),
                        "coop": {}, "lane": {}, "reqs": None, "event": None} for _ in range(slots)]
         self.sessions = {}
         self._next_sid = 0

@@ -145,3 +145,51 @@ def test_batch_rejects_entries_not_in_bundle():
         with pytest.raises(_lib.OdevecError) as e:
             b.solve_async([seg], stream().cuda_stream)
         assert e.value.code == _lib.ENOTFOUND
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_fp64_fallback(use_graph):
+    """Mixed segments with the FP64 fallback: handed-back cells carry the
+    FP64 form's result bit for bit (the single-network FP64 per-cell kernel
+    on the same cells), the others the batch's plain mixed result; an FP64
+    segment in the same batch is unaffected; graph replays agree."""
+    from odevec_amd.api import Batch, Odevec
+    import torch
+    pri = W.primordial(1500, seed=21)
+    pri["t_stop"][::13] = 0.0
+    rob = W.rober(100, seed=22)
+    limit = 8
+    say("GPU batch fallback, graph", use_graph)
+    plain = run("MIX3", [to_dev(pri, _lib.MIXED), to_dev(rob, _lib.FP64)])
+    b = Batch("MIX3", use_graph=use_graph, fp64_fallback=limit)
+    segs = [to_dev(pri, _lib.MIXED), to_dev(rob, _lib.FP64)]
+    fb = run("MIX3", segs, batch=b)
+    if use_graph:                        # a replay over fresh inputs gives the same
+        segs2 = [to_dev(pri, _lib.MIXED), to_dev(rob, _lib.FP64)]
+        fb2 = run("MIX3", segs2, batch=b)
+        assert all(np.array_equal(x, y) for p, q in zip(fb, fb2) for x, y in zip(p, q))
+    b.close()
+    # FP64 reference for the same cells: the single-network per-cell kernel
+    n = pri["y"].shape[1]
+    ode = Odevec(pri["network"], n, precision=0, mode=_lib.PERCELL)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = pri["rtol"], pri["atol"], pri["dt_min"]
+    dev = to_dev(pri, _lib.FP64)
+    torch.cuda.synchronize()
+    ode.solve_cells_async(n, dev["t_stop"].data_ptr(), dev["y"].data_ptr(), t0_ptr=dev["t0"].data_ptr(),
+                          dt0_ptr=dev["dt0"].data_ptr(), status_ptr=dev["status"].data_ptr(),
+                          dt_out_ptr=dev["dt_out"].data_ptr(), counters_ptr=dev["counters"].data_ptr(),
+                          stream=stream().cuda_stream)
+    torch.cuda.synchronize()
+    ode.CloseOdevec()
+    yd, sd, dd, cd = host(dev)
+    ym, sm, dm, cm = plain[0]
+    yf, sf, df, cf = fb[0]
+    prom = cm[5] >= limit
+    say("handed back", int(prom.sum()), "of", n)
+    assert 10 <= prom.sum() <= n // 2
+    assert np.array_equal(yf[:, prom], yd[:, prom]) and np.array_equal(sf[prom], sd[prom])
+    assert np.array_equal(df[prom], dd[prom]) and np.array_equal(cf[:, prom], cd[:, prom])
+    keep = ~prom
+    assert np.array_equal(yf[:, keep], ym[:, keep]) and np.array_equal(sf[keep], sm[keep])
+    assert np.array_equal(df[keep], dm[keep]) and np.array_equal(cf[:, keep], cm[:, keep])
+    assert all(np.array_equal(x, y) for x, y in zip(fb[1], plain[1]))
