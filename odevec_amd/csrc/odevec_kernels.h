@@ -508,15 +508,22 @@ __global__ void k_cost_zero(unsigned* count) {
   if (threadIdx.x < COST_BINS) count[threadIdx.x] = 0u;
 }
 
+// The bin counts and the scatter's positions are gathered per workgroup in
+// LDS first, so that each workgroup adds to a global counter once per bin it
+// holds instead of once per cell (one global atomic per cell on 256 counters
+// serialised the two kernels: 6.5 ms each for 1M cells).
 template <class Net>
 __global__ void k_cost_bin(KArgs a, unsigned char* bin, unsigned* count) {
+  __shared__ unsigned h[COST_BINS];
+  for (int t = threadIdx.x; t < COST_BINS; t += blockDim.x) h[t] = 0u;
+  __syncthreads();
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.ncells) return;
   constexpr int NF = Net::NEQ, N = Net::NACT;
   constexpr int NP = Net::NPARAM > 0 ? Net::NPARAM : 1;
-  const double t0 = a.t0[i * a.t0_s], ts = a.tstop[i * a.ts_s];
+  const bool in = i < a.ncells;
+  const double t0 = in ? a.t0[i * a.t0_s] : 0.0, ts = in ? a.tstop[i * a.ts_s] : 0.0;
   int b = COST_BINS - 1;   // skipped cells last
-  if (!(a.mask && !a.mask[i]) && t0 < ts) {
+  if (in && !(a.mask && !a.mask[i]) && t0 < ts) {
     double y[NF], prm[NP], k[Net::NRATE], ya[N], f[N];
 #pragma unroll
     for (int j = 0; j < NF; ++j) y[j] = a.y[(long long)j * a.ncells + i];
@@ -539,8 +546,13 @@ __global__ void k_cost_bin(KArgs a, unsigned char* bin, unsigned* count) {
     const int qb = (q == q) ? (int)fmin(fmax(q * 4.0, 0.0), (double)(COST_BINS - 2)) : 0;
     b = COST_BINS - 2 - qb;
   }
-  bin[i] = (unsigned char)b;
-  atomicAdd(&count[b], 1u);
+  if (in) {
+    bin[i] = (unsigned char)b;
+    atomicAdd(&h[b], 1u);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < COST_BINS; t += blockDim.x)
+    if (h[t]) atomicAdd(&count[t], h[t]);
 }
 
 // exclusive prefix sum of the bin counts (one workgroup of COST_BINS lanes)
@@ -559,12 +571,22 @@ __global__ void __launch_bounds__(COST_BINS) k_cost_scan(unsigned* count) {
   count[t] = s[t] - count[t];
 }
 
+// each workgroup reserves one range per bin (its cells' count) and places
+// its cells in it; offset[] starts as the exclusive scan of the bin counts
 template <int UNUSED = 0>
 __global__ void k_cost_scatter(long long ncells, const unsigned char* bin, unsigned* offset, int* order) {
+  __shared__ unsigned h[COST_BINS], base[COST_BINS];
+  for (int t = threadIdx.x; t < COST_BINS; t += blockDim.x) h[t] = 0u;
+  __syncthreads();
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ncells) return;
-  const unsigned p = atomicAdd(&offset[bin[i]], 1u);
-  order[p] = (int)i;
+  const bool in = i < ncells;
+  const int b = in ? (int)bin[i] : 0;
+  const unsigned local = in ? atomicAdd(&h[b], 1u) : 0u;
+  __syncthreads();
+  for (int t = threadIdx.x; t < COST_BINS; t += blockDim.x)
+    if (h[t]) base[t] = atomicAdd(&offset[t], h[t]);
+  __syncthreads();
+  if (in) order[base[b] + local] = (int)i;
 }
 
 // ---------------------------------------------------------------- batches
