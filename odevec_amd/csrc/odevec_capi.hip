@@ -789,6 +789,10 @@ struct odevec_batch_s {
   unsigned* d_promo_n = nullptr;   // MAXSEG counts
   double* d_ws2 = nullptr;         // workspace of the FP64 passes
   int64_t ws2_cap = 0;
+  int* d_order = nullptr;          // per-lane segments larger than the grid: queue in cost order
+  unsigned char* d_bin = nullptr;
+  int64_t order_cap = 0;
+  unsigned* d_count = nullptr;     // MAXSEG x COST_BINS
 };
 
 namespace {
@@ -1002,6 +1006,46 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     }
     d.kind = kind[k];
   }
+  // per-lane segments with more cells than the grid has lanes: the queue in
+  // cost order (costliest first, as odevec_opts_t.schedule = 1 does for a
+  // handle); results do not depend on it
+  {
+    int64_t ototal = 0;
+    std::vector<const odv::NetEntry*> onet(nseg, nullptr);
+    for (int k = 0; k < nseg; ++k) {
+      const odv::NetEntry* ne = find_net(segs[k].network);
+      if (ne && ne->lane_ok && segs[k].ncells > p.grid * p.block && !segs[k].hist) {
+        onet[k] = ne;
+        ototal += segs[k].ncells;
+      }
+    }
+    if (ototal > 0) {
+      if (ototal > b->order_cap) {
+        if (b->d_order) (void)hipFree(b->d_order);
+        if (b->d_bin) (void)hipFree(b->d_bin);
+        b->d_order = nullptr;
+        b->d_bin = nullptr;
+        b->order_cap = 0;
+        if ((he = hipMalloc((void**)&b->d_order, (size_t)ototal * sizeof(int))) != hipSuccess ||
+            (he = hipMalloc((void**)&b->d_bin, (size_t)ototal)) != hipSuccess)
+          return fail(ODEVEC_ENOMEM, std::string("schedule allocation failed: ") + hipGetErrorString(he));
+        b->order_cap = ototal;
+      }
+      if (!b->d_count &&
+          (he = hipMalloc((void**)&b->d_count, (size_t)odv::MAXSEG * odv::COST_BINS * sizeof(unsigned))) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("schedule allocation failed: ") + hipGetErrorString(he));
+      int64_t off = 0;
+      for (int k = 0; k < nseg; ++k) {
+        if (!onet[k]) continue;
+        odv::KArgs& a = b->h_segs[k].a;
+        if ((he = onet[k]->order(a, b->d_bin + off, b->d_count + (size_t)k * odv::COST_BINS, b->d_order + off, st)) !=
+            hipSuccess)
+          return hip_fail(he, "cost order");
+        a.order = b->d_order + off;
+        off += segs[k].ncells;
+      }
+    }
+  }
   *b->h_nseg = nseg;
   if ((he = hipMemcpyAsync(b->d_segs, b->h_segs, sizeof(odv::SegDev) * nseg, hipMemcpyHostToDevice, st)) != hipSuccess ||
       (he = hipMemcpyAsync(b->d_nseg, b->h_nseg, sizeof(int), hipMemcpyHostToDevice, st)) != hipSuccess ||
@@ -1065,7 +1109,8 @@ int32_t odevec_batch_destroy(odevec_batch_t* b) {
   batch_drop_graph(b);
   if (b->h_segs) (void)hipHostFree(b->h_segs);
   if (b->h_nseg) (void)hipHostFree(b->h_nseg);
-  void* ps[] = {b->d_segs, b->d_nseg, b->d_queues, b->d_scal, b->d_ws, b->d_promo, b->d_promo_n, b->d_ws2};
+  void* ps[] = {b->d_segs, b->d_nseg, b->d_queues, b->d_scal, b->d_ws, b->d_promo, b->d_promo_n, b->d_ws2,
+                b->d_order, b->d_bin, b->d_count};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (b->ev_copied) (void)hipEventDestroy(b->ev_copied);
