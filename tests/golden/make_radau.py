@@ -5,7 +5,11 @@ independent of the BDF code under test), rtol 1e-12.
 The right-hand sides and analytic Jacobians are evaluated from the network
 term tables (pinned to the reference's network definitions by
 reference_rhs.json); PRIMORDIAL's rate coefficients come from the oracle
-(pinned to the reference's rate text by reference_rates.json).
+(pinned to the reference's rate text by reference_rates.json). SYNTH64 has
+no reference definition: its right-hand side and Jacobian are the mass-action
+law of the generator's reaction list (A + B -> C + D, k = A exp(-Ea/T)),
+evaluated here directly from that list, without the preprocessor's term
+table or the oracle.
 """
 import json
 import os
@@ -68,6 +72,49 @@ def solve(net_id, w, idx, t_stop=None, rtol=1e-12, atol=1e-30):
     return out
 
 
+def synth_fun(reactions, A, Ea, T):
+    """Mass-action f and J of two-body reactions (a, b) -> (c, d) with
+    k_r = A_r exp(-Ea_r / T), straight from the reaction list."""
+    ra = np.array([r[0][0] for r in reactions])
+    rb = np.array([r[0][1] for r in reactions])
+    pc = np.array([r[1][0] for r in reactions])
+    pd = np.array([r[1][1] for r in reactions])
+    k = np.asarray(A) * np.exp(-np.asarray(Ea) / T)
+
+    def f(t, y):
+        v = k * y[ra] * y[rb]
+        out = np.zeros(y.size)
+        for idx, sign in ((ra, -1.0), (rb, -1.0), (pc, 1.0), (pd, 1.0)):
+            np.add.at(out, idx, sign * v)
+        return out
+
+    def jac(t, y):
+        J = np.zeros((y.size, y.size))
+        da, db = k * y[rb], k * y[ra]     # d(rate)/dy_a, d(rate)/dy_b
+        for idx, sign in ((ra, -1.0), (rb, -1.0), (pc, 1.0), (pd, 1.0)):
+            np.add.at(J, (idx, ra), sign * da)
+            np.add.at(J, (idx, rb), sign * db)
+        return J
+
+    return f, jac
+
+
+def solve_synth(w, idx, rtol=1e-12, atol=1e-30):
+    from odevec_amd.pre_odevec.networks import synth_reactions
+    reactions, A, Ea = synth_reactions(64, 300)
+    out = []
+    for i in idx:
+        y0 = w["y"][:, i].copy()
+        T = float(w["params"][0, i])
+        f, jac = synth_fun(reactions, A, Ea, T)
+        ts = float(w["t_stop"][i])
+        r = solve_ivp(f, (0.0, ts), y0, method="Radau", rtol=rtol, atol=atol, jac=jac)
+        assert r.success, r.message
+        out.append({"cell": int(i), "y0": y0.tolist(), "t_stop": ts, "params": [T], "y": r.y[:, -1].tolist(),
+                    "nfev": int(r.nfev)})
+    return out
+
+
 def main():
     res = {"method": "scipy.integrate.solve_ivp Radau", "rtol": 1e-12, "cases": {}}
     res["cases"]["ROBER"] = {"network": "ROBER/NONE/DENSE", "atol": 1e-30,
@@ -76,6 +123,8 @@ def main():
                              "cells": solve("OREGO/NONE/DENSE", W.orego(32), range(4), t_stop=30.0)}
     res["cases"]["PRIMORDIAL"] = {"network": "PRIMORDIAL/CMK/CSC", "atol": 1e-40,
                                   "cells": solve("PRIMORDIAL/CMK/CSC", W.primordial(64), range(6), atol=1e-40)}
+    res["cases"]["SYNTH64"] = {"network": "SYNTH64/FF/CSC", "atol": 1e-30,
+                               "cells": solve_synth(W.synth64(64, seed=7), range(4))}
     with open(os.path.join(HERE, "radau_known_answers.json"), "w") as fh:
         json.dump(res, fh, indent=1)
     print("wrote radau_known_answers.json")
