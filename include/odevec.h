@@ -127,6 +127,25 @@ int32_t odevec_set_check_negatives(odevec_t* h, int32_t flag);
  * off. */
 int32_t odevec_set_fp64_fallback(odevec_t* h, int32_t convfail_limit);
 
+/* Invariant test of the mixed form's corrector (no reference counterpart):
+ * in the mixed form a corrector iteration that passes the convergence test
+ * but leaves one of the network's linear invariants c (element and charge
+ * conservation, sum_j c_j f_j = 0) with a residual |c^T (y_NS(:,1) + en)|
+ * above `tolerances` x sqrt(sum_j c_j^2 (rtol |y_j| + atol)^2) counts as a
+ * convergence failure (a quarter step and fresh factors, the reference's
+ * path at src/odevec.F90:375-393). Exact factors of P = I - dt*beta*J keep
+ * the invariants at every corrector step (c^T P = c^T); FP32 factors of a
+ * very stiff P do not, and without the test the cell's invariants drift.
+ * With the FP64 fallback on (odevec_set_fp64_fallback), a cell whose test
+ * fails `fallback_fails` times is handed to the FP64 pass like one that
+ * reaches the convergence-failure limit (0: never; the two counts are
+ * separate). Defaults ODEVEC_MIXED_INVARIANT_TOL and
+ * ODEVEC_FALLBACK_INVARIANT_FAILS; tolerances 0 turns the test off. The FP64
+ * form and networks without linear invariants (OREGO) are unaffected. */
+#define ODEVEC_MIXED_INVARIANT_TOL 3.0
+#define ODEVEC_FALLBACK_INVARIANT_FAILS 32
+int32_t odevec_set_invariant_test(odevec_t* h, double tolerances, int32_t fallback_fails);
+
 /* Replaces SolveODE(this,time,dt,t_stop,y,GetRHS,GetJac,GetLU,Mask)
  * (src/odevec.F90:158-214) for the handle's nvector cells: integrate y from
  * *time to t_stop; *dt is the initial-step hint on entry and the last step
@@ -262,6 +281,8 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
  * persistent kernel solves such cells on the same stream (one launch per
  * segment, outside the batch's captured graph). 0 (default): off. */
 int32_t odevec_batch_set_fp64_fallback(odevec_batch_t* b, int32_t convfail_limit);
+/* The invariant test of the batch's mixed segments (see odevec_set_invariant_test). */
+int32_t odevec_batch_set_invariant_test(odevec_batch_t* b, double tolerances, int32_t fallback_fails);
 int32_t odevec_batch_kernel_info(const char* bundle, int32_t block, odevec_kernel_info_t* info);
 int32_t odevec_batch_destroy(odevec_batch_t* b);
 

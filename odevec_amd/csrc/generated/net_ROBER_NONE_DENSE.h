@@ -11,6 +11,10 @@ struct odv_net_ROBER_NONE_DENSE {
   static constexpr int NACT = 3;  // species integrated; the others stay constant
   // ACT[a] = species index of integrated species a
   static constexpr int ACT[3] = {0, 1, 2};
+  // linear invariants over the integrated species: sum_a INV[k][a] f_a(y) = 0 for all y
+  static constexpr int NINV = 1;
+  static constexpr signed char INV[1][3] = {{1, 1, 1}};
+  ODV_HD static constexpr double inv(int k, int a) { return INV[k][a]; }
   static constexpr int NNZ = 8;
   static constexpr int NRATE = 3;
   static constexpr int NPARAM = 3;
@@ -51,6 +55,7 @@ struct odv_net_ROBER_NONE_DENSE {
   template <int S = 1>
   ODV_HD static inline void rhs(const double* __restrict__ y, const double* __restrict__ k,
                                double* __restrict__ f) {
+#pragma clang fp contract(off)
     (void)k;
     {
       const double F = k[0 * S] * y[0];

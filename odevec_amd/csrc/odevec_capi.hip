@@ -101,6 +101,8 @@ struct odevec_s {
   unsigned* d_count = nullptr;       // COST_BINS counters / offsets
   int64_t order_cap = 0;
   int promote_cf = 0;                // mixed form: FP64 fallback after this many convergence failures (0: off)
+  double inv_tol = ODEVEC_MIXED_INVARIANT_TOL;   // mixed form: invariant test of the corrector (0: off)
+  int promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;   // fallback after this many failed tests (0: never)
   int* d_promo = nullptr;            // cells handed to the FP64 pass
   unsigned* d_promo_n = nullptr;     // their count
   int64_t promo_cap = 0;
@@ -194,6 +196,7 @@ int launch(odevec_s* h, const Req& r) {
   a.ctl.dt_min = h->dt_min;
   a.ctl.check_negatives = h->check_neg;
   a.ctl.max_attempts = h->opts.max_attempts > 0 ? h->opts.max_attempts : 1000000;
+  a.ctl.inv_tol = h->inv_tol;
   int grp = h->opts.group;
   if (grp <= 0) grp = (int)(h->nvector < 256 ? h->nvector : 256);
   if (grp < 1) grp = 1;
@@ -256,6 +259,7 @@ int launch(odevec_s* h, const Req& r) {
   }
   if (promote) {
     a.ctl.promote_cf = h->promote_cf;
+    a.ctl.promote_inv = h->promote_inv;
     a.promo = h->d_promo;
     a.promo_n = h->d_promo_n;
     hipLaunchKernelGGL(odv::k_setu, dim3(1), dim3(1), 0, r.stream, h->d_promo_n, 0u);
@@ -479,6 +483,15 @@ int32_t odevec_set_fp64_fallback(odevec_t* h, int32_t convfail_limit) {
   if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
   if (convfail_limit < 0) return fail(ODEVEC_EINVAL, "convfail_limit must be >= 0");
   h->promote_cf = convfail_limit;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_set_invariant_test(odevec_t* h, double tolerances, int32_t fallback_fails) {
+  if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
+  if (!(tolerances >= 0.0)) return fail(ODEVEC_EINVAL, "tolerances must be >= 0");
+  if (fallback_fails < 0) return fail(ODEVEC_EINVAL, "fallback_fails must be >= 0");
+  h->inv_tol = tolerances;
+  h->promote_inv = fallback_fails;
   return ODEVEC_OK;
 }
 
@@ -784,6 +797,8 @@ struct odevec_batch_s {
   hipStream_t gstream = nullptr;
   int ghist = 0;                   // the captured launch is the history variant
   int promote_cf = 0;              // FP64 fallback of mixed segments (0: off)
+  double inv_tol = ODEVEC_MIXED_INVARIANT_TOL;   // mixed segments: invariant test of the corrector (0: off)
+  int promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;   // fallback after this many failed tests (0: never)
   int* d_promo = nullptr;          // handed-back cells of every mixed segment, one region per segment
   int64_t promo_cap = 0;
   unsigned* d_promo_n = nullptr;   // MAXSEG counts
@@ -997,10 +1012,12 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     a.ctl.dt_min = g.dt_min;
     a.ctl.check_negatives = g.check_negatives;
     a.ctl.max_attempts = g.max_attempts > 0 ? g.max_attempts : 1000000;
+    a.ctl.inv_tol = b->inv_tol;
     a.group = 1;
     a.hist = g.hist;
     if (fbnet[k]) {
       a.ctl.promote_cf = b->promote_cf;
+      a.ctl.promote_inv = b->promote_inv;
       a.promo = b->d_promo + fboff[k];
       a.promo_n = b->d_promo_n + k;
     }
@@ -1087,6 +1104,15 @@ int32_t odevec_batch_set_fp64_fallback(odevec_batch_t* b, int32_t convfail_limit
   if (!b) return fail(ODEVEC_EINVAL, "batch is NULL");
   if (convfail_limit < 0) return fail(ODEVEC_EINVAL, "convfail_limit must be >= 0");
   b->promote_cf = convfail_limit;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_batch_set_invariant_test(odevec_batch_t* b, double tolerances, int32_t fallback_fails) {
+  if (!b) return fail(ODEVEC_EINVAL, "batch is NULL");
+  if (!(tolerances >= 0.0)) return fail(ODEVEC_EINVAL, "tolerances must be >= 0");
+  if (fallback_fails < 0) return fail(ODEVEC_EINVAL, "fallback_fails must be >= 0");
+  b->inv_tol = tolerances;
+  b->promote_inv = fallback_fails;
   return ODEVEC_OK;
 }
 

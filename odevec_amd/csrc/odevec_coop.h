@@ -90,6 +90,7 @@ struct CoopNet {
   static constexpr int ACT[1] = {0};
   static constexpr int NNZ = NB;            // a lane's row of P / the factors
   static constexpr bool FD_JAC = false;     // the symbolic Jacobian (tables below)
+  static constexpr int NINV = Net_::NINV;   // the cell's linear invariants (mixed form's convergence test)
   static constexpr int NRATE = 1;           // the rates live in the group's LDS block
   static constexpr int NPARAM = Net_::NPARAM;
   static constexpr int RATE_KIND = Net_::RATE_KIND;
@@ -101,6 +102,12 @@ struct CoopNet {
 
   __device__ static double* blk() { return odv_coop_lds + coop_grp<G>() * Y::WORDS; }
 
+  // coefficient of this lane's species in invariant k (0 on padding lanes)
+  __device__ static double inv(int k, int) {
+    const int l = coop_l<G>();
+    return l < NB ? (double)Base::INV[k][l] : 0.0;
+  }
+
   // once per cell: lane 0 evaluates the rates into the LDS block
   __device__ static void rates(const double* y0, const double* prm, double* k) {
     k[0] = 0.0;
@@ -111,6 +118,9 @@ struct CoopNet {
   // publish the lanes' species values, then each lane gathers its species' terms
   template <int S = 1>
   __device__ __forceinline__ static void rhs(const double* y, const double* k, double* f) {
+    // each term rounded before it is added, as in the per-lane RHS, so the
+    // rows' sums see the same rounded fluxes
+#pragma clang fp contract(off)
     (void)k;
     double* b = blk();
     const int l = coop_l<G>();
@@ -259,6 +269,14 @@ struct CoopGroup {
   }
   __device__ bool any_valid(bool b) const { return any_of(b); }
   __device__ bool any_active(bool b) const { return any_of(b); }
+  // the largest of the lanes' values
+  __device__ double species_max(double v) const {
+    const int base = coop_base<G>();
+    double m = coop_bcast<G>(v, base, 0);
+#pragma unroll
+    for (int j = 1; j < NB; ++j) m = fmax(m, coop_bcast<G>(v, base, j));
+    return m;
+  }
   // the lanes' partial sums added in species order: s = v_0; s = s + v_1; ...
   __device__ double species_sum(double v) const {
     const int base = coop_base<G>();

@@ -50,6 +50,13 @@ struct Controls {
   int check_negatives;
   int max_attempts;
   int promote_cf;      // > 0: stop a cell at this many convergence failures (ST_PROMOTE)
+  // mixed form: a corrector that passes the convergence test but leaves a
+  // linear invariant's residual above inv_tol tolerances has not converged
+  // (0: no such test); see odv_invariants_broken
+  double inv_tol;
+  // with promote_cf > 0: also stop a cell (ST_PROMOTE) at this many failed
+  // invariant tests (0: never)
+  int promote_inv;
 };
 
 // Per-cell event counters (feed the flop model and diagnostics).
@@ -62,6 +69,7 @@ struct Counters {
   int convfail;    // convergence failures
   int stepsel;     // step-size/order selections
   int qsum;        // sum of q over attempts
+  int invfail;     // of the convergence failures, those of the invariant test (mixed form; not stored)
 };
 
 // Build-time configuration of one lane.
@@ -122,6 +130,7 @@ struct LaneGroup {
   ODV_HD bool any_valid(bool b) const { return b; }
   ODV_HD bool any_active(bool b) const { return b; }
   ODV_HD double species_sum(double v) const { return v; }
+  ODV_HD double species_max(double v) const { return v; }
 };
 
 // Which species a lane integrates. For the generated networks a lane holds
@@ -278,6 +287,66 @@ ODV_HD inline double odv_wnorm_f(const L& s, const G& g, X x) {
 template <class L, class G>
 ODV_HD inline double odv_wnorm(const L& s, const G& g, const double* x) {
   return odv_wnorm_f(s, g, [&](int j) { return x[j]; });
+}
+
+// max_j |x_j| sqrt(w_j): the largest weighted component (NaN if one is, or
+// if a frozen species' weight is not finite, as in the RMS norm)
+template <class L, class G>
+ODV_HD inline double odv_wmax(const L& s, const G& g, const double* x) {
+  constexpr int N = L::N;
+  double m = (x[0] * x[0]) * s.w(0);
+  bool nan = m != m;
+#pragma unroll
+  for (int j = 1; j < N; ++j) {
+    const double v = (x[j] * x[j]) * s.w(j);
+    nan = nan || (v != v);
+    m = fmax(m, v);
+  }
+  if (nan || (s.fz & 1)) m = NAN;
+  return sqrt(g.species_max(m));
+}
+
+// Mixed form: whether the converged correction leaves one of the network's
+// linear invariants c (sum_j c_j f_j = 0, element and charge conservation)
+// with a residual above `tol` tolerances. With exact factors of P = I -
+// dt*beta*J every corrector step restores sum_j c_j (y_NS(1,j) + en_j) = 0
+// (c^T P = c^T since c^T J = 0, and the residual's invariant part is
+// -c^T (y_NS(1) + en)); FP32 factors of a very stiff P (dt*beta*|J| far
+// above 1/eps_32) lose that structure: P~^-1 maps the invariant part of the
+// residual to almost nothing, further iterations cannot reduce it, the
+// convergence test passes and the invariant drifts (deuterium nuclei up to
+// 10% on the 1M-cell PRIMORDIAL workload, against 1.1e-4 with FP64 factors).
+// Such a step counts as a convergence failure (the reference's path: a
+// quarter step and fresh factors, src/odevec.F90:375-393), which brings
+// dt*beta*|J| back to where FP32 factors keep the invariants. The test
+// compares |c^T (y_NS(1) + en)| with tol * sqrt(sum_j c_j^2 (rtol |y_j| +
+// atol)^2), the size of the smallest correction that would restore the
+// invariant, in units of the error weights. Net::inv(k, j) is the
+// coefficient of the lane's j-th species in invariant k; the group adds the
+// species' parts (the identity when a lane holds its whole cell).
+template <class L, class G>
+ODV_HD inline bool odv_invariants_broken(const L& s, const G& g, const double* y, double rtol, double atol,
+                                         double tol) {
+  using Net = typename L::Net;
+  constexpr int N = L::N;
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < Net::NINV; ++k) {
+    double b = 0.0, sc = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const double c = Net::inv(k, j);
+      if (c != 0.0) {
+        const double t = rtol * fabs(y[j]) + atol;
+        b = b + c * (s.z(1, j) + s.en[j]);
+        sc = sc + (c * c) * (t * t);
+      }
+    }
+    b = g.species_sum(b);
+    sc = g.species_sum(sc);
+    bad = bad || (b * b > (tol * tol) * sc);
+  }
+  return bad;
 }
 
 // one of the lane's weights is NaN
@@ -800,11 +869,33 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       rtmp = e / s.cerr;
       s.rate = fmax(0.2 * s.rate, rtmp);
     }
-    const double crit = e * fmin(1.0, 1.5 * s.rate) / s.crit_den;
+    // CheckConvergence's test estimates the remaining error as the last
+    // correction's RMS norm times min(1, 1.5*rate), the contraction seen in
+    // that norm. FP32 factors contract the slow directions of a stiff P
+    // (trace species, invariants) much more slowly than the norm's large
+    // components show, so the mixed form asks every weighted component of
+    // the last correction to be within the divisor, with no rate discount
+    // (measured on PRIMORDIAL cells: the 99th percentile of the distance to
+    // an rtol-1e-12 solution 9.2e3 -> 5.7e3 tolerances, below the FP64
+    // form's 8.0e3, with 8% fewer step attempts)
+    double crit;
+    if constexpr (MIXED) crit = g.max_active(odv_wmax(s, g, f), s.active) / s.crit_den;
+    else crit = e * fmin(1.0, 1.5 * s.rate) / s.crit_den;
     if (g.any_valid(odv_has_nan<N>(f) || odv_w_nan(s) || (s.fz & 2))) {
       failed = true;
     } else if (crit < 1.0) {
       converged = true;
+      if constexpr (MIXED && L::Net::NINV > 0) {
+        if (c.inv_tol > 0.0) {
+          double y[N];
+          odv_cur_y(s, beta, y);
+          if (g.any_valid(odv_invariants_broken(s, g, y, c.rtol, c.atol, c.inv_tol) && s.active)) {
+            converged = false;
+            failed = true;
+            s.cnt.invfail++;
+          }
+        }
+      }
     } else if (it >= 2 && s.rate > 2.0 * rtmp) {
       failed = true;
     } else if (it == 3) {
@@ -829,7 +920,10 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       s.t = told;
       s.fc++;
       s.cnt.convfail++;
-      if (c.promote_cf > 0 && s.cnt.convfail >= c.promote_cf) {
+      // the FP64 fallback counts the failures of the iteration itself and
+      // the failed invariant tests separately
+      if (c.promote_cf > 0 && (s.cnt.convfail - s.cnt.invfail >= c.promote_cf ||
+                               (c.promote_inv > 0 && s.cnt.invfail >= c.promote_inv))) {
         s.status = ST_PROMOTE;
         return true;
       }

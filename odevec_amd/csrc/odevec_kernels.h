@@ -686,6 +686,7 @@ struct BlockGroup {
   __device__ bool any_valid(bool b) const { return any_of(b && valid); }
   __device__ bool any_active(bool b) const { return any_of(b); }
   __device__ double species_sum(double v) const { return v; }
+  __device__ double species_max(double v) const { return v; }
 };
 
 // Lockstep form. Every decision is taken from group-reduced values, so all

@@ -56,6 +56,11 @@ module odevec_main
     ! mixed form (ODEVEC_PRECISION=1): cells reaching this many convergence
     ! failures are solved again in FP64 (0: off); no reference counterpart
     integer :: FP64Fallback = 0
+    ! mixed form: a corrector that leaves an element or charge invariant off
+    ! by more than this many tolerances has not converged (0: off)
+    real(dp) :: InvariantTolerance = 3.0_dp
+    ! ... and with FP64Fallback on, cells failing it this many times go to FP64 (0: never)
+    integer :: InvariantFallbackFails = 32
     real(dp), allocatable :: y(:,:)
     integer(c_int32_t), allocatable :: status(:)
     type(c_ptr) :: handle = c_null_ptr
@@ -96,6 +101,13 @@ module odevec_main
       import :: c_ptr, c_int32_t
       type(c_ptr), value :: h
       integer(c_int32_t), value :: limit
+      integer(c_int32_t) :: rc
+    end function
+    function c_set_invariant_test(h, tol, fails) bind(C, name='odevec_set_invariant_test') result(rc)
+      import :: c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: tol
+      integer(c_int32_t), value :: fails
       integer(c_int32_t) :: rc
     end function
     function c_solve(h, time, dt, t_stop, y, mask, status) bind(C, name='odevec_solve') result(rc)
@@ -182,6 +194,8 @@ contains
     if (rc == 0) rc = c_set_dt_min(this%handle, this%MinimalTimestep)
     if (rc == 0) rc = c_set_check_negatives(this%handle, merge(1_c_int32_t, 0_c_int32_t, this%CheckNegatives))
     if (rc == 0) rc = c_set_fp64_fallback(this%handle, int(this%FP64Fallback, c_int32_t))
+    if (rc == 0) rc = c_set_invariant_test(this%handle, real(this%InvariantTolerance, c_double), &
+                                           int(this%InvariantFallbackFails, c_int32_t))
     if (rc /= 0) call fail(rc, "SolveODE (settings)")
     mp = c_null_ptr
     if (present(Mask)) then

@@ -101,6 +101,60 @@ def p_pattern(neq: int, terms: List[Term]) -> np.ndarray:
     return P
 
 
+def _size(v: List[int]) -> Tuple[int, int]:
+    return sum(map(abs, v)), sum(1 for x in v if x)
+
+
+def linear_invariants(neq: int, terms: List[Term]) -> List[List[int]]:
+    """Integer basis of the network's linear invariants: the vectors c with
+    sum_j c_j f_j(y) = 0 for every y (element and charge conservation of a
+    mass-action network; PRIMORDIAL: H, He and D nuclei and charge; ROBER:
+    y1 + y2 + y3; SYNTH64: the molecule count).
+
+    f is a sum of monomials (rate x product of species), so c is an invariant
+    exactly when it is orthogonal to every monomial's column of
+    stoichiometric coefficients: the left null space of the species x
+    monomial coefficient matrix, computed in exact rational arithmetic. The
+    basis is the reduced row echelon form scaled to integers, then
+    size-reduced (a vector is replaced by its sum or difference with another
+    while that lowers its L1 norm, then its count of nonzeros), so that each
+    vector involves few species (PRIMORDIAL: H, He and D nuclei, and charge
+    combined with helium, E + H- + He - H+ - H2+ - D+ - He++)."""
+    cols = {}
+    for t in terms:
+        col = cols.setdefault((t.rate, t.factors), {})
+        col[t.row] = col.get(t.row, 0) + sp.nsimplify(t.coef, rational=True)
+    M = sp.zeros(neq, max(len(cols), 1))
+    for c, rows in enumerate(cols.values()):
+        for r, v in rows.items():
+            M[r, c] = v
+    ns = M.T.nullspace()
+    if not ns:
+        return []
+    R = sp.Matrix.hstack(*ns).T.rref()[0]
+    basis = []
+    for i in range(R.rows):
+        row = list(R.row(i))
+        den = sp.ilcm(*[sp.fraction(v)[1] for v in row])
+        basis.append([int(v * den) for v in row])
+    changed = True
+    while changed:
+        changed = False
+        for a in range(len(basis)):
+            for b in range(len(basis)):
+                if a == b:
+                    continue
+                for sgn in (1, -1):
+                    cand = [x + sgn * y for x, y in zip(basis[a], basis[b])]
+                    if _size(cand) < _size(basis[a]):
+                        basis[a] = cand
+                        changed = True
+    for v in basis:   # leading nonzero positive
+        if next(x for x in v if x != 0) < 0:
+            v[:] = [-x for x in v]
+    return basis
+
+
 def _colsorted_entries(P: np.ndarray) -> List[Tuple[int, int]]:
     n = P.shape[0]
     return [(i, j) for j in range(n) for i in range(n) if P[i, j]]

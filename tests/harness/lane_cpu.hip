@@ -4,6 +4,8 @@
 // library; built by tests/conftest.py with --offload-host-only.
 #include <cstring>
 
+#include "../../include/odevec.h"
+
 #include "odevec_integrator.h"
 #include "generated/net_ROBER_NONE_DENSE.h"
 #include "generated/net_ROBER_NONE_CSC.h"
@@ -89,7 +91,8 @@ extern "C" int lane_solve_cells(const char* id, int mixed, long n, const double*
                                 const double* dt0, double* y, const double* prm, double rtol, double atol,
                                 double dt_min, int check_neg, int max_att, int* status, double* dtout,
                                 int* cnt) {
-  odv::Controls c{rtol, atol, dt_min, check_neg, max_att};
+  // the mixed form runs with the library's default invariant test
+  odv::Controls c{rtol, atol, dt_min, check_neg, max_att, 0, (mixed & 1) ? ODEVEC_MIXED_INVARIANT_TOL : 0.0};
 #define ODV_CASE(NET) \
   if (!std::strcmp(id, NET::ID)) { run_prec<NET>(mixed, n, t0, ts, dt0, y, prm, c, status, dtout, cnt); return 0; }
   ODV_CASE(odv_net_ROBER_NONE_DENSE)
