@@ -268,3 +268,117 @@ def test_failure_paths_on_cpu(oracle):
     y, st, _, _ = net.solve_cells(yneg, w["t_stop"], dt0=w["dt0"], rtol=1e-6, atol=1e-20, dt_min=1e300,
                                   check_negatives=True)
     assert (st == 1).all()
+
+
+# --------------------------------------------------------------- SYNTH64
+# SYNTH64 has no reference definition beyond its specification (64 species,
+# 300 seeded two-body reactions A + B -> C + D, k = A exp(-Ea/T) with
+# log10 A ~ U[-12,-8] and Ea/k ~ U[0, 5e4] K, FF ordering). These tests pin
+# the oracle's SYNTH64 tables (which come from the preprocessor's term
+# table) against that specification, evaluated here from the reaction list
+# itself, and against the textbook identities the other networks are pinned
+# by.
+
+def _synth_list():
+    from odevec_amd.pre_odevec.networks import synth_reactions
+    reactions, A, Ea = synth_reactions(64, 300)
+    ra = np.array([r[0][0] for r in reactions])
+    rb = np.array([r[0][1] for r in reactions])
+    pc = np.array([r[1][0] for r in reactions])
+    pd = np.array([r[1][1] for r in reactions])
+    return ra, rb, pc, pd, np.array(A), np.array(Ea)
+
+
+def _synth_rhs(y, k):
+    ra, rb, pc, pd, _, _ = _synth_list()
+    v = k * y[ra] * y[rb]
+    f = np.zeros(64)
+    for idx, sgn in ((ra, -1.0), (rb, -1.0), (pc, 1.0), (pd, 1.0)):
+        np.add.at(f, idx, sgn * v)
+    return f
+
+
+def test_synth64_generator_meets_its_specification():
+    ra, rb, pc, pd, A, Ea = _synth_list()
+    assert len(ra) == 300
+    for r in zip(ra, rb, pc, pd):
+        assert len(set(int(x) for x in r)) == 4 and all(0 <= x < 64 for x in r)
+    assert np.all((np.log10(A) >= -12) & (np.log10(A) <= -8)) and np.all((Ea >= 0) & (Ea <= 5e4))
+    # spread over the stated ranges (not a degenerate draw)
+    assert np.ptp(np.log10(A)) > 3.9 and np.ptp(Ea) > 4.9e4
+
+
+def test_synth64_rates_are_arrhenius(oracle):
+    net = _net(oracle, "SYNTH64/FF/CSC")
+    _, _, _, _, A, Ea = _synth_list()
+    for T in (100.0, 1234.5, 1e4):
+        k = net.rates(np.ones(64), np.array([T]))
+        assert np.allclose(k, A * np.exp(-Ea / T), rtol=1e-13, atol=0.0), T
+
+
+def test_synth64_rhs_is_mass_action_of_its_reactions(oracle):
+    net = _net(oracle, "SYNTH64/FF/CSC")
+    rng = np.random.default_rng(64)
+    for _ in range(10):
+        y = np.exp(rng.uniform(-10, 5, 64))
+        k = np.exp(rng.uniform(-20, 0, 300))
+        f = net.rhs(y, k)
+        ref = _synth_rhs(y, k)
+        scale = np.abs(k * y[_synth_list()[0]] * y[_synth_list()[1]]).max()
+        assert np.allclose(f, ref, rtol=1e-12, atol=1e-14 * scale)
+        # every reaction conserves the molecule count
+        assert abs(f.sum()) <= 1e-13 * np.abs(f).sum() + 1e-300
+
+
+def test_synth64_jacobian_and_lu(oracle):
+    """Central-difference Jacobian against the oracle's P, and L*U = P on the
+    3,155-entry FF pattern, with random states and rates."""
+    net = _net(oracle, "SYNTH64/FF/CSC")
+    rng = np.random.default_rng(65)
+    perm = net.ir["perm"]
+    for _ in range(3):
+        y = np.exp(rng.uniform(-2, 2, 64))
+        k = np.exp(rng.uniform(-3, 0, 300))
+        hb = float(np.exp(rng.uniform(-3, 0)))
+        P = net.p_dense(y, k, hb)
+        J = np.zeros((64, 64))
+        for j in range(64):
+            h = 1e-6 * y[j]
+            yp, ym = y.copy(), y.copy()
+            yp[j] += h
+            ym[j] -= h
+            J[:, j] = (_synth_rhs(yp, k) - _synth_rhs(ym, k)) / (2 * h)
+        Pfd = (np.eye(64) - hb * J)[np.ix_(perm, perm)]
+        assert np.allclose(P, Pfd, rtol=1e-6, atol=1e-8)
+        lu = net.factor(y, k, hb)
+        Lm, Um = _unpack(net, lu)
+        assert np.allclose(Lm @ Um, P, rtol=1e-11, atol=1e-11 * np.abs(P).max())
+        res = rng.standard_normal(64)
+        den = net.lin_solve(lu, res)
+        assert np.allclose(P @ den[perm], res[perm], rtol=1e-9, atol=1e-9)
+
+
+def test_synth64_ff_ordering_and_fill_from_the_reaction_list(oracle):
+    """The FF permutation is NumPy's default argsort of nnz(column) + nnz(row)
+    - 1 of P's pattern, as the reference calls it (pre_odevec.py:343-350; not
+    a stable sort, so ties keep NumPy's order), and the LU pattern is the
+    structural fill of pivot-free elimination of the permuted pattern; both
+    recomputed here from the reaction list (the Jacobian pattern: row r reads
+    species s when s is a reactant of a reaction that changes r)."""
+    net = _net(oracle, "SYNTH64/FF/CSC")
+    ra, rb, pc, pd, _, _ = _synth_list()
+    Pat = np.eye(64, dtype=bool)
+    for a, b, c, d in zip(ra, rb, pc, pd):
+        for row in (a, b, c, d):
+            Pat[row, a] = Pat[row, b] = True
+    score = Pat.sum(axis=0) + Pat.sum(axis=1) - 1
+    perm = np.argsort(score)
+    assert list(perm) == list(net.ir["perm"])
+    assert np.all(np.diff(score[perm]) >= 0)
+    F = Pat[np.ix_(perm, perm)].copy()
+    for kk in range(64):
+        rows = np.where(F[kk + 1:, kk])[0] + kk + 1
+        cols = np.where(F[kk, kk + 1:])[0] + kk + 1
+        F[np.ix_(rows, cols)] = True
+    assert int(F.sum()) == net.ir["nnz"] == 3155
+    assert sorted(map(tuple, net.ir["lu_entries"])) == sorted(zip(*np.where(F.T)[::-1]))

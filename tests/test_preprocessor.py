@@ -146,3 +146,36 @@ def test_synth64_seeded():
     reac, A, Ea = a
     assert len(reac) == 300 and all(len(set(r + p)) == 4 for r, p, _ in reac)
     assert all(1e-12 <= v <= 1e-8 for v in A) and all(0 <= v <= 5e4 for v in Ea)
+
+
+def test_linear_invariants():
+    """The invariants the generator emits (the mixed form's corrector test):
+    ROBER's total, none for OREGO, SYNTH64's molecule count, and for
+    PRIMORDIAL a basis of the same space as H, He and D nuclei and charge;
+    each satisfies sum_j c_j f_j = 0 at random states and rates."""
+    def inv(name):
+        st = S.reduce_frozen(S.analyse(N.load(name), "None", "CSC"))
+        return st, S.linear_invariants(st.neq, st.terms)
+    assert inv("ROBER")[1] == [[1, 1, 1]]
+    assert inv("OREGO")[1] == []
+    assert inv("SYNTH64")[1] == [[1] * 64]
+    st, B = inv("PRIMORDIAL")
+    phys = np.array([[0, 1, 1, 0, 2, 0, 1, 1, 0, 2, 0, 0], [0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 1],
+                     [0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 0], [-1, -1, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2]])
+    B = np.array(B)
+    assert B.shape == (4, 12)
+    assert np.linalg.matrix_rank(np.vstack([B, phys])) == 4 == np.linalg.matrix_rank(B)
+    rng = np.random.default_rng(8)
+    for _ in range(5):
+        y = np.exp(rng.uniform(-5, 5, 12))
+        k = np.exp(rng.uniform(-5, 0, 38))
+        f = np.zeros(12)
+        terms = np.zeros(12)
+        for t in st.terms:
+            v = t.coef * (k[t.rate] if t.rate >= 0 else 1.0) * np.prod([y[i] for i in t.factors])
+            f[t.row] += v
+            terms[t.row] += abs(v)
+        for c in B:
+            assert abs(c @ f) <= 1e-12 * (np.abs(c) @ terms)
+    hdr = emit.emit_header(S.analyse(N.load("PRIMORDIAL"), "CMK", "CSC"))
+    assert "static constexpr int NINV = 4;" in hdr
