@@ -66,7 +66,7 @@ def parse():
 class Runner:
     """One configuration of the solver on device-resident inputs."""
 
-    def __init__(self, w, precision, mode, fallback=0, schedule=0):
+    def __init__(self, w, precision, mode, fallback=0, schedule=0, max_attempts=0):
         import torch
         from odevec_amd.api import Odevec
         self.torch = torch
@@ -80,7 +80,8 @@ class Runner:
         self.prm = torch.tensor(w["params"], dtype=torch.float64, device=dev) if w.get("params") is not None else None
         self.st = torch.zeros(n, dtype=torch.int32, device=dev)
         self.flops = torch.zeros(2, dtype=torch.float64, device=dev)
-        self.ode = Odevec(w["network"], n, precision=precision, mode=mode, schedule=schedule)
+        self.ode = Odevec(w["network"], n, precision=precision, mode=mode, schedule=schedule,
+                          max_attempts=max_attempts)
         self.ode.RelativeTolerance = w["rtol"]
         self.ode.AbsoluteTolerance = w["atol"]
         self.ode.MinimalTimestep = w["dt_min"]
@@ -125,6 +126,74 @@ class Runner:
         self.torch.cuda.synchronize()
         st = self.st.cpu().numpy()
         return self.flops.cpu().numpy(), st
+
+
+# linear invariants of PRIMORDIAL (H, He and D nuclei; species order of the state)
+NUCLEI = {"H": [0, 1, 1, 0, 2, 0, 1, 1, 0, 2, 0, 0, 0, 0, 0, 0],
+          "He": [0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0],
+          "D": [0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 0, 0, 0, 0, 0]}
+TIGHT_CELLS = 8192
+
+
+def cell_errors(y, yref, rtol, atol):
+    """Per cell: the largest over its integrated species of |y - y_ref| /
+    (rtol |y_ref| + atol), the reference's error-weight scale
+    (src/odevec.F90:706-716); 1 = one tolerance."""
+    e = np.abs(y[:12] - yref[:12]) / (rtol * np.abs(yref[:12]) + atol)
+    return e.max(axis=0)
+
+
+def error_summary(e):
+    return {"max": float(e.max()), "q999": float(np.quantile(e, 0.999)), "q99": float(np.quantile(e, 0.99)),
+            "q50": float(np.median(e)), "within_10": float((e <= 10.0).mean()), "cells": int(e.size)}
+
+
+def drift(y0, y):
+    out = {}
+    for name, v in NUCLEI.items():
+        v = np.array(v, dtype=float)
+        r = np.abs(v @ y0 - v @ y) / (v @ y0)
+        out[name] = {"max": float(r.max()), "q999": float(np.quantile(r, 0.999))}
+    return out
+
+
+def accuracy(w, results):
+    """Errors of each solve in `results` ({name: (y, status)}, full workload)
+    against (a) the FP64 reproducible build, which equals the oracle -- the
+    reference's algorithm -- bit for bit, over every cell, and (b) an FP64
+    run at rtol 1e-12, atol 1e-26 (the reproducible build again, attempt
+    budget 200,000) on TIGHT_CELLS seeded cells; errors in units of the
+    configuration's tolerance scale rtol |y_ref| + atol. Also the largest
+    drift of the H, He and D nuclei over the solve."""
+    import torch
+    n = w["y"].shape[1]
+    ref = Runner(dict(w, network=w["network"] + "+REPRO"), precision=0, mode=2, schedule=1)
+    ref.restore()
+    ref.solve()
+    torch.cuda.synchronize()
+    y_ref, s_ref = ref.y.cpu().numpy(), ref.st.cpu().numpy()
+    ref.ode.CloseOdevec()
+    idx = np.sort(np.random.default_rng(33).choice(n, min(TIGHT_CELLS, n), replace=False))
+    from odevec_amd import workloads as W
+    s = W.subset(w, idx)
+    tight = Runner(dict(s, network=w["network"] + "+REPRO", rtol=1e-12, atol=1e-26), precision=0, mode=2,
+                   schedule=1, max_attempts=200000)
+    tight.restore()
+    tight.solve()
+    torch.cuda.synchronize()
+    y_t, s_t = tight.y.cpu().numpy(), tight.st.cpu().numpy()
+    tight.ode.CloseOdevec()
+    out = {"reference": "FP64 +REPRO build (bit-identical to the oracle), every cell",
+           "tight": "FP64 +REPRO build at rtol 1e-12, atol 1e-26, %d seeded cells (%d finished)" % (
+               idx.size, int((s_t == 0).sum())),
+           "units": "largest over a cell's species of |y - y_ref| / (rtol |y_ref| + atol), rtol 1e-6, atol 1e-20"}
+    for name, (y, st) in list(results.items()) + [("fp64_reproducible", (y_ref, s_ref))]:
+        ok = (st == 0) & (s_ref == 0)
+        okt = (s_t == 0) & (st[idx] == 0)
+        out[name] = {"err_vs_reference": error_summary(cell_errors(y[:, ok], y_ref[:, ok], w["rtol"], w["atol"])),
+                     "err_vs_tight": error_summary(cell_errors(y[:, idx][:, okt], y_t[:, okt], w["rtol"], w["atol"])),
+                     "drift": drift(w["y"][:, st == 0], y[:, st == 0])}
+    return out
 
 
 def batching(steps, warmup):
@@ -369,7 +438,14 @@ def main():
     mixed = Runner(w, precision=1, mode=2, fallback=FALLBACK, schedule=SCHEDULE)
     fp64 = Runner(w, precision=0, mode=0)
     wall_m, ms_m = mixed.timed(a.steps, a.warmup)
+    y_m, st_mixed = mixed.y.cpu().numpy(), mixed.st.cpu().numpy()
     wall_d, ms_d = fp64.timed(a.steps, a.warmup)
+    y_d, st_fp64 = fp64.y.cpu().numpy(), fp64.st.cpu().numpy()
+    # the FP64 form in the persistent kernel with the same cost-ordered
+    # queue: the mixed form's gain from precision alone
+    fp64p = Runner(w, precision=0, mode=2, schedule=SCHEDULE)
+    wall_p, ms_p = fp64p.timed(a.steps, a.warmup)
+    fp64p.ode.CloseOdevec()
     # the mixed form without the FP64 fallback, for comparison (not the value)
     wall_n, ms_n = Runner(w, precision=1, mode=2).timed(min(a.steps, 2), 0)
     fl_m, st_m = mixed.count_flops()
@@ -407,9 +483,14 @@ def main():
                        "kernel": "one lane per cell, FP64, no compaction"},
         "mixed_over_fp64": value / value_d,
         "cells_ok": {"mixed": ok_m, "fp64": ok_d},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+        "fp64_persistent": {"value": a.steps * a.cells / wall_p, "unit": "cell-solves/s",
+                            "ms_per_step": wall_p / a.steps * 1e3,
+                            "kernel": "persistent, queue in cost order, FP64 (the mixed form's scheduling)"},
+        "mixed_over_fp64_persistent": wall_p / wall_m,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": frac, "traffic": read_traffic(),
-                     "note": "compute (FP64/FP32 VALU) bound; achieved = FP64 + FP32*peak64/peak32 algorithmic "
+                     "note": "FP64/FP32 vector-ALU bound (no MFMA: per-cell systems are tiny sparse LU "
+                             "solves; neither hbm nor mfma applies); achieved = FP64 + FP32*peak64/peak32 algorithmic "
                              "flops per solve (op-count model; the mixed kernel and the FP64 pass) / mean solve "
                              "time (HIP events around both launches); traffic: the mixed kernel's HBM bytes per "
                              "launch; no MFMA is used"},
@@ -418,6 +499,9 @@ def main():
         "mixed_without_fallback": {"value": min(a.steps, 2) * a.cells / wall_n, "unit": "cell-solves/s",
                                    "kernel_ms": ms_n},
     }
+    res["accuracy"] = accuracy(w, {"mixed": (y_m, st_mixed), "fp64_plain": (y_d, st_fp64)})
+    res["err_vs_fp64"] = res["accuracy"]["mixed"]["err_vs_reference"]
+    res["err_vs_tight"] = res["accuracy"]["mixed"]["err_vs_tight"]
     res["end_to_end"] = end_to_end(w, a.steps, float(np.mean(ms_m)))
     res["timeline"] = timeline(w)
     if not a.no_batch:

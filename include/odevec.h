@@ -138,11 +138,13 @@ int32_t odevec_set_fp64_fallback(odevec_t* h, int32_t convfail_limit);
  * very stiff P do not, and without the test the cell's invariants drift.
  * With the FP64 fallback on (odevec_set_fp64_fallback), a cell whose test
  * fails `fallback_fails` times is handed to the FP64 pass like one that
- * reaches the convergence-failure limit (0: never; the two counts are
- * separate). Defaults ODEVEC_MIXED_INVARIANT_TOL and
+ * reaches the convergence-failure limit (the two counts are separate);
+ * without a fallback the test is no longer applied to that cell after
+ * `fallback_fails` failures, which bounds the work of a cell whose FP32
+ * factors keep failing it (0: neither). Defaults ODEVEC_MIXED_INVARIANT_TOL and
  * ODEVEC_FALLBACK_INVARIANT_FAILS; tolerances 0 turns the test off. The FP64
  * form and networks without linear invariants (OREGO) are unaffected. */
-#define ODEVEC_MIXED_INVARIANT_TOL 3.0
+#define ODEVEC_MIXED_INVARIANT_TOL 1.0
 #define ODEVEC_FALLBACK_INVARIANT_FAILS 32
 int32_t odevec_set_invariant_test(odevec_t* h, double tolerances, int32_t fallback_fails);
 

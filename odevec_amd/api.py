@@ -124,7 +124,7 @@ class Odevec:
         it off (the default)."""
         L.check(L.lib().odevec_set_fp64_fallback(self._h, int(convfail_limit)))
 
-    def set_invariant_test(self, tolerances: float = 3.0, fallback_fails: int = 32):
+    def set_invariant_test(self, tolerances: float = 1.0, fallback_fails: int = 32):
         """Mixed form: a corrector that passes the convergence test but leaves
         an element or charge invariant's residual above `tolerances`
         error-weight units counts as a convergence failure (0 turns the test
@@ -299,7 +299,7 @@ class Batch:
         failures are solved again in FP64 after the batch launch (0: off)."""
         L.check(L.lib().odevec_batch_set_fp64_fallback(self._h, int(convfail_limit)))
 
-    def set_invariant_test(self, tolerances: float = 3.0, fallback_fails: int = 32):
+    def set_invariant_test(self, tolerances: float = 1.0, fallback_fails: int = 32):
         """The invariant test of mixed segments (see Odevec.set_invariant_test)."""
         L.check(L.lib().odevec_batch_set_invariant_test(self._h, float(tolerances), int(fallback_fails)))
 

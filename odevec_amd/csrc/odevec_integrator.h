@@ -54,8 +54,9 @@ struct Controls {
   // linear invariant's residual above inv_tol tolerances has not converged
   // (0: no such test); see odv_invariants_broken
   double inv_tol;
-  // with promote_cf > 0: also stop a cell (ST_PROMOTE) at this many failed
-  // invariant tests (0: never)
+  // failed invariant tests after which a cell is handed to the FP64 fallback
+  // (ST_PROMOTE, with promote_cf > 0) or, without a fallback, no longer
+  // tested (0: neither)
   int promote_inv;
 };
 
@@ -886,7 +887,7 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
     } else if (crit < 1.0) {
       converged = true;
       if constexpr (MIXED && L::Net::NINV > 0) {
-        if (c.inv_tol > 0.0) {
+        if (c.inv_tol > 0.0 && (c.promote_inv == 0 || s.cnt.invfail < c.promote_inv)) {
           double y[N];
           odv_cur_y(s, beta, y);
           if (g.any_valid(odv_invariants_broken(s, g, y, c.rtol, c.atol, c.inv_tol) && s.active)) {

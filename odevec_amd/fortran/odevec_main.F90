@@ -58,8 +58,9 @@ module odevec_main
     integer :: FP64Fallback = 0
     ! mixed form: a corrector that leaves an element or charge invariant off
     ! by more than this many tolerances has not converged (0: off)
-    real(dp) :: InvariantTolerance = 3.0_dp
-    ! ... and with FP64Fallback on, cells failing it this many times go to FP64 (0: never)
+    real(dp) :: InvariantTolerance = 1.0_dp
+    ! ... cells failing it this many times go to FP64 (FP64Fallback on) or are no
+    ! longer tested (0: neither)
     integer :: InvariantFallbackFails = 32
     real(dp), allocatable :: y(:,:)
     integer(c_int32_t), allocatable :: status(:)

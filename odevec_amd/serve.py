@@ -428,6 +428,7 @@ def run_sessions(server: Server, nsess: int = 8, ncells: int = 65536, nadv: int 
     sess = [server.open_session(network, w["y"], w.get("params"), w["rtol"], w["atol"], w["dt_min"], w["dt0"],
                                 precision=precision) if warm else None for w in blocks]
     state = [np.ascontiguousarray(w["y"]) for w in blocks]
+    cell_ok = [np.ones(ncells, dtype=bool) for _ in blocks]   # OK in every advance so far
     step = [0] * nsess
     live = {}
     lat = []
@@ -459,6 +460,7 @@ def run_sessions(server: Server, nsess: int = 8, ncells: int = 65536, nadv: int 
                 continue
             lat.append((r.done - r.arrival) * 1e3)
             ok += int((r.status == 0).sum())
+            cell_ok[k] &= r.status == 0
             state[k] = np.ascontiguousarray(r.result)
             step[k] += 1
             if step[k] < nadv:
@@ -473,7 +475,7 @@ def run_sessions(server: Server, nsess: int = 8, ncells: int = 65536, nadv: int 
     return {"mode": "warm" if warm else "cold", "sessions": nsess, "cells": ncells, "advances": nadv,
             "cells_ok": ok, "wall_s": wall, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "cell_solves_per_s": cells / wall,
-            "final": state}
+            "final": state, "final_ok": cell_ok}
 
 
 def main(argv=None) -> int:

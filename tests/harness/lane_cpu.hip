@@ -92,7 +92,8 @@ extern "C" int lane_solve_cells(const char* id, int mixed, long n, const double*
                                 double dt_min, int check_neg, int max_att, int* status, double* dtout,
                                 int* cnt) {
   // the mixed form runs with the library's default invariant test
-  odv::Controls c{rtol, atol, dt_min, check_neg, max_att, 0, (mixed & 1) ? ODEVEC_MIXED_INVARIANT_TOL : 0.0};
+  odv::Controls c{rtol, atol, dt_min, check_neg, max_att, 0, (mixed & 1) ? ODEVEC_MIXED_INVARIANT_TOL : 0.0,
+                  ODEVEC_FALLBACK_INVARIANT_FAILS};
 #define ODV_CASE(NET) \
   if (!std::strcmp(id, NET::ID)) { run_prec<NET>(mixed, n, t0, ts, dt0, y, prm, c, status, dtout, cnt); return 0; }
   ODV_CASE(odv_net_ROBER_NONE_DENSE)

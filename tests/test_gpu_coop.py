@@ -86,9 +86,12 @@ def test_coop_default_build_accuracy(oracle, case, precision):
     yr, sr, dr, cr = oracle_cells(oracle, w["network"], w)
     ok = (sr == 0)
     assert (st[ok] == 0).mean() > 0.99
-    big = np.abs(yr) > 1e-6 * np.abs(yr).max(axis=0)
-    rel = (np.abs(y - yr)[big & ok[None, :]] / np.abs(yr[big & ok[None, :]]))
-    assert np.median(rel) < 1e-4, np.median(rel)
+    ok &= st == 0
+    # every species, in units of the error weights' scale rtol |y| + atol
+    e = (np.abs(y - yr)[:, ok] / (w["rtol"] * np.abs(yr[:, ok]) + w["atol"])).max(axis=0)
+    print("coop %s precision %d: median %.3g, within 10 tolerances %.4f" % (case, precision, np.median(e),
+                                                                         (e <= 10).mean()), flush=True)
+    assert np.median(e) < 1.0 and (e <= 10).mean() > 0.8
 
 
 @pytest.mark.parametrize("case", ["ROBER", "SYNTH64"])
