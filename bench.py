@@ -272,6 +272,7 @@ def sessions(nsess=8, ncells=65536, nadv=5):
         r = S.run_sessions(srv, nsess=nsess, ncells=ncells, nadv=nadv, warm=warm, seed=7)
         srv.close()
         r.pop("final")
+        r.pop("final_ok")
         out["warm" if warm else "cold"] = r
     out["warm_over_cold"] = out["warm"]["cell_solves_per_s"] / out["cold"]["cell_solves_per_s"]
     out["workload"] = ("%d closed-loop clients x %d PRIMORDIAL cells (seeds 7..), %d advances each through "
