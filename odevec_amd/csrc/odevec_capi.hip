@@ -102,7 +102,7 @@ struct odevec_s {
   int64_t order_cap = 0;
   int promote_cf = 0;                // mixed form: FP64 fallback after this many convergence failures (0: off)
   double inv_tol = ODEVEC_MIXED_INVARIANT_TOL;   // mixed form: invariant test of the corrector (0: off)
-  int promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;   // fallback after this many failed tests (0: never)
+  int promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;   // after this many failed tests: FP64 fallback, or no test (0: neither)
   int* d_promo = nullptr;            // cells handed to the FP64 pass
   unsigned* d_promo_n = nullptr;     // their count
   int64_t promo_cap = 0;
@@ -197,6 +197,7 @@ int launch(odevec_s* h, const Req& r) {
   a.ctl.check_negatives = h->check_neg;
   a.ctl.max_attempts = h->opts.max_attempts > 0 ? h->opts.max_attempts : 1000000;
   a.ctl.inv_tol = h->inv_tol;
+  a.ctl.promote_inv = h->promote_inv;
   int grp = h->opts.group;
   if (grp <= 0) grp = (int)(h->nvector < 256 ? h->nvector : 256);
   if (grp < 1) grp = 1;
@@ -259,7 +260,6 @@ int launch(odevec_s* h, const Req& r) {
   }
   if (promote) {
     a.ctl.promote_cf = h->promote_cf;
-    a.ctl.promote_inv = h->promote_inv;
     a.promo = h->d_promo;
     a.promo_n = h->d_promo_n;
     hipLaunchKernelGGL(odv::k_setu, dim3(1), dim3(1), 0, r.stream, h->d_promo_n, 0u);
@@ -798,7 +798,7 @@ struct odevec_batch_s {
   int ghist = 0;                   // the captured launch is the history variant
   int promote_cf = 0;              // FP64 fallback of mixed segments (0: off)
   double inv_tol = ODEVEC_MIXED_INVARIANT_TOL;   // mixed segments: invariant test of the corrector (0: off)
-  int promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;   // fallback after this many failed tests (0: never)
+  int promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;   // after this many failed tests: FP64 fallback, or no test (0: neither)
   int* d_promo = nullptr;          // handed-back cells of every mixed segment, one region per segment
   int64_t promo_cap = 0;
   unsigned* d_promo_n = nullptr;   // MAXSEG counts
@@ -1013,11 +1013,11 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     a.ctl.check_negatives = g.check_negatives;
     a.ctl.max_attempts = g.max_attempts > 0 ? g.max_attempts : 1000000;
     a.ctl.inv_tol = b->inv_tol;
+    a.ctl.promote_inv = b->promote_inv;
     a.group = 1;
     a.hist = g.hist;
     if (fbnet[k]) {
       a.ctl.promote_cf = b->promote_cf;
-      a.ctl.promote_inv = b->promote_inv;
       a.promo = b->d_promo + fboff[k];
       a.promo_n = b->d_promo_n + k;
     }

@@ -165,7 +165,8 @@ template <class Net>
 struct Hist {
   static constexpr int N = Net::NACT, NF = Net::NEQ, NP = Net::NPARAM;
   static constexpr int VALID = 0, TSTOP = 1, T = 2, DT = 3, RATE = 4, JCH = 5, CERR = 6;
-  static constexpr int INTS = 7, NINT = 7;   // q, steps since Jacobian, lju, failed tests, conv fails, flags, fz
+  // q, steps since Jacobian, lju, failed tests, conv fails, flags, fz, failed invariant tests
+  static constexpr int INTS = 7, NINT = 8;
   static constexpr int Y = INTS + NINT;      // the state returned (all NEQ species)
   static constexpr int P = Y + NF;           // the parameters the rates came from
   static constexpr int Z = P + NP;           // Nordsieck history, column-major (c*N + j)
@@ -204,6 +205,7 @@ __device__ inline bool odv_hist_resume(const KArgs& a, long long i, L& s, const 
   s.upd_jac = (fl & 2) != 0;
   s.neglect_up = (fl & 4) != 0;
   s.fz = (int)h[H::INTS + 6];
+  s.cnt.invfail = (int)h[H::INTS + 7];
   s.status = ST_RUNNING;
   s.tstop = ts;
 #pragma unroll
@@ -251,6 +253,7 @@ __device__ inline void odv_hist_save(const KArgs& a, long long i, const L& s) {
   h[H::INTS + 4] = (double)s.fc;
   h[H::INTS + 5] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0));
   h[H::INTS + 6] = (double)s.fz;
+  h[H::INTS + 7] = (double)s.cnt.invfail;
 #pragma unroll
   for (int j = 0; j < H::NF; ++j) h[H::Y + j] = a.y[(long long)j * n + i];
 #pragma unroll
