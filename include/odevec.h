@@ -180,6 +180,31 @@ int32_t odevec_solve_cells_async(odevec_t* h, int64_t ncells, const double* d_t0
                                  int64_t params_cell_stride, int32_t* d_status, double* d_dt_out,
                                  int32_t* d_counters, double* d_flops, void* stream);
 
+/* Per-cell solve that allocates nothing. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * solve of odevec_solve_cells_async in persistent mode (cooperative mode for
+ * networks of more than 32 species), without a handle: every device buffer
+ * beyond the caller's arrays -- the queue counter, the per-wavefront
+ * workspace and the default values -- is a fixed file-scope device array of
+ * ODEVEC_NOALLOC_SCRATCH_BYTES, one per GPU, held until the process ends.
+ * All pointers are device pointers with one value per cell (d_t0 NULL -> 0,
+ * d_dt0 NULL -> 1e-10, d_params NULL -> the network's defaults, d_status /
+ * d_dt_out NULL -> not wanted); the call enqueues on `stream` (a hipStream_t)
+ * and returns. grid: workgroups, 0 = fill the device or the largest grid
+ * whose workspace fits the scratch; a grid whose workspace does not fit, or
+ * ncells >= 2^31, is rejected with ODEVEC_EINVAL. No FP64 fallback, no cost
+ * order, check_negatives off, the default invariant test. Every call on one
+ * GPU shares that GPU's scratch: calls that overlap on one GPU (on different
+ * streams, or from different threads) race; calls in one stream are
+ * ordered. Results equal odevec_solve_cells_async's in persistent mode bit
+ * for bit. */
+#define ODEVEC_NOALLOC_SCRATCH_BYTES (24u << 20)
+int32_t odevec_solve_cells_noalloc(const char* network, int32_t precision, int64_t ncells, const double* d_t0,
+                                   const double* d_t_stop, const double* d_dt0, double* d_y, const double* d_params,
+                                   int32_t* d_status, double* d_dt_out, double rtol, double atol, double dt_min,
+                                   int32_t grid, void* stream);
+
 /* The generated plugin's network functions, evaluated on the GPU for
  * ncells cells of the handle's network, in the reference's layouts (arrays
  * are host arrays, or device arrays with opts.device_ptrs):

@@ -104,6 +104,8 @@ def lib():
         "odevec_solve_cells_async": (_I32, [_VP, _I64, _VP, _I64, _VP, _I64, _VP, _I64, _VP, _VP, _I64,
                                             _VP, _VP, _VP, _VP, _VP]),
         "odevec_destroy": (_I32, [_VP]),
+        "odevec_solve_cells_noalloc": (_I32, [C.c_char_p, _I32, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP,
+                                              C.c_double, C.c_double, C.c_double, _I32, _VP]),
         "odevec_bundle_count": (_I32, []),
         "odevec_bundle_name": (C.c_char_p, [_I32]),
         "odevec_bundle_entry": (_I32, [C.c_char_p, _I32, C.POINTER(C.c_char_p), C.POINTER(_I32)]),
@@ -129,7 +131,7 @@ EXPORTED = [
     "odevec_bundle_count",
     "odevec_bundle_name", "odevec_bundle_entry", "odevec_batch_create", "odevec_batch_solve_async",
     "odevec_batch_kernel_info", "odevec_batch_destroy", "odevec_batch_set_fp64_fallback",
-    "odevec_set_invariant_test", "odevec_batch_set_invariant_test",
+    "odevec_set_invariant_test", "odevec_batch_set_invariant_test", "odevec_solve_cells_noalloc",
 ]
 
 

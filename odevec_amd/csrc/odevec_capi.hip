@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -751,6 +752,126 @@ int32_t odevec_destroy(odevec_t* h) {
   if (h->e1) (void)hipEventDestroy(h->e1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
+  return ODEVEC_OK;
+}
+
+}  // extern "C"
+
+/* ------------------------------------------------ solve without allocation */
+
+// Device memory of odevec_solve_cells_noalloc beyond the caller's arrays, in
+// one file-scope array per GPU: [0, 256) the persistent queue's counter,
+// [256, 1024) default values (t0 = 0, the dt hint 1e-10, the network's
+// parameter defaults), [1024, end) the per-wavefront workspace. In host code
+// `scratch` names a host-side stand-in; its device address comes from
+// hipGetSymbolAddress (per device) and only that address is used.
+static __device__ __align__(256) unsigned char scratch[ODEVEC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr size_t NOALLOC_HEAD = 1024;
+constexpr int NOALLOC_MAXP = (int)((NOALLOC_HEAD - 256) / 8) - 2;
+constexpr int NOALLOC_DEVICES = 64;
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_DEVICES] = {};
+
+// the device address of `scratch` on the current device, looked up once per device
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= NOALLOC_DEVICES) return fail(ODEVEC_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ODEVEC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t odevec_solve_cells_noalloc(const char* network, int32_t precision, int64_t ncells, const double* d_t0,
+                                   const double* d_t_stop, const double* d_dt0, double* d_y, const double* d_params,
+                                   int32_t* d_status, double* d_dt_out, double rtol, double atol, double dt_min,
+                                   int32_t grid, void* stream) {
+  const odv::NetEntry* net = find_net(network);
+  if (!net) return fail(ODEVEC_ENOTFOUND, std::string("unknown network ") + (network ? network : "(null)"));
+  if (!d_t_stop || !d_y) return fail(ODEVEC_EINVAL, "NULL argument");
+  if (precision != ODEVEC_FP64 && precision != ODEVEC_MIXED) return fail(ODEVEC_EINVAL, "bad precision");
+  if (ncells < 0 || ncells > 0x7fffffffLL) return fail(ODEVEC_EINVAL, "ncells must be in [0, 2^31)");
+  if (grid < 0) return fail(ODEVEC_EINVAL, "grid must be >= 0");
+  if (!(rtol > 0.0) || !(atol >= 0.0)) return fail(ODEVEC_EINVAL, "need rtol > 0 and atol >= 0");
+  if (ncells == 0) return ODEVEC_OK;
+  const int mode = net->lane_ok ? ODEVEC_PERSISTENT : (net->coop_ok ? ODEVEC_COOP : -1);
+  if (mode < 0) return fail(ODEVEC_EINVAL, "network has no persistent kernel");
+  if (!d_params && net->nparam > NOALLOC_MAXP)
+    return fail(ODEVEC_EINVAL, "network has more parameters than the scratch holds defaults for; pass d_params");
+  const size_t ws_cap = (ODEVEC_NOALLOC_SCRATCH_BYTES - NOALLOC_HEAD) / 8;   // doubles
+  odv::Plan p{};
+  hipError_t e = net->plan(ncells, mode, precision, 0, grid, 1, &p);
+  if (e != hipSuccess) return hip_fail(e, "launch plan");
+  if ((size_t)p.ws_doubles > ws_cap) {
+    const long long per_wg = p.ws_doubles / p.grid;
+    if (grid > 0 || per_wg <= 0 || (long long)ws_cap < per_wg) {
+      char buf[200];
+      std::snprintf(buf, sizeof buf, "%lld workgroups need %lld bytes of workspace; the scratch holds %zu",
+                    (long long)p.grid, p.ws_doubles * 8, ws_cap * 8);
+      return fail(ODEVEC_EINVAL, buf);
+    }
+    // the full-device grid does not fit: the largest grid that does
+    e = net->plan(ncells, mode, precision, 0, (int)((long long)ws_cap / per_wg), 1, &p);
+    if (e != hipSuccess) return hip_fail(e, "launch plan");
+  }
+  void* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc != ODEVEC_OK) return rc;
+  unsigned char* b = static_cast<unsigned char*>(base);
+  double* defaults = reinterpret_cast<double*>(b + 256);
+  hipStream_t st = (hipStream_t)stream;
+  odv::KArgs a{};
+  a.y = d_y;
+  a.ncells = ncells;
+  a.tstop = d_t_stop;
+  a.ts_s = 1;
+  if (d_t0) { a.t0 = d_t0; a.t0_s = 1; }
+  else {
+    hipLaunchKernelGGL(odv::k_fill_f64, dim3(1), dim3(64), 0, st, defaults, 0.0, 1LL);
+    a.t0 = defaults; a.t0_s = 0;
+  }
+  if (d_dt0) { a.dt0 = d_dt0; a.dt_s = 1; }
+  else {
+    hipLaunchKernelGGL(odv::k_fill_f64, dim3(1), dim3(64), 0, st, defaults + 1, 1e-10, 1LL);
+    a.dt0 = defaults + 1; a.dt_s = 0;
+  }
+  if (d_params && net->nparam > 0) {
+    a.prm = d_params; a.ps_cell = 1; a.ps_plane = ncells;
+  } else {
+    double pd[NOALLOC_MAXP > 0 ? NOALLOC_MAXP : 1] = {};
+    if (net->nparam > 0) net->param_defaults(pd);
+    for (int k = 0; k < net->nparam; ++k)
+      hipLaunchKernelGGL(odv::k_fill_f64, dim3(1), dim3(64), 0, st, defaults + 2 + k, pd[k], 1LL);
+    a.prm = defaults + 2; a.ps_cell = 0; a.ps_plane = 1;
+  }
+  a.status = d_status;
+  a.dt_out = d_dt_out;
+  a.queue = reinterpret_cast<unsigned long long*>(b);
+  a.ws = reinterpret_cast<double*>(b + NOALLOC_HEAD);
+  a.ctl.rtol = rtol;
+  a.ctl.atol = atol;
+  a.ctl.dt_min = dt_min;
+  a.ctl.max_attempts = 1000000;
+  a.ctl.inv_tol = ODEVEC_MIXED_INVARIANT_TOL;
+  a.ctl.promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;
+  a.group = 1;
+  e = net->launch(a, mode, precision, p, st);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return ODEVEC_OK;
 }
 
