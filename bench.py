@@ -19,7 +19,8 @@ at 1 request/s and the three small ones at 40/s), and (skipped with
 --no-cpu) the CPU baseline; and serving with warm restart (sessions whose
 blocks stay on the device and are advanced in closed loop) against the same
 advances cold-started. `--only batching|synth64|sessions` runs that part
-alone and prints its object (for profiling one kernel).
+alone and prints its object, `--only headline` the headline solve alone (for
+profiling one kernel).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batch] [--no-cpu] [--only PART]
 """
@@ -57,7 +58,7 @@ def parse():
     p.add_argument("--cells", type=int, default=NCELLS)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
-    p.add_argument("--only", choices=["batching", "synth64", "sessions", "tail"], default=None)
+    p.add_argument("--only", choices=["headline", "batching", "synth64", "sessions", "tail"], default=None)
     p.add_argument("--fallback", type=int, default=FALLBACK,
                    help="mixed form: FP64 fallback after this many convergence failures (0: off)")
     return p.parse_args()
@@ -437,6 +438,12 @@ def main():
         return
     w = W.primordial(a.cells, seed=3)
     mixed = Runner(w, precision=1, mode=2, fallback=FALLBACK, schedule=SCHEDULE)
+    if a.only == "headline":
+        # the headline solve alone (for profiling its kernels)
+        wall_m, ms_m = mixed.timed(a.steps, a.warmup)
+        print(json.dumps({"headline": {"value": a.steps * a.cells / wall_m, "unit": "cell-solves/s",
+                                       "ms_per_step": wall_m / a.steps * 1e3, "kernel_ms": ms_m}}))
+        return
     fp64 = Runner(w, precision=0, mode=0)
     wall_m, ms_m = mixed.timed(a.steps, a.warmup)
     y_m, st_mixed = mixed.y.cpu().numpy(), mixed.st.cpu().numpy()

@@ -211,6 +211,9 @@ int launch(odevec_s* h, const Req& r) {
   // by the FP64 persistent kernel over the list of such cells (second launch)
   const bool promote = h->promote_cf > 0 && h->opts.precision == ODEVEC_MIXED && !a.hist &&
                        (h->opts.mode == ODEVEC_PERCELL || h->opts.mode == ODEVEC_PERSISTENT);
+  // the fallback's list and the cost-ordered queue hold cell indices as int
+  if ((promote || (h->opts.mode == ODEVEC_PERSISTENT && h->opts.schedule == 1)) && r.ncells > 0x7fffffffLL)
+    return fail(ODEVEC_EINVAL, "the FP64 fallback and the cost-ordered queue take at most 2^31 - 1 cells per solve");
   odv::Plan p2{};
   if (promote) {
     e = h->net->plan(r.ncells, ODEVEC_PERSISTENT, ODEVEC_FP64, h->opts.block, h->opts.grid, grp, &p2);
@@ -1089,6 +1092,8 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     for (int k = 0; k < nseg; ++k) {
       const odevec_segment_t& g = segs[k];
       if (g.precision != ODEVEC_MIXED || g.hist || g.ncells <= 0) continue;
+      if (g.ncells > 0x7fffffffLL)
+        return fail(ODEVEC_EINVAL, "the FP64 fallback takes at most 2^31 - 1 cells per segment");
       const odv::NetEntry* ne = find_net(g.network);
       if (!ne || !ne->lane_ok) continue;
       fbnet[k] = ne;
@@ -1153,6 +1158,8 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     for (int k = 0; k < nseg; ++k) {
       const odv::NetEntry* ne = find_net(segs[k].network);
       if (ne && ne->lane_ok && segs[k].ncells > p.grid * p.block && !segs[k].hist) {
+        if (segs[k].ncells > 0x7fffffffLL)
+          return fail(ODEVEC_EINVAL, "the cost-ordered queue takes at most 2^31 - 1 cells per segment");
         onet[k] = ne;
         ototal += segs[k].ncells;
       }

@@ -564,3 +564,20 @@ def test_mixed_fp64_fallback_on_invariant_failures(mode):
     say("handed back", int((as_fp64 & ~as_mixed).sum()), "kept", int(as_mixed.sum()))
     assert (as_fp64 | as_mixed).all()
     assert (as_fp64 & ~as_mixed).sum() >= 1 and as_mixed.sum() >= len(sf) // 2
+
+
+def test_int_index_lists_refuse_2pow31_cells():
+    """The FP64 fallback's list and the cost-ordered queue store cell indices
+    as int: a solve of 2^31 cells that would use them is refused before
+    anything is launched (the pointers are never read)."""
+    from odevec_amd.api import Odevec
+    fake = 1 << 12
+    for prec, sched, fb in ((_lib.MIXED, 0, 128), (_lib.FP64, 1, 0)):
+        say("refusal of 2^31 cells, precision", prec, "schedule", sched, "fallback", fb)
+        ode = Odevec("PRIMORDIAL/CMK/CSC", 16, precision=prec, mode=_lib.PERSISTENT, schedule=sched)
+        if fb:
+            ode.set_fp64_fallback(fb)
+        with pytest.raises(_lib.OdevecError) as ex:
+            ode.solve_cells_async(1 << 31, fake, fake)
+        assert ex.value.code == _lib.EINVAL and "2^31" in str(ex.value)
+        ode.CloseOdevec()
