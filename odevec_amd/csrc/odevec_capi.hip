@@ -66,6 +66,15 @@ const odv::NetEntry* find_net(const char* id) {
   return nullptr;
 }
 
+// A batch segment's network in the bundle's own build (a reproducible
+// bundle's FP64 passes and cost ranking use the reproducible kernels)
+const odv::NetEntry* find_net_build(const char* id, int repro) {
+  if (!id) return nullptr;
+  for (const odv::NetEntry* e : odv::registry())
+    if (e->repro == repro && std::strcmp(id, e->id) == 0) return e;
+  return nullptr;
+}
+
 }  // namespace
 
 struct odevec_s {
@@ -1094,7 +1103,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       if (g.precision != ODEVEC_MIXED || g.hist || g.ncells <= 0) continue;
       if (g.ncells > 0x7fffffffLL)
         return fail(ODEVEC_EINVAL, "the FP64 fallback takes at most 2^31 - 1 cells per segment");
-      const odv::NetEntry* ne = find_net(g.network);
+      const odv::NetEntry* ne = find_net_build(g.network, b->b->repro);
       if (!ne || !ne->lane_ok) continue;
       fbnet[k] = ne;
       fboff[k] = fbtotal;
@@ -1156,7 +1165,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     int64_t ototal = 0;
     std::vector<const odv::NetEntry*> onet(nseg, nullptr);
     for (int k = 0; k < nseg; ++k) {
-      const odv::NetEntry* ne = find_net(segs[k].network);
+      const odv::NetEntry* ne = find_net_build(segs[k].network, b->b->repro);
       if (ne && ne->lane_ok && segs[k].ncells > p.grid * p.block && !segs[k].hist) {
         if (segs[k].ncells > 0x7fffffffLL)
           return fail(ODEVEC_EINVAL, "the cost-ordered queue takes at most 2^31 - 1 cells per segment");

@@ -7,6 +7,12 @@ precisions and sizes (odevec_batch_*).
   (bitwise: per-cell step control, no data shared between cells).
 * graph mode: replays of the captured launch, with eager work and
   synchronizations between them, each equal the eager launch.
+* cost order: segments with more cells than the launch has lanes get their
+  queue ranked by cost; on a one-workgroup grid the +REPRO bundle still
+  equals the oracle, and with the FP64 fallback results equal a full grid's.
+* the batching workload at full size (2,097,152 cells) through the
+  reproducible build of the tuned bundle, fallback and graph replay: seeded
+  samples equal the single-network kernels bit for bit.
 """
 import numpy as np
 import pytest
@@ -160,8 +166,14 @@ def test_batch_fp64_fallback(use_graph):
     rob = W.rober(100, seed=22)
     limit = 8
     say("GPU batch fallback, graph", use_graph)
-    plain = run("MIX3", [to_dev(pri, _lib.MIXED), to_dev(rob, _lib.FP64)])
+    # the invariant test off: the handed-back cells are then exactly those
+    # whose convergence failures reach the limit
+    pb = Batch("MIX3")
+    pb.set_invariant_test(0.0, 0)
+    plain = run("MIX3", [to_dev(pri, _lib.MIXED), to_dev(rob, _lib.FP64)], batch=pb)
+    pb.close()
     b = Batch("MIX3", use_graph=use_graph, fp64_fallback=limit)
+    b.set_invariant_test(0.0, 0)
     segs = [to_dev(pri, _lib.MIXED), to_dev(rob, _lib.FP64)]
     fb = run("MIX3", segs, batch=b)
     if use_graph:                        # a replay over fresh inputs gives the same
@@ -193,3 +205,86 @@ def test_batch_fp64_fallback(use_graph):
     assert np.array_equal(yf[:, keep], ym[:, keep]) and np.array_equal(sf[keep], sm[keep])
     assert np.array_equal(df[keep], dm[keep]) and np.array_equal(cf[:, keep], cm[:, keep])
     assert all(np.array_equal(x, y) for x, y in zip(fb[1], plain[1]))
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_cost_order_small_grid_equals_oracle(oracle, use_graph):
+    """One workgroup of 64 lanes: every per-lane segment has more cells than
+    the launch has lanes, so each gets its queue in cost order (ranked before
+    the launch); the +REPRO bundle still equals the oracle bit for bit."""
+    from odevec_amd.api import Batch
+    ws = mixed_workload()
+    say("GPU batch cost order, grid 1 x 64, graph", use_graph)
+    b = Batch("MIX3+REPRO", block=64, grid=1, use_graph=use_graph)
+    out = run("MIX3+REPRO", [to_dev(w, _lib.FP64) for w in ws], batch=b)
+    b.close()
+    assert sum(w["y"].shape[1] > 64 for w in ws) >= 2       # PRIMORDIAL and ROBER get ranked queues
+    for w, (y, st, dto, cnt) in zip(ws, out):
+        if w["y"].shape[1] == 0:
+            continue
+        yr, sr, dr, cr = oracle.Network.load(w["network"]).solve_cells(
+            w["y"], w["t_stop"], t0=w["t0"], dt0=w["dt0"], params=w["params"], rtol=w["rtol"], atol=w["atol"],
+            dt_min=w["dt_min"])
+        assert np.array_equal(st, sr) and np.array_equal(y, yr), w["network"]
+        assert np.array_equal(dto, dr) and np.array_equal(cnt, cr), w["network"]
+
+
+def test_batch_cost_order_with_fallback_equals_full_grid():
+    """Cost-ordered segment queues on a small grid, together with the FP64
+    fallback (which reuses the ranked list's machinery): results equal a
+    full-grid launch (index order, no ranking) bit for bit."""
+    from odevec_amd.api import Batch
+    pri = W.primordial(2000, seed=31)
+    rob = W.rober(500, seed=32)
+    say("GPU batch cost order + fallback, grid 2 x 64")
+    outs = []
+    for grid in (2, 0):
+        b = Batch("MIX3", block=64, grid=grid, fp64_fallback=8)
+        outs.append(run("MIX3", [to_dev(pri, _lib.MIXED), to_dev(rob, _lib.FP64)], batch=b))
+        b.close()
+    for p, q in zip(*outs):
+        for x, y in zip(p, q):
+            assert np.array_equal(x, y)
+    assert (outs[0][0][1] == 0).mean() > 0.99
+
+
+def test_batching_configuration_full_size():
+    """The batching workload at full size (2,097,152 cells, 40/30/30
+    ROBER/OREGO/PRIMORDIAL, seed 4) through the reproducible build of the
+    tuned bundle and precisions, with its FP64 fallback and graph replay:
+    every cell the reference's algorithm finishes finishes, and seeded
+    samples of each segment equal the single-network persistent kernels
+    (same precision, same fallback) bit for bit."""
+    import torch
+    from odevec_amd import autotune as A
+    from odevec_amd.api import Batch, Odevec
+    segs = W.batch_mix(2097152, seed=4)
+    nets = [w["network"] for w in segs]
+    t = A.lookup(nets) or {"bundle": "MIX3_MIXED", "block": 64, "fp64_fallback": 128,
+                           "precision": {"ROBER/NONE/DENSE": 0, "OREGO/NONE/DENSE": 0, "PRIMORDIAL/CMK/CSC": 1}}
+    bundle = t["bundle"] + "+REPRO"
+    fb = t.get("fp64_fallback", 0)
+    say("GPU batching configuration:", bundle, "block", t["block"], "fallback", fb, "cells", [w["y"].shape[1] for w in segs])
+    b = Batch(bundle, block=t["block"], use_graph=True, fp64_fallback=fb)
+    dsegs = [to_dev(w, t["precision"][w["network"]]) for w in segs]
+    out = run(bundle, dsegs, batch=b)
+    b.close()
+    del dsegs
+    rng = np.random.default_rng(44)
+    for w, (y, st, dto, cnt) in zip(segs, out):
+        n = w["y"].shape[1]
+        prec = t["precision"][w["network"]]
+        say(w["network"], "precision", prec, "status", np.bincount(st))
+        assert (st == 0).mean() > 0.9999
+        idx = np.sort(rng.choice(n, 4096, replace=False))
+        s = W.subset(w, idx)
+        ode = Odevec(w["network"] + "+REPRO", idx.size, precision=prec, mode=_lib.PERSISTENT)
+        ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = s["rtol"], s["atol"], s["dt_min"]
+        if prec == _lib.MIXED and fb:
+            ode.set_fp64_fallback(fb)
+        ys = np.array(s["y"], order="C", copy=True)
+        ss, ds, _ = ode.solve_cells(ys, s["t_stop"], t0=s["t0"], dt0=s["dt0"], params=s["params"], raise_on_fail=False)
+        ode.CloseOdevec()
+        assert np.array_equal(ss, st[idx]) and np.array_equal(ys, y[:, idx]), w["network"]
+        assert np.array_equal(ds, dto[idx]), w["network"]
+    torch.cuda.empty_cache()

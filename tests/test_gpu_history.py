@@ -55,7 +55,9 @@ CASES = {
 @pytest.mark.parametrize("case", sorted(CASES))
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
 @pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
-def test_continue_equals_one_solve(case, precision, mode):
+def test_continue_equals_one_solve(oracle, case, precision, mode):
+    """FP64: the single solve, and so the continued one, is also the
+    oracle's solve over the same span bit for bit."""
     w = CASES[case]()
     nid = w["network"] + "+REPRO"
     n = w["y"].shape[1]
@@ -75,6 +77,10 @@ def test_continue_equals_one_solve(case, precision, mode):
     assert np.array_equal(dt_b, dt_one)
     for k in ("attempts", "steps", "newton", "jac"):
         assert s_a[k] + s_b[k] == s_one[k], k
+    if precision == _lib.FP64:
+        yo, so, do, _ = oracle.Network.load(w["network"]).solve_cells(
+            w["y"], t2, t0=t0, dt0=w["dt0"], params=w["params"], rtol=w["rtol"], atol=w["atol"], dt_min=w["dt_min"])
+        assert np.array_equal(so, st_b) and np.array_equal(yo, y) and np.array_equal(do, dt_b)
 
 
 @pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])

@@ -55,6 +55,65 @@ def test_serving_results_same_alone_and_in_mix():
 
 
 @pytest.mark.gpu
+def test_serving_repro_equals_oracle(oracle):
+    """The serving loop over the reproducible bundle (FP64, no contraction):
+    requests batched together into shared launches give, on a seeded sample
+    of each request's cells, the oracle's per-cell solve bit for bit (state
+    and status)."""
+    print("GPU serving +REPRO vs oracle", flush=True)
+    nets = ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"]
+    trace = S.make_trace(400.0, 0.03, seed=12, networks=nets, lo=1e3, hi=8e3)
+    assert len(trace) >= 6 and len({r.network for r in trace}) == 3
+    srv = S.Server("MIX3+REPRO", slots=2)
+    S.run_trace(srv, trace)
+    srv.close()
+    print("requests", len(trace), "launches", srv.launches, flush=True)
+    rng = np.random.default_rng(12)
+    for r in trace:
+        idx = np.sort(rng.choice(r.ncells, 48, replace=False))
+        yo, so, _, _ = oracle.Network.load(r.network).solve_cells(
+            r.y[:, idx], r.t_stop[idx], dt0=r.dt0[idx], params=None if r.params is None else r.params[:, idx],
+            rtol=r.rtol, atol=r.atol, dt_min=r.dt_min)
+        assert np.array_equal(so, r.status[idx]) and np.array_equal(yo, r.result[:, idx]), (r.rid, r.network)
+
+
+@pytest.mark.gpu
+def test_serving_bench_configuration_same_alone_and_in_mix():
+    """The serving configuration of the bench (the tuned bundle and
+    precisions, mixed PRIMORDIAL with its FP64 fallback at 128 convergence
+    failures, seed-5 arrivals of blocks of up to 256k cells of the three
+    bundled networks): every request's result is the same alone and in the
+    mix, bit for bit (handing a cell back depends only on that cell)."""
+    from odevec_amd import autotune as A
+    print("GPU serving bench configuration", flush=True)
+    nets = ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"]
+    tuned = A.lookup(nets) or {}
+    kw = dict(precision=tuned.get("precision", {"PRIMORDIAL/CMK/CSC": 1}), block=tuned.get("block", 64),
+              fp64_fallback=tuned.get("fp64_fallback", 128))
+    bundle = tuned.get("bundle", "MIX3_MIXED")
+    assert kw["precision"].get("PRIMORDIAL/CMK/CSC") == 1 and kw["fp64_fallback"] > 0
+
+    def trace():
+        return S.make_trace(100.0, 0.1, seed=5, networks=nets)
+
+    mix = trace()
+    assert len(mix) >= 10 and max(r.ncells for r in mix if r.network.startswith("PRIMORDIAL")) > 1.5e5
+    srv = S.Server(bundle, slots=2, **kw)
+    S.run_trace(srv, mix)
+    srv.close()
+    for r in mix:
+        alone = trace()[r.rid]
+        one = S.Server(bundle, slots=1, **kw)
+        one.submit(alone)
+        one.drain()
+        one.close()
+        assert np.array_equal(alone.result, r.result) and np.array_equal(alone.status, r.status), r.rid
+    ok = sum(int((r.status == 0).sum()) for r in mix) / sum(r.ncells for r in mix)
+    print("cells finished: %.6f" % ok, flush=True)
+    assert ok > 0.999
+
+
+@pytest.mark.gpu
 def test_serving_admission_respects_hbm_budget():
     trace = S.make_trace(400.0, 0.03, seed=12, lo=1e3, hi=4e3)
     budget = max(r.footprint() for r in trace)       # one request at a time
