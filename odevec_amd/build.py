@@ -37,6 +37,8 @@ BUILTIN_VARIANTS = [
     "SYNTH64/FF/CSC",     # 64 species: cooperative kernels only
     "ROBER/NONE/DENSE/LU3",        # LUmethod 3: Jacobian by forward differences
     "PRIMORDIAL/NONE/DENSE/LU3",
+    "ROBER/NONE/DENSE/LU2",        # LUmethod 2: L and U in closed form
+    "OREGO/NONE/DENSE/LU2",
 ]
 
 # bundles: one batch kernel (odv_k_batch) over several (network, precision)
@@ -82,23 +84,23 @@ def generate(variants: List[str]) -> List[str]:
         name, ordering, packaging, lu = _parts(v)
         ordering = "None" if ordering.upper() == "NONE" else ordering
         cli.generate(name, ordering, packaging, GEN, lu_method=lu)
-        bases.append("%s_%s_%s%s" % (name, ordering.upper(), packaging, "_LU3" if lu == 3 else ""))
+        bases.append("%s_%s_%s%s" % (name, ordering.upper(), packaging, "" if lu == 1 else "_LU%d" % lu))
     return bases
 
 
 def _parts(variant: str):
-    """NAME/ORDERING/PACKAGING[/LU3] -> (name, ordering, packaging, LUmethod)."""
+    """NAME/ORDERING/PACKAGING[/LU2|/LU3] -> (name, ordering, packaging, LUmethod)."""
     p = variant.split("/")
-    if len(p) == 4 and p[3] == "LU3":
-        return p[0], p[1], p[2], 3
+    if len(p) == 4 and p[3] in ("LU2", "LU3"):
+        return p[0], p[1], p[2], int(p[3][2])
     if len(p) != 3:
-        raise ValueError("variant %r: expected NAME/ORDERING/PACKAGING[/LU3]" % variant)
+        raise ValueError("variant %r: expected NAME/ORDERING/PACKAGING[/LU2|/LU3]" % variant)
     return p[0], p[1], p[2], 1
 
 
 def _struct(variant: str) -> str:
     name, ordering, packaging, lu = _parts(variant)
-    return "odv_net_%s_%s_%s%s" % (name, ordering.upper(), packaging, "_LU3" if lu == 3 else "")
+    return "odv_net_%s_%s_%s%s" % (name, ordering.upper(), packaging, "" if lu == 1 else "_LU%d" % lu)
 
 
 def generate_bundle(name: str, entries) -> str:

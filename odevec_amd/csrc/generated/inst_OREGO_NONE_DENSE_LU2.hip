@@ -1,0 +1,4 @@
+// Generated by odevec_amd.pre_odevec -- do not edit.
+#include "../odevec_registry.h"
+#include "net_OREGO_NONE_DENSE_LU2.h"
+ODV_REGISTER(odv_net_OREGO_NONE_DENSE_LU2)

@@ -25,6 +25,9 @@ struct odv_net_ROBER_NONE_DENSE_LU3 {
   // = storage index of P_order(r, c), -1 outside the LU pattern (there the difference
   // is exactly zero: no RHS term of row r reads species c)
   static constexpr bool FD_JAC = true;
+  // LUmethod 2: jac() writes the closed-form L and U factors of P_order (GetLU),
+  // lu() does nothing and lu_r() only inverts the diagonal
+  static constexpr int LU_METHOD = 3;
   static constexpr short AIDX[9] = {0, 2, 5, 1, 3, 6, -1, 4, 7};
   static constexpr int FLOPS_RHS = 8, FLOPS_JAC = 95, FLOPS_LU = 8, FLOPS_SOLVE = 13;
   static constexpr int FLOPS_LU_R = 11, FLOPS_SOLVE_R = 13;

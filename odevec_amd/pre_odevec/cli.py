@@ -12,13 +12,16 @@ library (``python -m odevec_amd.build --extra <id>``) adds the network to it.
 Flag notes:
 * ``--nvector`` is accepted for compatibility; the cell count is a runtime
   size of the library, not a code-generation constant.
-* ``--LUmethod`` 1 (symbolic Jacobian + generated LU) and 3 (Jacobian by
-  forward differences of the RHS, src/odevec.F90:469-506, then the generated
-  LU; variant id ``NAME/NONE/DENSE/LU3``) are generated. 3 needs DENSE
-  packaging (the reference aborts for CSC, src/odevec.F90:509-523) and no
-  ordering (the reference permutes the residual but not this Jacobian,
-  src/odevec.F90:306-308, 328-342, which is only consistent without one).
-  2 (closed-form LU) is rejected with a message.
+* ``--LUmethod`` 1 (symbolic Jacobian + generated LU), 2 (the L and U
+  factors in closed form, pre_odevec.py:132-139, variant id
+  ``NAME/ORDERING/DENSE/LU2``) and 3 (Jacobian by forward differences of the
+  RHS, src/odevec.F90:469-506, then the generated LU; variant id
+  ``NAME/NONE/DENSE/LU3``) are generated. 2 needs DENSE packaging: the
+  reference's CSC form writes the factors in SymPy's column-list order, which
+  is not the order its CSC solve reads them in. 3 needs DENSE packaging (the
+  reference aborts for CSC, src/odevec.F90:509-523) and no ordering (the
+  reference permutes the residual but not this Jacobian, src/odevec.F90:306-308,
+  328-342, which is only consistent without one).
 * ``--heatcool 1`` needs KROME's heating/cooling Fortran and is rejected.
 * ``--equilibrium_H`` is validated exactly as the reference validates it
   (pre_odevec.py:455-460: DENSE, no ordering, LUmethod 3) and, as there,
@@ -41,13 +44,16 @@ def generate(example: str, ordering: str, packaging: str, outdir: str,
              quiet: bool = True, lu_method: int = 1) -> dict:
     net = networks.load(example, krome_setupfile)
     st = structure.analyse(net, ordering, packaging)
-    if lu_method not in (1, 3):
-        raise ValueError("LUmethod %d is not generated (1: symbolic Jacobian, 3: finite differences)" % lu_method)
+    if lu_method not in (1, 2, 3):
+        raise ValueError("LUmethod %d is not generated (1: symbolic Jacobian, 2: closed-form LU, "
+                         "3: finite differences)" % lu_method)
+    if lu_method == 2 and st.packaging != "DENSE":
+        raise ValueError("LUmethod 2 needs DENSE packaging")
     if lu_method == 3 and (st.packaging != "DENSE" or st.ordering != "None"):
         raise ValueError("LUmethod 3 needs DENSE packaging and no ordering")
     st.lu_method = lu_method
     os.makedirs(outdir, exist_ok=True)
-    base = "%s_%s_%s%s" % (net.name, st.ordering.upper(), st.packaging, "_LU3" if lu_method == 3 else "")
+    base = "%s_%s_%s%s" % (net.name, st.ordering.upper(), st.packaging, emit.lu_suffix(lu_method, "_"))
     hdr = emit.emit_header(st)
     ir = emit.ir_dict(st)
     if dt_min is not None:
@@ -102,9 +108,11 @@ def main(argv: Optional[List[str]] = None) -> int:
         sys.exit("args.equilibrium_H is only allowed without ordering.")
     if args.equilibrium_H == 1 and args.LUmethod != 3:
         sys.exit("args.equilibrium_H is currently only usable with LUmethod==3.")
-    if args.LUmethod not in (1, 3):
-        sys.exit("LUmethod %d is not generated; 1 (symbolic Jacobian) and 3 (finite differences) are."
-                 % args.LUmethod)
+    if args.LUmethod not in (1, 2, 3):
+        sys.exit("LUmethod %d is not generated; 1 (symbolic Jacobian), 2 (closed-form LU) and 3 "
+                 "(finite differences) are." % args.LUmethod)
+    if args.LUmethod == 2 and args.packaging != "DENSE":
+        sys.exit("LUmethod 2 needs DENSE packaging.")
     if args.LUmethod == 3 and (args.packaging != "DENSE" or args.ordering not in ("None", "NONE")):
         sys.exit("LUmethod 3 needs DENSE packaging and no ordering.")
     if args.heatcool:

@@ -32,6 +32,7 @@ import math
 import numpy as np
 from typing import Dict, List, Tuple
 
+from . import lu_closed
 from .structure import Structure, Term, linear_invariants, reduce_frozen
 
 RATE_KIND_ID = {"none": 0, "param": 1, "primordial": 2, "arrhenius": 3}
@@ -47,13 +48,17 @@ def _lit(v: float) -> str:
     return r
 
 
+def lu_suffix(lu_method: int, sep: str) -> str:
+    """Variant suffix of a LUmethod: none for 1, ``/LU2`` or ``/LU3``."""
+    return "" if lu_method == 1 else "%sLU%d" % (sep, lu_method)
+
+
 def struct_name(st: Structure) -> str:
-    return "odv_net_%s_%s_%s%s" % (st.net.name, st.ordering.upper(), st.packaging,
-                                   "_LU3" if st.lu_method == 3 else "")
+    return "odv_net_%s_%s_%s%s" % (st.net.name, st.ordering.upper(), st.packaging, lu_suffix(st.lu_method, "_"))
 
 
 def network_id(st: Structure) -> str:
-    return "%s/%s/%s%s" % (st.net.name, st.ordering.upper(), st.packaging, "/LU3" if st.lu_method == 3 else "")
+    return "%s/%s/%s%s" % (st.net.name, st.ordering.upper(), st.packaging, lu_suffix(st.lu_method, "/"))
 
 
 class _Counter:
@@ -314,6 +319,12 @@ def opcounts(st: Structure) -> Dict[str, Dict[str, int]]:
         c = _Counter()
         fn(c)
         out[name] = c.as_dict()
+    if st.lu_method == 2:
+        # the closed-form factors: the program's operations, no runtime
+        # factorization; the reciprocal form inverts the diagonal
+        out["jac"] = lu_closed.opcount(lu_closed.closed_form_lu(st))
+        out["lu"] = {"add": 0, "mul": 0, "div": 0, "flops": 0}
+        out["lu_r"] = {"add": 0, "mul": 0, "div": st.neq, "flops": st.neq}
     if st.lu_method == 3:
         # forward differences: N+1 RHS evaluations; the weighted norm of f
         # (3N+2), r0 (4), per column 9 (|y|, two products, sqrt, quotient,
@@ -428,6 +439,8 @@ def emit_header(full: Structure) -> str:
     lu = lu_fn(st, _Counter(), False)
     sv = sv_fn(st, _Counter(), False)
     lur = lu_fn(st, _Counter(), True)
+    if st.lu_method == 2:
+        jac, lu, lur = _emit_lu_closed(st)
     svr = sv_fn(st, _Counter(), True)
     ops = opcounts(st)
     perm = ", ".join(str(int(p)) for p in st.perm)
@@ -464,6 +477,9 @@ def emit_header(full: Structure) -> str:
     hdr.append("  // = storage index of P_order(r, c), -1 outside the LU pattern (there the difference")
     hdr.append("  // is exactly zero: no RHS term of row r reads species c)")
     hdr.append("  static constexpr bool FD_JAC = %s;" % ("true" if st.lu_method == 3 else "false"))
+    hdr.append("  // LUmethod 2: jac() writes the closed-form L and U factors of P_order (GetLU),")
+    hdr.append("  // lu() does nothing and lu_r() only inverts the diagonal")
+    hdr.append("  static constexpr int LU_METHOD = %d;" % st.lu_method)
     if st.lu_method == 3:
         pos = {tuple(map(int, e)): k for k, e in enumerate(st.csc.entries)}
         aidx = [pos.get((r, c), -1) for r in range(st.neq) for c in range(st.neq)]
@@ -541,6 +557,21 @@ def emit_header(full: Structure) -> str:
     return "\n".join(hdr)
 
 
+def _emit_lu_closed(st: Structure):
+    """LUmethod 2 bodies: (jac, lu, lu_r). jac() evaluates the closed-form
+    program and writes every storage slot (zero where the symbolic factor is
+    zero); lu() is empty; lu_r() turns the U diagonal into reciprocals."""
+    idx = st.csc.index()
+    prog = lu_closed.closed_form_lu(st)
+    lit = lambda v: _lit(v) if v >= 0 else "(%s)" % _lit(v)
+    body, _ = lu_closed.emit_c(prog, lit, lambda r, c, e: "    a[%d] = (M)(%s);" % (idx[(r, c)], e))
+    written = {idx[(r, c)] for r, c, _ in prog["out"]}
+    zero = ["    a[%d] = (M)0.0;" % s for s in range(st.csc.nnz) if s not in written]
+    jac = body + zero
+    lur = ["    a[%d] = (M)1 / a[%d];" % (idx[(k, k)], idx[(k, k)]) for k in range(st.neq)]
+    return jac, ["    (void)a;"], lur
+
+
 def _emit_jmap(full: Structure, st: Structure) -> List[str]:
     """The reference's GetJac storage (src/odevec_commons.F90:32-51,
     pre_odevec.py:144-180) from the generated code's: entry e of the full
@@ -603,6 +634,8 @@ def ir_dict(st: Structure) -> dict:
         "l_col_start": st.csc.l_col_start,
         "nnz": st.csc.nnz,
         "lu_method": st.lu_method,
+        # LUmethod 2: the closed-form factors' program (row, col in P_order)
+        "lu_closed": lu_closed.closed_form_lu(st) if st.lu_method == 2 else None,
         # the generated code integrates the active species only
         "active": reduce_frozen(analyse_copy(st)).active,
         # linear invariants of the full system (zero on frozen species)

@@ -15,10 +15,13 @@
  * differentiating those monomials (or, LUmethod 3, by CalcJac_dense's
  * forward differences of the RHS, src/odevec.F90:469-506), the factorizations with the CSC index
  * arrays exactly as the reference loops over them (src/odevec.F90:853-971)
- * or dense (src/odevec.F90:797-849, 898-922).
+ * or dense (src/odevec.F90:797-849, 898-922); for LUmethod 2 (GetLU) the
+ * factors come from the preprocessor's closed-form program, executed
+ * operation by operation (closed_lu below).
  *
  * Parity pins (see tests/): BDF l-vectors against their closed form,
- * LU factors against L*U = P, RHS against the reference network
+ * LU factors (numeric and closed-form) against L*U = P, the closed-form
+ * program against the exact rational factorization, RHS against the reference network
  * definitions' own values, rates against the reference formulas, whole
  * solves against an independent Radau integrator at rtol 1e-12, the
  * forward-difference Jacobian against the analytic one within its error
@@ -55,7 +58,18 @@ typedef struct {
   const int* l_col_start;/* [neq] */
   const double* arr_A;   /* arrhenius */
   const double* arr_Ea;
-  int lu_method;         /* 1: GetJac (symbolic P); 3: CalcJac_dense (forward differences) */
+  int lu_method;         /* 1: GetJac (symbolic P); 2: GetLU (closed-form factors); 3: CalcJac_dense */
+  /* LUmethod 2: the closed-form factors as a straight-line program over
+   * y[a], k[a], hb, constants and earlier results (code 0 y, 1 k, 2 hb,
+   * 3 constant, 4 add, 5 sub, 6 mul, 7 div, 8 negate), and its outputs
+   * (row, col, node) in P_order coordinates */
+  int lc_nops;
+  const int* lc_code;
+  const int* lc_a;
+  const int* lc_b;
+  const double* lc_c;
+  int lc_nout;
+  const int* lc_out;     /* [lc_nout*3] */
 } onet_t;
 
 
@@ -519,7 +533,37 @@ static void fd_p_all(ovec_t* v, const double* y, double dt, double beta, double*
   free(yy); free(f0); free(D); free(r); free(fac);
 }
 
-/* GetJac (or CalcJac, LUmethod 3) + LUDecompose for all cells */
+/* GetLU (src/odevec_commons.F90:55-72, entries written by pre_odevec.py:132-139),
+ * LUmethod 2: the dense factors of one cell by the closed-form program, each
+ * operation one correctly rounded IEEE operation in program order; entries
+ * outside the outputs are zero. A(m,j) at A[m*N+j]. */
+static void closed_lu(const onet_t* n, const double* y, const double* k, double hb, double* A) {
+  const int N = n->neq;
+  double* v = (double*)malloc(sizeof(double) * (size_t)(n->lc_nops > 0 ? n->lc_nops : 1));
+  for (int i = 0; i < n->lc_nops; ++i) {
+    const int a = n->lc_a[i], b = n->lc_b[i];
+    switch (n->lc_code[i]) {
+      case 0: v[i] = y[a]; break;
+      case 1: v[i] = k[a]; break;
+      case 2: v[i] = hb; break;
+      case 3: v[i] = n->lc_c[i]; break;
+      case 4: v[i] = v[a] + v[b]; break;
+      case 5: v[i] = v[a] - v[b]; break;
+      case 6: v[i] = v[a] * v[b]; break;
+      case 7: v[i] = v[a] / v[b]; break;
+      default: v[i] = -v[a]; break;
+    }
+  }
+  memset(A, 0, sizeof(double) * (size_t)N * N);
+  for (int o = 0; o < n->lc_nout; ++o) {
+    const int* e = n->lc_out + 3 * o;
+    A[e[0] * N + e[1]] = v[e[2]];
+  }
+  free(v);
+}
+
+/* GetJac (or CalcJac, LUmethod 3) + LUDecompose for all cells; GetLU for
+ * LUmethod 2 */
 static void factor_all(ovec_t* v, const double* y, double dt) {
   const onet_t* n = v->n;
   const int G = v->G, N = v->N;
@@ -533,6 +577,10 @@ static void factor_all(ovec_t* v, const double* y, double dt) {
   }
   for (int i = 0; i < G; ++i) {
     for (int j = 0; j < N; ++j) yc[j] = A2(y, j, i);
+    if (n->lu_method == 2 && !n->csc) {
+      closed_lu(n, yc, v->k + (size_t)i * (n->nrate > 0 ? n->nrate : 1), hb, v->LU + (size_t)i * N * N);
+      continue;
+    }
     if (Pfd) memcpy(P, Pfd + (size_t)i * N * N, sizeof(double) * (size_t)N * N);
     else oracle_p_dense(n, yc, v->k + (size_t)i * (n->nrate > 0 ? n->nrate : 1), hb, P);
     double* A = v->LU + (size_t)i * (n->csc ? n->nnz : N * N);

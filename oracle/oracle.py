@@ -29,7 +29,9 @@ class ONet(C.Structure):
                 ("t_row", _IP), ("t_coef", _DP), ("t_rate", _IP), ("t_nfac", _IP), ("t_fac", _IP),
                 ("perm", _IP), ("csc", C.c_int), ("nnz", C.c_int),
                 ("row_index", _IP), ("u_col_start", _IP), ("l_col_start", _IP),
-                ("arr_A", _DP), ("arr_Ea", _DP), ("lu_method", C.c_int)]
+                ("arr_A", _DP), ("arr_Ea", _DP), ("lu_method", C.c_int),
+                ("lc_nops", C.c_int), ("lc_code", _IP), ("lc_a", _IP), ("lc_b", _IP), ("lc_c", _DP),
+                ("lc_nout", C.c_int), ("lc_out", _IP)]
 
 
 class OCtl(C.Structure):
@@ -112,10 +114,19 @@ class Network:
         self.csc = ir["packaging"] == "CSC"
         self.nnz = ir["nnz"]
         self.param_defaults = np.array(ir.get("param_defaults") or [], dtype=np.float64)
+        lc = ir.get("lu_closed") or {"ops": [], "out": []}
+        ops = lc["ops"] or [[3, 0, 0, 0.0]]
+        self._lc_code = np.array([o[0] for o in ops], dtype=np.int32)
+        self._lc_a = np.array([o[1] for o in ops], dtype=np.int32)
+        self._lc_b = np.array([o[2] for o in ops], dtype=np.int32)
+        self._lc_c = np.array([o[3] for o in ops], dtype=np.float64)
+        self._lc_out = np.array(lc["out"] or [[0, 0, 0]], dtype=np.int32).reshape(-1)
         self.s = ONet(self.neq, self.nrate, self.nparam, RATE_KIND[ir["rate_kind"]], ir["tgas_index"], nt,
                       _ip(self._row), _dp(self._coef), _ip(self._rate), _ip(self._nfac), _ip(self._fac),
                       _ip(self._perm), 1 if self.csc else 0, self.nnz, _ip(self._ri), _ip(self._ucs),
-                      _ip(self._lcs), _dp(self._A), _dp(self._Ea), int(ir.get("lu_method", 1)))
+                      _ip(self._lcs), _dp(self._A), _dp(self._Ea), int(ir.get("lu_method", 1)),
+                      len(lc["ops"]), _ip(self._lc_code), _ip(self._lc_a), _ip(self._lc_b), _dp(self._lc_c),
+                      len(lc["out"]), _ip(self._lc_out))
 
     @classmethod
     def load(cls, net_id: str) -> "Network":

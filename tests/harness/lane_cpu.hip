@@ -16,6 +16,8 @@
 #include "generated/net_PRIMORDIAL_FF_CSC.h"
 #include "generated/net_ROBER_NONE_DENSE_LU3.h"
 #include "generated/net_PRIMORDIAL_NONE_DENSE_LU3.h"
+#include "generated/net_ROBER_NONE_DENSE_LU2.h"
+#include "generated/net_OREGO_NONE_DENSE_LU2.h"
 
 namespace {
 
@@ -105,6 +107,8 @@ extern "C" int lane_solve_cells(const char* id, int mixed, long n, const double*
   ODV_CASE(odv_net_PRIMORDIAL_FF_CSC)
   ODV_CASE(odv_net_ROBER_NONE_DENSE_LU3)
   ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE_LU3)
+  ODV_CASE(odv_net_ROBER_NONE_DENSE_LU2)
+  ODV_CASE(odv_net_OREGO_NONE_DENSE_LU2)
 #undef ODV_CASE
   return -1;
 }
@@ -131,6 +135,8 @@ extern "C" int lane_netfun(const char* id, int what, const double* y, const doub
   ODV_CASE(odv_net_PRIMORDIAL_FF_CSC)
   ODV_CASE(odv_net_ROBER_NONE_DENSE_LU3)
   ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE_LU3)
+  ODV_CASE(odv_net_ROBER_NONE_DENSE_LU2)
+  ODV_CASE(odv_net_OREGO_NONE_DENSE_LU2)
 #undef ODV_CASE
   return -1;
 }

@@ -18,6 +18,9 @@ NETS = {
     # LUmethod 3: Jacobian by forward differences of the RHS
     "ROBER/NONE/DENSE/LU3": lambda n: dict(W.rober(n), network="ROBER/NONE/DENSE/LU3"),
     "PRIMORDIAL/NONE/DENSE/LU3": lambda n: W.primordial(n, network="PRIMORDIAL/NONE/DENSE/LU3"),
+    # LUmethod 2: the L and U factors in closed form
+    "ROBER/NONE/DENSE/LU2": lambda n: dict(W.rober(n), network="ROBER/NONE/DENSE/LU2"),
+    "OREGO/NONE/DENSE/LU2": lambda n: dict(W.orego(n), network="OREGO/NONE/DENSE/LU2"),
 }
 
 

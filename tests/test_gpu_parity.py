@@ -57,6 +57,9 @@ CASES = {
     # LUmethod 3: the Jacobian by forward differences of the RHS
     "ROBER_LU3": lambda: dict(W.rober(32, seed=1), network="ROBER/NONE/DENSE/LU3"),
     "PRIMORDIAL_LU3": lambda: W.primordial(256, seed=3, network="PRIMORDIAL/NONE/DENSE/LU3"),
+    # LUmethod 2: the L and U factors in closed form
+    "ROBER_LU2": lambda: dict(W.rober(32, seed=1), network="ROBER/NONE/DENSE/LU2"),
+    "OREGO_LU2": lambda: dict(W.orego(512, seed=2), network="OREGO/NONE/DENSE/LU2"),
 }
 
 
@@ -97,7 +100,7 @@ def test_repro_nan_state_bitwise(oracle, case, mode):
 
 
 @pytest.mark.parametrize("case,G", [("ROBER", 32), ("OREGO", 64), ("PRIMORDIAL", 64), ("ROBER_LU3", 32),
-                                    ("PRIMORDIAL_LU3", 64)])
+                                    ("PRIMORDIAL_LU3", 64), ("ROBER_LU2", 32), ("OREGO_LU2", 64)])
 def test_repro_bitwise_lockstep(oracle, case, G):
     """Lockstep groups reproduce the reference at nvector = G (shared t_stop)."""
     w = CASES[case]()
@@ -459,12 +462,15 @@ def test_solveode_mask_per_cell(oracle):
     assert np.array_equal(y[:, on], yr[:, on])
 
 
-@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "OREGO/NONE/CSC", "PRIMORDIAL/CMK/CSC", "PRIMORDIAL/NONE/DENSE"])
+@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "OREGO/NONE/CSC", "PRIMORDIAL/CMK/CSC", "PRIMORDIAL/NONE/DENSE",
+                                 "ROBER/NONE/DENSE/LU2", "OREGO/NONE/DENSE/LU2"])
 def test_plugin_getrhs_getjac(oracle, nid):
     """The generated plugin's GetRHS / GetJac through the C ABI (evaluated
     on the GPU, +REPRO build) equal the oracle's RHS and permuted P bit for
     bit, in the reference's layouts (DENSE: all neq x neq entries; CSC: the
-    full LU pattern, fill entries 0; PRIMORDIAL's frozen species included)."""
+    full LU pattern, fill entries 0; PRIMORDIAL's frozen species included).
+    For a LUmethod 2 variant the same entry point returns GetLU's closed-form
+    factors in the DENSE layout."""
     from odevec_amd.api import Odevec
     w = {"ROBER": lambda: W.rober(40, seed=16), "OREGO": lambda: W.orego(40, seed=17),
          "PRIMORDIAL": lambda: W.primordial(40, seed=18)}[nid.split("/")[0]]()
@@ -481,6 +487,9 @@ def test_plugin_getrhs_getjac(oracle, nid):
         k = net.rates(w["y"][:, i])
         assert np.array_equal(f[:, i], net.rhs(w["y"][:, i], k))
         P = net.p_dense(w["y"][:, i], k, beta * dt)
+        if net.ir["lu_method"] == 2:
+            # GetLU: the closed-form factors (L below the diagonal, U on and above)
+            P = net.factor(w["y"][:, i], k, beta * dt).reshape(neq, neq)
         if net.csc:
             ref = np.array([P[r, c] for r, c in net.ir["lu_entries"]])
         else:

@@ -206,6 +206,113 @@ def test_lu_identity_and_solve(oracle, nid):
         assert np.allclose(P @ den[perm], res[perm], rtol=1e-10, atol=1e-10)
 
 
+@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE/LU2", "OREGO/NONE/DENSE/LU2"])
+def test_closed_form_lu_reproduces_p(oracle, nid):
+    """LUmethod 2 (GetLU): the oracle's closed-form factors -- unit L below
+    the diagonal, U on and above it -- multiply back to P_order, evaluated
+    independently term by term (p_dense), to within a few roundings of
+    |L| |U|, over stiff and non-stiff points (hb from 1e-6 to 1e4, ROBER rate
+    constants scattered over a factor e^3 around their defaults); a wrong
+    sign, index or dropped term fails by orders of magnitude. The factors
+    also agree with the numeric Doolittle factorization of the same P
+    (LUmethod 1) to rounding."""
+    net = _net(oracle, nid)
+    sym = _net(oracle, nid[:-4])
+    n = net.neq
+    assert net.ir["lu_method"] == 2
+    pat = {tuple(e) for e in net.ir["lu_entries"]}
+    assert {(r, c) for r, c, _ in net.ir["lu_closed"]["out"]} <= pat
+    rng = np.random.default_rng(9)
+    for _ in range(500):
+        y = 10.0 ** rng.uniform(-8, 0, n)
+        prm = net.param_defaults * np.exp(rng.uniform(-1.5, 1.5, net.nparam)) if net.nparam else None
+        k = net.rates(y, prm)
+        hb = 10.0 ** rng.uniform(-6, 4)
+        A = net.factor(y, k, hb).reshape(n, n)
+        Lm, Um = np.tril(A, -1) + np.eye(n), np.triu(A)
+        P = sym.p_dense(y, k, hb)
+        scale = np.abs(Lm) @ np.abs(Um)
+        assert (np.abs(Lm @ Um - P) <= 8 * 2.0 ** -52 * scale).all()
+        B = sym.factor(y, k, hb).reshape(n, n)
+        assert np.allclose(A, B, rtol=1e-13, atol=1e-15 * np.abs(B).max())
+        res = rng.standard_normal(n)
+        assert np.allclose(net.lin_solve(A.ravel(), res), sym.lin_solve(B.ravel(), res), rtol=1e-10, atol=1e-12)
+
+
+def test_closed_form_lu_program_is_the_symbolic_factorization():
+    """The program's expressions, evaluated in exact rational arithmetic at
+    rational points, equal the exact Doolittle factors of P_order there
+    (Fractions, no SymPy): the program is the factorization, not just close
+    to it."""
+    from fractions import Fraction
+    from odevec_amd.pre_odevec import lu_closed, networks, structure
+    for name in ("ROBER", "OREGO"):
+        st = structure.analyse(networks.load(name), "None", "DENSE")
+        prog = lu_closed.closed_form_lu(st)
+        rng = np.random.default_rng(4)
+        n = st.neq
+        for _ in range(3):
+            y = [Fraction(int(v), 97) for v in rng.integers(1, 500, n)]
+            k = [Fraction(int(v), 13) for v in rng.integers(1, 50, 3)]
+            hb = Fraction(int(rng.integers(1, 40)), 7)
+            # P = I - hb*J from the structure's terms (derivative of each monomial)
+            P = [[Fraction(int(r == c)) for c in range(n)] for r in range(n)]
+            for t in st.terms:
+                for j in set(t.factors):
+                    red = list(t.factors)
+                    red.remove(j)
+                    v = Fraction(t.coef) * t.factors.count(j) * (k[t.rate] if t.rate >= 0 else 1)
+                    for f in red:
+                        v *= y[f]
+                    P[t.row][j] -= hb * v
+            A = [row[:] for row in P]
+            for q in range(n - 1):
+                for i in range(q + 1, n):
+                    A[i][q] /= A[q][q]
+                    for j in range(q + 1, n):
+                        A[i][j] -= A[i][q] * A[q][j]
+            val = []
+            for code, a, b, c in prog["ops"]:
+                val.append([lambda: y[a], lambda: k[a], lambda: hb, lambda: Fraction(c),
+                            lambda: val[a] + val[b], lambda: val[a] - val[b], lambda: val[a] * val[b],
+                            lambda: val[a] / val[b], lambda: -val[a]][code]())
+            got = {(r, c): val[node] for r, c, node in prog["out"]}
+            for r in range(n):
+                for c in range(n):
+                    exact = A[r][c]
+                    if (r, c) in got:
+                        # constants are doubles: the only inexact inputs
+                        assert abs(got[(r, c)] - exact) <= Fraction(1, 10 ** 12) * (abs(exact) + 1), (name, r, c)
+                    else:
+                        assert exact == 0, (name, r, c)
+
+
+def test_lumethod2_solution_agrees_with_lumethod1(oracle):
+    """Solves with the closed-form factors stay within the tolerance of the
+    LUmethod 1 solves (only the rounding of the iteration matrix differs)
+    and converge to the Radau solutions like them."""
+    gold = json.load(open(os.path.join(GOLD, "radau_known_answers.json")))
+    for name in ("ROBER", "OREGO"):
+        case = gold["cases"][name]
+        net = _net(oracle, case["network"] + "/LU2")
+        for c in case["cells"]:
+            y0 = np.array(c["y0"])[:, None]
+            prm = None if c["params"] is None else np.array(c["params"])[:, None]
+            ref = np.array(c["y"])
+            big = np.abs(ref) > 1e-6 * np.abs(ref).max()
+            y, st, _, _ = net.solve_cells(y0, c["t_stop"], params=prm, rtol=1e-8, atol=1e-12, dt_min=1e-30)
+            assert st[0] == 0
+            assert np.max(np.abs(y[:, 0] - ref)[big] / np.abs(ref[big])) < 2e-6, (name, c["cell"])
+    from odevec_amd import workloads as W
+    w = W.rober(16, seed=3)
+    a = _net(oracle, "ROBER/NONE/DENSE").solve_cells(w["y"], w["t_stop"], params=w["params"], rtol=1e-6,
+                                                     atol=1e-10)
+    b = _net(oracle, "ROBER/NONE/DENSE/LU2").solve_cells(w["y"], w["t_stop"], params=w["params"], rtol=1e-6,
+                                                         atol=1e-10)
+    assert (a[1] == 0).all() and (b[1] == 0).all()
+    assert np.all(np.abs(a[0] - b[0]) <= 10 * 1e-6 * np.abs(a[0]) + 1e-9)
+
+
 def test_known_answers_radau(oracle):
     """The oracle converges to SciPy Radau (rtol 1e-12) solutions."""
     gold = json.load(open(os.path.join(GOLD, "radau_known_answers.json")))

@@ -112,8 +112,15 @@ def test_opcounts_primordial_cmk():
 def test_cli_flags(tmp_path):
     assert cli.main(["--example=ROBER", "--nvector=32", "--outdir", str(tmp_path)]) == 0
     assert os.path.exists(os.path.join(str(tmp_path), "net_ROBER_NONE_DENSE.h"))
+    with pytest.raises(SystemExit):   # closed-form LU: DENSE only
+        cli.main(["--example=ROBER", "--LUmethod=2", "--packaging=CSC", "--outdir", str(tmp_path)])
     with pytest.raises(SystemExit):
-        cli.main(["--example=ROBER", "--LUmethod=2", "--outdir", str(tmp_path)])
+        cli.main(["--example=ROBER", "--LUmethod=4", "--outdir", str(tmp_path)])
+    assert cli.main(["--example=ROBER", "--LUmethod=2", "--outdir", str(tmp_path)]) == 0
+    hdr = open(os.path.join(str(tmp_path), "net_ROBER_NONE_DENSE_LU2.h")).read()
+    assert "LU_METHOD = 2" in hdr and '"ROBER/NONE/DENSE/LU2"' in hdr
+    ir = json.load(open(os.path.join(str(tmp_path), "ROBER_NONE_DENSE_LU2.json")))
+    assert ir["lu_method"] == 2 and len(ir["lu_closed"]["out"]) == 8 and ir["opcounts"]["lu"]["flops"] == 0
     with pytest.raises(SystemExit):
         cli.main(["--example=ROBER", "--packaging=COO", "--outdir", str(tmp_path)])
     with pytest.raises(SystemExit):   # pre_odevec.py:455-456
