@@ -116,6 +116,11 @@ struct odevec_s {
   int* d_promo = nullptr;            // cells handed to the FP64 pass
   unsigned* d_promo_n = nullptr;     // their count
   int64_t promo_cap = 0;
+  double* d_ho = nullptr;            // their hand-off records (NetEntry::handoff_doubles per cell)
+  int64_t ho_cap = 0;
+  double* d_ws2 = nullptr;           // persistent mode with the fallback: the FP64 lanes' workspace
+  int64_t ws2_cap = 0;
+  unsigned* d_done = nullptr;        // ... and its count of finished queue wavefronts
   std::vector<int> h_status;
   std::vector<int> h_cnt;
 };
@@ -215,19 +220,44 @@ int launch(odevec_s* h, const Req& r) {
   odv::Plan p{};
   hipError_t e = h->net->plan(r.ncells, h->opts.mode, h->opts.precision, h->opts.block, h->opts.grid, grp, &p);
   if (e != hipSuccess) return hip_fail(e, "launch plan");
-  // mixed form with the FP64 fallback: cells that reach promote_cf convergence
-  // failures stop, keep their input state and are solved again from the start
-  // by the FP64 persistent kernel over the list of such cells (second launch)
-  const bool promote = h->promote_cf > 0 && h->opts.precision == ODEVEC_MIXED && !a.hist &&
+  // the FP64 fallback: cells that reach promote_cf convergence failures stop
+  // (their input state untouched, their integrator state in a hand-off
+  // record) and the FP64 persistent kernel continues them over the list of
+  // such cells (second launch); on an FP64 handle the cells only move
+  // between the two launches, with unchanged results
+  const bool promote = h->promote_cf > 0 && !a.hist &&
                        (h->opts.mode == ODEVEC_PERCELL || h->opts.mode == ODEVEC_PERSISTENT);
   // the fallback's list and the cost-ordered queue hold cell indices as int
   if ((promote || (h->opts.mode == ODEVEC_PERSISTENT && h->opts.schedule == 1)) && r.ncells > 0x7fffffffLL)
     return fail(ODEVEC_EINVAL, "the FP64 fallback and the cost-ordered queue take at most 2^31 - 1 cells per solve");
   odv::Plan p2{};
+  // persistent mode: the FP64 lanes run in the same launch (odv_k_persistent_fb),
+  // and the pass after it only takes what they left
+  const bool one_launch = promote && h->opts.mode == ODEVEC_PERSISTENT;
+  odv::Plan pf{};
+  int nprod = 0;
+  long long ws64 = 0;
   if (promote) {
     e = h->net->plan(r.ncells, ODEVEC_PERSISTENT, ODEVEC_FP64, h->opts.block, h->opts.grid, grp, &p2);
     if (e != hipSuccess) return hip_fail(e, "launch plan (FP64 pass)");
-    if (p2.ws_doubles > p.ws_doubles) p.ws_doubles = p2.ws_doubles;
+    if (one_launch) {
+      if ((e = h->net->plan_fb(r.ncells, h->opts.precision, &pf, &nprod, &ws64)) != hipSuccess)
+        return hip_fail(e, "launch plan (one-launch fallback)");
+      p = pf;
+      if (p2.ws_doubles > ws64) ws64 = p2.ws_doubles;
+      if (ws64 > h->ws2_cap) {
+        if (h->d_ws2) (void)hipFree(h->d_ws2);
+        h->d_ws2 = nullptr;
+        h->ws2_cap = 0;
+        if ((e = hipMalloc((void**)&h->d_ws2, (size_t)ws64 * 8)) != hipSuccess)
+          return fail(ODEVEC_ENOMEM, std::string("fallback workspace allocation failed: ") + hipGetErrorString(e));
+        h->ws2_cap = ws64;
+      }
+      if (!h->d_done && (e = hipMalloc((void**)&h->d_done, sizeof(unsigned))) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("fallback counter allocation failed: ") + hipGetErrorString(e));
+    } else if (p2.ws_doubles > p.ws_doubles) {
+      p.ws_doubles = p2.ws_doubles;
+    }
     if (r.ncells > h->promo_cap) {
       if (h->d_promo) (void)hipFree(h->d_promo);
       h->d_promo = nullptr;
@@ -238,6 +268,15 @@ int launch(odevec_s* h, const Req& r) {
     }
     if (!h->d_promo_n && (e = hipMalloc((void**)&h->d_promo_n, sizeof(unsigned))) != hipSuccess)
       return fail(ODEVEC_ENOMEM, std::string("fallback list allocation failed: ") + hipGetErrorString(e));
+    const int64_t need = r.ncells * (int64_t)h->net->handoff_doubles;
+    if (need > h->ho_cap) {
+      if (h->d_ho) (void)hipFree(h->d_ho);
+      h->d_ho = nullptr;
+      h->ho_cap = 0;
+      if ((e = hipMalloc((void**)&h->d_ho, (size_t)need * sizeof(double))) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("fallback record allocation failed: ") + hipGetErrorString(e));
+      h->ho_cap = need;
+    }
   }
   if (p.ws_doubles > h->ws_cap) {
     // hipFree waits for the device, so no earlier launch still reads the old workspace
@@ -271,10 +310,39 @@ int launch(odevec_s* h, const Req& r) {
       return hip_fail(e, "cost order");
     a.order = h->d_order;
   }
+  if (one_launch) {
+    a.ctl.promote_cf = h->promote_cf;
+    a.promo = h->d_promo;
+    a.promo_n = h->d_promo_n;
+    a.ho = h->d_ho;
+    a.ho_mode = 1;
+    // the FP64 lanes: the list as their queue, claimed through their own counter
+    odv::KArgs b = a;
+    b.ctl.promote_cf = 0;
+    b.promo = nullptr;
+    b.promo_n = nullptr;
+    b.order = h->d_promo;
+    b.qlen = h->d_promo_n;
+    b.ws = h->d_ws2;
+    b.ho_mode = 2;
+    h->qslot = (h->qslot + 1) % 64;
+    b.queue = h->d_queue + h->qslot;
+    e = h->net->launch_fb(a, b, h->d_done, nprod, h->opts.precision, pf, r.stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch (one-launch fallback)");
+    // what the FP64 lanes did not take (normally nothing): the FP64 pass
+    odv::KArgs a2 = b;
+    h->qslot = (h->qslot + 1) % 64;
+    a2.queue = h->d_queue + h->qslot;
+    e = h->net->launch(a2, ODEVEC_PERSISTENT, ODEVEC_FP64, p2, r.stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch (FP64 pass)");
+    return ODEVEC_OK;
+  }
   if (promote) {
     a.ctl.promote_cf = h->promote_cf;
     a.promo = h->d_promo;
     a.promo_n = h->d_promo_n;
+    a.ho = h->d_ho;
+    a.ho_mode = 1;
     hipLaunchKernelGGL(odv::k_setu, dim3(1), dim3(1), 0, r.stream, h->d_promo_n, 0u);
   }
   e = h->net->launch(a, h->opts.mode, h->opts.precision, p, r.stream);
@@ -286,6 +354,7 @@ int launch(odevec_s* h, const Req& r) {
     a2.promo_n = nullptr;
     a2.order = h->d_promo;
     a2.qlen = h->d_promo_n;
+    a2.ho_mode = 2;
     h->qslot = (h->qslot + 1) % 64;
     a2.queue = h->d_queue + h->qslot;
     e = h->net->launch(a2, ODEVEC_PERSISTENT, ODEVEC_FP64, p2, r.stream);
@@ -758,6 +827,9 @@ int32_t odevec_destroy(odevec_t* h) {
   if (h->d_order) (void)hipFree(h->d_order);
   if (h->d_promo) (void)hipFree(h->d_promo);
   if (h->d_promo_n) (void)hipFree(h->d_promo_n);
+  if (h->d_ho) (void)hipFree(h->d_ho);
+  if (h->d_ws2) (void)hipFree(h->d_ws2);
+  if (h->d_done) (void)hipFree(h->d_done);
   if (h->d_bin) (void)hipFree(h->d_bin);
   if (h->d_count) (void)hipFree(h->d_count);
   if (h->e0) (void)hipEventDestroy(h->e0);
@@ -934,6 +1006,8 @@ struct odevec_batch_s {
   int promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;   // after this many failed tests: FP64 fallback, or no test (0: neither)
   int* d_promo = nullptr;          // handed-back cells of every mixed segment, one region per segment
   int64_t promo_cap = 0;
+  double* d_ho = nullptr;          // their hand-off records, one region per segment
+  int64_t ho_cap = 0;
   unsigned* d_promo_n = nullptr;   // MAXSEG counts
   double* d_ws2 = nullptr;         // workspace of the FP64 passes
   int64_t ws2_cap = 0;
@@ -974,6 +1048,7 @@ int batch_fallback_passes(odevec_batch_s* b, const odevec_segment_t* segs, int n
     a2.ctl.promote_cf = 0;
     a2.order = a2.promo;
     a2.qlen = a2.promo_n;
+    a2.ho_mode = 2;
     a2.promo = nullptr;
     a2.promo_n = nullptr;
     a2.ws = b->d_ws2;
@@ -1095,8 +1170,8 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
   // promote_cf convergence failures; after the batch launch, the network's
   // FP64 persistent kernel solves each segment's handed-back cells
   std::vector<const odv::NetEntry*> fbnet(nseg, nullptr);
-  std::vector<int64_t> fboff(nseg, 0);
-  int64_t fbtotal = 0;
+  std::vector<int64_t> fboff(nseg, 0), hooff(nseg, 0);
+  int64_t fbtotal = 0, hototal = 0;
   if (b->promote_cf > 0) {
     for (int k = 0; k < nseg; ++k) {
       const odevec_segment_t& g = segs[k];
@@ -1108,6 +1183,8 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       fbnet[k] = ne;
       fboff[k] = fbtotal;
       fbtotal += g.ncells;
+      hooff[k] = hototal;
+      hototal += g.ncells * (int64_t)ne->handoff_doubles;
     }
     if (fbtotal > b->promo_cap) {
       if (b->d_promo) (void)hipFree(b->d_promo);
@@ -1120,6 +1197,14 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     if (fbtotal > 0 && !b->d_promo_n &&
         (he = hipMalloc((void**)&b->d_promo_n, odv::MAXSEG * sizeof(unsigned))) != hipSuccess)
       return fail(ODEVEC_ENOMEM, std::string("fallback list allocation failed: ") + hipGetErrorString(he));
+    if (hototal > b->ho_cap) {
+      if (b->d_ho) (void)hipFree(b->d_ho);
+      b->d_ho = nullptr;
+      b->ho_cap = 0;
+      if ((he = hipMalloc((void**)&b->d_ho, (size_t)hototal * sizeof(double))) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("fallback record allocation failed: ") + hipGetErrorString(he));
+      b->ho_cap = hototal;
+    }
   }
   // the previous call's table copy must have left the staging buffer
   if (b->ev_pending && (he = hipEventSynchronize(b->ev_copied)) != hipSuccess) return hip_fail(he, "batch staging");
@@ -1155,6 +1240,8 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       a.ctl.promote_cf = b->promote_cf;
       a.promo = b->d_promo + fboff[k];
       a.promo_n = b->d_promo_n + k;
+      a.ho = b->d_ho + hooff[k];
+      a.ho_mode = 1;
     }
     d.kind = kind[k];
   }
@@ -1273,7 +1360,7 @@ int32_t odevec_batch_destroy(odevec_batch_t* b) {
   if (b->h_segs) (void)hipHostFree(b->h_segs);
   if (b->h_nseg) (void)hipHostFree(b->h_nseg);
   void* ps[] = {b->d_segs, b->d_nseg, b->d_queues, b->d_scal, b->d_ws, b->d_promo, b->d_promo_n, b->d_ws2,
-                b->d_order, b->d_bin, b->d_count};
+                b->d_order, b->d_bin, b->d_count, b->d_ho};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (b->ev_copied) (void)hipEventDestroy(b->ev_copied);

@@ -679,14 +679,12 @@ ODV_HD inline void odv_rhs(const L& s, const double* y, double* f) {
 // Start of SolveODE for one lane (src/odevec.F90:170-204). `y` is the caller's
 // state, `dt_hint` the caller's dt (it only scales y_NS(:,:,1), Q2). The
 // external columns (KS > 0) must be set before the call.
-template <class L, class G>
-ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, double t0, double tstop,
-                             double dt_hint, const Controls& c, const G& g) {
+// What a cell's solve takes from its input state once: the frozen species'
+// contributions to the reductions (s.fz bits 0..2) and the rate coefficients
+// (computed once per cell from its input state and parameters).
+template <class L>
+ODV_HD inline void odv_cell_constants(L& s, const double* yfull, const double* prm, const Controls& c) {
   using Net = typename L::Net;
-  constexpr int N = L::N;
-  double y[N];
-#pragma unroll
-  for (int j = 0; j < N; ++j) y[j] = yfull[SpeciesMap<Net>::lane(j)];
   // frozen species: constant values, constant weights, zero corrections
   s.fz = 0;
   {
@@ -706,6 +704,24 @@ ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, doubl
       if (yfull[j] < 0.0) s.fz |= 4;
     }
   }
+  double kk[Net::NRATE];
+  Net::rates(yfull, prm, kk);
+#pragma unroll
+  for (int r = 0; r < Net::NRATE; ++r) {
+    if (L::KS == 0) s.kl[r] = kk[r];
+    else s.kx[r * L::KS] = kk[r];
+  }
+}
+
+template <class L, class G>
+ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, double t0, double tstop,
+                             double dt_hint, const Controls& c, const G& g) {
+  using Net = typename L::Net;
+  constexpr int N = L::N;
+  double y[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) y[j] = yfull[SpeciesMap<Net>::lane(j)];
+  odv_cell_constants(s, yfull, prm, c);
   s.first = true;
   odv_set_order(s, 1);
   s.swu = 0;
@@ -721,15 +737,6 @@ ODV_HD inline void odv_begin(L& s, const double* yfull, const double* prm, doubl
   s.t = t0;
   s.tstop = tstop;
   s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
-  {
-    double kk[Net::NRATE];
-    Net::rates(yfull, prm, kk);
-#pragma unroll
-    for (int r = 0; r < Net::NRATE; ++r) {
-      if (L::KS == 0) s.kl[r] = kk[r];
-      else s.kx[r * L::KS] = kk[r];
-    }
-  }
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     s.en[j] = 0.0;
@@ -947,13 +954,6 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
       s.t = told;
       s.fc++;
       s.cnt.convfail++;
-      // the FP64 fallback counts the failures of the iteration itself and
-      // the failed invariant tests separately
-      if (c.promote_cf > 0 && (s.cnt.convfail - s.cnt.invfail >= c.promote_cf ||
-                               (c.promote_inv > 0 && s.cnt.invfail >= c.promote_inv))) {
-        s.status = ST_PROMOTE;
-        return true;
-      }
       if (s.fc > 10) {
         s.status = ST_CONV_FAIL;
         return true;
@@ -965,6 +965,15 @@ ODV_HD inline bool odv_attempt(L& s, const Controls& c, const G& g) {
         return true;
       }
       s.upd_jac = true;
+      // the FP64 fallback (counting the failures of the iteration itself and
+      // the failed invariant tests separately): the cell leaves the launch in
+      // the state this failure leaves it in -- the next attempt, with a fresh
+      // Jacobian -- and the FP64 pass continues it from there
+      if (c.promote_cf > 0 && (s.cnt.convfail - s.cnt.invfail >= c.promote_cf ||
+                               (c.promote_inv > 0 && s.cnt.invfail >= c.promote_inv))) {
+        s.status = ST_PROMOTE;
+        return true;
+      }
       ODV_BLOCK_SPLIT()
       return false;
     }

@@ -60,6 +60,11 @@ struct KArgs {
   int* promo;             // ctl.promote_cf > 0: cells stopped with ST_PROMOTE are appended here ...
   unsigned* promo_n;      // ... at promo[atomicAdd(promo_n, 1)]; their y, status, counters are not written
   const unsigned* qlen;   // persistent: number of queue positions (device count), or null = ncells
+  // FP64 fallback hand-off records, Handoff<Net>::REC doubles per cell:
+  // ho_mode 1 -- a cell stopped with ST_PROMOTE leaves its state in its
+  // record; ho_mode 2 -- the cells of the queue continue from theirs (FP64)
+  double* ho;
+  int ho_mode;
   Controls ctl;
   int group;              // lockstep group size
 };
@@ -176,6 +181,106 @@ struct Hist {
   static constexpr int K = A + Net::NNZ;     // rate coefficients
   static constexpr int PLANES = K + Net::NRATE;
 };
+
+// Hand-off record of the mixed form's FP64 fallback (no counterpart in the
+// reference): the integrator state in which a cell's convergence failure
+// left it when it reached the fallback's limit -- Nordsieck history already
+// reset, step size cut, a Jacobian refresh pending -- from which the FP64
+// pass continues the solve; cell i at ho[i*REC ...]. The rates and the
+// frozen species' flags are recomputed from the cell's input state, which a
+// handed-back cell leaves untouched, and the LU factors are not kept (the
+// next attempt refreshes them). The event counters travel with the cell.
+template <class Net>
+struct Handoff {
+  static constexpr int N = Net::NACT;
+  static constexpr int T = 0, DT = 1, RATE = 2, JCH = 3, CERR = 4;
+  static constexpr int INTS = 5, NINT = 7;   // q, swu, lju, fet, fc, flags, failed invariant tests
+  static constexpr int CNT = INTS + NINT;    // the 8 event counters
+  static constexpr int Z = CNT + 8;          // Nordsieck history, column-major (c*N + j)
+  static constexpr int W = Z + NCOL * N;     // error weights
+  static constexpr int EO = W + N;           // previous accepted correction
+  static constexpr int REC = EO + N;
+};
+
+template <class L>
+__device__ inline void odv_handoff_save(const KArgs& a, long long i, const L& s) {
+  using H = Handoff<typename L::Net>;
+  double* h = a.ho + i * H::REC;
+  h[H::T] = s.t;
+  h[H::DT] = s.dt;
+  h[H::RATE] = s.rate;
+  h[H::JCH] = s.jchange;
+  h[H::CERR] = s.cerr;
+  h[H::INTS] = (double)s.q;
+  h[H::INTS + 1] = (double)s.swu;
+  h[H::INTS + 2] = (double)s.lju;
+  h[H::INTS + 3] = (double)s.fet;
+  h[H::INTS + 4] = (double)s.fc;
+  h[H::INTS + 5] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0));
+  h[H::INTS + 6] = (double)s.cnt.invfail;
+  const Counters& c = s.cnt;
+  const int cv[8] = {c.attempts, c.steps, c.newton, c.jac, c.errfail, c.convfail, c.stepsel, c.qsum};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) h[H::CNT + k] = (double)cv[k];
+#pragma unroll
+  for (int c2 = 0; c2 < NCOL; ++c2)
+#pragma unroll
+    for (int j = 0; j < H::N; ++j) h[H::Z + c2 * H::N + j] = s.z(c2, j);
+#pragma unroll
+  for (int j = 0; j < H::N; ++j) {
+    h[H::W + j] = s.w(j);
+    h[H::EO + j] = s.eo(j);
+  }
+}
+
+// the counters a handed-back cell brought from the launch that stopped it
+template <class Net>
+__device__ inline Counters odv_handoff_counters(const KArgs& a, long long i) {
+  using H = Handoff<Net>;
+  const double* h = a.ho + i * H::REC + H::CNT;
+  return Counters{(int)h[0], (int)h[1], (int)h[2], (int)h[3], (int)h[4], (int)h[5], (int)h[6], (int)h[7], 0};
+}
+
+template <class L>
+__device__ inline void odv_handoff_resume(const KArgs& a, long long i, L& s, const double* yfull,
+                                          const double* prm, double ts) {
+  using H = Handoff<typename L::Net>;
+  const double* h = a.ho + i * H::REC;
+  odv_cell_constants(s, yfull, prm, a.ctl);
+  s.t = h[H::T];
+  s.dt = h[H::DT];
+  s.rate = h[H::RATE];
+  s.jchange = h[H::JCH];
+  s.cerr = h[H::CERR];
+  odv_set_order(s, (int)h[H::INTS]);
+  s.swu = (int)h[H::INTS + 1];
+  s.lju = (int)h[H::INTS + 2];
+  s.fet = (int)h[H::INTS + 3];
+  s.fc = (int)h[H::INTS + 4];
+  const int fl = (int)h[H::INTS + 5];
+  s.first = (fl & 1) != 0;
+  s.upd_jac = (fl & 2) != 0;
+  s.neglect_up = (fl & 4) != 0;
+  s.cnt = odv_handoff_counters<typename L::Net>(a, i);
+  s.cnt.invfail = (int)h[H::INTS + 6];
+  s.status = ST_RUNNING;
+  s.tstop = ts;
+#pragma unroll
+  for (int c = 0; c < NCOL; ++c)
+#pragma unroll
+    for (int j = 0; j < H::N; ++j) s.set_z(c, j, h[H::Z + c * H::N + j]);
+  bool nan = false;
+#pragma unroll
+  for (int j = 0; j < H::N; ++j) {
+    const double wj = h[H::W + j];
+    s.set_w(j, wj);
+    nan = nan || (wj != wj);
+    s.set_eo(j, h[H::EO + j]);
+    s.en[j] = 0.0;
+  }
+  // weights held in registers carry their NaN flag (as odv_weights sets it)
+  if constexpr (!L::WX_) s.fz = (s.fz & ~8) | (nan ? 8 : 0);
+}
 
 template <class L>
 __device__ inline bool odv_hist_resume(const KArgs& a, long long i, L& s, const double* y, const double* prm,
@@ -307,6 +412,14 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
       return true;
     }
   }
+  if constexpr (sizeof(typename L::M) == sizeof(double)) {
+    // the FP64 fallback's pass: the cell continues where the launch that
+    // handed it back stopped it
+    if (a.ho_mode == 2 && a.ho) {
+      odv_handoff_resume(a, i, s, y, prm, ts);
+      return false;
+    }
+  }
   LaneGroup g;
   odv_begin(s, y, prm, t0, ts, dh, a.ctl, g);
   return false;
@@ -316,8 +429,13 @@ template <class L, bool HIST = false>
 __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   if (a.times) a.times[a.ncells + i] = (long long)wall_clock64();
   if (s.status == ST_PROMOTE) {
-    // the caller's state stays untouched for the FP64 pass over promo[]
-    a.promo[atomicAdd(a.promo_n, 1u)] = (int)i;
+    // the caller's state stays untouched for the FP64 pass over promo[],
+    // which continues the cell from its hand-off record
+    if (a.ho) odv_handoff_save(a, i, s);
+    // the entry is published after the record (an FP64 lane of the same
+    // launch may be waiting for it)
+    const unsigned k = atomicAdd(a.promo_n, 1u);
+    __hip_atomic_store(&a.promo[k], (int)i, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   if (s.status == ST_SKIPPED) {
@@ -373,20 +491,22 @@ __global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
 // leaves. `lds` holds the workgroup's wavefront slices,
 // LdsPolicy<Net, M>::WORDS doubles each.
 template <class Net, class M, bool RCP, bool HIST = false>
-__device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
+__device__ inline void odv_persistent_run(const KArgs& a, double* lds, long long nl = 0) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
-  const long long nlanes = (long long)gridDim.x * blockDim.x;
+  // lanes taking part (the first nl of the grid; 0: all)
+  const long long nlanes = nl > 0 ? nl : (long long)gridDim.x * blockDim.x;
   const long long gl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const long long nq = a.qlen ? (long long)*a.qlen : a.ncells;   // queue positions
   const long long p0 = a.order ? (long long)lane * (nlanes >> 6) + (gl >> 6) : gl;
   long long i = p0 < nq ? (a.order ? (long long)a.order[p0] : p0) : a.ncells;
+  // a list entry below 0: a cell already taken elsewhere (the fallback's list)
   L s;
   odv_bind_columns<P>(s, a, lds);
   LaneGroup g;
   double fl64 = 0.0, fl32 = 0.0;
-  bool have = p0 < nq;
+  bool have = p0 < nq && i >= 0;
   bool done = true;
   bool exhausted = nlanes >= nq;   // wave-uniform: no cell left to hand out
   if (have) done = odv_load_begin<L, HIST>(a, i, s);
@@ -396,7 +516,14 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
       odv_finish<L, HIST>(a, i, s);
       if (a.flops && s.status != ST_SKIPPED) {
         double f64, f32;
-        odv_flops<L>(s.cnt, f64, f32);
+        Counters c = s.cnt;
+        if (a.ho_mode == 2 && a.ho) {
+          // a continued cell: count the work of this launch only
+          const Counters b = odv_handoff_counters<Net>(a, i);
+          c = Counters{c.attempts - b.attempts, c.steps - b.steps, c.newton - b.newton, c.jac - b.jac,
+                       c.errfail - b.errfail, c.convfail - b.convfail, c.stepsel - b.stepsel, c.qsum - b.qsum, 0};
+        }
+        odv_flops<L>(c, f64, f32);
         fl64 += f64;
         fl32 += f32;
       }
@@ -418,8 +545,10 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
         const long long ni = (long long)base + rank;
         if (ni < nq) {
           i = a.order ? (long long)a.order[ni] : ni;
-          have = true;
-          done = odv_load_begin<L, HIST>(a, i, s);
+          if (i >= 0) {
+            have = true;
+            done = odv_load_begin<L, HIST>(a, i, s);
+          }
         }
       }
     }
@@ -434,6 +563,137 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds) {
       atomicAdd(&a.flops[1], fl32);
     }
   }
+}
+
+// The FP64 fallback inside the launch: cells handed back by the wavefronts
+// of the mixed persistent loop (their list b.order, filled in hand-back
+// order, entries -1 until written; count *b.qlen) are continued from their
+// hand-off records by FP64 lanes while the mixed loop still runs. Each lane
+// claims the next list position (b.queue, one atomic per wavefront), waits
+// until the position's entry is written, marks the entry taken (-2) and
+// runs the cell. A claim past the final count (known once all `nprod`
+// mixed wavefronts have counted themselves in *done) is void; a wavefront
+// with nothing to run sleeps between looks at the list, and one that has
+// waited for `idle_ticks` of the device wall clock drops its pending claims
+// and leaves -- the entries it did not take stay in the list for a pass
+// after the launch, so a wait can never lose a cell. Columns: b.ws (a
+// workspace for the FP64 layout) and the wavefront's LDS slice.
+template <class Net>
+__device__ inline void odv_fb_consume(const KArgs& b, double* lds, const unsigned* done, unsigned nprod,
+                                      long long idle_ticks) {
+  using P = LdsPolicy<Net, double>;
+  using L = PerCellLane<Net, double, false>;
+  const int lane = threadIdx.x & 63;
+  int* list = const_cast<int*>(b.order);
+  L s;
+  odv_bind_columns<P>(s, b, lds);
+  LaneGroup g;
+  double fl64 = 0.0, fl32 = 0.0;
+  long long i = 0;
+  bool have = false, fin = true;
+  long long pos = -1;      // claimed list position; -1 none, -2 void
+  bool closed = false;     // wave-uniform: claim no more
+  long long idle0 = (long long)wall_clock64();
+  while (true) {
+    if (have && fin) {
+      odv_finish<L>(b, i, s);
+      if (b.flops && s.status != ST_SKIPPED) {
+        const Counters c0 = odv_handoff_counters<Net>(b, i);
+        const Counters& c = s.cnt;
+        const Counters d{c.attempts - c0.attempts, c.steps - c0.steps, c.newton - c0.newton, c.jac - c0.jac,
+                         c.errfail - c0.errfail, c.convfail - c0.convfail, c.stepsel - c0.stepsel,
+                         c.qsum - c0.qsum, 0};
+        double f64, f32;
+        odv_flops<L>(d, f64, f32);
+        fl64 += f64;
+        fl32 += f32;
+      }
+      have = false;
+    }
+    const bool need = !have && pos == -1;
+    const unsigned long long ball = __ballot(need);
+    if (ball && !closed) {
+      const int cnt = __popcll(ball);
+      const int rank = __popcll(ball & ((1ull << lane) - 1ull));
+      const int leader = __ffsll((long long)ball) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(b.queue, (unsigned long long)cnt);
+      base = __shfl(base, leader, 64);
+      if (need) pos = (long long)base + rank < b.ncells ? (long long)base + rank : -2;
+    }
+    // look at the claimed entries (relaxed loads; the acquire fence only
+    // once something has arrived, before its record is read)
+    int v = -1;
+    bool all_in = false;
+    if (pos >= 0) {
+      v = __hip_atomic_load(&list[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (v < 0) all_in = __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nprod;
+    }
+    if (__ballot(v >= 0 || all_in)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (pos >= 0) {
+      if (v < 0 && all_in) {
+        // every mixed wavefront has finished: an entry below the final count
+        // is written by now, one at or past it never will be (void claim)
+        v = __hip_atomic_load(&list[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v < 0 && pos >= (long long)__hip_atomic_load(b.qlen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+          pos = -2;
+      }
+      if (v >= 0) {
+        __hip_atomic_store(&list[pos], -2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        i = v;
+        have = true;
+        fin = odv_load_begin<L>(b, i, s);
+        pos = -1;
+      }
+    }
+    // claims are handed out in order: once one is void, all later ones are
+    if (__ballot(pos == -2)) closed = true;
+    if (pos == -2) pos = -3;   // settled
+    const bool running = have && !fin;
+    if (__ballot(running)) {
+      idle0 = (long long)wall_clock64();
+      if (running) fin = odv_attempt(s, b.ctl, g);
+    } else {
+      if (closed && __ballot(pos >= 0) == 0ull && __ballot(have) == 0ull) break;
+      if ((long long)wall_clock64() - idle0 > idle_ticks) {
+        closed = true;    // leave the pending claims to the pass after the launch
+        if (pos >= 0) pos = -3;
+      }
+      __builtin_amdgcn_s_sleep(4);
+    }
+  }
+  if (b.flops) {
+    fl64 = odv_wave_sum(fl64);
+    fl32 = odv_wave_sum(fl32);
+    if (lane == 0) {
+      atomicAdd(&b.flops[0], fl64);
+      atomicAdd(&b.flops[1], fl32);
+    }
+  }
+}
+
+// Mixed (or FP64) persistent loop with its FP64 fallback in the same launch:
+// the first nprod wavefronts run the persistent loop over a's queue, handing
+// cells back (a.ctl.promote_cf, a.ho_mode 1), then join the last wavefronts,
+// which continue handed-back cells from the start (odv_fb_consume, args b).
+// One wavefront per workgroup: both layouts use the slice at the LDS base.
+template <class Net, class M, bool RCP>
+__global__ void __launch_bounds__(WAVE) odv_k_persistent_fb(KArgs a, KArgs b, unsigned* done, unsigned nprod,
+                                                            long long idle_ticks) {
+  if (blockIdx.x < nprod) {
+    odv_persistent_run<Net, M, RCP>(a, odv_lane_lds, (long long)nprod * WAVE);
+    // the wavefront's hand-backs (list entries and records) are written
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  odv_fb_consume<Net>(b, odv_lane_lds, done, nprod, idle_ticks);
+}
+
+// Fill n ints with v (the fallback list before a one-launch solve).
+template <int UNUSED = 0>
+__global__ void k_fill_i(int* p, long long n, int v) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
 }
 
 // With a.order and a.qlen the queue is the first *a.qlen entries of a.order
@@ -500,10 +760,14 @@ __global__ void k_get_jac(KArgs a, double hb, double* jac) {
 // Proxy for a cell's cost: log2((t_stop - t0)/h0), h0 the reference's
 // initial step size (CalcInitialStepSize, src/odevec.F90:216-238): a stiff
 // cell over a long span needs many steps (correlation 0.89 with the log of
-// the attempts on that workload). The cells are binned by the proxy (4 bins
-// per octave, costliest bin first) and counting-sorted into `order`; within
-// a bin the order is arbitrary. Only the order in which cells start
-// changes: every cell's arithmetic is the same.
+// the attempts on that workload). The cells are binned by the proxy (2 bins
+// per octave over 0..127 octaves, costliest bin first; that workload's
+// proxies span 10..128 octaves, and with 4 bins per octave, saturating at 63,
+// half of its cells -- among them every cell the mixed form hands to its
+// FP64 fallback -- shared the top bin in arbitrary order) and
+// counting-sorted into `order`; within a bin the order is arbitrary. Only
+// the order in which cells start changes: every cell's arithmetic is the
+// same.
 constexpr int COST_BINS = 256;
 
 template <int UNUSED = 0>
@@ -546,7 +810,7 @@ __global__ void k_cost_bin(KArgs a, unsigned char* bin, unsigned* count) {
     const double W0 = fmax(fabs(t0), fabs(ts));
     const double h0 = fmin(sqrt(tol / (1.0 / (W0 * W0) + acc / NF)), ts - t0);
     const double q = log2((ts - t0) / h0);
-    const int qb = (q == q) ? (int)fmin(fmax(q * 4.0, 0.0), (double)(COST_BINS - 2)) : 0;
+    const int qb = (q == q) ? (int)fmin(fmax(q * 2.0, 0.0), (double)(COST_BINS - 2)) : 0;
     b = COST_BINS - 2 - qb;
   }
   if (in) {

@@ -155,10 +155,11 @@ def test_batch_rejects_entries_not_in_bundle():
 
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_batch_fp64_fallback(use_graph):
-    """Mixed segments with the FP64 fallback: handed-back cells carry the
-    FP64 form's result bit for bit (the single-network FP64 per-cell kernel
-    on the same cells), the others the batch's plain mixed result; an FP64
-    segment in the same batch is unaffected; graph replays agree."""
+    """Mixed segments with the FP64 fallback: every cell carries the result
+    of the single-network mixed kernel with the same fallback bit for bit
+    (handed-back cells continued in FP64 from their hand-off records); cells
+    below the limit the batch's plain mixed result; an FP64 segment in the
+    same batch is unaffected; graph replays agree."""
     from odevec_amd.api import Batch, Odevec
     import torch
     pri = W.primordial(1500, seed=21)
@@ -181,11 +182,14 @@ def test_batch_fp64_fallback(use_graph):
         fb2 = run("MIX3", segs2, batch=b)
         assert all(np.array_equal(x, y) for p, q in zip(fb, fb2) for x, y in zip(p, q))
     b.close()
-    # FP64 reference for the same cells: the single-network per-cell kernel
+    # the same cells through a handle: the single-network mixed per-cell
+    # kernel with the same fallback
     n = pri["y"].shape[1]
-    ode = Odevec(pri["network"], n, precision=0, mode=_lib.PERCELL)
+    ode = Odevec(pri["network"], n, precision=1, mode=_lib.PERCELL)
     ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = pri["rtol"], pri["atol"], pri["dt_min"]
-    dev = to_dev(pri, _lib.FP64)
+    ode.set_fp64_fallback(limit)
+    ode.set_invariant_test(0.0, 0)
+    dev = to_dev(pri, _lib.MIXED)
     torch.cuda.synchronize()
     ode.solve_cells_async(n, dev["t_stop"].data_ptr(), dev["y"].data_ptr(), t0_ptr=dev["t0"].data_ptr(),
                           dt0_ptr=dev["dt0"].data_ptr(), status_ptr=dev["status"].data_ptr(),
@@ -199,8 +203,8 @@ def test_batch_fp64_fallback(use_graph):
     prom = cm[5] >= limit
     say("handed back", int(prom.sum()), "of", n)
     assert 10 <= prom.sum() <= n // 2
-    assert np.array_equal(yf[:, prom], yd[:, prom]) and np.array_equal(sf[prom], sd[prom])
-    assert np.array_equal(df[prom], dd[prom]) and np.array_equal(cf[:, prom], cd[:, prom])
+    assert np.array_equal(yf, yd) and np.array_equal(sf, sd)
+    assert np.array_equal(df, dd) and np.array_equal(cf, cd)
     keep = ~prom
     assert np.array_equal(yf[:, keep], ym[:, keep]) and np.array_equal(sf[keep], sm[keep])
     assert np.array_equal(df[keep], dm[keep]) and np.array_equal(cf[:, keep], cm[:, keep])
