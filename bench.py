@@ -36,13 +36,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 NCELLS = 1048576
-# mixed form: cells reaching this many convergence failures are solved again in
-# FP64 (odevec_set_fp64_fallback); on the headline workload 128, 256, 512 and
-# 1024 gave 221, 232, 255 and 281 ms per solve against 505 ms without
+# mixed form: cells reaching this many convergence failures (or 32 failed
+# invariant tests) continue in FP64 from where they stopped, in the same launch
+# (odevec_set_fp64_fallback); on the headline workload limits 128 / 64 / 32
+# (invariant tests 32 / 16 / 8) gave 119-121 / 121-124 / 132-137 ms per solve
 FALLBACK = 128
-# persistent queue in cost order (odevec_opts_t.schedule = 1): with the
-# fallback the launch is bound by its bulk, and starting the costliest cells
-# first took it from 212 to 202 ms per solve (results unchanged)
+# persistent queue in cost order (odevec_opts_t.schedule = 1): starting the
+# costliest cells first -- among them every cell the fallback takes -- keeps
+# the launch bound by its bulk (results unchanged)
 SCHEDULE = 1
 PEAK_FP64 = 78.6e12     # MI355X vector/matrix FP64, FLOP/s
 PEAK_FP32 = 157.3e12    # MI355X FP32, FLOP/s
@@ -484,7 +485,8 @@ def main():
         "config": {"workload": "PRIMORDIAL 16-species, CSC+CMK, %d cells, per-cell t_stop logU[1e8,1e14] s, rtol 1e-6, "
                                "mixed precision (FP32 P/LU/solves, FP64 residual/history/control), persistent "
                                "work-compacting kernel with the queue in cost order; cells reaching %d "
-                               "convergence failures are solved again in FP64 (second launch)" % (a.cells, FALLBACK),
+                               "convergence failures continue in FP64 from where they stopped, in the same launch "
+                               "(1 in 32 wavefronts starts as FP64 lanes)" % (a.cells, FALLBACK),
                    "cells": a.cells, "rtol": 1e-6, "atol": 1e-20, "network": w["network"],
                    "fp64_fallback_convfail": FALLBACK, "schedule": SCHEDULE},
         "fp64_plain": {"value": value_d, "unit": "cell-solves/s", "ms_per_step": wall_d / a.steps * 1e3,
@@ -499,7 +501,7 @@ def main():
                      "frac": frac, "traffic": read_traffic(),
                      "note": "FP64/FP32 vector-ALU bound (no MFMA: per-cell systems are tiny sparse LU "
                              "solves; neither hbm nor mfma applies); achieved = FP64 + FP32*peak64/peak32 algorithmic "
-                             "flops per solve (op-count model; the mixed kernel and the FP64 pass) / mean solve "
+                             "flops per solve (op-count model; mixed and FP64 lanes of the launch) / mean solve "
                              "time (HIP events around both launches); traffic: the mixed kernel's HBM bytes per "
                              "launch; no MFMA is used"},
         "flops_per_launch": {"fp64": float(fl_m[0]), "fp32": float(fl_m[1])},

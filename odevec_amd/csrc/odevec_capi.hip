@@ -25,6 +25,10 @@ std::vector<const BundleEntry*>& bundles() {
 
 __global__ void k_setq(unsigned long long* q, unsigned long long v) { *q = v; }
 __global__ void k_setu(unsigned* p, unsigned v) { *p = v; }
+__global__ void k_fill_int(int* p, long long n, int v) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
 __global__ void k_zero_u(unsigned* p, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0u;
@@ -118,9 +122,11 @@ struct odevec_s {
   int64_t promo_cap = 0;
   double* d_ho = nullptr;            // their hand-off records (NetEntry::handoff_doubles per cell)
   int64_t ho_cap = 0;
-  double* d_ws2 = nullptr;           // persistent mode with the fallback: the FP64 lanes' workspace
+  double* d_ws2 = nullptr;           // persistent mode with the fallback: the FP64 lanes' workspace,
   int64_t ws2_cap = 0;
-  unsigned* d_done = nullptr;        // ... and its count of finished queue wavefronts
+  unsigned* d_done = nullptr;        // the count of finished queue wavefronts,
+  hipStream_t fb_stream = nullptr;   // the stream the FP64 lanes run on beside the queue kernel,
+  hipEvent_t fb_fork = nullptr, fb_join = nullptr;   // and its fork / join with the caller's stream
   std::vector<int> h_status;
   std::vector<int> h_cnt;
 };
@@ -231,19 +237,22 @@ int launch(odevec_s* h, const Req& r) {
   if ((promote || (h->opts.mode == ODEVEC_PERSISTENT && h->opts.schedule == 1)) && r.ncells > 0x7fffffffLL)
     return fail(ODEVEC_EINVAL, "the FP64 fallback and the cost-ordered queue take at most 2^31 - 1 cells per solve");
   odv::Plan p2{};
-  // persistent mode: the FP64 lanes run in the same launch (odv_k_persistent_fb),
-  // and the pass after it only takes what they left
-  const bool one_launch = promote && h->opts.mode == ODEVEC_PERSISTENT;
-  odv::Plan pf{};
+  // persistent mode: the FP64 lanes run beside the queue kernel, on a stream
+  // of the handle, in 1 in FB_SHARE of the machine's wavefront slots; the pass
+  // after both only takes what they left
+  const bool beside = promote && h->opts.mode == ODEVEC_PERSISTENT;
+  long long nfb = 0;
   int nprod = 0;
-  long long ws64 = 0;
   if (promote) {
     e = h->net->plan(r.ncells, ODEVEC_PERSISTENT, ODEVEC_FP64, h->opts.block, h->opts.grid, grp, &p2);
     if (e != hipSuccess) return hip_fail(e, "launch plan (FP64 pass)");
-    if (one_launch) {
-      if ((e = h->net->plan_fb(r.ncells, h->opts.precision, &pf, &nprod, &ws64)) != hipSuccess)
-        return hip_fail(e, "launch plan (one-launch fallback)");
-      p = pf;
+    if (beside) {
+      const long long wpb = p.block / 64, waves = p.grid * wpb;
+      nfb = waves / odv::FB_SHARE > 0 ? waves / odv::FB_SHARE : 1;
+      const long long keep = (waves - nfb) / wpb;
+      p.grid = keep > 0 ? keep : 1;
+      nprod = (int)(p.grid * wpb);
+      long long ws64 = nfb * h->net->ws64_per_wave;
       if (p2.ws_doubles > ws64) ws64 = p2.ws_doubles;
       if (ws64 > h->ws2_cap) {
         if (h->d_ws2) (void)hipFree(h->d_ws2);
@@ -255,6 +264,12 @@ int launch(odevec_s* h, const Req& r) {
       }
       if (!h->d_done && (e = hipMalloc((void**)&h->d_done, sizeof(unsigned))) != hipSuccess)
         return fail(ODEVEC_ENOMEM, std::string("fallback counter allocation failed: ") + hipGetErrorString(e));
+      if (!h->fb_stream && (e = hipStreamCreateWithFlags(&h->fb_stream, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "fallback stream");
+      if (!h->fb_fork && (e = hipEventCreateWithFlags(&h->fb_fork, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "fallback event");
+      if (!h->fb_join && (e = hipEventCreateWithFlags(&h->fb_join, hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "fallback event");
     } else if (p2.ws_doubles > p.ws_doubles) {
       p.ws_doubles = p2.ws_doubles;
     }
@@ -310,25 +325,41 @@ int launch(odevec_s* h, const Req& r) {
       return hip_fail(e, "cost order");
     a.order = h->d_order;
   }
-  if (one_launch) {
+  if (beside) {
     a.ctl.promote_cf = h->promote_cf;
     a.promo = h->d_promo;
     a.promo_n = h->d_promo_n;
     a.ho = h->d_ho;
     a.ho_mode = 1;
+    a.fb_done = h->d_done;
     // the FP64 lanes: the list as their queue, claimed through their own counter
     odv::KArgs b = a;
     b.ctl.promote_cf = 0;
     b.promo = nullptr;
     b.promo_n = nullptr;
+    b.fb_done = nullptr;
     b.order = h->d_promo;
     b.qlen = h->d_promo_n;
     b.ws = h->d_ws2;
     b.ho_mode = 2;
     h->qslot = (h->qslot + 1) % 64;
     b.queue = h->d_queue + h->qslot;
-    e = h->net->launch_fb(a, b, h->d_done, nprod, h->opts.precision, pf, r.stream);
-    if (e != hipSuccess) return hip_fail(e, "kernel launch (one-launch fallback)");
+    // the list's entries read -1 until written; the counters start at 0
+    hipLaunchKernelGGL(odv::k_fill_int, dim3((unsigned)((r.ncells + 255) / 256)), dim3(256), 0, r.stream, h->d_promo,
+                       (long long)r.ncells, -1);
+    hipLaunchKernelGGL(odv::k_setu, dim3(1), dim3(1), 0, r.stream, h->d_promo_n, 0u);
+    hipLaunchKernelGGL(odv::k_setu, dim3(1), dim3(1), 0, r.stream, h->d_done, 0u);
+    hipLaunchKernelGGL(odv::k_setq, dim3(1), dim3(1), 0, r.stream, b.queue, 0ull);
+    if ((e = hipEventRecord(h->fb_fork, r.stream)) != hipSuccess ||
+        (e = hipStreamWaitEvent(h->fb_stream, h->fb_fork, 0)) != hipSuccess)
+      return hip_fail(e, "fallback fork");
+    e = h->net->launch(a, ODEVEC_PERSISTENT, h->opts.precision, p, r.stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    e = h->net->launch_fbc(b, h->d_done, nprod, nfb, h->fb_stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch (FP64 lanes)");
+    if ((e = hipEventRecord(h->fb_join, h->fb_stream)) != hipSuccess ||
+        (e = hipStreamWaitEvent(r.stream, h->fb_join, 0)) != hipSuccess)
+      return hip_fail(e, "fallback join");
     // what the FP64 lanes did not take (normally nothing): the FP64 pass
     odv::KArgs a2 = b;
     h->qslot = (h->qslot + 1) % 64;
@@ -830,6 +861,9 @@ int32_t odevec_destroy(odevec_t* h) {
   if (h->d_ho) (void)hipFree(h->d_ho);
   if (h->d_ws2) (void)hipFree(h->d_ws2);
   if (h->d_done) (void)hipFree(h->d_done);
+  if (h->fb_stream) (void)hipStreamDestroy(h->fb_stream);
+  if (h->fb_fork) (void)hipEventDestroy(h->fb_fork);
+  if (h->fb_join) (void)hipEventDestroy(h->fb_join);
   if (h->d_bin) (void)hipFree(h->d_bin);
   if (h->d_count) (void)hipFree(h->d_count);
   if (h->e0) (void)hipEventDestroy(h->e0);

@@ -65,6 +65,10 @@ struct KArgs {
   // record; ho_mode 2 -- the cells of the queue continue from theirs (FP64)
   double* ho;
   int ho_mode;
+  // persistent mode: a wavefront that leaves the loop adds one here once its
+  // list entries are stored (the FP64 lanes of a concurrent kernel wait for
+  // all of them before declaring the list complete)
+  unsigned* fb_done;
   Controls ctl;
   int group;              // lockstep group size
 };
@@ -202,34 +206,47 @@ struct Handoff {
   static constexpr int REC = EO + N;
 };
 
+// The record and the list entry pass between wavefronts of one launch that
+// may run on different XCDs, whose L2 caches are not coherent with each other:
+// they are written and read only with agent-scope atomic accesses (coherent
+// without writing back or invalidating the caches, which a release or
+// acquire fence would do for the whole L2 at every hand-off), and the entry
+// is stored only once the record's stores have completed (odv_finish).
+__device__ inline void ho_st(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double ho_ld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class L>
 __device__ inline void odv_handoff_save(const KArgs& a, long long i, const L& s) {
   using H = Handoff<typename L::Net>;
   double* h = a.ho + i * H::REC;
-  h[H::T] = s.t;
-  h[H::DT] = s.dt;
-  h[H::RATE] = s.rate;
-  h[H::JCH] = s.jchange;
-  h[H::CERR] = s.cerr;
-  h[H::INTS] = (double)s.q;
-  h[H::INTS + 1] = (double)s.swu;
-  h[H::INTS + 2] = (double)s.lju;
-  h[H::INTS + 3] = (double)s.fet;
-  h[H::INTS + 4] = (double)s.fc;
-  h[H::INTS + 5] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0));
-  h[H::INTS + 6] = (double)s.cnt.invfail;
+  ho_st(&h[H::T], s.t);
+  ho_st(&h[H::DT], s.dt);
+  ho_st(&h[H::RATE], s.rate);
+  ho_st(&h[H::JCH], s.jchange);
+  ho_st(&h[H::CERR], s.cerr);
+  ho_st(&h[H::INTS], (double)s.q);
+  ho_st(&h[H::INTS + 1], (double)s.swu);
+  ho_st(&h[H::INTS + 2], (double)s.lju);
+  ho_st(&h[H::INTS + 3], (double)s.fet);
+  ho_st(&h[H::INTS + 4], (double)s.fc);
+  ho_st(&h[H::INTS + 5], (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0)));
+  ho_st(&h[H::INTS + 6], (double)s.cnt.invfail);
   const Counters& c = s.cnt;
   const int cv[8] = {c.attempts, c.steps, c.newton, c.jac, c.errfail, c.convfail, c.stepsel, c.qsum};
 #pragma unroll
-  for (int k = 0; k < 8; ++k) h[H::CNT + k] = (double)cv[k];
+  for (int k = 0; k < 8; ++k) ho_st(&h[H::CNT + k], (double)cv[k]);
 #pragma unroll
   for (int c2 = 0; c2 < NCOL; ++c2)
 #pragma unroll
-    for (int j = 0; j < H::N; ++j) h[H::Z + c2 * H::N + j] = s.z(c2, j);
+    for (int j = 0; j < H::N; ++j) ho_st(&h[H::Z + c2 * H::N + j], s.z(c2, j));
 #pragma unroll
   for (int j = 0; j < H::N; ++j) {
-    h[H::W + j] = s.w(j);
-    h[H::EO + j] = s.eo(j);
+    ho_st(&h[H::W + j], s.w(j));
+    ho_st(&h[H::EO + j], s.eo(j));
   }
 }
 
@@ -238,7 +255,8 @@ template <class Net>
 __device__ inline Counters odv_handoff_counters(const KArgs& a, long long i) {
   using H = Handoff<Net>;
   const double* h = a.ho + i * H::REC + H::CNT;
-  return Counters{(int)h[0], (int)h[1], (int)h[2], (int)h[3], (int)h[4], (int)h[5], (int)h[6], (int)h[7], 0};
+  return Counters{(int)ho_ld(h), (int)ho_ld(h + 1), (int)ho_ld(h + 2), (int)ho_ld(h + 3), (int)ho_ld(h + 4),
+                  (int)ho_ld(h + 5), (int)ho_ld(h + 6), (int)ho_ld(h + 7), 0};
 }
 
 template <class L>
@@ -247,35 +265,35 @@ __device__ inline void odv_handoff_resume(const KArgs& a, long long i, L& s, con
   using H = Handoff<typename L::Net>;
   const double* h = a.ho + i * H::REC;
   odv_cell_constants(s, yfull, prm, a.ctl);
-  s.t = h[H::T];
-  s.dt = h[H::DT];
-  s.rate = h[H::RATE];
-  s.jchange = h[H::JCH];
-  s.cerr = h[H::CERR];
-  odv_set_order(s, (int)h[H::INTS]);
-  s.swu = (int)h[H::INTS + 1];
-  s.lju = (int)h[H::INTS + 2];
-  s.fet = (int)h[H::INTS + 3];
-  s.fc = (int)h[H::INTS + 4];
-  const int fl = (int)h[H::INTS + 5];
+  s.t = ho_ld(&h[H::T]);
+  s.dt = ho_ld(&h[H::DT]);
+  s.rate = ho_ld(&h[H::RATE]);
+  s.jchange = ho_ld(&h[H::JCH]);
+  s.cerr = ho_ld(&h[H::CERR]);
+  odv_set_order(s, (int)ho_ld(&h[H::INTS]));
+  s.swu = (int)ho_ld(&h[H::INTS + 1]);
+  s.lju = (int)ho_ld(&h[H::INTS + 2]);
+  s.fet = (int)ho_ld(&h[H::INTS + 3]);
+  s.fc = (int)ho_ld(&h[H::INTS + 4]);
+  const int fl = (int)ho_ld(&h[H::INTS + 5]);
   s.first = (fl & 1) != 0;
   s.upd_jac = (fl & 2) != 0;
   s.neglect_up = (fl & 4) != 0;
   s.cnt = odv_handoff_counters<typename L::Net>(a, i);
-  s.cnt.invfail = (int)h[H::INTS + 6];
+  s.cnt.invfail = (int)ho_ld(&h[H::INTS + 6]);
   s.status = ST_RUNNING;
   s.tstop = ts;
 #pragma unroll
   for (int c = 0; c < NCOL; ++c)
 #pragma unroll
-    for (int j = 0; j < H::N; ++j) s.set_z(c, j, h[H::Z + c * H::N + j]);
+    for (int j = 0; j < H::N; ++j) s.set_z(c, j, ho_ld(&h[H::Z + c * H::N + j]));
   bool nan = false;
 #pragma unroll
   for (int j = 0; j < H::N; ++j) {
-    const double wj = h[H::W + j];
+    const double wj = ho_ld(&h[H::W + j]);
     s.set_w(j, wj);
     nan = nan || (wj != wj);
-    s.set_eo(j, h[H::EO + j]);
+    s.set_eo(j, ho_ld(&h[H::EO + j]));
     s.en[j] = 0.0;
   }
   // weights held in registers carry their NaN flag (as odv_weights sets it)
@@ -432,10 +450,11 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
     // the caller's state stays untouched for the FP64 pass over promo[],
     // which continues the cell from its hand-off record
     if (a.ho) odv_handoff_save(a, i, s);
-    // the entry is published after the record (an FP64 lane of the same
-    // launch may be waiting for it)
+    // the entry is published once the record's stores have completed (an
+    // FP64 lane of the same launch may be waiting for it)
+    __builtin_amdgcn_s_waitcnt(0);
     const unsigned k = atomicAdd(a.promo_n, 1u);
-    __hip_atomic_store(&a.promo[k], (int)i, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&a.promo[k], (int)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   if (s.status == ST_SKIPPED) {
@@ -490,8 +509,11 @@ __global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
 // or, once the wavefront has seen the queue exhausted and holds no cell,
 // leaves. `lds` holds the workgroup's wavefront slices,
 // LdsPolicy<Net, M>::WORDS doubles each.
+// (forced inline: called from more than one kernel of a unit, the compiler
+// would otherwise emit it as a function, with the arguments in scratch memory
+// and a call per launch)
 template <class Net, class M, bool RCP, bool HIST = false>
-__device__ inline void odv_persistent_run(const KArgs& a, double* lds, long long nl = 0) {
+__device__ __forceinline__ void odv_persistent_run(const KArgs& a, double* lds, long long nl = 0) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
   // lanes taking part (the first nl of the grid; 0: all)
@@ -563,24 +585,34 @@ __device__ inline void odv_persistent_run(const KArgs& a, double* lds, long long
       atomicAdd(&a.flops[1], fl32);
     }
   }
+  if (a.fb_done) {
+    __builtin_amdgcn_s_waitcnt(0);
+    if (lane == 0) __hip_atomic_fetch_add(a.fb_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
-// The FP64 fallback inside the launch: cells handed back by the wavefronts
-// of the mixed persistent loop (their list b.order, filled in hand-back
-// order, entries -1 until written; count *b.qlen) are continued from their
-// hand-off records by FP64 lanes while the mixed loop still runs. Each lane
+// The FP64 fallback beside the launch: cells handed back by the wavefronts
+// of a persistent loop running at the same time (a kernel on another stream;
+// their list b.order, filled in hand-back order, entries -1 until written;
+// count *b.qlen) are continued from their hand-off records by FP64 lanes
+// while that loop still runs. Each lane
 // claims the next list position (b.queue, one atomic per wavefront), waits
 // until the position's entry is written, marks the entry taken (-2) and
 // runs the cell. A claim past the final count (known once all `nprod`
 // mixed wavefronts have counted themselves in *done) is void; a wavefront
 // with nothing to run sleeps between looks at the list, and one that has
-// waited for `idle_ticks` of the device wall clock drops its pending claims
-// and leaves -- the entries it did not take stay in the list for a pass
-// after the launch, so a wait can never lose a cell. Columns: b.ws (a
-// workspace for the FP64 layout) and the wavefront's LDS slice.
+// waited for `idle_rounds` such looks in a row drops its pending claims and
+// leaves -- the entries it did not take stay in the list for a pass after
+// the launch, so a wait can never lose a cell. The wait is bounded by a
+// count, not by the device clock: with the clock read (s_memrealtime) in
+// this loop, this compiler built a consumer kernel whose continued cells
+// differed from the same loop compiled as a called function (a wrong result
+// in the build without contraction: 29 of 113 PRIMORDIAL cells, the same
+// ones at every run); without the read they agree bit for bit. Columns: b.ws
+// (a workspace for the FP64 layout) and the wavefront's LDS slice.
 template <class Net>
-__device__ inline void odv_fb_consume(const KArgs& b, double* lds, const unsigned* done, unsigned nprod,
-                                      long long idle_ticks) {
+__device__ __forceinline__ void odv_fb_consume(const KArgs& b, double* lds, const unsigned* done, unsigned nprod,
+                                               long long idle_rounds) {
   using P = LdsPolicy<Net, double>;
   using L = PerCellLane<Net, double, false>;
   const int lane = threadIdx.x & 63;
@@ -593,7 +625,7 @@ __device__ inline void odv_fb_consume(const KArgs& b, double* lds, const unsigne
   bool have = false, fin = true;
   long long pos = -1;      // claimed list position; -1 none, -2 void
   bool closed = false;     // wave-uniform: claim no more
-  long long idle0 = (long long)wall_clock64();
+  long long idle = 0;       // looks at the list in a row with nothing to run
   while (true) {
     if (have && fin) {
       odv_finish<L>(b, i, s);
@@ -621,15 +653,14 @@ __device__ inline void odv_fb_consume(const KArgs& b, double* lds, const unsigne
       base = __shfl(base, leader, 64);
       if (need) pos = (long long)base + rank < b.ncells ? (long long)base + rank : -2;
     }
-    // look at the claimed entries (relaxed loads; the acquire fence only
-    // once something has arrived, before its record is read)
+    // look at the claimed entries (the record is read, by address from the
+    // entry, only once the entry is there)
     int v = -1;
     bool all_in = false;
     if (pos >= 0) {
       v = __hip_atomic_load(&list[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (v < 0) all_in = __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nprod;
     }
-    if (__ballot(v >= 0 || all_in)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (pos >= 0) {
       if (v < 0 && all_in) {
         // every mixed wavefront has finished: an entry below the final count
@@ -651,11 +682,11 @@ __device__ inline void odv_fb_consume(const KArgs& b, double* lds, const unsigne
     if (pos == -2) pos = -3;   // settled
     const bool running = have && !fin;
     if (__ballot(running)) {
-      idle0 = (long long)wall_clock64();
+      idle = 0;
       if (running) fin = odv_attempt(s, b.ctl, g);
     } else {
       if (closed && __ballot(pos >= 0) == 0ull && __ballot(have) == 0ull) break;
-      if ((long long)wall_clock64() - idle0 > idle_ticks) {
+      if (++idle > idle_rounds) {
         closed = true;    // leave the pending claims to the pass after the launch
         if (pos >= 0) pos = -3;
       }
@@ -677,23 +708,13 @@ __device__ inline void odv_fb_consume(const KArgs& b, double* lds, const unsigne
 // cells back (a.ctl.promote_cf, a.ho_mode 1), then join the last wavefronts,
 // which continue handed-back cells from the start (odv_fb_consume, args b).
 // One wavefront per workgroup: both layouts use the slice at the LDS base.
-template <class Net, class M, bool RCP>
-__global__ void __launch_bounds__(WAVE) odv_k_persistent_fb(KArgs a, KArgs b, unsigned* done, unsigned nprod,
-                                                            long long idle_ticks) {
-  if (blockIdx.x < nprod) {
-    odv_persistent_run<Net, M, RCP>(a, odv_lane_lds, (long long)nprod * WAVE);
-    // the wavefront's hand-backs (list entries and records) are written
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  odv_fb_consume<Net>(b, odv_lane_lds, done, nprod, idle_ticks);
-}
-
-// Fill n ints with v (the fallback list before a one-launch solve).
-template <int UNUSED = 0>
-__global__ void k_fill_i(int* p, long long n, int v) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = v;
+// The FP64 lanes of the fallback as a kernel of their own, launched on a
+// second stream beside the persistent kernel whose hand-backs they take
+// (nprod: that kernel's wavefronts); one wavefront per workgroup.
+template <class Net>
+__global__ void __launch_bounds__(WAVE) odv_k_fb_consume(KArgs b, unsigned* done, unsigned nprod,
+                                                         long long idle_rounds) {
+  odv_fb_consume<Net>(b, odv_lane_lds, done, nprod, idle_rounds);
 }
 
 // With a.order and a.qlen the queue is the first *a.qlen entries of a.order

@@ -44,13 +44,11 @@ struct NetEntry {
   // GetRHS (what 0) / GetJac (what 1, P = I - hb*J) for a.ncells cells into `out`
   hipError_t (*netfun)(int what, const KArgs& a, double hb, double* out, hipStream_t st);
   int jac_entries;        // entries per cell of the GetJac layout (0: per-lane code not built)
-  // persistent mode with the FP64 fallback in one launch (odv_k_persistent_fb):
-  // the shape (block 64; nprod wavefronts run the queue, the rest start as
-  // FP64 lanes) and its two workspaces (ws_doubles: the queue's precision;
-  // ws64: the FP64 lanes')
-  hipError_t (*plan_fb)(long long ncells, int prec, Plan* out, int* nprod, long long* ws64);
-  hipError_t (*launch_fb)(const KArgs& a, const KArgs& b, unsigned* done, int nprod, int prec, const Plan& p,
-                          hipStream_t st);
+  // persistent mode with the FP64 fallback: FP64 lanes beside the queue
+  // kernel (odv_k_fb_consume, `grid` one-wavefront workgroups) and the
+  // workspace doubles per wavefront they need
+  hipError_t (*launch_fbc)(const KArgs& b, unsigned* done, int nprod, long long grid, hipStream_t st);
+  long long ws64_per_wave;
 };
 
 std::vector<const NetEntry*>& registry();
@@ -86,7 +84,6 @@ struct Registrar {
 };
 
 __global__ void k_setq(unsigned long long* q, unsigned long long v);
-__global__ void k_zero_u(unsigned* p, int n);
 
 inline namespace ODV_VARIANT_NS {
 
@@ -211,79 +208,33 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
   }
 }
 
-// One-launch fallback: one wavefront per workgroup, one wave of resident
-// workgroups; 1 in FB_SHARE of them (at least one) starts as FP64 lanes.
+// FP64 lanes of the fallback beside a persistent launch: 1 in FB_SHARE of
+// the machine's wavefront slots (at least one).
 constexpr int FB_SHARE = 32;
-// an FP64 wavefront with nothing to run for this long (device wall clock,
-// 100 MHz) leaves the rest of the fallback to the pass after the launch
-constexpr long long FB_IDLE_TICKS = 50000000;
+// an FP64 wavefront that has looked at the list this many times in a row with
+// nothing to run (a sleep and a few loads each, about 1 us: about a second)
+// leaves the rest of the fallback to the pass after the launch
+constexpr long long FB_IDLE_ROUNDS = 1 << 20;
 
-template <class Net, class M, bool RCP>
-inline hipError_t plan_fb_t(long long ncells, Plan* p, int* nprod, long long* ws64) {
+template <class Net>
+hipError_t launch_fbc_net(const KArgs& b, unsigned* done, int nprod, long long grid, hipStream_t st) {
   if constexpr (!Net::LANE) {
     return hipErrorInvalidValue;
   } else {
-    constexpr int W1 = LdsPolicy<Net, M>::WORDS, W2 = LdsPolicy<Net, double>::WORDS;
-    const int lds = lane_lds_bytes(W1 > W2 ? W1 : W2, WAVE);
-    int dev = 0, ncu = 0, nb = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, reinterpret_cast<const void*>(odv_k_persistent_fb<Net, M, RCP>), WAVE, lds);
-    if (e != hipSuccess) return e;
-    const long long w = (long long)ncu * (nb > 0 ? nb : 1);
-    const long long r = w / FB_SHARE > 0 ? w / FB_SHARE : 1;
-    long long np = w - r > 1 ? w - r : 1;
-    const long long need = (ncells + WAVE - 1) / WAVE;
-    if (np > need) np = need > 0 ? need : 1;
-    p->block = WAVE;
-    p->grid = np + r;
-    p->ws_doubles = p->grid * LdsPolicy<Net, M>::WS_PER_SLOT;
-    *nprod = (int)np;
-    *ws64 = p->grid * LdsPolicy<Net, double>::WS_PER_SLOT;
-    return hipSuccess;
-  }
-}
-
-template <class Net, class M, bool RCP>
-inline hipError_t launch_fb_t(const KArgs& a, const KArgs& b, unsigned* done, int nprod, const Plan& p,
-                              hipStream_t st) {
-  if constexpr (!Net::LANE) {
-    return hipErrorInvalidValue;
-  } else {
-    if (a.ncells <= 0) return hipSuccess;
-    if (p.block != WAVE || p.grid <= nprod || nprod <= 0 || p.grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    if ((LdsPolicy<Net, M>::WS_PER_SLOT > 0 && !a.ws) || (LdsPolicy<Net, double>::WS_PER_SLOT > 0 && !b.ws) ||
-        !b.order || !b.qlen || !b.queue || !b.ho || !a.promo || !a.promo_n || !done)
+    if (grid <= 0 || grid > 0x7fffffffLL || nprod <= 0) return hipErrorInvalidValue;
+    if ((LdsPolicy<Net, double>::WS_PER_SLOT > 0 && !b.ws) || !b.order || !b.qlen || !b.queue || !b.ho || !done)
       return hipErrorInvalidValue;
-    constexpr int W1 = LdsPolicy<Net, M>::WORDS, W2 = LdsPolicy<Net, double>::WORDS;
-    const int lds = lane_lds_bytes(W1 > W2 ? W1 : W2, WAVE);
-    // the list's entries read -1 until written, the counters start at 0 and
-    // the queue past the positions the first wavefronts start on
-    hipLaunchKernelGGL(k_fill_i<0>, dim3((unsigned)((a.ncells + 255) / 256)), dim3(256), 0, st,
-                       const_cast<int*>(b.order), (long long)a.ncells, -1);
-    hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)nprod * WAVE);
-    hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, b.queue, 0ull);
-    hipLaunchKernelGGL(k_zero_u, dim3(1), dim3(64), 0, st, a.promo_n, 1);
-    hipLaunchKernelGGL(k_zero_u, dim3(1), dim3(64), 0, st, done, 1);
-    hipLaunchKernelGGL((odv_k_persistent_fb<Net, M, RCP>), dim3((unsigned)p.grid), dim3(WAVE), lds, st, a, b, done,
-                       (unsigned)nprod, FB_IDLE_TICKS);
+    hipLaunchKernelGGL(odv_k_fb_consume<Net>, dim3((unsigned)grid), dim3(WAVE),
+                       lane_lds_bytes(LdsPolicy<Net, double>::WORDS, WAVE), st, b, done, (unsigned)nprod,
+                       FB_IDLE_ROUNDS);
     return hipGetLastError();
   }
 }
 
 template <class Net>
-hipError_t plan_fb_net(long long ncells, int prec, Plan* p, int* nprod, long long* ws64) {
-  if (prec == PREC_MIXED) return plan_fb_t<Net, float, true>(ncells, p, nprod, ws64);
-  return plan_fb_t<Net, double, false>(ncells, p, nprod, ws64);
-}
-
-template <class Net>
-hipError_t launch_fb_net(const KArgs& a, const KArgs& b, unsigned* done, int nprod, int prec, const Plan& p,
-                         hipStream_t st) {
-  if (prec == PREC_MIXED) return launch_fb_t<Net, float, true>(a, b, done, nprod, p, st);
-  return launch_fb_t<Net, double, false>(a, b, done, nprod, p, st);
+constexpr long long ws64_per_wave_of() {
+  if constexpr (Net::LANE) return LdsPolicy<Net, double>::WS_PER_SLOT;
+  else return 0;
 }
 
 template <class Net>
@@ -422,5 +373,5 @@ struct Bundle {
   static const odv::NetEntry odv_entry_##NET = {  /* NOLINT */                                                \
       NET::ID, ODV_REPRO_FLAG, NET::NEQ, NET::NNZ, NET::NRATE, NET::NPARAM, NET::RATE_KIND, NET::TGAS,          \
       NET::FLOPS_RHS, NET::FLOPS_JAC, NET::FLOPS_LU, NET::FLOPS_SOLVE, NET::FLOPS_LU_R,         \
-      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::Hist<NET>::PLANES, odv::Handoff<NET>::REC, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults, &odv::order_net<NET>, &odv::netfun_net<NET>, odv::jac_entries_of<NET>(), &odv::plan_fb_net<NET>, &odv::launch_fb_net<NET>};                          \
+      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::Hist<NET>::PLANES, odv::Handoff<NET>::REC, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults, &odv::order_net<NET>, &odv::netfun_net<NET>, odv::jac_entries_of<NET>(), &odv::launch_fbc_net<NET>, odv::ws64_per_wave_of<NET>()};                          \
   static odv::Registrar odv_reg_##NET(&odv_entry_##NET);
