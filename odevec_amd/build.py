@@ -61,11 +61,6 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 instructions (hipcc's host pass warns about the feature)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # register spills go to scratch memory, not to AGPRs: the AGPR spill path
-    # gave wrong results (no fault) for the PRIMORDIAL/FF/CSC FP64 per-cell
-    # kernel under this compiler, in both builds, and right ones with this flag
-    # (AGPRs are still allocated as registers)
-    "-mllvm", "-amdgpu-spill-vgpr-to-agpr=0",
     "-Wno-unused-command-line-argument",
 ]
 
