@@ -1042,6 +1042,10 @@ struct odevec_batch_s {
   int64_t promo_cap = 0;
   double* d_ho = nullptr;          // their hand-off records, one region per segment
   int64_t ho_cap = 0;
+  unsigned* d_done = nullptr;      // MAXSEG counts of batch wavefronts past each segment
+  unsigned long long* d_cq = nullptr;   // MAXSEG claim counters of the FP64 lanes
+  hipStream_t fb_stream = nullptr;      // the FP64 lanes run beside the batch launch on this stream
+  hipEvent_t fb_fork = nullptr, fb_join = nullptr;
   unsigned* d_promo_n = nullptr;   // MAXSEG counts
   double* d_ws2 = nullptr;         // workspace of the FP64 passes
   int64_t ws2_cap = 0;
@@ -1083,6 +1087,7 @@ int batch_fallback_passes(odevec_batch_s* b, const odevec_segment_t* segs, int n
     a2.order = a2.promo;
     a2.qlen = a2.promo_n;
     a2.ho_mode = 2;
+    a2.fb_done = nullptr;
     a2.promo = nullptr;
     a2.promo_n = nullptr;
     a2.ws = b->d_ws2;
@@ -1090,6 +1095,35 @@ int batch_fallback_passes(odevec_batch_s* b, const odevec_segment_t* segs, int n
     if (he != hipSuccess) return hip_fail(he, "kernel launch (FP64 pass)");
   }
   return ODEVEC_OK;
+}
+
+// The FP64 lanes beside a batch launch: one odv_k_fb_consume per mixed
+// segment with the fallback, in table order on the batch's second stream
+// (each ends once every batch wavefront has passed its segment and its
+// handed-back cells are done), joined back into the batch's stream; then
+// the FP64 passes over what they left.
+int batch_fallback_lanes(odevec_batch_s* b, const odevec_segment_t* segs, int nseg,
+                         const std::vector<const odv::NetEntry*>& fbnet, int nprod, long long nfb, hipStream_t st) {
+  hipError_t he;
+  for (int k = 0; k < nseg; ++k) {
+    if (!fbnet[k]) continue;
+    odv::KArgs c = b->h_segs[k].a;
+    c.ctl.promote_cf = 0;
+    c.order = c.promo;
+    c.qlen = c.promo_n;
+    c.promo = nullptr;
+    c.promo_n = nullptr;
+    c.fb_done = nullptr;
+    c.ws = b->d_ws2;
+    c.ho_mode = 2;
+    c.queue = b->d_cq + k;
+    he = fbnet[k]->launch_fbc(c, b->d_done + k, nprod, nfb, b->fb_stream);
+    if (he != hipSuccess) return hip_fail(he, "kernel launch (FP64 lanes)");
+  }
+  if ((he = hipEventRecord(b->fb_join, b->fb_stream)) != hipSuccess ||
+      (he = hipStreamWaitEvent(st, b->fb_join, 0)) != hipSuccess)
+    return hip_fail(he, "fallback join");
+  return batch_fallback_passes(b, segs, nseg, fbnet, st);
 }
 
 }  // namespace
@@ -1185,24 +1219,11 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     if (g.hist) hist = 1;
   }
   if (total == 0) return ODEVEC_OK;
-  // a graph is captured for one launch shape: in graph mode the grid does not
-  // depend on the batch's size (workgroups without cells leave at once), so
-  // one captured launch serves every batch
-  odv::Plan p{};
-  hipError_t he = e->plan(b->use_graph ? (int64_t)1 << 40 : total, b->block, b->grid, &p);
-  if (he != hipSuccess) return hip_fail(he, "batch plan");
-  if (p.ws_doubles > b->ws_cap) {
-    batch_drop_graph(b);   // the captured launch holds the old workspace
-    if (b->d_ws) (void)hipFree(b->d_ws);
-    b->d_ws = nullptr;
-    b->ws_cap = 0;
-    if ((he = hipMalloc((void**)&b->d_ws, (size_t)p.ws_doubles * 8)) != hipSuccess)
-      return fail(ODEVEC_ENOMEM, std::string("rate workspace allocation failed: ") + hipGetErrorString(he));
-    b->ws_cap = p.ws_doubles;
-  }
   // FP64 fallback: mixed segments (no history) hand back the cells that reach
-  // promote_cf convergence failures; after the batch launch, the network's
-  // FP64 persistent kernel solves each segment's handed-back cells
+  // promote_cf convergence failures; FP64 lanes beside the batch launch (one
+  // kernel per such segment, on a second stream, in 1 in FB_SHARE of the
+  // machine's wavefront slots) continue them from their hand-off records, and
+  // after both the network's FP64 persistent kernel takes what they left
   std::vector<const odv::NetEntry*> fbnet(nseg, nullptr);
   std::vector<int64_t> fboff(nseg, 0), hooff(nseg, 0);
   int64_t fbtotal = 0, hototal = 0;
@@ -1220,6 +1241,32 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       hooff[k] = hototal;
       hototal += g.ncells * (int64_t)ne->handoff_doubles;
     }
+  }
+  // a graph is captured for one launch shape: in graph mode the grid does not
+  // depend on the batch's size (workgroups without cells leave at once), so
+  // one captured launch serves every batch
+  odv::Plan p{};
+  hipError_t he = e->plan(b->use_graph ? (int64_t)1 << 40 : total, b->block, b->grid, &p);
+  if (he != hipSuccess) return hip_fail(he, "batch plan");
+  long long nfb = 0;
+  int nprod = 0;
+  if (fbtotal > 0) {
+    const long long wpb = p.block / 64, waves = p.grid * wpb;
+    nfb = waves / odv::FB_SHARE > 0 ? waves / odv::FB_SHARE : 1;
+    const long long keep = (waves - nfb) / wpb;
+    p.grid = keep > 0 ? keep : 1;
+    nprod = (int)(p.grid * wpb);
+  }
+  if (p.ws_doubles > b->ws_cap) {
+    batch_drop_graph(b);   // the captured launch holds the old workspace
+    if (b->d_ws) (void)hipFree(b->d_ws);
+    b->d_ws = nullptr;
+    b->ws_cap = 0;
+    if ((he = hipMalloc((void**)&b->d_ws, (size_t)p.ws_doubles * 8)) != hipSuccess)
+      return fail(ODEVEC_ENOMEM, std::string("rate workspace allocation failed: ") + hipGetErrorString(he));
+    b->ws_cap = p.ws_doubles;
+  }
+  if (fbtotal > 0) {
     if (fbtotal > b->promo_cap) {
       if (b->d_promo) (void)hipFree(b->d_promo);
       b->d_promo = nullptr;
@@ -1238,6 +1285,29 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       if ((he = hipMalloc((void**)&b->d_ho, (size_t)hototal * sizeof(double))) != hipSuccess)
         return fail(ODEVEC_ENOMEM, std::string("fallback record allocation failed: ") + hipGetErrorString(he));
       b->ho_cap = hototal;
+    }
+    // the FP64 lanes beside the launch: their counters, stream and workspace
+    // (one segment's kernel at a time on that stream)
+    if (!b->d_done && (he = hipMalloc((void**)&b->d_done, odv::MAXSEG * sizeof(unsigned))) != hipSuccess)
+      return fail(ODEVEC_ENOMEM, std::string("fallback counter allocation failed: ") + hipGetErrorString(he));
+    if (!b->d_cq && (he = hipMalloc((void**)&b->d_cq, odv::MAXSEG * sizeof(unsigned long long))) != hipSuccess)
+      return fail(ODEVEC_ENOMEM, std::string("fallback counter allocation failed: ") + hipGetErrorString(he));
+    if (!b->fb_stream && (he = hipStreamCreateWithFlags(&b->fb_stream, hipStreamNonBlocking)) != hipSuccess)
+      return hip_fail(he, "fallback stream");
+    if (!b->fb_fork && (he = hipEventCreateWithFlags(&b->fb_fork, hipEventDisableTiming)) != hipSuccess)
+      return hip_fail(he, "fallback event");
+    if (!b->fb_join && (he = hipEventCreateWithFlags(&b->fb_join, hipEventDisableTiming)) != hipSuccess)
+      return hip_fail(he, "fallback event");
+    long long ws64 = 0;
+    for (int k = 0; k < nseg; ++k)
+      if (fbnet[k] && nfb * fbnet[k]->ws64_per_wave > ws64) ws64 = nfb * fbnet[k]->ws64_per_wave;
+    if (ws64 > b->ws2_cap) {
+      if (b->d_ws2) (void)hipFree(b->d_ws2);
+      b->d_ws2 = nullptr;
+      b->ws2_cap = 0;
+      if ((he = hipMalloc((void**)&b->d_ws2, (size_t)ws64 * 8)) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("fallback workspace allocation failed: ") + hipGetErrorString(he));
+      b->ws2_cap = ws64;
     }
   }
   // the previous call's table copy must have left the staging buffer
@@ -1276,6 +1346,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       a.promo_n = b->d_promo_n + k;
       a.ho = b->d_ho + hooff[k];
       a.ho_mode = 1;
+      a.fb_done = b->d_done + k;
     }
     d.kind = kind[k];
   }
@@ -1327,11 +1398,22 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       (he = hipEventRecord(b->ev_copied, st)) != hipSuccess)
     return hip_fail(he, "batch table upload");
   b->ev_pending = true;
-  if (fbtotal > 0) hipLaunchKernelGGL(odv::k_zero_u, dim3((nseg + 63) / 64), dim3(64), 0, st, b->d_promo_n, nseg);
+  if (fbtotal > 0) {
+    // lists read -1 until written; counts, wavefronts past, claims start at 0
+    hipLaunchKernelGGL(odv::k_fill_int, dim3((unsigned)((fbtotal + 255) / 256)), dim3(256), 0, st, b->d_promo,
+                       (long long)fbtotal, -1);
+    hipLaunchKernelGGL(odv::k_zero_u, dim3((nseg + 63) / 64), dim3(64), 0, st, b->d_promo_n, nseg);
+    hipLaunchKernelGGL(odv::k_zero_u, dim3((nseg + 63) / 64), dim3(64), 0, st, b->d_done, nseg);
+    for (int k = 0; k < nseg; ++k)
+      if (fbnet[k]) hipLaunchKernelGGL(odv::k_setq, dim3(1), dim3(1), 0, st, b->d_cq + k, 0ull);
+    if ((he = hipEventRecord(b->fb_fork, st)) != hipSuccess ||
+        (he = hipStreamWaitEvent(b->fb_stream, b->fb_fork, 0)) != hipSuccess)
+      return hip_fail(he, "fallback fork");
+  }
   if (!b->use_graph) {
     he = e->launch(b->d_segs, b->d_nseg, odv::MAXSEG, p, hist, st);
     if (he != hipSuccess) return hip_fail(he, "batch launch");
-    return fbtotal > 0 ? batch_fallback_passes(b, segs, nseg, fbnet, st) : ODEVEC_OK;
+    return fbtotal > 0 ? batch_fallback_lanes(b, segs, nseg, fbnet, nprod, nfb, st) : ODEVEC_OK;
   }
   // graph: the queue reset and the batch kernel, captured once per launch
   // shape and stream and replayed; the table is read from device memory, so
@@ -1355,7 +1437,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     b->ghist = hist;
   }
   if ((he = hipGraphLaunch(b->exec, st)) != hipSuccess) return hip_fail(he, "graph launch");
-  return fbtotal > 0 ? batch_fallback_passes(b, segs, nseg, fbnet, st) : ODEVEC_OK;
+  return fbtotal > 0 ? batch_fallback_lanes(b, segs, nseg, fbnet, nprod, nfb, st) : ODEVEC_OK;
 }
 
 int32_t odevec_batch_set_fp64_fallback(odevec_batch_t* b, int32_t convfail_limit) {
@@ -1394,10 +1476,13 @@ int32_t odevec_batch_destroy(odevec_batch_t* b) {
   if (b->h_segs) (void)hipHostFree(b->h_segs);
   if (b->h_nseg) (void)hipHostFree(b->h_nseg);
   void* ps[] = {b->d_segs, b->d_nseg, b->d_queues, b->d_scal, b->d_ws, b->d_promo, b->d_promo_n, b->d_ws2,
-                b->d_order, b->d_bin, b->d_count, b->d_ho};
+                b->d_order, b->d_bin, b->d_count, b->d_ho, b->d_done, b->d_cq};
   for (void* q : ps)
     if (q) (void)hipFree(q);
   if (b->ev_copied) (void)hipEventDestroy(b->ev_copied);
+  if (b->fb_stream) (void)hipStreamDestroy(b->fb_stream);
+  if (b->fb_fork) (void)hipEventDestroy(b->fb_fork);
+  if (b->fb_join) (void)hipEventDestroy(b->fb_join);
   delete b;
   return ODEVEC_OK;
 }
