@@ -116,15 +116,21 @@ int32_t odevec_set_dt_min(odevec_t* h, double dt_min);
 int32_t odevec_set_check_negatives(odevec_t* h, int32_t flag);
 
 /* FP64 fallback of the mixed form (no reference counterpart). With
- * convfail_limit > 0, a cell solved in the mixed form (ODEVEC_MIXED, per-cell
- * or persistent mode, no history attached) that reaches convfail_limit
- * convergence failures stops, keeps its input state, and is solved again
- * from its start by the FP64 persistent kernel in a second launch on the same
- * stream; its state, status, last step and counters are then those of the
- * FP64 form. A few cells fall into long runs of convergence failures with FP32
- * factors (tens of thousands of attempts where FP64 needs about a thousand)
- * and set the whole launch's length; the fallback bounds them. 0 (default):
- * off. */
+ * convfail_limit > 0, a cell (per-cell or persistent mode, no history
+ * attached) that reaches convfail_limit convergence failures stops after that
+ * failure's normal handling (history reset, step size x 0.25, a Jacobian
+ * refresh due), leaves its integrator state in a hand-off record and keeps
+ * its input state; FP64 lanes continue it from there. In persistent mode they
+ * run beside the launch, on a second stream of the handle (forked from and
+ * joined back into `stream`, in 1 in 32 of the machine's wavefront slots),
+ * taking each cell as it is handed back; in per-cell mode, and for whatever
+ * they leave, the FP64 persistent kernel continues the cells after the launch
+ * on the same stream. Counters are those of both precisions together. A few
+ * cells fall into long runs of convergence failures with FP32 factors (tens
+ * of thousands of attempts where FP64 needs about a thousand) and would set
+ * the whole launch's length; the fallback bounds them. On an FP64 handle the
+ * cells only move between lanes: results are those without the fallback, bit
+ * for bit. 0 (default): off. */
 int32_t odevec_set_fp64_fallback(odevec_t* h, int32_t convfail_limit);
 
 /* Invariant test of the mixed form's corrector (no reference counterpart):
@@ -304,9 +310,11 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
 /* FP64 fallback for the batch's mixed segments (no reference counterpart;
  * see odevec_set_fp64_fallback): with convfail_limit > 0, a cell of a mixed
  * segment (without history) that reaches convfail_limit convergence failures
- * keeps its input state, and after the batch launch the network's FP64
- * persistent kernel solves such cells on the same stream (one launch per
- * segment, outside the batch's captured graph). 0 (default): off. */
+ * is handed off as for a handle; one FP64 kernel per such segment runs beside
+ * the batch launch on the batch's second stream (outside the captured graph)
+ * and continues the segment's cells as they are handed back, and the
+ * network's FP64 persistent kernel takes what they left after the launch.
+ * 0 (default): off. */
 int32_t odevec_batch_set_fp64_fallback(odevec_batch_t* b, int32_t convfail_limit);
 /* The invariant test of the batch's mixed segments (see odevec_set_invariant_test). */
 int32_t odevec_batch_set_invariant_test(odevec_batch_t* b, double tolerances, int32_t fallback_fails);

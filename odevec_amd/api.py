@@ -120,8 +120,9 @@ class Odevec:
 
     def set_fp64_fallback(self, convfail_limit: int):
         """Mixed form: cells that reach `convfail_limit` convergence failures
-        are solved again from their start in FP64 (a second launch); 0 turns
-        it off (the default)."""
+        continue in FP64 from where they stopped (FP64 lanes beside the
+        launch in persistent mode, a second launch in per-cell mode); 0 turns
+        it off (the default). On an FP64 handle results are unchanged."""
         L.check(L.lib().odevec_set_fp64_fallback(self._h, int(convfail_limit)))
 
     def set_invariant_test(self, tolerances: float = 1.0, fallback_fails: int = 32):
@@ -296,7 +297,8 @@ class Batch:
 
     def set_fp64_fallback(self, convfail_limit: int):
         """Mixed segments: cells that reach `convfail_limit` convergence
-        failures are solved again in FP64 after the batch launch (0: off)."""
+        failures continue in FP64 from where they stopped, in FP64 lanes
+        beside the batch launch (0: off)."""
         L.check(L.lib().odevec_batch_set_fp64_fallback(self._h, int(convfail_limit)))
 
     def set_invariant_test(self, tolerances: float = 1.0, fallback_fails: int = 32):
