@@ -72,13 +72,16 @@ class Odevec:
         self._times = None
 
     # -- the generated plugin's network functions (src/odevec_commons.F90:32-95)
-    def GetRHS(self, y: np.ndarray) -> np.ndarray:
+    def GetRHS(self, y: np.ndarray, params: Optional[np.ndarray] = None) -> np.ndarray:
         """f(y) for y of shape (neq, ncells) (rates from each cell's y and
-        the network's default parameters); returns (neq, ncells)."""
+        its parameters, shape (nparam, ncells), or the network's defaults);
+        returns (neq, ncells)."""
         self._require()
         y = np.ascontiguousarray(y, dtype=np.float64)
         out = np.zeros_like(y)
-        L.check(L.lib().odevec_get_rhs(self._h, y.shape[1], y.ctypes.data, None, out.ctypes.data))
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        L.check(L.lib().odevec_get_rhs(self._h, y.shape[1], y.ctypes.data, None if p is None else p.ctypes.data,
+                                       out.ctypes.data))
         return out
 
     def GetJac(self, beta: float, y: np.ndarray, dt: float, params: Optional[np.ndarray] = None) -> np.ndarray:

@@ -39,7 +39,12 @@ BUILTIN_VARIANTS = [
     "PRIMORDIAL/NONE/DENSE/LU3",
     "ROBER/NONE/DENSE/LU2",        # LUmethod 2: L and U in closed form
     "OREGO/NONE/DENSE/LU2",
+    "KROME/FF/CSC",                # the KROME setup file KROME_SETUPFILE
 ]
+
+# the network that KROME/... variants are generated from: a KROME-style setup
+# file (Python syntax, read as pre_odevec.py:97-105 reads one)
+KROME_SETUPFILE = os.path.join(HERE, "pre_odevec", "krome_ism.py")
 
 # bundles: one batch kernel (odv_k_batch) over several (network, precision)
 # entries; every network listed must also be a variant above. The kernel's
@@ -72,13 +77,14 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def generate(variants: List[str]) -> List[str]:
+def generate(variants: List[str], krome_setupfile: str = None) -> List[str]:
     from .pre_odevec import cli
     bases = []
     for v in variants:
         name, ordering, packaging, lu = _parts(v)
         ordering = "None" if ordering.upper() == "NONE" else ordering
-        cli.generate(name, ordering, packaging, GEN, lu_method=lu)
+        kf = (krome_setupfile or KROME_SETUPFILE) if name.upper() == "KROME" else None
+        cli.generate(name, ordering, packaging, GEN, krome_setupfile=kf, lu_method=lu)
         bases.append("%s_%s_%s%s" % (name, ordering.upper(), packaging, "" if lu == 1 else "_LU%d" % lu))
     return bases
 
@@ -137,9 +143,9 @@ def _compile(src: str, obj: str, extra: List[str]) -> str:
 
 
 def build(variants: List[str] = None, jobs: int = 8, verbose: bool = True, repro: bool = True,
-          bundles: bool = True) -> str:
+          bundles: bool = True, krome_setupfile: str = None) -> str:
     variants = list(BUILTIN_VARIANTS if variants is None else variants)
-    bases = generate(variants)
+    bases = generate(variants, krome_setupfile)
     os.makedirs(BUILD, exist_ok=True)
     units = [(os.path.join(CSRC, "odevec_capi.hip"), os.path.join(BUILD, "odevec_capi.o"), [])]
     for b in bases:
@@ -217,8 +223,10 @@ def main(argv=None) -> int:
     p = argparse.ArgumentParser()
     p.add_argument("--extra", action="append", default=[], help="NETWORK/ORDERING/PACKAGING to add")
     p.add_argument("-j", "--jobs", type=int, default=8)
+    p.add_argument("--krome_setupfile", default=None,
+                   help="KROME-style setup file for the KROME/... variants (default %s)" % KROME_SETUPFILE)
     a = p.parse_args(argv)
-    build(BUILTIN_VARIANTS + a.extra, a.jobs)
+    build(BUILTIN_VARIANTS + a.extra, a.jobs, krome_setupfile=a.krome_setupfile)
     return 0
 
 

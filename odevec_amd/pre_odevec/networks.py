@@ -27,6 +27,7 @@ Networks follow the reference definitions:
 """
 from __future__ import annotations
 
+import re
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
 
@@ -213,9 +214,12 @@ def load_krome_setupfile(path: str) -> Network:
 
     The file is executed with SymPy's names in scope and must define ``y``
     (a sequence of symbols), ``rhs`` (a sequence or Matrix of expressions),
-    ``neq`` and ``nrea``; rate symbols are any symbols named ``k...`` and
-    become per-cell parameters (KROME's own rate code is not part of this
-    preprocessor)."""
+    ``neq`` and ``nrea``. Every symbol of the RHS that is not a species is a
+    rate and becomes a per-cell parameter (KROME computes its rate
+    coefficients ``k(:,i)`` itself, outside the solver): if the file defines
+    ``k`` (a sequence of symbols), parameter i is ``k[i]``, used or not, so
+    that parameter indices are KROME's reaction indices; any other symbols
+    follow in natural order of their names (``k(:,2)`` before ``k(:,10)``)."""
     scope: Dict[str, object] = {}
     exec("from sympy import *", scope)
     with open(path) as fh:
@@ -228,7 +232,10 @@ def load_krome_setupfile(path: str) -> Network:
     yn = _ysyms(neq)
     sub = {ys[i]: yn[i] for i in range(neq)}
     rhs = [sp.expand(sp.sympify(e).xreplace(sub)) for e in rhs_in]
-    ksyms = sorted({s for e in rhs for s in e.free_symbols if s not in yn}, key=lambda s: s.name)
+    declared = [sp.sympify(s) for s in scope["k"]] if "k" in scope else []
+    extra = sorted({s for e in rhs for s in e.free_symbols if s not in yn and s not in declared},
+                   key=lambda s: [int(t) if t.isdigit() else t for t in re.split(r"(\d+)", s.name)])
+    ksyms = declared + extra
     kn = _ksyms(len(ksyms))
     rhs = [e.xreplace(dict(zip(ksyms, kn))) for e in rhs]
     return Network("KROME", [str(s) for s in ys], yn, kn, rhs, rate_kind="param",

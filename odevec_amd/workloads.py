@@ -18,6 +18,8 @@ Each generator returns a dict with the network id, the state ``y`` (shape
   t_stop ~ logU[1e8, 1e14] s; dt hint 2.2954626201190183e-26 as the single
   precision literal of tests/primordial.f90:73; rtol 1e-6, atol 1e-20,
   MinimalTimestep 1e-20 (tests/primordial.f90:44-45).
+* KROME_ISM (seed 5): the KROME setup file's interstellar carbon/oxygen
+  network, conditions and rates as krome_ism() states.
 * SYNTH64 (serving): abundances logU[1e-6, 1] of n, n ~ logU[1, 1e6],
   T ~ logU[100, 1e4] K.
 * Batching mix (seed 4): categorical 40/30/30 ROBER/OREGO/PRIMORDIAL cells,
@@ -97,6 +99,82 @@ def synth64(n: int, seed: int = 7, network: str = "SYNTH64/FF/CSC", t_stop: floa
     T = _logu(rng, 100.0, 1e4, n)
     return dict(network=network, y=np.ascontiguousarray(y), t0=np.zeros(n), t_stop=np.full(n, float(t_stop)),
                 dt0=np.full(n, 1e-10), params=T[None, :].copy(), rtol=rtol, atol=1e-20, dt_min=1e-20)
+
+
+# KROME_ISM rate coefficients: (alpha, beta, gamma, kind) per reaction of the
+# setup file odevec_amd/pre_odevec/krome_ism.py, in its order. kind "T":
+# alpha (T/300)^beta exp(-gamma/T) [cm^3 s^-1]; "grain": alpha (T/100)^0.5
+# [cm^3 s^-1]; "cr": alpha zeta/1.3e-17 [s^-1]; "uv": alpha G0 exp(-gamma Av)
+# [s^-1]. Magnitudes are typical interstellar values.
+KROME_ISM_RATES = [
+    (3.0e-17, 0.0, 0.0, "grain"),        # 1  H + H -> H2 (grains)
+    (1.2e-17, 0.0, 0.0, "cr"),           # 2  H2 -> H3+ + H + e- (CR)
+    (6.0e-18, 0.0, 0.0, "cr"),           # 3  H -> H+ + e- (CR)
+    (3.5e-12, -0.75, 0.0, "T"),          # 4  H+ + e- -> H
+    (6.7e-8, -0.52, 0.0, "T"),           # 5  H3+ + e- -> H2 + H
+    (8.0e-10, 0.0, 0.0, "T"),            # 6  H3+ + O -> OH + H + H+
+    (2.05e-12, 1.52, 1736.0, "T"),       # 7  OH + H2 -> H2O + H
+    (5.9e-10, 0.0, 1.7, "uv"),           # 8  H2O -> OH + H
+    (3.9e-10, 0.0, 2.24, "uv"),          # 9  OH -> O + H
+    (2.1e-19, 0.0, 0.0, "T"),            # 10 C + O -> CO
+    (1.0e-10, 0.0, 0.0, "T"),            # 11 C + OH -> CO + H
+    (7.7e-10, -0.5, 0.0, "T"),           # 12 C+ + OH -> CO + H+
+    (3.5e-10, 0.0, 3.76, "uv"),          # 13 C -> C+ + e-
+    (4.67e-12, -0.6, 0.0, "T"),          # 14 C+ + e- -> C
+    (2.4e-10, 0.0, 3.88, "uv"),          # 15 CO -> C + O
+    (1.7e-9, 0.0, 0.0, "T"),             # 16 H3+ + CO -> HCO+ + H2
+    (2.4e-7, -0.69, 0.0, "T"),           # 17 HCO+ + e- -> CO + H
+    (9.0e-10, -0.5, 0.0, "T"),           # 18 C+ + H2O -> HCO+ + H
+    (5.7e-11, 0.0, 3.7, "uv"),           # 19 H2 -> H + H
+    (3.14e-13, 2.7, 3150.0, "T"),        # 20 O + H2 -> OH + H
+    (7.0e-14, 2.8, 1950.0, "T"),         # 21 H + OH -> O + H2
+]
+KROME_ISM_C, KROME_ISM_O = 1.4e-4, 3.2e-4     # carbon and oxygen nuclei per H nucleus
+
+
+def krome_ism_rates(T, Av, G0, zeta):
+    """Rate coefficients k(:,1..21) of the KROME_ISM network per cell, shape
+    (21, ncells): the values KROME's rate code hands the solver."""
+    T, Av, G0, zeta = (np.asarray(v, dtype=np.float64) for v in (T, Av, G0, zeta))
+    out = np.empty((len(KROME_ISM_RATES), T.size))
+    for i, (a, b, g, kind) in enumerate(KROME_ISM_RATES):
+        if kind == "T":
+            out[i] = a * (T / 300.0) ** b * np.exp(-g / T)
+        elif kind == "grain":
+            out[i] = a * np.sqrt(T / 100.0)
+        elif kind == "cr":
+            out[i] = a * zeta / 1.3e-17
+        else:
+            out[i] = a * G0 * np.exp(-g * Av)
+    return out
+
+
+def krome_ism(n: int, seed: int = 5, network: str = "KROME/FF/CSC", t_lo: float = 1e9, t_hi: float = 1e14,
+              rtol: float = 1e-6):
+    """KROME_ISM cells: n_H ~ logU[10, 1e5] cm^-3, T ~ logU[10, 3000] K,
+    A_V ~ logU[0.01, 20], G0 ~ logU[0.1, 100], zeta ~ 1.3e-17 logU[0.1, 10]
+    s^-1; x(H2) ~ logU[1e-4, 0.4], carbon all C+, oxygen all O, x(H+) =
+    1e-8, electrons balance the charge, other species at 1e-20 n_H;
+    t_stop ~ logU[1e9, 1e14] s; dt hint 1e-6 s; rtol 1e-6, atol 1e-20,
+    MinimalTimestep 1e-20."""
+    rng = np.random.default_rng(seed)
+    nH = _logu(rng, 10.0, 1e5, n)
+    T = _logu(rng, 10.0, 3000.0, n)
+    Av = _logu(rng, 0.01, 20.0, n)
+    G0 = _logu(rng, 0.1, 100.0, n)
+    zeta = 1.3e-17 * _logu(rng, 0.1, 10.0, n)
+    xH2 = _logu(rng, 1e-4, 0.4, n)
+    ts = _logu(rng, t_lo, t_hi, n)
+    y = np.full((12, n), 1e-20) * nH
+    y[2] = xH2 * nH                       # H2
+    y[3] = 1e-8 * nH                      # H+
+    y[6] = KROME_ISM_C * nH               # C+
+    y[7] = KROME_ISM_O * nH               # O
+    y[0] = y[3] + y[6]                    # e-
+    y[1] = nH - 2.0 * y[2] - y[3]         # H
+    return dict(network=network, y=np.ascontiguousarray(y), t0=np.zeros(n), t_stop=ts, dt0=np.full(n, 1e-6),
+                params=np.ascontiguousarray(krome_ism_rates(T, Av, G0, zeta)), rtol=rtol, atol=1e-20,
+                dt_min=1e-20)
 
 
 def subset(w: dict, idx) -> dict:

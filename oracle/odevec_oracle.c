@@ -839,10 +839,10 @@ int oracle_solve_vector(const onet_t* n, int G, double* time, double* dt, double
   /* rates once per cell from its initial state */
   {
     double* yc = v.tmp;
-    double pc[8];
+    double pc[n->nparam > 0 ? n->nparam : 1];
     for (int i = 0; i < G; ++i) {
       for (int j = 0; j < N; ++j) yc[j] = A2(y, j, i);
-      for (int q = 0; q < n->nparam && q < 8; ++q) pc[q] = prm[(size_t)q * G + i];
+      for (int q = 0; q < n->nparam; ++q) pc[q] = prm[(size_t)q * G + i];
       cell_rates(n, yc, pc, v.k + (size_t)i * nr);
     }
   }
@@ -918,9 +918,9 @@ void oracle_solve_cells(const onet_t* n, long ncells, const double* t0, const do
 #pragma omp parallel for schedule(dynamic, 16) if (nthreads != 1)
 #endif
   for (long i = 0; i < ncells; ++i) {
-    double yc[256], pc[8];
+    double yc[N], pc[n->nparam > 0 ? n->nparam : 1];
     for (int j = 0; j < N; ++j) yc[j] = y[(size_t)j * ncells + i];
-    for (int q = 0; q < n->nparam && q < 8; ++q) pc[q] = prm[(size_t)q * ncells + i];
+    for (int q = 0; q < n->nparam; ++q) pc[q] = prm[(size_t)q * ncells + i];
     double t = t0[i], dt = dt0[i];
     ocnt_t cnt;
     int st = oracle_solve_vector(n, 1, &t, &dt, tstop[i], yc, pc, NULL, c, &cnt);

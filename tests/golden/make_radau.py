@@ -9,7 +9,8 @@ reference_rhs.json); PRIMORDIAL's rate coefficients come from the oracle
 no reference definition: its right-hand side and Jacobian are the mass-action
 law of the generator's reaction list (A + B -> C + D, k = A exp(-Ea/T)),
 evaluated here directly from that list, without the preprocessor's term
-table or the oracle.
+table or the oracle. KROME_ISM's come from the KROME setup file's own SymPy
+expressions (lambdified), again without the term table or the oracle.
 """
 import json
 import os
@@ -115,6 +116,37 @@ def solve_synth(w, idx, rtol=1e-12, atol=1e-30):
     return out
 
 
+def krome_fun(path):
+    """f and J of a KROME-style setup file straight from its SymPy ``rhs``
+    (executed as pre_odevec.py:97-105 executes it), without the
+    preprocessor's term table or the oracle; k(:,i) are the file's ``k``."""
+    import sympy as sp
+    scope = {}
+    exec("from sympy import *", scope)
+    with open(path) as fh:
+        exec(fh.read(), scope)
+    y, k, rhs = list(scope["y"]), list(scope["k"]), sp.Matrix(scope["rhs"])
+    fl = sp.lambdify([y, k], list(rhs), "numpy")
+    jl = sp.lambdify([y, k], rhs.jacobian(y).tolist(), "numpy")
+    return (lambda kv: (lambda t, yv: np.array(fl(yv, kv), dtype=float)),
+            lambda kv: (lambda t, yv: np.array(jl(yv, kv), dtype=float)))
+
+
+def solve_krome(w, idx, rtol=1e-12, atol=1e-30):
+    from odevec_amd.build import KROME_SETUPFILE
+    mk_f, mk_j = krome_fun(KROME_SETUPFILE)
+    out = []
+    for i in idx:
+        y0 = w["y"][:, i].copy()
+        kv = w["params"][:, i].copy()
+        ts = float(w["t_stop"][i])
+        r = solve_ivp(mk_f(kv), (0.0, ts), y0, method="Radau", rtol=rtol, atol=atol, jac=mk_j(kv))
+        assert r.success, r.message
+        out.append({"cell": int(i), "y0": y0.tolist(), "t_stop": ts, "params": kv.tolist(),
+                    "y": r.y[:, -1].tolist(), "nfev": int(r.nfev)})
+    return out
+
+
 def main():
     res = {"method": "scipy.integrate.solve_ivp Radau", "rtol": 1e-12, "cases": {}}
     res["cases"]["ROBER"] = {"network": "ROBER/NONE/DENSE", "atol": 1e-30,
@@ -125,6 +157,8 @@ def main():
                                   "cells": solve("PRIMORDIAL/CMK/CSC", W.primordial(64), range(6), atol=1e-40)}
     res["cases"]["SYNTH64"] = {"network": "SYNTH64/FF/CSC", "atol": 1e-30,
                                "cells": solve_synth(W.synth64(64, seed=7), range(4))}
+    res["cases"]["KROME_ISM"] = {"network": "KROME/FF/CSC", "atol": 1e-30,
+                                 "cells": solve_krome(W.krome_ism(64, seed=5), range(6))}
     with open(os.path.join(HERE, "radau_known_answers.json"), "w") as fh:
         json.dump(res, fh, indent=1)
     print("wrote radau_known_answers.json")

@@ -18,6 +18,7 @@
 #include "generated/net_PRIMORDIAL_NONE_DENSE_LU3.h"
 #include "generated/net_ROBER_NONE_DENSE_LU2.h"
 #include "generated/net_OREGO_NONE_DENSE_LU2.h"
+#include "generated/net_KROME_FF_CSC.h"
 
 namespace {
 
@@ -109,6 +110,7 @@ extern "C" int lane_solve_cells(const char* id, int mixed, long n, const double*
   ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE_LU3)
   ODV_CASE(odv_net_ROBER_NONE_DENSE_LU2)
   ODV_CASE(odv_net_OREGO_NONE_DENSE_LU2)
+  ODV_CASE(odv_net_KROME_FF_CSC)
 #undef ODV_CASE
   return -1;
 }
@@ -137,6 +139,7 @@ extern "C" int lane_netfun(const char* id, int what, const double* y, const doub
   ODV_CASE(odv_net_PRIMORDIAL_NONE_DENSE_LU3)
   ODV_CASE(odv_net_ROBER_NONE_DENSE_LU2)
   ODV_CASE(odv_net_OREGO_NONE_DENSE_LU2)
+  ODV_CASE(odv_net_KROME_FF_CSC)
 #undef ODV_CASE
   return -1;
 }

@@ -21,6 +21,8 @@ NETS = {
     # LUmethod 2: the L and U factors in closed form
     "ROBER/NONE/DENSE/LU2": lambda n: dict(W.rober(n), network="ROBER/NONE/DENSE/LU2"),
     "OREGO/NONE/DENSE/LU2": lambda n: dict(W.orego(n), network="OREGO/NONE/DENSE/LU2"),
+    # a network from a KROME setup file: 21 rate coefficients as per-cell parameters
+    "KROME/FF/CSC": lambda n: W.krome_ism(n),
 }
 
 
@@ -65,7 +67,7 @@ def test_lane_skips_empty_span(harness):
     assert np.array_equal(y[:, [0, 1, 3]], w["y"][:, [0, 1, 3]])
 
 
-@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"])
+@pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC", "KROME/FF/CSC"])
 def test_lane_mixed_precision_accuracy(harness, oracle, nid):
     """FP32 P/LU/solves, FP64 residual and control: every cell finishes, and
     its distance to a tight FP64 solve is of the size of the FP64 run's own."""

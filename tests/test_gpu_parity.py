@@ -60,6 +60,8 @@ CASES = {
     # LUmethod 2: the L and U factors in closed form
     "ROBER_LU2": lambda: dict(W.rober(32, seed=1), network="ROBER/NONE/DENSE/LU2"),
     "OREGO_LU2": lambda: dict(W.orego(512, seed=2), network="OREGO/NONE/DENSE/LU2"),
+    # a network built from a KROME setup file; its 21 rate coefficients per cell
+    "KROME": lambda: W.krome_ism(512, seed=5),
 }
 
 
@@ -100,7 +102,7 @@ def test_repro_nan_state_bitwise(oracle, case, mode):
 
 
 @pytest.mark.parametrize("case,G", [("ROBER", 32), ("OREGO", 64), ("PRIMORDIAL", 64), ("ROBER_LU3", 32),
-                                    ("PRIMORDIAL_LU3", 64), ("ROBER_LU2", 32), ("OREGO_LU2", 64)])
+                                    ("PRIMORDIAL_LU3", 64), ("ROBER_LU2", 32), ("OREGO_LU2", 64), ("KROME", 64)])
 def test_repro_bitwise_lockstep(oracle, case, G):
     """Lockstep groups reproduce the reference at nvector = G (shared t_stop)."""
     w = CASES[case]()
@@ -286,13 +288,14 @@ def test_default_build_accuracy_vs_radau(oracle, precision):
     oracle's own (the problems amplify rounding differences, so two builds
     that round differently cannot agree to 10*rtol with each other)."""
     gold = json.load(open(os.path.join(GOLD, "radau_known_answers.json")))["cases"]
-    for name in ("ROBER", "PRIMORDIAL"):
+    dt0 = {"ROBER": 1e-10, "PRIMORDIAL": W.PRIMORDIAL_DT_HINT, "KROME_ISM": 1e-6}
+    for name in ("ROBER", "PRIMORDIAL", "KROME_ISM"):
         case = gold[name]
         cells = case["cells"]
         n = len(cells)
         w = {"network": case["network"], "y": np.array([c["y0"] for c in cells]).T.copy(),
              "t_stop": np.array([c["t_stop"] for c in cells]), "t0": np.zeros(n),
-             "dt0": np.full(n, 1e-10 if name == "ROBER" else W.PRIMORDIAL_DT_HINT),
+             "dt0": np.full(n, dt0[name]),
              "params": None if cells[0]["params"] is None else np.array([c["params"] for c in cells]).T.copy(),
              "rtol": 1e-6, "atol": 1e-10 if name == "ROBER" else 1e-20, "dt_min": 1e-20}
         y, st, _, _ = gpu_solve(case["network"], w, precision=precision)
@@ -309,6 +312,26 @@ def test_mixed_precision_primordial_all_cells_finish():
     w = W.primordial(8192, seed=3)
     y, st, dto, stats = gpu_solve("PRIMORDIAL/CMK/CSC", w, precision=1, mode=_lib.PERSISTENT)
     assert (st == 0).all() and stats["flops32"] > 0
+
+
+def test_krome_mixed_and_fp64_against_tight_solve():
+    """The KROME setup-file network in the default build: FP64 and mixed
+    precision (with the FP64 fallback) both finish every cell, and the mixed
+    form's distance to a tight FP64 solve (rtol 1e-11, atol 1e-26), in units
+    of rtol|y| + atol, is of the size of the FP64 run's own (median and 90th
+    percentile within twice FP64's, or 10)."""
+    w = W.krome_ism(4096, seed=5)
+    yt, stt, _, _ = gpu_solve("KROME/FF/CSC", dict(w, rtol=1e-11, atol=1e-26, dt_min=1e-30),
+                              mode=_lib.PERSISTENT)
+    yd, sd, _, _ = gpu_solve("KROME/FF/CSC", w, mode=_lib.PERSISTENT)
+    ym, sm, _, stats = gpu_solve("KROME/FF/CSC", w, precision=1, mode=_lib.PERSISTENT, fallback=128)
+    assert (stt == 0).all() and (sd == 0).all() and (sm == 0).all() and stats["flops32"] > 0
+    sc = w["rtol"] * np.abs(yt) + w["atol"]
+    ed = (np.abs(yd - yt) / sc).max(axis=0)
+    em = (np.abs(ym - yt) / sc).max(axis=0)
+    for q in (0.5, 0.9):
+        say("KROME q%.2f: FP64 %.3g mixed %.3g" % (q, np.quantile(ed, q), np.quantile(em, q)))
+        assert np.quantile(em, q) <= max(2 * np.quantile(ed, q), 10.0)
 
 
 def test_async_device_pointers_equal_sync():
@@ -463,7 +486,7 @@ def test_solveode_mask_per_cell(oracle):
 
 
 @pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "OREGO/NONE/CSC", "PRIMORDIAL/CMK/CSC", "PRIMORDIAL/NONE/DENSE",
-                                 "ROBER/NONE/DENSE/LU2", "OREGO/NONE/DENSE/LU2"])
+                                 "ROBER/NONE/DENSE/LU2", "OREGO/NONE/DENSE/LU2", "KROME/FF/CSC"])
 def test_plugin_getrhs_getjac(oracle, nid):
     """The generated plugin's GetRHS / GetJac through the C ABI (evaluated
     on the GPU, +REPRO build) equal the oracle's RHS and permuted P bit for
@@ -473,18 +496,21 @@ def test_plugin_getrhs_getjac(oracle, nid):
     factors in the DENSE layout."""
     from odevec_amd.api import Odevec
     w = {"ROBER": lambda: W.rober(40, seed=16), "OREGO": lambda: W.orego(40, seed=17),
-         "PRIMORDIAL": lambda: W.primordial(40, seed=18)}[nid.split("/")[0]]()
+         "PRIMORDIAL": lambda: W.primordial(40, seed=18), "KROME": lambda: W.krome_ism(40, seed=19)}[
+        nid.split("/")[0]]()
     n = w["y"].shape[1]
+    # KROME: the rate coefficients are the caller's per-cell parameters
+    prm = w["params"] if nid.startswith("KROME") else None
     say("GPU GetRHS/GetJac", nid)
     ode = Odevec(nid + "+REPRO", n)
-    f = ode.GetRHS(w["y"])
+    f = ode.GetRHS(w["y"], prm)
     beta, dt = 6.0 / 11.0, 3e-4
-    jac = ode.GetJac(beta, w["y"], dt)
+    jac = ode.GetJac(beta, w["y"], dt, prm)
     ode.CloseOdevec()
     net = oracle.Network.load(nid)
     neq = net.neq
     for i in range(n):
-        k = net.rates(w["y"][:, i])
+        k = net.rates(w["y"][:, i], None if prm is None else prm[:, i])
         assert np.array_equal(f[:, i], net.rhs(w["y"][:, i], k))
         P = net.p_dense(w["y"][:, i], k, beta * dt)
         if net.ir["lu_method"] == 2:

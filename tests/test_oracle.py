@@ -186,7 +186,7 @@ def _unpack(net, lu):
 
 
 @pytest.mark.parametrize("nid", ["ROBER/NONE/DENSE", "OREGO/NONE/CSC", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC",
-                                 "PRIMORDIAL/FF/CSC", "PRIMORDIAL/NONE/DENSE"])
+                                 "PRIMORDIAL/FF/CSC", "PRIMORDIAL/NONE/DENSE", "KROME/FF/CSC"])
 def test_lu_identity_and_solve(oracle, nid):
     """L*U = P_order and P * den = res for the permuted system."""
     net = _net(oracle, nid)
@@ -323,7 +323,7 @@ def test_known_answers_radau(oracle):
             prm = None if c["params"] is None else np.array(c["params"])[:, None]
             ref = np.array(c["y"])
             big = np.abs(ref) > 1e-6 * np.abs(ref).max()
-            atol = 1e-20 if name == "PRIMORDIAL" else 1e-12
+            atol = 1e-20 if name in ("PRIMORDIAL", "KROME_ISM") else 1e-12
             y, st, _, _ = net.solve_cells(y0, c["t_stop"], params=prm, rtol=1e-8, atol=atol, dt_min=1e-30)
             assert st[0] == 0
             err = np.max(np.abs(y[:, 0] - ref)[big] / np.abs(ref[big]))
@@ -489,3 +489,55 @@ def test_synth64_ff_ordering_and_fill_from_the_reaction_list(oracle):
         F[np.ix_(rows, cols)] = True
     assert int(F.sum()) == net.ir["nnz"] == 3155
     assert sorted(map(tuple, net.ir["lu_entries"])) == sorted(zip(*np.where(F.T)[::-1]))
+
+
+def _krome_file():
+    """The KROME setup file executed as pre_odevec.py:97-105 executes it:
+    its own y, k and rhs (SymPy), independent of the preprocessor."""
+    import sympy as sp
+    from odevec_amd.build import KROME_SETUPFILE
+    scope = {}
+    exec("from sympy import *", scope)
+    with open(KROME_SETUPFILE) as fh:
+        exec(fh.read(), scope)
+    y, k, rhs = list(scope["y"]), list(scope["k"]), sp.Matrix(scope["rhs"])
+    return (scope, sp.lambdify([y, k], list(rhs), "numpy"),
+            sp.lambdify([y, k], rhs.jacobian(y).tolist(), "numpy"))
+
+
+def test_krome_setupfile_rhs_jacobian_and_conservation(oracle):
+    """KROME/FF/CSC: the oracle's RHS and P equal the setup file's own SymPy
+    expressions at random states and rate coefficients (rate i is k(:,i) of
+    the file, so all 21 parameters reach the network), and every RHS
+    conserves H, C and O nuclei and charge (species e-, H, H2, H+, H3+, C,
+    C+, O, OH, H2O, CO, HCO+)."""
+    scope, f_sym, j_sym = _krome_file()
+    assert (scope["neq"], scope["nrea"]) == (12, 21)
+    net = _net(oracle, "KROME/FF/CSC")
+    assert (net.neq, net.nrate, net.nparam) == (12, 21, 21)
+    nuc = np.array([[0, 1, 2, 1, 3, 0, 0, 0, 1, 2, 0, 1],       # H
+                    [0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 1],       # C
+                    [0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1],       # O
+                    [-1, 0, 0, 1, 1, 0, 1, 0, 0, 0, 0, 1]], float)  # charge
+    perm = net.ir["perm"]
+    rng = np.random.default_rng(21)
+    for _ in range(10):
+        y = np.exp(rng.uniform(-8, 8, 12))
+        k = np.exp(rng.uniform(-20, 0, 21))
+        assert np.array_equal(net.rates(y, k), k)
+        f = net.rhs(y, k)
+        ref = np.array(f_sym(y, k), dtype=float)
+        assert np.allclose(f, ref, rtol=1e-13, atol=1e-14 * np.abs(ref).max())
+        for v in nuc:
+            assert abs(v @ f) <= 1e-13 * (np.abs(v) @ np.abs(f)) + 1e-300
+        hb = float(np.exp(rng.uniform(-4, 1)))
+        P = np.eye(12) - hb * np.array(j_sym(y, k), dtype=float)
+        assert np.allclose(net.p_dense(y, k, hb), P[np.ix_(perm, perm)], rtol=1e-13, atol=1e-13 * np.abs(P).max())
+    # each rate coefficient moves exactly the species of its reaction
+    y = np.exp(rng.uniform(-3, 3, 12))
+    base = net.rhs(y, np.zeros(21))
+    assert np.all(base == 0.0)
+    for i in range(21):
+        e = np.zeros(21)
+        e[i] = 1.0
+        assert np.allclose(net.rhs(y, e), np.array(f_sym(y, e), dtype=float), rtol=1e-14, atol=0.0), i

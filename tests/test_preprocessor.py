@@ -92,9 +92,11 @@ def test_primordial_reactions_conserve_elements_and_charge():
 
 def test_emitted_files_are_current(tmp_path):
     """The committed generated headers/IR are what the preprocessor emits."""
-    for base in ("ROBER_NONE_DENSE", "OREGO_NONE_DENSE", "PRIMORDIAL_CMK_CSC", "PRIMORDIAL_FF_CSC"):
+    from odevec_amd.build import KROME_SETUPFILE
+    for base in ("ROBER_NONE_DENSE", "OREGO_NONE_DENSE", "PRIMORDIAL_CMK_CSC", "PRIMORDIAL_FF_CSC", "KROME_FF_CSC"):
         name, ordering, packaging = base.split("_")
-        cli.generate(name, "None" if ordering == "NONE" else ordering, packaging, str(tmp_path))
+        cli.generate(name, "None" if ordering == "NONE" else ordering, packaging, str(tmp_path),
+                     krome_setupfile=KROME_SETUPFILE if name == "KROME" else None)
         for f in ("net_%s.h" % base, "%s.json" % base, "inst_%s.hip" % base):
             with open(os.path.join(GEN, f)) as a, open(os.path.join(str(tmp_path), f)) as b:
                 assert a.read() == b.read(), f
@@ -144,6 +146,18 @@ def test_krome_setupfile(tmp_path):
     assert net.neq == 2 and net.nparam == 1 and net.rate_kind == "param"
     st = S.analyse(net, "None", "DENSE")
     assert len(st.terms) == 2
+    # parameters follow the file's k (KROME's reaction indices), used or not
+    g = tmp_path / "net12.py"
+    g.write_text("neq = 2\nnrea = 12\ny = symbols('y(\\\\:\\\\,1:3)')\nk = symbols('k(\\\\:\\\\,1:13)')\n"
+                 "rhs = Matrix([-k[9]*y[0] + k[1]*y[1], k[9]*y[0] - k[1]*y[1]])\n")
+    net = N.load("KROME", str(g))
+    assert net.nparam == 12 and net.param_names[9] == "k(:,10)" and net.param_names[1] == "k(:,2)"
+    st = S.analyse(net, "None", "DENSE")
+    assert sorted(t.rate for t in st.terms) == [1, 1, 9, 9]
+    # without k: the other symbols in natural order of their names
+    h = tmp_path / "netx.py"
+    h.write_text("neq = 1\nnrea = 2\ny = [Symbol('a')]\nrhs = [Symbol('k10')*y[0] - Symbol('k2')]\n")
+    assert N.load("KROME", str(h)).param_names == ["k2", "k10"]
 
 
 def test_synth64_seeded():
