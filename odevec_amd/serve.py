@@ -10,9 +10,13 @@ target) and tolerances. The server
     its own batch handle (segment table, queue counters, rate workspace) and
     a share of the device's workgroups, so a request that arrives while
     others run starts on a free slot instead of waiting for them: their work
-    overlaps on the device; requests of networks too large for one lane
-    (SYNTH64, 64 species) go through the cooperative kernel on the same
-    slot's stream;
+    overlaps on the device;
+  * runs requests of networks too large for one lane (SYNTH64, 64 species)
+    through the cooperative kernel on a slot of their own, one chunk of at
+    most `coop_chunk` cells per launch: the cooperative kernel fills the
+    device for as long as a launch lasts, so a whole 256k-cell request in
+    one launch would keep the other slots' launches off the device for
+    seconds; between chunks they get it;
   * forms each launch on a free slot from the waiting requests that fit
     (continuous batching: requests join at the next free slot, not when the
     previous batch drains), ordered by cost per cell;
@@ -55,6 +59,11 @@ COST_ORDER = ["PRIMORDIAL/CMK/CSC", "OREGO/NONE/DENSE", "ROBER/NONE/DENSE"]
 # networks served by the cooperative kernel (one launch per request, on the
 # same slot stream as the slot's batch launch) rather than the bundle
 COOP_NETWORKS = {"SYNTH64/FF/CSC"}
+
+
+def is_coop(network: str) -> bool:
+    """Served by the cooperative kernel (the +REPRO build of one too)."""
+    return network.split("+")[0] in COOP_NETWORKS
 
 
 @dataclass
@@ -107,7 +116,7 @@ class Request:
 
 class Server:
     def __init__(self, bundle: str = "MIX3", slots: int = 2, hbm_budget: int = 16 << 30, block: int = 256,
-                 precision: Optional[dict] = None, fp64_fallback: int = 0):
+                 precision: Optional[dict] = None, fp64_fallback: int = 0, coop_chunk: int = 16384):
         import torch
         from .api import Batch
         self.torch = torch
@@ -120,7 +129,11 @@ class Server:
         self.grid = max(1, full // slots)
         self.slots = [{"stream": torch.cuda.Stream(),
                        "batch": Batch(bundle, block=block, grid=self.grid, fp64_fallback=fp64_fallback),
-                       "coop": {}, "lane": {}, "reqs": None, "event": None} for _ in range(slots)]
+                       "lane": {}, "reqs": None, "event": None} for _ in range(slots)]
+        # the cooperative kernel's slot: one request at a time, chunk by chunk
+        self.coop_slot = {"stream": torch.cuda.Stream(), "coop": {}, "req": None, "next": 0, "event": None}
+        self.coop_chunk = int(coop_chunk)
+        self.coop_waiting: deque = deque()
         self.sessions = {}
         self._next_sid = 0
         self.budget = hbm_budget
@@ -143,7 +156,7 @@ class Server:
         requests integrate further; its arrays and history stay on the device
         until close_session."""
         torch = self.torch
-        if network in COOP_NETWORKS:
+        if is_coop(network):
             raise ValueError("%s runs on the cooperative kernel, which has no warm restart" % network)
         n = y.shape[1]
         planes = L.history_planes(network)
@@ -184,7 +197,7 @@ class Server:
         del self.sessions[ss.sid]
 
     def submit(self, r: Request):
-        if r.network in COOP_NETWORKS:
+        if is_coop(r.network):
             if r.precision is None:
                 r.precision = self.precision.get(r.network, L.MIXED)
         else:
@@ -194,7 +207,7 @@ class Server:
                 raise ValueError("network %s precision %d not in bundle %s" % (r.network, r.precision, self.bundle))
         if r.footprint() > self.budget:
             raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.footprint()))
-        self.waiting.append(r)
+        (self.coop_waiting if is_coop(r.network) else self.waiting).append(r)
 
     def _admit(self, stream) -> List[Request]:
         """Upload waiting requests that fit the budget (arrival order, at most
@@ -229,14 +242,79 @@ class Server:
         self.waiting.extendleft(reversed(skipped))
         return new
 
-    def _coop(self, slot, network, precision):
+    def _coop(self, network, precision):
         from .api import Odevec
         key = (network, precision)
-        h = slot["coop"].get(key)
+        h = self.coop_slot["coop"].get(key)
         if h is None:
             h = Odevec(network, 1, precision=precision, mode=L.COOP)
-            slot["coop"][key] = h
+            self.coop_slot["coop"][key] = h
         return h
+
+    def _coop_admit(self) -> bool:
+        """Upload the next cooperative-kernel request, one array set per
+        chunk of at most coop_chunk cells; False if it does not fit yet."""
+        torch = self.torch
+        r = self.coop_waiting[0]
+        if self.in_use + r.footprint() > self.budget:
+            return False
+        self.coop_waiting.popleft()
+        dev = torch.device("cuda:0")
+        chunks = []
+        with torch.cuda.stream(self.coop_slot["stream"]):
+            for c0 in range(0, r.ncells, self.coop_chunk):
+                c1 = min(r.ncells, c0 + self.coop_chunk)
+                up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dev, non_blocking=True)
+                d = {"n": c1 - c0, "y": up(r.y[:, c0:c1]), "t_stop": up(r.t_stop[c0:c1]), "dt0": up(r.dt0[c0:c1]),
+                     "status": torch.empty(c1 - c0, dtype=torch.int32, device=dev)}
+                if r.params is not None:
+                    d["params"] = up(r.params[:, c0:c1])
+                chunks.append(d)
+        r.dev = {"chunks": chunks}
+        self.in_use += r.footprint()
+        self.coop_slot["req"], self.coop_slot["next"] = r, 0
+        return True
+
+    def _coop_launch(self):
+        """Enqueue the current cooperative request's next chunk and the copy
+        of its results to pinned host memory."""
+        torch = self.torch
+        cs = self.coop_slot
+        r = cs["req"]
+        d = r.dev["chunks"][cs["next"]]
+        st = cs["stream"]
+        h = self._coop(r.network, r.precision)
+        h.RelativeTolerance, h.AbsoluteTolerance, h.MinimalTimestep = r.rtol, r.atol, r.dt_min
+        h.solve_cells_async(d["n"], d["t_stop"].data_ptr(), d["y"].data_ptr(), dt0_ptr=d["dt0"].data_ptr(),
+                            params_ptr=d["params"].data_ptr() if "params" in d else None,
+                            status_ptr=d["status"].data_ptr(), stream=st.cuda_stream)
+        with torch.cuda.stream(st):
+            d["y_host"] = torch.empty(tuple(d["y"].shape), dtype=torch.float64, pin_memory=True)
+            d["st_host"] = torch.empty(d["n"], dtype=torch.int32, pin_memory=True)
+            d["y_host"].copy_(d["y"], non_blocking=True)
+            d["st_host"].copy_(d["status"], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        cs["event"] = ev
+        cs["next"] += 1
+        self.launches += 1
+
+    def _coop_poll(self, done: List[Request]):
+        cs = self.coop_slot
+        if cs["req"] is not None and cs["event"].query():
+            r = cs["req"]
+            if cs["next"] < len(r.dev["chunks"]):
+                self._coop_launch()
+                return
+            r.result = np.concatenate([d["y_host"].numpy() for d in r.dev["chunks"]], axis=1)
+            r.status = np.concatenate([d["st_host"].numpy() for d in r.dev["chunks"]])
+            r.done = self.now()
+            self.in_use -= r.footprint()
+            r.dev = {}
+            cs["req"], cs["event"] = None, None
+            done.append(r)
+        if cs["req"] is None and self.coop_waiting and self._coop_admit():
+            self._coop_launch()
 
     def _lane(self, slot, network, precision):
         """The slot's persistent-mode handle for (network, precision): runs
@@ -289,14 +367,6 @@ class Server:
                 r.dev["y"], r.dev["status"] = d["y"], d["status"]
                 advanced.append(r)
                 continue
-            if r.network in COOP_NETWORKS:
-                h = self._coop(slot, r.network, r.precision)
-                h.RelativeTolerance, h.AbsoluteTolerance, h.MinimalTimestep = r.rtol, r.atol, r.dt_min
-                h.solve_cells_async(r.ncells, r.dev["t_stop"].data_ptr(), r.dev["y"].data_ptr(),
-                                    dt0_ptr=r.dev["dt0"].data_ptr(),
-                                    params_ptr=r.dev["params"].data_ptr() if r.params is not None else None,
-                                    status_ptr=r.dev["status"].data_ptr(), stream=st.cuda_stream)
-                continue
             s = {"network": r.network, "precision": r.precision, "y": r.dev["y"], "t_stop": r.dev["t_stop"],
                  "dt0": r.dev["dt0"], "status": r.dev["status"], "rtol": r.rtol, "atol": r.atol, "dt_min": r.dt_min}
             if r.params is not None:
@@ -340,12 +410,15 @@ class Server:
                 if reqs:
                     self._launch(slot, reqs)
                     self.launches += 1
-        self.max_inflight = max(self.max_inflight, sum(s["reqs"] is not None for s in self.slots))
+        self._coop_poll(done)
+        self.max_inflight = max(self.max_inflight, sum(s["reqs"] is not None for s in self.slots) +
+                                (self.coop_slot["req"] is not None))
         self.completed.extend(done)
         return done
 
     def busy(self) -> bool:
-        return bool(self.waiting or any(s["reqs"] is not None for s in self.slots))
+        return bool(self.waiting or self.coop_waiting or self.coop_slot["req"] is not None or
+                    any(s["reqs"] is not None for s in self.slots))
 
     def drain(self):
         while self.busy():
@@ -356,8 +429,10 @@ class Server:
         self.drain()
         for s in self.slots:
             s["batch"].close()
-            for h in list(s["coop"].values()) + list(s["lane"].values()):
+            for h in s["lane"].values():
                 h.CloseOdevec()
+        for h in self.coop_slot["coop"].values():
+            h.CloseOdevec()
 
 
 L_MAXSEG = 1024
@@ -376,11 +451,12 @@ def make_trace(rate: float, seconds: float, seed: int = 5, networks=None, lo: fl
         if t > seconds:
             break
         net = networks[rng.integers(len(networks))]
+        base = net.split("+")[0]            # the +REPRO build of a network: the same inputs
         n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         sd = int(rng.integers(1 << 30))
         w = {"ROBER/NONE/DENSE": W.rober, "OREGO/NONE/DENSE": W.orego, "PRIMORDIAL/CMK/CSC": W.primordial,
-             "SYNTH64/FF/CSC": W.synth64}[net](n, seed=sd)
-        lo_t, hi_t = SERVE_SPANS[net]
+             "SYNTH64/FF/CSC": W.synth64}[base](n, seed=sd)
+        lo_t, hi_t = SERVE_SPANS[base]
         ts = float(np.exp(rng.uniform(np.log(lo_t), np.log(hi_t))))
         rtol = float(rtols[rng.integers(len(rtols))])
         out.append(Request(rid, net, np.ascontiguousarray(w["y"]), np.full(n, ts), rtol,
@@ -404,11 +480,16 @@ def run_trace(server: Server, trace: List[Request]) -> dict:
     lat = np.array([r.done - r.arrival for r in trace]) * 1e3
     cells = sum(r.ncells for r in trace)
     ok = sum(int((r.status == 0).sum()) for r in trace)
+    per_net = {}
+    for net in sorted({r.network for r in trace}):
+        ln = np.array([(r.done - r.arrival) * 1e3 for r in trace if r.network == net])
+        per_net[net] = {"requests": int(ln.size), "p50_ms": float(np.percentile(ln, 50)),
+                        "p99_ms": float(np.percentile(ln, 99))}
     return {"requests": len(trace), "cells": cells, "cells_ok": ok, "wall_s": wall,
             "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
             "mean_ms": float(lat.mean()) if len(lat) else None,
-            "cell_solves_per_s": cells / wall if wall > 0 else None}
+            "cell_solves_per_s": cells / wall if wall > 0 else None, "per_network": per_net}
 
 
 def run_sessions(server: Server, nsess: int = 8, ncells: int = 65536, nadv: int = 5, warm: bool = True,

@@ -38,14 +38,18 @@ def test_serving_results_same_alone_and_in_mix():
     print("GPU serving mix invariance", flush=True)
     trace = S.make_trace(400.0, 0.05, seed=11, lo=1e3, hi=2e4)
     assert len(trace) >= 8
-    srv = S.Server("MIX3", slots=2, precision={"PRIMORDIAL/CMK/CSC": 1})
+    # SYNTH64 requests run through the cooperative slot in chunks of 4096
+    # cells; alone, each runs in one launch
+    assert any(S.is_coop(r.network) and r.ncells > 4096 for r in trace)
+    srv = S.Server("MIX3", slots=2, precision={"PRIMORDIAL/CMK/CSC": 1}, coop_chunk=4096)
     res = S.run_trace(srv, trace)
     srv.close()
     assert res["requests"] == len(trace) and res["p99_ms"] is not None
-    assert srv.max_inflight == 2          # launches overlapped on the device
+    assert set(res["per_network"]) == {r.network for r in trace}
+    assert srv.max_inflight >= 2          # launches overlapped on the device
     for r in trace:
         alone = S.make_trace(400.0, 0.05, seed=11, lo=1e3, hi=2e4)[r.rid]
-        one = S.Server("MIX3", slots=1, precision={"PRIMORDIAL/CMK/CSC": 1})
+        one = S.Server("MIX3", slots=1, precision={"PRIMORDIAL/CMK/CSC": 1}, coop_chunk=1 << 30)
         one.submit(alone)
         one.drain()
         one.close()
@@ -61,17 +65,18 @@ def test_serving_repro_equals_oracle(oracle):
     of each request's cells, the oracle's per-cell solve bit for bit (state
     and status)."""
     print("GPU serving +REPRO vs oracle", flush=True)
-    nets = ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC"]
-    trace = S.make_trace(400.0, 0.03, seed=12, networks=nets, lo=1e3, hi=8e3)
-    assert len(trace) >= 6 and len({r.network for r in trace}) == 3
-    srv = S.Server("MIX3+REPRO", slots=2)
+    nets = ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC", "SYNTH64/FF/CSC+REPRO"]
+    trace = S.make_trace(400.0, 0.04, seed=14, networks=nets, lo=1e3, hi=8e3)
+    assert len(trace) >= 6 and len({r.network for r in trace}) == 4
+    # SYNTH64 through the cooperative slot (FP64: its default is mixed), chunks of 2048 cells
+    srv = S.Server("MIX3+REPRO", slots=2, coop_chunk=2048, precision={"SYNTH64/FF/CSC+REPRO": 0})
     S.run_trace(srv, trace)
     srv.close()
     print("requests", len(trace), "launches", srv.launches, flush=True)
-    rng = np.random.default_rng(12)
+    rng = np.random.default_rng(14)
     for r in trace:
-        idx = np.sort(rng.choice(r.ncells, 48, replace=False))
-        yo, so, _, _ = oracle.Network.load(r.network).solve_cells(
+        idx = np.sort(rng.choice(r.ncells, 16 if S.is_coop(r.network) else 48, replace=False))
+        yo, so, _, _ = oracle.Network.load(r.network.split("+")[0]).solve_cells(
             r.y[:, idx], r.t_stop[idx], dt0=r.dt0[idx], params=None if r.params is None else r.params[:, idx],
             rtol=r.rtol, atol=r.atol, dt_min=r.dt_min)
         assert np.array_equal(so, r.status[idx]) and np.array_equal(yo, r.result[:, idx]), (r.rid, r.network)
