@@ -116,8 +116,8 @@ int32_t odevec_set_dt_min(odevec_t* h, double dt_min);
 int32_t odevec_set_check_negatives(odevec_t* h, int32_t flag);
 
 /* FP64 fallback of the mixed form (no reference counterpart). With
- * convfail_limit > 0, a cell (per-cell or persistent mode, no history
- * attached) that reaches convfail_limit convergence failures stops after that
+ * convfail_limit > 0, a cell (per-cell or persistent mode) that reaches
+ * convfail_limit convergence failures stops after that
  * failure's normal handling (history reset, step size x 0.25, a Jacobian
  * refresh due), leaves its integrator state in a hand-off record and keeps
  * its input state; FP64 lanes continue it from there. In persistent mode they
@@ -241,7 +241,11 @@ int32_t odevec_destroy(odevec_t* h);
  * call continues a cell from it when the call starts at the saved t_stop on
  * the state it returned (t0 and every species and parameter bitwise equal;
  * any other cell starts cold). A solve to t1 continued to t2 then takes the
- * same steps as one solve to t2 and returns the same result bit for bit.
+ * same steps as one solve to t2 and returns the same result bit for bit,
+ * with the FP64 fallback too: the convergence- and invariant-failure counts
+ * the fallback decides on carry over (their counters then count from the
+ * cell's first call), and a cell that FP64 lanes continued is continued by
+ * FP64 lanes again, from its history.
  *
  * odevec_history_planes: doubles per cell of a history for `network`.
  * odevec_set_history: attach d_hist, a device array of planes x ncells
@@ -309,7 +313,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
 
 /* FP64 fallback for the batch's mixed segments (no reference counterpart;
  * see odevec_set_fp64_fallback): with convfail_limit > 0, a cell of a mixed
- * segment (without history) that reaches convfail_limit convergence failures
+ * segment (with or without history) that reaches convfail_limit convergence failures
  * is handed off as for a handle; one FP64 kernel per such segment runs beside
  * the batch launch on the batch's second stream (outside the captured graph)
  * and continues the segment's cells as they are handed back, and the

@@ -231,8 +231,7 @@ int launch(odevec_s* h, const Req& r) {
   // record) and the FP64 persistent kernel continues them over the list of
   // such cells (second launch); on an FP64 handle the cells only move
   // between the two launches, with unchanged results
-  const bool promote = h->promote_cf > 0 && !a.hist &&
-                       (h->opts.mode == ODEVEC_PERCELL || h->opts.mode == ODEVEC_PERSISTENT);
+  const bool promote = h->promote_cf > 0 && (h->opts.mode == ODEVEC_PERCELL || h->opts.mode == ODEVEC_PERSISTENT);
   // the fallback's list and the cost-ordered queue hold cell indices as int
   if ((promote || (h->opts.mode == ODEVEC_PERSISTENT && h->opts.schedule == 1)) && r.ncells > 0x7fffffffLL)
     return fail(ODEVEC_EINVAL, "the FP64 fallback and the cost-ordered queue take at most 2^31 - 1 cells per solve");
@@ -1219,7 +1218,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     if (g.hist) hist = 1;
   }
   if (total == 0) return ODEVEC_OK;
-  // FP64 fallback: mixed segments (no history) hand back the cells that reach
+  // FP64 fallback: mixed segments hand back the cells that reach
   // promote_cf convergence failures; FP64 lanes beside the batch launch (one
   // kernel per such segment, on a second stream, in 1 in FB_SHARE of the
   // machine's wavefront slots) continue them from their hand-off records, and
@@ -1230,7 +1229,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
   if (b->promote_cf > 0) {
     for (int k = 0; k < nseg; ++k) {
       const odevec_segment_t& g = segs[k];
-      if (g.precision != ODEVEC_MIXED || g.hist || g.ncells <= 0) continue;
+      if (g.precision != ODEVEC_MIXED || g.ncells <= 0) continue;
       if (g.ncells > 0x7fffffffLL)
         return fail(ODEVEC_EINVAL, "the FP64 fallback takes at most 2^31 - 1 cells per segment");
       const odv::NetEntry* ne = find_net_build(g.network, b->b->repro);

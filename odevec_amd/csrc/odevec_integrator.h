@@ -61,6 +61,8 @@ enum Status : int {
   ST_SKIPPED = 5,      // masked out / empty span: y returned unchanged
   ST_PROMOTE = 6,      // mixed form: too many convergence failures, the cell is re-solved in FP64
                        // (never returned to the caller)
+  ST_PROMOTE_HIST = 7, // mixed form with a warm-restart history: the cell went on in FP64 in the
+                       // last call, so FP64 lanes continue it from its history (never returned)
 };
 
 constexpr int MAXORD = 5;          // pre_odevec.py:83

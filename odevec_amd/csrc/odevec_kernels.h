@@ -170,12 +170,17 @@ __device__ inline void odv_store(const KArgs& a, long long i, const L& s, const 
 // the saved ones), else the cell starts cold. Steps are never clamped to
 // t_stop (the last step overshoots and the result is interpolated), so a
 // solve to t1 continued to t2 takes exactly the steps of one solve to t2.
+// The counts the mixed form's FP64 fallback decides on (convergence and
+// invariant failures) carry over, and a cell that FP64 lanes continued
+// (flag 8) is continued by FP64 lanes again, from its history, so with the
+// fallback too a continued solve equals one solve.
 template <class Net>
 struct Hist {
   static constexpr int N = Net::NACT, NF = Net::NEQ, NP = Net::NPARAM;
   static constexpr int VALID = 0, TSTOP = 1, T = 2, DT = 3, RATE = 4, JCH = 5, CERR = 6;
-  // q, steps since Jacobian, lju, failed tests, conv fails, flags, fz, failed invariant tests
-  static constexpr int INTS = 7, NINT = 8;
+  // q, steps since Jacobian, lju, failed tests, conv fails, flags, fz, failed invariant tests,
+  // convergence failures
+  static constexpr int INTS = 7, NINT = 9;
   static constexpr int Y = INTS + NINT;      // the state returned (all NEQ species)
   static constexpr int P = Y + NF;           // the parameters the rates came from
   static constexpr int Z = P + NP;           // Nordsieck history, column-major (c*N + j)
@@ -250,6 +255,14 @@ __device__ inline void odv_handoff_save(const KArgs& a, long long i, const L& s)
   }
 }
 
+template <class Net>
+__device__ inline void odv_handoff_no_counters(const KArgs& a, long long i) {
+  using H = Handoff<Net>;
+  double* h = a.ho + i * H::REC + H::CNT;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ho_st(&h[k], 0.0);
+}
+
 // the counters a handed-back cell brought from the launch that stopped it
 template <class Net>
 __device__ inline Counters odv_handoff_counters(const KArgs& a, long long i) {
@@ -300,11 +313,11 @@ __device__ inline void odv_handoff_resume(const KArgs& a, long long i, L& s, con
   if constexpr (!L::WX_) s.fz = (s.fz & ~8) | (nan ? 8 : 0);
 }
 
-template <class L>
-__device__ inline bool odv_hist_resume(const KArgs& a, long long i, L& s, const double* y, const double* prm,
-                                       double t0, double ts) {
-  using H = Hist<typename L::Net>;
-  using M = typename L::M;
+// cell i's history continues this call: valid, saved where this call starts,
+// on the state and parameters it returned
+template <class Net>
+__device__ inline bool odv_hist_match(const KArgs& a, long long i, const double* y, const double* prm, double t0) {
+  using H = Hist<Net>;
   const double* h = a.hist + i * H::PLANES;
   if (h[H::VALID] != 1.0 || h[H::TSTOP] != t0) return false;
   bool same = true;
@@ -312,7 +325,35 @@ __device__ inline bool odv_hist_resume(const KArgs& a, long long i, L& s, const 
   for (int j = 0; j < H::NF; ++j) same = same && (h[H::Y + j] == y[j]);
 #pragma unroll
   for (int p = 0; p < H::NP; ++p) same = same && (h[H::P + p] == prm[p]);
-  if (!same) return false;
+  return same;
+}
+
+// FP64 lanes of the fallback continued cell i in the call that saved its history
+template <class Net>
+__device__ inline bool odv_hist_fp64(const KArgs& a, long long i) {
+  using H = Hist<Net>;
+  return (((int)a.hist[i * H::PLANES + H::INTS + 5]) & 8) != 0;
+}
+
+// the rates saved with cell i's history (computed once, from the state the
+// cell's integration started from)
+template <class L>
+__device__ inline void odv_hist_rates(const KArgs& a, long long i, L& s) {
+  using H = Hist<typename L::Net>;
+  const double* h = a.hist + i * H::PLANES;
+#pragma unroll
+  for (int r = 0; r < L::Net::NRATE; ++r) {
+    const double v = h[H::K + r];
+    if (L::KS == 0) s.kl[r] = v;
+    else s.kx[r * L::KS] = v;
+  }
+}
+
+template <class L>
+__device__ inline void odv_hist_resume(const KArgs& a, long long i, L& s, double ts) {
+  using H = Hist<typename L::Net>;
+  using M = typename L::M;
+  const double* h = a.hist + i * H::PLANES;
   s.t = h[H::T];
   s.dt = h[H::DT];
   s.rate = h[H::RATE];
@@ -329,6 +370,7 @@ __device__ inline bool odv_hist_resume(const KArgs& a, long long i, L& s, const 
   s.neglect_up = (fl & 4) != 0;
   s.fz = (int)h[H::INTS + 6];
   s.cnt.invfail = (int)h[H::INTS + 7];
+  s.cnt.convfail = (int)h[H::INTS + 8];
   s.status = ST_RUNNING;
   s.tstop = ts;
 #pragma unroll
@@ -343,18 +385,13 @@ __device__ inline bool odv_hist_resume(const KArgs& a, long long i, L& s, const 
   }
 #pragma unroll
   for (int e = 0; e < L::Net::NNZ; ++e) s.a[e] = (M)h[H::A + e];
-#pragma unroll
-  for (int r = 0; r < L::Net::NRATE; ++r) {
-    const double v = h[H::K + r];
-    if (L::KS == 0) s.kl[r] = v;
-    else s.kx[r * L::KS] = v;
-  }
-  return true;
+  odv_hist_rates(a, i, s);
 }
 
-// after odv_store: a.y holds the state returned to the caller
+// after odv_store: a.y holds the state returned to the caller; fp64: the
+// cell ran on an FP64 lane of the fallback
 template <class L>
-__device__ inline void odv_hist_save(const KArgs& a, long long i, const L& s) {
+__device__ inline void odv_hist_save(const KArgs& a, long long i, const L& s, bool fp64) {
   using H = Hist<typename L::Net>;
   const long long n = a.ncells;
   double* h = a.hist + i * H::PLANES;
@@ -374,9 +411,10 @@ __device__ inline void odv_hist_save(const KArgs& a, long long i, const L& s) {
   h[H::INTS + 2] = (double)s.lju;
   h[H::INTS + 3] = (double)s.fet;
   h[H::INTS + 4] = (double)s.fc;
-  h[H::INTS + 5] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0));
+  h[H::INTS + 5] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0) | (fp64 ? 8 : 0));
   h[H::INTS + 6] = (double)s.fz;
   h[H::INTS + 7] = (double)s.cnt.invfail;
+  h[H::INTS + 8] = (double)s.cnt.convfail;
 #pragma unroll
   for (int j = 0; j < H::NF; ++j) h[H::Y + j] = a.y[(long long)j * n + i];
 #pragma unroll
@@ -424,10 +462,21 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
     return true;
   }
   if constexpr (HIST) {
-    if (a.hist && odv_hist_resume(a, i, s, y, prm, t0, ts)) {
-      if (s.t < s.tstop) return false;
-      s.status = ST_OK;   // the last step already passed the new t_stop: interpolate
-      return true;
+    if (a.hist && odv_hist_match<Net>(a, i, y, prm, t0)) {
+      const bool f64 = odv_hist_fp64<Net>(a, i);
+      if (a.ho_mode == 1 && f64) {
+        // FP64 lanes continued the cell last call: they continue it again
+        s.status = ST_PROMOTE_HIST;
+        return true;
+      }
+      // (an FP64 lane of the fallback takes a cell handed back in this call
+      // from its hand-off record, below)
+      if (a.ho_mode != 2 || f64) {
+        odv_hist_resume(a, i, s, ts);
+        if (s.t < s.tstop) return false;
+        s.status = ST_OK;   // the last step already passed the new t_stop: interpolate
+        return true;
+      }
     }
   }
   if constexpr (sizeof(typename L::M) == sizeof(double)) {
@@ -435,6 +484,12 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
     // handed it back stopped it
     if (a.ho_mode == 2 && a.ho) {
       odv_handoff_resume(a, i, s, y, prm, ts);
+      if constexpr (HIST) {
+        // a cell this call continued from its history before handing it
+        // back: its rates are the history's (the rates come from the state
+        // the integration started from, not from this call's input state)
+        if (a.hist && odv_hist_match<Net>(a, i, y, prm, t0)) odv_hist_rates(a, i, s);
+      }
       return false;
     }
   }
@@ -446,10 +501,14 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
 template <class L, bool HIST = false>
 __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   if (a.times) a.times[a.ncells + i] = (long long)wall_clock64();
-  if (s.status == ST_PROMOTE) {
+  if (s.status == ST_PROMOTE || s.status == ST_PROMOTE_HIST) {
     // the caller's state stays untouched for the FP64 pass over promo[],
-    // which continues the cell from its hand-off record
-    if (a.ho) odv_handoff_save(a, i, s);
+    // which continues the cell from its hand-off record (or, ST_PROMOTE_HIST,
+    // from its history; the record then only says that no work came with it)
+    if (a.ho) {
+      if (s.status == ST_PROMOTE) odv_handoff_save(a, i, s);
+      else odv_handoff_no_counters<typename L::Net>(a, i);
+    }
     // the entry is published once the record's stores have completed (an
     // FP64 lane of the same launch may be waiting for it)
     __builtin_amdgcn_s_waitcnt(0);
@@ -467,7 +526,7 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   odv_result(s, y);
   odv_store(a, i, s, y);
   if constexpr (HIST) {
-    if (a.hist) odv_hist_save(a, i, s);
+    if (a.hist) odv_hist_save(a, i, s, a.ho_mode == 2);
   }
 }
 
@@ -610,7 +669,7 @@ __device__ __forceinline__ void odv_persistent_run(const KArgs& a, double* lds, 
 // in the build without contraction: 29 of 113 PRIMORDIAL cells, the same
 // ones at every run); without the read they agree bit for bit. Columns: b.ws
 // (a workspace for the FP64 layout) and the wavefront's LDS slice.
-template <class Net>
+template <class Net, bool HIST = false>
 __device__ __forceinline__ void odv_fb_consume(const KArgs& b, double* lds, const unsigned* done, unsigned nprod,
                                                long long idle_rounds) {
   using P = LdsPolicy<Net, double>;
@@ -628,7 +687,7 @@ __device__ __forceinline__ void odv_fb_consume(const KArgs& b, double* lds, cons
   long long idle = 0;       // looks at the list in a row with nothing to run
   while (true) {
     if (have && fin) {
-      odv_finish<L>(b, i, s);
+      odv_finish<L, HIST>(b, i, s);
       if (b.flops && s.status != ST_SKIPPED) {
         const Counters c0 = odv_handoff_counters<Net>(b, i);
         const Counters& c = s.cnt;
@@ -673,7 +732,7 @@ __device__ __forceinline__ void odv_fb_consume(const KArgs& b, double* lds, cons
         __hip_atomic_store(&list[pos], -2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         i = v;
         have = true;
-        fin = odv_load_begin<L>(b, i, s);
+        fin = odv_load_begin<L, HIST>(b, i, s);
         pos = -1;
       }
     }
@@ -711,10 +770,10 @@ __device__ __forceinline__ void odv_fb_consume(const KArgs& b, double* lds, cons
 // The FP64 lanes of the fallback as a kernel of their own, launched on a
 // second stream beside the persistent kernel whose hand-backs they take
 // (nprod: that kernel's wavefronts); one wavefront per workgroup.
-template <class Net>
+template <class Net, bool HIST = false>
 __global__ void __launch_bounds__(WAVE) odv_k_fb_consume(KArgs b, unsigned* done, unsigned nprod,
                                                          long long idle_rounds) {
-  odv_fb_consume<Net>(b, odv_lane_lds, done, nprod, idle_rounds);
+  odv_fb_consume<Net, HIST>(b, odv_lane_lds, done, nprod, idle_rounds);
 }
 
 // With a.order and a.qlen the queue is the first *a.qlen entries of a.order

@@ -224,9 +224,14 @@ hipError_t launch_fbc_net(const KArgs& b, unsigned* done, int nprod, long long g
     if (grid <= 0 || grid > 0x7fffffffLL || nprod <= 0) return hipErrorInvalidValue;
     if ((LdsPolicy<Net, double>::WS_PER_SLOT > 0 && !b.ws) || !b.order || !b.qlen || !b.queue || !b.ho || !done)
       return hipErrorInvalidValue;
-    hipLaunchKernelGGL(odv_k_fb_consume<Net>, dim3((unsigned)grid), dim3(WAVE),
-                       lane_lds_bytes(LdsPolicy<Net, double>::WORDS, WAVE), st, b, done, (unsigned)nprod,
-                       FB_IDLE_ROUNDS);
+    if (b.hist)
+      hipLaunchKernelGGL((odv_k_fb_consume<Net, true>), dim3((unsigned)grid), dim3(WAVE),
+                         lane_lds_bytes(LdsPolicy<Net, double>::WORDS, WAVE), st, b, done, (unsigned)nprod,
+                         FB_IDLE_ROUNDS);
+    else
+      hipLaunchKernelGGL((odv_k_fb_consume<Net, false>), dim3((unsigned)grid), dim3(WAVE),
+                         lane_lds_bytes(LdsPolicy<Net, double>::WORDS, WAVE), st, b, done, (unsigned)nprod,
+                         FB_IDLE_ROUNDS);
     return hipGetLastError();
   }
 }

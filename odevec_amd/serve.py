@@ -35,7 +35,9 @@ each cell's integration where the previous one stopped (Nordsieck history,
 order, step size, LU factors), so it does not pay the cold start's order-1
 steps, fresh Jacobian and initial step-size search again, and nothing but
 the end times travels to the device. One session's advances run in order;
-different sessions' advances overlap like other requests.
+different sessions' advances overlap like other requests. Mixed-precision
+sessions keep the FP64 fallback: a cell that FP64 lanes took over goes on
+in FP64 in later advances, so advances equal one solve there too.
 
     python -m odevec_amd.serve --rates 4,8,16,32 --seconds 10
 """
@@ -141,6 +143,7 @@ class Server:
         self.waiting: deque = deque()      # arrived, not yet admitted
         self.completed: List[Request] = []
         self.precision = precision or {}
+        self.fp64_fallback = int(fp64_fallback)
         self.launches = 0
         self.max_inflight = 0      # most launches seen in flight at once
         self.t0 = time.perf_counter()
@@ -328,6 +331,8 @@ class Server:
             ncu = self.torch.cuda.get_device_properties(0).multi_processor_count
             grid = max(1, ncu * max(k["blocks_per_cu"], 1) // len(self.slots))
             h = Odevec(network, 1, precision=precision, mode=L.PERSISTENT, grid=grid)
+            if self.fp64_fallback and precision == L.MIXED:
+                h.set_fp64_fallback(self.fp64_fallback)
             slot["lane"][key] = h
         return h
 

@@ -292,3 +292,45 @@ def test_batching_configuration_full_size():
         assert np.array_equal(ss, st[idx]) and np.array_equal(ys, y[:, idx]), w["network"]
         assert np.array_equal(ds, dto[idx]), w["network"]
     torch.cuda.empty_cache()
+
+
+def test_batch_history_with_fallback_equals_one_solve():
+    """A mixed segment with a warm-restart history and the FP64 fallback
+    (limit 2): the batch solve split at t1 -- cells handed to FP64 lanes
+    before t1 are handed straight to them again in the second launch and
+    continued from the history -- equals the single batch solve with the
+    fallback bit for bit (states, status, last steps; attempts, steps,
+    Newton iterations and Jacobians add up). Autonomous network over
+    [-S, 0], so the first step does not depend on t_stop. The reproducible
+    bundle: the history's kernel is another instantiation than the plain
+    one, and without contraction both do the same arithmetic."""
+    from odevec_amd.api import Batch
+    import torch
+    w = W.primordial(2000, seed=48)
+    n = w["y"].shape[1]
+    span = np.asarray(w["t_stop"], dtype=np.float64)
+    limit = 2
+    say("GPU batch history + fallback")
+    single = dict(w, t0=-span, t_stop=np.zeros(n))
+    b = Batch("MIX3+REPRO", fp64_fallback=limit)
+    one = run("MIX3+REPRO", [to_dev(single, _lib.MIXED)], batch=b)[0]
+    planes = _lib.history_planes(w["network"])
+    hist = torch.zeros(planes * n, dtype=torch.float64, device="cuda")
+    seg = to_dev(dict(w, t0=-span, t_stop=-span * 0.55), _lib.MIXED)
+    seg["hist"] = hist
+    ya, sa, _, ca = run("MIX3+REPRO", [seg], batch=b)[0]
+    seg["t0"] = seg["t_stop"].clone()
+    seg["t_stop"] = torch.zeros(n, dtype=torch.float64, device="cuda")
+    seg["counters"].zero_()
+    yb, sb, db, cb = run("MIX3+REPRO", [seg], batch=b)[0]
+    b.close()
+    y1, s1, d1, c1 = one
+    ok = (sa == 0) & (s1 == 0)
+    say("status: single", np.bincount(s1, minlength=7).tolist(), "to t1", np.bincount(sa, minlength=7).tolist())
+    assert ok.mean() > 0.99 and np.array_equal(sb[sa == 0], s1[sa == 0])
+    before, handed = ca[5] >= limit, c1[5] >= limit
+    say("handed over: before t1", int((before & handed & ok).sum()), "after", int((~before & handed & ok).sum()))
+    assert (before & handed & ok).sum() >= 5 and (~before & handed & ok).sum() >= 5
+    assert np.array_equal(yb[:, ok], y1[:, ok]) and np.array_equal(db[ok], d1[ok])
+    for k, name in enumerate(("attempts", "steps", "newton", "jac")):
+        assert np.array_equal(ca[k][ok] + cb[k][ok], c1[k][ok]), name

@@ -151,3 +151,77 @@ def test_history_rejects_other_sizes_and_modes():
     with pytest.raises(_lib.OdevecError):
         lock.enable_history(64)
     lock.CloseOdevec()
+
+
+def _dev_solve(ode, w, y_dev, t0, ts):
+    """One device-resident solve on `ode` (torch tensors): status, last
+    steps and the 8 per-cell counters."""
+    import torch
+    n = y_dev.shape[1]
+    dev = y_dev.device
+    stream = torch.cuda.Stream()
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    t0d = torch.tensor(np.ascontiguousarray(t0, dtype=np.float64), device=dev)
+    tsd = torch.tensor(np.ascontiguousarray(ts, dtype=np.float64), device=dev)
+    dt0 = torch.tensor(np.broadcast_to(w["dt0"], (n,)).astype(np.float64), device=dev)
+    st = torch.zeros(n, dtype=torch.int32, device=dev)
+    dto = torch.zeros(n, dtype=torch.float64, device=dev)
+    cnt = torch.zeros(8 * n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    ode.solve_cells_async(n, tsd.data_ptr(), y_dev.data_ptr(), t0_ptr=t0d.data_ptr(), dt0_ptr=dt0.data_ptr(),
+                          status_ptr=st.data_ptr(), dt_out_ptr=dto.data_ptr(), counters_ptr=cnt.data_ptr(),
+                          stream=stream.cuda_stream)
+    stream.synchronize()
+    return st.cpu().numpy(), dto.cpu().numpy(), cnt.cpu().numpy().reshape(8, n)
+
+
+@pytest.mark.parametrize("mode", [_lib.PERCELL, _lib.PERSISTENT])
+def test_continue_with_fp64_fallback_equals_one_solve(mode):
+    """Mixed form with the FP64 fallback (limit 2 convergence failures, so
+    that many cells go on in FP64) and a history: a solve split at t1 equals
+    the single solve with the fallback bit for bit (states, status, last
+    steps; attempts, steps, Newton iterations and Jacobians add up), both
+    for cells handed to FP64 lanes before t1 -- the second call hands them
+    straight to FP64 lanes, which continue from the history -- and for cells
+    handed over only after t1, whose convergence-failure count carried over."""
+    import torch
+    w = W.primordial(3000, seed=47)
+    nid = w["network"] + "+REPRO"
+    n = w["y"].shape[1]
+    t0, t1, t2 = split_times(w, 0.45)
+    limit = 2
+    say("history + fallback", nid, "mode", mode, "cells", n)
+    dev = torch.device("cuda:0")
+    one = make(nid, n, _lib.MIXED, mode, hist=False)
+    one.set_fp64_fallback(limit)
+    y_one = torch.tensor(np.ascontiguousarray(w["y"]), device=dev)
+    st1, dt1, c1 = _dev_solve(one, w, y_one, t0, t2)
+    one.CloseOdevec()
+    ode = make(nid, n, _lib.MIXED, mode)
+    ode.set_fp64_fallback(limit)
+    y = torch.tensor(np.ascontiguousarray(w["y"]), device=dev)
+    sta, _, ca = _dev_solve(ode, w, y, t0, t1)
+    ya = y.cpu().numpy()
+    stb, dtb, cb = _dev_solve(ode, w, y, t1, t2)
+    ode.CloseOdevec()
+    say("status counts: single", np.bincount(st1, minlength=7).tolist(), "to t1",
+        np.bincount(sta, minlength=7).tolist(), "t1 to t2", np.bincount(stb, minlength=7).tolist())
+    # a cell the algorithm fails on before t1 starts cold in the second call
+    ok = (sta == 0) & (st1 == 0)
+    assert ok.mean() > 0.99
+    assert np.array_equal(stb[sta == 0], st1[sta == 0])
+    before = ca[5] >= limit                                      # convergence failures by t1
+    handed = c1[5] >= limit                                      # the single solve handed them over
+    say("handed over: before t1", int((before & handed & ok).sum()), "after t1", int((~before & handed & ok).sum()))
+    assert (before & handed & ok).sum() >= 5 and (~before & handed & ok).sum() >= 5
+    assert np.array_equal(y.cpu().numpy()[:, ok], y_one.cpu().numpy()[:, ok])
+    assert np.array_equal(dtb[ok], dt1[ok])
+    for k, name in enumerate(("attempts", "steps", "newton", "jac")):
+        assert np.array_equal(ca[k][ok] + cb[k][ok], c1[k][ok]), name
+    # the first call's states are those of a single solve to t1
+    one = make(nid, n, _lib.MIXED, mode, hist=False)
+    one.set_fp64_fallback(limit)
+    y_t1 = torch.tensor(np.ascontiguousarray(w["y"]), device=dev)
+    st_t1, _, _ = _dev_solve(one, w, y_t1, t0, t1)
+    one.CloseOdevec()
+    assert np.array_equal(st_t1, sta) and np.array_equal(ya[:, ok], y_t1.cpu().numpy()[:, ok])
