@@ -235,7 +235,7 @@ int32_t odevec_destroy(odevec_t* h);
  * No reference counterpart: the reference's SolveODE cold-starts every call
  * (src/odevec.F90:170-182: order 1, fresh Jacobian, initial step size). With
  * a history attached, every per-cell solve on the handle (sync or async,
- * per-cell or persistent mode) saves each successfully solved cell's
+ * per-cell, persistent or cooperative mode) saves each successfully solved cell's
  * integrator state (Nordsieck history, order, step size, LU factors, error
  * weights, rates, Jacobian-refresh counters) into the history, and a later
  * call continues a cell from it when the call starts at the saved t_stop on
@@ -246,6 +246,10 @@ int32_t odevec_destroy(odevec_t* h);
  * the fallback decides on carry over (their counters then count from the
  * cell's first call), and a cell that FP64 lanes continued is continued by
  * FP64 lanes again, from its history.
+ *
+ * The cooperative mode keeps histories (in a layout of its own) for networks
+ * too large for one lane, which only it runs (SYNTH64); for the others it
+ * refuses one.
  *
  * odevec_history_planes: doubles per cell of a history for `network`.
  * odevec_set_history: attach d_hist, a device array of planes x ncells

@@ -454,7 +454,8 @@ int32_t odevec_history_planes(const char* network, int64_t* planes) {
   const odv::NetEntry* e = find_net(network);
   if (!e) return fail(ODEVEC_ENOTFOUND, std::string("unknown network ") + (network ? network : "(null)"));
   if (!planes) return fail(ODEVEC_EINVAL, "planes is NULL");
-  if (!e->lane_ok) return fail(ODEVEC_EINVAL, std::string(e->id) + ": warm restart needs the per-lane kernels");
+  if (!e->lane_ok && !e->coop_ok)
+    return fail(ODEVEC_EINVAL, std::string(e->id) + ": warm restart needs the per-lane or the cooperative kernels");
   *planes = e->hist_planes;
   return ODEVEC_OK;
 }
@@ -467,8 +468,10 @@ int32_t odevec_set_history(odevec_t* h, double* d_hist, int64_t ncells) {
     return ODEVEC_OK;
   }
   if (ncells <= 0) return fail(ODEVEC_EINVAL, "ncells must be > 0");
-  if (h->opts.mode != ODEVEC_PERCELL && h->opts.mode != ODEVEC_PERSISTENT)
-    return fail(ODEVEC_EINVAL, "warm restart needs the per-cell or persistent mode (one step control per cell)");
+  const bool coop_only = h->opts.mode == ODEVEC_COOP && !h->net->lane_ok;
+  if (h->opts.mode != ODEVEC_PERCELL && h->opts.mode != ODEVEC_PERSISTENT && !coop_only)
+    return fail(ODEVEC_EINVAL, "warm restart needs the per-cell or persistent mode (one step control per cell), "
+                               "or the cooperative mode for a network too large for one lane");
   h->d_hist = d_hist;
   h->hist_cells = ncells;
   return ODEVEC_OK;

@@ -290,6 +290,131 @@ struct CoopGroup {
 template <class Net, class M, bool RCP>
 using CoopLane = Lane<Cfg<CoopNet<Net, CoopWidth<Net::NACT>::G>, M, RCP, 0, false>>;
 
+// Warm restart of the cooperative kernel (no reference counterpart; the
+// semantics of the per-lane kernels' Hist): cell i's record at
+// hist[i*PLANES ...]. The group's scalars once; each lane's own state (the
+// padding lanes' too, so a continued cell repeats their arithmetic exactly)
+// in planes of G, lane l at [plane*G + l], so the lanes of a group write and
+// read consecutive doubles.
+template <class Net>
+struct CoopHist {
+  static constexpr int G = CoopWidth<Net::NACT>::G;
+  static constexpr int NB = Net::NACT, NF = Net::NEQ, NP = Net::NPARAM;
+  static constexpr int VALID = 0, TSTOP = 1, T = 2, DT = 3, RATE = 4, JCH = 5, CERR = 6;
+  // q, steps since Jacobian, lju, failed tests, conv fails, flags, failed
+  // invariant tests, convergence failures
+  static constexpr int INTS = 7, NINT = 8;
+  static constexpr int Y = INTS + NINT;        // the state returned (all NEQ species)
+  static constexpr int P = Y + NF;             // the parameters
+  static constexpr int Z = P + NP;             // Nordsieck column c of lane l: Z + c*G + l
+  static constexpr int W = Z + NCOL * G;       // error weight
+  static constexpr int EO = W + G;             // previous accepted correction
+  static constexpr int FZ = EO + G;            // frozen-species flags (bit 3 is the lane's own)
+  static constexpr int A = FZ + G;             // entry j of lane l's row of the factors: A + j*G + l
+  static constexpr int K = A + NB * G;         // rate coefficients
+  static constexpr int PLANES = K + Net::NRATE;
+};
+
+// any lane of this lane's group
+template <int G>
+__device__ inline bool coop_any(bool b) {
+  const unsigned long long ball = __ballot(b);
+  const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << coop_base<G>());
+  return (ball & gm) != 0ull;
+}
+
+// continue cell i from its history if the call starts where it stopped, on
+// the state and parameters it returned (in the group's LDS block)
+template <class Net, class L>
+__device__ inline bool coop_hist_resume(const KArgs& a, long long i, L& s, const double* b, double t0,
+                                        double ts) {
+  using H = CoopHist<Net>;
+  using Y = CoopLayout<Net>;
+  using M = typename L::M;
+  constexpr int G = H::G;
+  const int l = coop_l<G>();
+  const double* h = a.hist + i * H::PLANES;
+  bool ok = h[H::VALID] == 1.0 && h[H::TSTOP] == t0;
+  for (int j = l; j < H::NF; j += G) ok = ok && h[H::Y + j] == b[Y::YF + j];
+  for (int p = l; p < H::NP; p += G) ok = ok && h[H::P + p] == b[Y::PR + p];
+  if (coop_any<G>(!ok)) return false;
+  s.t = h[H::T];
+  s.dt = h[H::DT];
+  s.rate = h[H::RATE];
+  s.jchange = h[H::JCH];
+  s.cerr = h[H::CERR];
+  odv_set_order(s, (int)h[H::INTS]);
+  s.swu = (int)h[H::INTS + 1];
+  s.lju = (int)h[H::INTS + 2];
+  s.fet = (int)h[H::INTS + 3];
+  s.fc = (int)h[H::INTS + 4];
+  const int fl = (int)h[H::INTS + 5];
+  s.first = (fl & 1) != 0;
+  s.upd_jac = (fl & 2) != 0;
+  s.neglect_up = (fl & 4) != 0;
+  s.cnt.invfail = (int)h[H::INTS + 6];
+  s.cnt.convfail = (int)h[H::INTS + 7];
+  s.status = ST_RUNNING;
+  s.tstop = ts;
+#pragma unroll
+  for (int c = 0; c < NCOL; ++c) s.set_z(c, 0, h[H::Z + c * G + l]);
+  s.set_w(0, h[H::W + l]);
+  s.set_eo(0, h[H::EO + l]);
+  s.en[0] = 0.0;
+  s.fz = (int)h[H::FZ + l];
+#pragma unroll
+  for (int j = 0; j < H::NB; ++j) s.a[j] = (M)h[H::A + j * G + l];
+  s.kl[0] = 0.0;   // the rates live in the group's LDS block
+  double* bw = const_cast<double*>(b);
+  for (int r = l; r < Net::NRATE; r += G) bw[Y::K + r] = h[H::K + r];
+  odv_wave_sync();
+  return true;
+}
+
+// after a successful solve: yv this lane's result (its species', if active)
+template <class Net, class L>
+__device__ inline void coop_hist_save(const KArgs& a, long long i, const L& s, double* b, double yv) {
+  using H = CoopHist<Net>;
+  using Y = CoopLayout<Net>;
+  constexpr int G = H::G;
+  const int l = coop_l<G>();
+  double* h = a.hist + i * H::PLANES;
+  if (s.status != ST_OK) {
+    if (l == 0) h[H::VALID] = 0.0;
+    return;
+  }
+  // the state returned: the block's full state with the integrated species' results
+  if (l < H::NB) b[Y::YF + Net::ACT[l]] = yv;
+  odv_wave_sync();
+  for (int j = l; j < H::NF; j += G) h[H::Y + j] = b[Y::YF + j];
+  for (int p = l; p < H::NP; p += G) h[H::P + p] = b[Y::PR + p];
+  for (int r = l; r < Net::NRATE; r += G) h[H::K + r] = b[Y::K + r];
+#pragma unroll
+  for (int c = 0; c < NCOL; ++c) h[H::Z + c * G + l] = s.z(c, 0);
+  h[H::W + l] = s.w(0);
+  h[H::EO + l] = s.eo(0);
+  h[H::FZ + l] = (double)s.fz;
+#pragma unroll
+  for (int j = 0; j < H::NB; ++j) h[H::A + j * G + l] = (double)s.a[j];
+  if (l == 0) {
+    h[H::TSTOP] = s.tstop;
+    h[H::T] = s.t;
+    h[H::DT] = s.dt;
+    h[H::RATE] = s.rate;
+    h[H::JCH] = s.jchange;
+    h[H::CERR] = s.cerr;
+    h[H::INTS] = (double)s.q;
+    h[H::INTS + 1] = (double)s.swu;
+    h[H::INTS + 2] = (double)s.lju;
+    h[H::INTS + 3] = (double)s.fet;
+    h[H::INTS + 4] = (double)s.fc;
+    h[H::INTS + 5] = (double)((s.first ? 1 : 0) | (s.upd_jac ? 2 : 0) | (s.neglect_up ? 4 : 0));
+    h[H::INTS + 6] = (double)s.cnt.invfail;
+    h[H::INTS + 7] = (double)s.cnt.convfail;
+    h[H::VALID] = 1.0;
+  }
+}
+
 // Persistent: each group takes cells from a queue (its first cell is its
 // global group id; the counter starts at the number of groups launched)
 // until none is left, so a long cell does not hold a finished group idle.
@@ -309,7 +434,9 @@ using CoopLane = Lane<Cfg<CoopNet<Net, CoopWidth<Net::NACT>::G>, M, RCP, 0, fals
 constexpr int COOP_WAVES_F64 = 2;
 constexpr int COOP_WAVES_F32 = 3;
 
-template <class Net, class M, bool RCP>
+// HIST: the variant that continues cells from a warm-restart history and
+// saves it (for networks that only this kernel runs)
+template <class Net, class M, bool RCP, bool HIST = false>
 __global__ void __launch_bounds__(COOP_BLOCK)
 __attribute__((amdgpu_waves_per_eu(CoopWidth<Net::NACT>::G < 64 ? 1 : (sizeof(M) == 8 ? COOP_WAVES_F64 : COOP_WAVES_F32))))
 odv_k_coop(KArgs a) {
@@ -342,12 +469,23 @@ odv_k_coop(KArgs a) {
         if (a.counters) odv_store_counters(a, i, s.cnt);
       }
     } else {
-      odv_begin(s, b + Y::YF, b + Y::PR, t0, ts, dh, a.ctl, g);
-      while (!odv_attempt(s, a.ctl, g)) {
+      bool resumed = false;
+      if constexpr (HIST) {
+        if (a.hist) resumed = coop_hist_resume<Net>(a, i, s, b, t0, ts);
+      }
+      if (!resumed) odv_begin(s, b + Y::YF, b + Y::PR, t0, ts, dh, a.ctl, g);
+      if (resumed && !(s.t < s.tstop)) {
+        s.status = ST_OK;   // the last step already passed the new t_stop: interpolate
+      } else {
+        while (!odv_attempt(s, a.ctl, g)) {
+        }
       }
       double yv[1];
       odv_result(s, yv);
       if (l < Net::NACT) a.y[(long long)Net::ACT[l] * a.ncells + i] = yv[0];
+      if constexpr (HIST) {
+        if (a.hist) coop_hist_save<Net>(a, i, s, b, yv[0]);
+      }
       if (l == 0) {
         if (a.status) a.status[i] = s.status;
         if (a.dt_out) a.dt_out[i] = s.dt;

@@ -33,7 +33,8 @@ struct NetEntry {
   int neq, nnz, nrate, nparam, rate_kind, tgas;
   int flops_rhs, flops_jac, flops_lu, flops_solve, flops_lu_r, flops_solve_r;
   int lane_ok, coop_ok;   // per-lane modes (percell, lockstep, persistent) / cooperative mode compiled
-  int hist_planes;        // warm-restart history planes per cell (per-lane modes)
+  int hist_planes;        // warm-restart history planes per cell (per-lane modes; the cooperative
+                          // mode's layout for a network only it runs)
   int handoff_doubles;    // FP64 fallback hand-off record per cell (per-lane modes)
   hipError_t (*plan)(long long ncells, int mode, int prec, int block, int grid, int group, Plan* out);
   hipError_t (*launch)(const KArgs& a, int mode, int prec, const Plan& p, hipStream_t st);
@@ -178,8 +179,16 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
     if constexpr (Net::COOP) {
       if (p.block != COOP_BLOCK) return hipErrorInvalidValue;
       constexpr long long GPB = COOP_BLOCK / CoopWidth<Net::NACT>::G;
+      // a history in this mode only for networks no per-lane kernel runs
+      // (a network has one history layout)
+      if (a.hist && Net::LANE) return hipErrorInvalidValue;
       hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * GPB));
-      hipLaunchKernelGGL((odv_k_coop<Net, M, RCP>), dim3((unsigned)p.grid), dim3(COOP_BLOCK), coop_lds_bytes<Net>(), st, a);
+      if (a.hist)
+        hipLaunchKernelGGL((odv_k_coop<Net, M, RCP, true>), dim3((unsigned)p.grid), dim3(COOP_BLOCK),
+                           coop_lds_bytes<Net>(), st, a);
+      else
+        hipLaunchKernelGGL((odv_k_coop<Net, M, RCP>), dim3((unsigned)p.grid), dim3(COOP_BLOCK), coop_lds_bytes<Net>(),
+                           st, a);
       return hipGetLastError();
     }
     return hipErrorInvalidValue;
@@ -234,6 +243,14 @@ hipError_t launch_fbc_net(const KArgs& b, unsigned* done, int nprod, long long g
                          FB_IDLE_ROUNDS);
     return hipGetLastError();
   }
+}
+
+// the per-lane kernels' history layout, or the cooperative kernel's for a
+// network only it runs
+template <class Net>
+constexpr int hist_planes_of() {
+  if constexpr (Net::LANE || !Net::COOP) return Hist<Net>::PLANES;
+  else return CoopHist<Net>::PLANES;
 }
 
 template <class Net>
@@ -378,5 +395,5 @@ struct Bundle {
   static const odv::NetEntry odv_entry_##NET = {  /* NOLINT */                                                \
       NET::ID, ODV_REPRO_FLAG, NET::NEQ, NET::NNZ, NET::NRATE, NET::NPARAM, NET::RATE_KIND, NET::TGAS,          \
       NET::FLOPS_RHS, NET::FLOPS_JAC, NET::FLOPS_LU, NET::FLOPS_SOLVE, NET::FLOPS_LU_R,         \
-      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::Hist<NET>::PLANES, odv::Handoff<NET>::REC, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults, &odv::order_net<NET>, &odv::netfun_net<NET>, odv::jac_entries_of<NET>(), &odv::launch_fbc_net<NET>, odv::ws64_per_wave_of<NET>()};                          \
+      NET::FLOPS_SOLVE_R, NET::LANE ? 1 : 0, NET::COOP ? 1 : 0, odv::hist_planes_of<NET>(), odv::Handoff<NET>::REC, &odv::plan_net<NET>, &odv::launch_net<NET>, &odv::info_net<NET>, &NET::param_defaults, &odv::order_net<NET>, &odv::netfun_net<NET>, odv::jac_entries_of<NET>(), &odv::launch_fbc_net<NET>, odv::ws64_per_wave_of<NET>()};                          \
   static odv::Registrar odv_reg_##NET(&odv_entry_##NET);

@@ -159,8 +159,6 @@ class Server:
         requests integrate further; its arrays and history stay on the device
         until close_session."""
         torch = self.torch
-        if is_coop(network):
-            raise ValueError("%s runs on the cooperative kernel, which has no warm restart" % network)
         n = y.shape[1]
         planes = L.history_planes(network)
         prec = self.precision.get(network, L.FP64) if precision is None else precision
@@ -320,11 +318,15 @@ class Server:
             self._coop_launch()
 
     def _lane(self, slot, network, precision):
-        """The slot's persistent-mode handle for (network, precision): runs
+        """The slot's persistent-mode handle for (network, precision) (the
+        cooperative mode's for a network too large for one lane): runs
         session advances, each with its session's history attached."""
         from .api import Odevec
         key = (network, precision)
         h = slot["lane"].get(key)
+        if h is None and is_coop(network):
+            h = Odevec(network, 1, precision=precision, mode=L.COOP)
+            slot["lane"][key] = h
         if h is None:
             # the slot's share of the device's resident workgroups of this kernel
             k = L.kernel_info(network, L.PERSISTENT, precision)

@@ -112,3 +112,72 @@ def test_repro_coop_nan_state_bitwise(oracle, case):
     assert np.array_equal(yg, yr, equal_nan=True)
     assert np.array_equal(dto, dr, equal_nan=True)
     assert stats["attempts"] == int(cr[0].sum())
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
+@pytest.mark.parametrize("grid", [0, 1])
+def test_coop_continue_equals_one_solve(oracle, precision, grid):
+    """Warm restart of the cooperative kernel (SYNTH64, a network no
+    per-lane kernel runs): over [-S, 0] (the first step then does not depend
+    on t_stop), solve(-S -> t1) continued to 0 equals one solve bit for bit
+    (states, status, last steps; attempts, steps, Newton iterations and
+    Jacobians add up), and in FP64 equals the oracle's solve; a cell whose
+    state changed between the calls starts cold and equals a fresh solve."""
+    from odevec_amd.api import Odevec
+    w = W.synth64(48, seed=36)
+    nid = w["network"] + "+REPRO"
+    n = w["y"].shape[1]
+    span = np.broadcast_to(np.asarray(w["t_stop"], dtype=np.float64), (n,)).copy()
+    t0, t1, t2 = -span, -span * 0.63, np.zeros(n)
+    say("GPU coop history, precision", precision, "grid", grid)
+
+    def run(hist, pieces, y, prm):
+        ode = Odevec(nid, n, precision=precision, mode=_lib.COOP, grid=grid)
+        ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+        if hist:
+            ode.enable_history(n)
+        out = []
+        for a, b, yy in pieces:
+            st, dto, stats = ode.solve_cells(yy, b, t0=a, dt0=w["dt0"], params=prm, stats=True, raise_on_fail=False)
+            out.append((st, dto, stats))
+        ode.CloseOdevec()
+        return out
+
+    y_one = np.array(w["y"], order="C", copy=True)
+    (st1, dt1, s1), = run(False, [(t0, t2, y_one)], None, w["params"])
+    y = np.array(w["y"], order="C", copy=True)
+    (sta, _, sa), (stb, dtb, sb) = run(True, [(t0, t1, y), (t1, t2, y)], None, w["params"])
+    assert (st1 == 0).all() and (sta == 0).all() and (stb == 0).all()
+    assert np.array_equal(y, y_one) and np.array_equal(dtb, dt1)
+    for k in ("attempts", "steps", "newton", "jac"):
+        assert sa[k] + sb[k] == s1[k], k
+    if precision == _lib.FP64:
+        yo, so, do, _ = oracle.Network.load(w["network"]).solve_cells(
+            w["y"], t2, t0=t0, dt0=w["dt0"], params=w["params"], rtol=w["rtol"], atol=w["atol"], dt_min=w["dt_min"])
+        assert np.array_equal(so, stb) and np.array_equal(yo, y) and np.array_equal(do, dtb)
+    # changed cells (every third) start cold at t1
+    ode = Odevec(nid, n, precision=precision, mode=_lib.COOP, grid=grid)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    ode.enable_history(n)
+    y = np.array(w["y"], order="C", copy=True)
+    ode.solve_cells(y, t1, t0=t0, dt0=w["dt0"], params=w["params"], raise_on_fail=False)
+    y[5, 0::3] *= 1.0 + 1e-12
+    y_mid = y.copy()
+    st2, dt2, _ = ode.solve_cells(y, t2, t0=t1, dt0=w["dt0"], params=w["params"], raise_on_fail=False)
+    ode.CloseOdevec()
+    y_cold = y_mid.copy()
+    (stc, dtc, _), = run(False, [(t1, t2, y_cold)], None, w["params"])
+    ch = np.zeros(n, dtype=bool)
+    ch[0::3] = True
+    assert np.array_equal(y[:, ch], y_cold[:, ch]) and np.array_equal(dt2[ch], dtc[ch])
+    assert np.array_equal(y[:, ~ch], y_one[:, ~ch])
+
+
+def test_coop_history_only_for_networks_without_per_lane_kernels():
+    """A network the per-lane kernels run keeps their history layout: the
+    cooperative mode refuses a history for it."""
+    from odevec_amd.api import Odevec
+    ode = Odevec("PRIMORDIAL/CMK/CSC", 64, mode=_lib.COOP)
+    with pytest.raises(_lib.OdevecError):
+        ode.enable_history(64)
+    ode.CloseOdevec()

@@ -185,6 +185,34 @@ def test_session_advances_continue_the_integration(bundle):
 
 
 @pytest.mark.gpu
+def test_synth64_session_advances_continue_the_integration():
+    """A SYNTH64 session (the cooperative kernel's history): three advances
+    over [-S, 0] equal one solve over it bit for bit."""
+    from odevec_amd import _lib as L
+    from odevec_amd import workloads as W
+    from odevec_amd.api import Odevec
+    print("GPU SYNTH64 session advances", flush=True)
+    w = W.synth64(300, seed=37)
+    nid = w["network"] + "+REPRO"
+    n = w["y"].shape[1]
+    span = np.broadcast_to(np.asarray(w["t_stop"], dtype=np.float64), (n,)).copy()
+    srv = S.Server("MIX3+REPRO", slots=2)
+    ss = srv.open_session(nid, w["y"], w["params"], w["rtol"], w["atol"], w["dt_min"], w["dt0"], t0=-span,
+                          precision=L.FP64)
+    rs = [srv.advance(ss, -span * f) for f in (0.6, 0.2, 0.0)]
+    srv.drain()
+    srv.close_session(ss)
+    srv.close()
+    assert all((r.status == 0).all() for r in rs)
+    ode = Odevec(nid, n, precision=L.FP64, mode=L.COOP)
+    ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    y = np.array(w["y"], order="C", copy=True)
+    st, _, _ = ode.solve_cells(y, np.zeros(n), t0=-span, dt0=w["dt0"], params=w["params"], raise_on_fail=False)
+    ode.CloseOdevec()
+    assert (st == 0).all() and np.array_equal(rs[2].result, y)
+
+
+@pytest.mark.gpu
 def test_sessions_overlap_with_requests_and_match_cold_within_tolerance():
     """Two sessions advanced in closed loop next to each other give the same
     results as each alone (bitwise), and agree with cold-restarted advances
