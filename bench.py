@@ -141,12 +141,38 @@ NUCLEI = {"H": [0, 1, 1, 0, 2, 0, 1, 1, 0, 2, 0, 0, 0, 0, 0, 0],
 TIGHT_CELLS = 8192
 
 
-def cell_errors(y, yref, rtol, atol):
-    """Per cell: the largest over its integrated species of |y - y_ref| /
+def cell_errors(y, yref, rtol, atol, nsp=12):
+    """Per cell: the largest over its integrated species (the first nsp:
+    PRIMORDIAL's 12; its last 4 are constant) of |y - y_ref| /
     (rtol |y_ref| + atol), the reference's error-weight scale
     (src/odevec.F90:706-716); 1 = one tolerance."""
-    e = np.abs(y[:12] - yref[:12]) / (rtol * np.abs(yref[:12]) + atol)
+    e = np.abs(y[:nsp] - yref[:nsp]) / (rtol * np.abs(yref[:nsp]) + atol)
     return e.max(axis=0)
+
+
+# integrated species of each network (the rest of the state is constant)
+ACTIVE = {"ROBER/NONE/DENSE": 3, "OREGO/NONE/DENSE": 3, "PRIMORDIAL/CMK/CSC": 12, "SYNTH64/FF/CSC": 64}
+
+
+def err_vs_tight(w, y, st, mode, sample=2048, seed=34):
+    """Errors of a solve's results (y, st over w's cells) against the FP64
+    reproducible build at rtol 1e-12 (atol 1e-6 of w's) on `sample` seeded
+    cells, in units of w's rtol |y_ref| + atol."""
+    from odevec_amd import workloads as W
+    n = w["y"].shape[1]
+    idx = np.sort(np.random.default_rng(seed).choice(n, min(sample, n), replace=False))
+    s = W.subset(w, idx)
+    tight = Runner(dict(s, network=w["network"] + "+REPRO", rtol=1e-12, atol=w["atol"] * 1e-6), precision=0,
+                   mode=mode, max_attempts=200000 if mode != 3 else 0)
+    tight.restore()
+    tight.solve()
+    tight.torch.cuda.synchronize()
+    y_t, s_t = tight.y.cpu().numpy(), tight.st.cpu().numpy()
+    tight.ode.CloseOdevec()
+    ok = (s_t == 0) & (st[idx] == 0)
+    out = error_summary(cell_errors(y[:, idx][:, ok], y_t[:, ok], w["rtol"], w["atol"], ACTIVE[w["network"]]))
+    out["sampled"] = int(idx.size)
+    return out
 
 
 def error_summary(e):
@@ -223,7 +249,13 @@ def batching(steps, warmup):
     sec = A.time_batch(t["bundle"], d, t["block"], use_graph=True, reps=max(steps, 1),
                        fp64_fallback=t.get("fp64_fallback", 0))
     ok = sum(int((x["status"] == 0).sum()) for x in d)
+    # each network's cells against an rtol-1e-12 solve of a sample of them
+    errs = {}
+    for x in d:
+        w = next(v for v in segs if v["network"] == x["network"])
+        errs[x["network"]] = err_vs_tight(w, x["y"].cpu().numpy(), x["status"].cpu().numpy(), 2)
     return {"value": 2097152 / sec, "unit": "cell-solves/s", "ms_per_launch": sec * 1e3,
+            "err_vs_tight": errs,
             "workload": "2,097,152 cells, 40/30/30 ROBER/OREGO/PRIMORDIAL (seed 4), t_stop logU over 6 decades "
                         "per network, one persistent launch per step, hipGraph replay",
             "bundle": t["bundle"], "block": t["block"], "fp64_fallback": t.get("fp64_fallback", 0), "precision": t["precision"], "cells_ok": ok,
@@ -243,7 +275,8 @@ def synth64(steps, warmup, cells=65536):
         r = Runner(w, precision=prec, mode=L.COOP)
         wall, ms = r.timed(steps, warmup)
         out[name] = {"value": steps * cells / wall, "unit": "cell-solves/s", "kernel_ms": ms,
-                     "cells_ok": int((r.st == 0).sum())}
+                     "cells_ok": int((r.st == 0).sum()),
+                     "err_vs_tight": err_vs_tight(w, r.y.cpu().numpy(), r.st.cpu().numpy(), L.COOP, sample=1024)}
         r.ode.CloseOdevec()
     return out
 
