@@ -127,6 +127,18 @@ struct odevec_s {
   unsigned* d_done = nullptr;        // the count of finished queue wavefronts,
   hipStream_t fb_stream = nullptr;   // the stream the FP64 lanes run on beside the queue kernel,
   hipEvent_t fb_fork = nullptr, fb_join = nullptr;   // and its fork / join with the caller's stream
+  // persistent mode, queue in cost order: the inputs and outputs in queue
+  // order (k_gather / k_scatter), g_cap cells
+  double* d_gy = nullptr;
+  double* d_gts = nullptr;
+  double* d_gdt = nullptr;
+  double* d_gt0 = nullptr;
+  double* d_gprm = nullptr;
+  double* d_gdto = nullptr;
+  int* d_gst = nullptr;
+  int* d_gcnt = nullptr;
+  int* d_ginv = nullptr;             // queue position of each cell
+  int64_t g_cap = 0;
   std::vector<int> h_status;
   std::vector<int> h_cnt;
 };
@@ -324,6 +336,57 @@ int launch(odevec_s* h, const Req& r) {
       return hip_fail(e, "cost order");
     a.order = h->d_order;
   }
+  // the solve on copies in queue order (coalesced state accesses in the
+  // persistent loop), unless per-cell side arrays indexed by cell are attached
+  odv::KArgs src = a;
+  const bool gather = a.order && !a.hist && !a.times && !a.mask;
+  if (gather) {
+    const int64_t n = r.ncells, neq = h->net->neq, np = h->net->nparam;
+    if (n > h->g_cap) {
+      void* ps[] = {h->d_gy, h->d_gts, h->d_gdt, h->d_gt0, h->d_gprm, h->d_gdto, h->d_gst, h->d_gcnt, h->d_ginv};
+      for (void* q : ps)
+        if (q) (void)hipFree(q);
+      h->d_gy = h->d_gts = h->d_gdt = h->d_gt0 = h->d_gprm = h->d_gdto = nullptr;
+      h->d_gst = h->d_gcnt = h->d_ginv = nullptr;
+      h->g_cap = 0;
+      if ((e = hipMalloc((void**)&h->d_gy, (size_t)(neq * n * 8))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_gts, (size_t)(n * 8))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_gdt, (size_t)(n * 8))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_gt0, (size_t)(n * 8))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_gprm, (size_t)((np > 0 ? np : 1) * n * 8))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_gdto, (size_t)(n * 8))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_gst, (size_t)(n * 4))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_gcnt, (size_t)(8 * n * 4))) != hipSuccess ||
+          (e = hipMalloc((void**)&h->d_ginv, (size_t)(n * 4))) != hipSuccess)
+        return fail(ODEVEC_ENOMEM, std::string("queue-order copies allocation failed: ") + hipGetErrorString(e));
+      h->g_cap = n;
+    }
+    double* gts = a.ts_s ? h->d_gts : nullptr;
+    double* gdt = a.dt_s ? h->d_gdt : nullptr;
+    double* gt0 = a.t0_s ? h->d_gt0 : nullptr;
+    double* gprm = (np > 0 && a.ps_cell) ? h->d_gprm : nullptr;
+    hipLaunchKernelGGL(odv::k_invert<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, r.stream, (long long)n,
+                       (const int*)h->d_order, h->d_ginv);
+    hipLaunchKernelGGL(odv::k_gather<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, r.stream, a, (int)neq,
+                       (int)np, (const int*)h->d_ginv, h->d_gy, gts, gdt, gt0, gprm);
+    a.y = h->d_gy;
+    if (gts) { a.tstop = gts; a.ts_s = 1; }
+    if (gdt) { a.dt0 = gdt; a.dt_s = 1; }
+    if (gt0) { a.t0 = gt0; a.t0_s = 1; }
+    if (gprm) { a.prm = gprm; a.ps_plane = n; a.ps_cell = 1; }
+    if (a.status) a.status = h->d_gst;
+    if (a.dt_out) a.dt_out = h->d_gdto;
+    if (a.counters) a.counters = h->d_gcnt;
+    a.order = nullptr;
+    a.spread = 1;
+  }
+  auto scatter = [&]() {
+    if (!gather) return;
+    hipLaunchKernelGGL(odv::k_scatter<0>, dim3((unsigned)((r.ncells + 255) / 256)), dim3(256), 0, r.stream,
+                       (long long)r.ncells, (int)h->net->neq, (const int*)h->d_order, (const double*)h->d_gy,
+                       (const int*)h->d_gst, (const double*)h->d_gdto, (const int*)h->d_gcnt, src.y, src.status,
+                       src.dt_out, src.counters);
+  };
   if (beside) {
     a.ctl.promote_cf = h->promote_cf;
     a.promo = h->d_promo;
@@ -365,6 +428,7 @@ int launch(odevec_s* h, const Req& r) {
     a2.queue = h->d_queue + h->qslot;
     e = h->net->launch(a2, ODEVEC_PERSISTENT, ODEVEC_FP64, p2, r.stream);
     if (e != hipSuccess) return hip_fail(e, "kernel launch (FP64 pass)");
+    scatter();
     return ODEVEC_OK;
   }
   if (promote) {
@@ -390,6 +454,7 @@ int launch(odevec_s* h, const Req& r) {
     e = h->net->launch(a2, ODEVEC_PERSISTENT, ODEVEC_FP64, p2, r.stream);
     if (e != hipSuccess) return hip_fail(e, "kernel launch (FP64 pass)");
   }
+  scatter();
   return ODEVEC_OK;
 }
 
@@ -868,6 +933,11 @@ int32_t odevec_destroy(odevec_t* h) {
   if (h->fb_join) (void)hipEventDestroy(h->fb_join);
   if (h->d_bin) (void)hipFree(h->d_bin);
   if (h->d_count) (void)hipFree(h->d_count);
+  {
+    void* ps[] = {h->d_gy, h->d_gts, h->d_gdt, h->d_gt0, h->d_gprm, h->d_gdto, h->d_gst, h->d_gcnt, h->d_ginv};
+    for (void* q : ps)
+      if (q) (void)hipFree(q);
+  }
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
   if (h->stream) (void)hipStreamDestroy(h->stream);

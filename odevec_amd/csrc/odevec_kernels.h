@@ -71,6 +71,9 @@ struct KArgs {
   unsigned* fb_done;
   Controls ctl;
   int group;              // lockstep group size
+  // persistent mode without an order: the arrays are already in queue order
+  // (k_gather), and lanes start spread over the wavefronts as with an order
+  int spread;
 };
 
 constexpr int WAVE = 64;         // lanes per wavefront: the column stride of a lane's external state
@@ -580,7 +583,7 @@ __device__ __forceinline__ void odv_persistent_run(const KArgs& a, double* lds, 
   const long long gl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const long long nq = a.qlen ? (long long)*a.qlen : a.ncells;   // queue positions
-  const long long p0 = a.order ? (long long)lane * (nlanes >> 6) + (gl >> 6) : gl;
+  const long long p0 = (a.order || a.spread) ? (long long)lane * (nlanes >> 6) + (gl >> 6) : gl;
   long long i = p0 < nq ? (a.order ? (long long)a.order[p0] : p0) : a.ncells;
   // a list entry below 0: a cell already taken elsewhere (the fallback's list)
   L s;
@@ -934,6 +937,51 @@ __global__ void k_cost_scatter(long long ncells, const unsigned char* bin, unsig
     if (h[t]) base[t] = atomicAdd(&offset[t], h[t]);
   __syncthreads();
   if (in) order[base[b] + local] = (int)i;
+}
+
+// position of each cell in the queue: inv[order[p]] = p
+template <int UNUSED = 0>
+__global__ void k_invert(long long n, const int* order, int* inv) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) inv[order[p]] = (int)p;
+}
+
+// The cost-ordered queue's cells copied into queue order: position p of
+// the copies holds cell order[p], so the cells a wavefront takes at one
+// refill (consecutive positions) are consecutive in memory, and the
+// persistent kernel's state loads and stores are coalesced. The permutation
+// happens here once, in streaming passes that read coalesced and write
+// scattered (partial cache lines of scattered writes merge in the caches;
+// scattered reads would fetch a whole line per value). Broadcast inputs
+// (stride 0) and absent outputs are left out (null destination).
+template <int UNUSED = 0>
+__global__ void k_gather(KArgs a, int neq, int np, const int* inv, double* gy, double* gts, double* gdt,
+                         double* gt0, double* gprm) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = a.ncells;
+  if (i >= n) return;
+  const long long p = inv[i];
+  for (int j = 0; j < neq; ++j) gy[(long long)j * n + p] = a.y[(long long)j * n + i];
+  if (gts) gts[p] = a.tstop[i * a.ts_s];
+  if (gdt) gdt[p] = a.dt0[i * a.dt_s];
+  if (gt0) gt0[p] = a.t0[i * a.t0_s];
+  if (gprm)
+    for (int q = 0; q < np; ++q) gprm[(long long)q * n + p] = a.prm[q * a.ps_plane + i * a.ps_cell];
+}
+
+// the results back to the caller's cell order
+template <int UNUSED = 0>
+__global__ void k_scatter(long long n, int neq, const int* order, const double* gy, const int* gst,
+                          const double* gdto, const int* gcnt, double* y, int* status, double* dt_out,
+                          int* counters) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const long long i = order[p];
+  for (int j = 0; j < neq; ++j) y[(long long)j * n + i] = gy[(long long)j * n + p];
+  if (status) status[i] = gst[p];
+  if (dt_out) dt_out[i] = gdto[p];
+  if (counters)
+    for (int m = 0; m < 8; ++m) counters[(long long)m * n + i] = gcnt[(long long)m * n + p];
 }
 
 // ---------------------------------------------------------------- batches

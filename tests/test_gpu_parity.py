@@ -245,15 +245,19 @@ def test_workgroup_sizes_bitwise(oracle, block, precision, mode):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_cost_ordered_schedule_changes_nothing(oracle, precision):
+@pytest.mark.parametrize("fallback", [0, 2])
+def test_cost_ordered_schedule_changes_nothing(oracle, precision, fallback):
     """The persistent queue in cost order (costliest cells first, spread one
-    per wavefront) and in index order give the same cells' results and
-    counters, bit for bit; few lanes (grid 2) so the queue does the work."""
+    per wavefront; the solve runs on copies of the inputs in queue order and
+    scatters the results back) and in index order give the same cells'
+    results and counters, bit for bit, with and without the FP64 fallback
+    (limit 2: many hand-backs); few lanes (grid 2) so the queue does the
+    work."""
     w = W.primordial(3000, seed=12)
     w["t_stop"][::7] = 0.0                        # skipped cells rank last
     nid = "PRIMORDIAL/CMK/CSC+REPRO"
-    a = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=1)
-    b = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=0)
+    a = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=1, fallback=fallback)
+    b = gpu_solve(nid, w, precision=precision, mode=_lib.PERSISTENT, grid=2, schedule=0, fallback=fallback)
     assert all(np.array_equal(x, y) for x, y in zip(a[:3], b[:3]))
     assert a[3]["attempts"] == b[3]["attempts"] and a[3]["newton"] == b[3]["newton"]
     assert (a[1][::7] == 5).all()
