@@ -49,6 +49,8 @@ extern "C" {
 #define ODEVEC_STATUS_ORDER 3      /* order/step mismatch (src/odevec.F90:584-587, 595-598) */
 #define ODEVEC_STATUS_MAX_STEPS 4  /* attempt budget exhausted (no reference counterpart) */
 #define ODEVEC_STATUS_SKIPPED 5    /* masked out or t_stop <= t0: y unchanged */
+#define ODEVEC_STATUS_PAUSED 8     /* batch segment with a quantum: not finished in this launch (y unchanged) */
+#define ODEVEC_STATUS_FRESH 9      /* set by the caller before a quantum segment's first launch */
 
 /* precision forms */
 #define ODEVEC_FP64 0   /* everything FP64 */
@@ -295,7 +297,12 @@ typedef struct {
   double* flops;            /* device: [2] accumulated (FP64, FP32) or NULL */
   double rtol, atol, dt_min;
   int32_t max_attempts;     /* 0 = 1000000 */
-  int32_t reserved;
+  int32_t quantum;          /* > 0 (needs hist and status, no flops): a cell runs at most this many
+                               step attempts per launch, then pauses with status ODEVEC_STATUS_PAUSED
+                               and its state in hist; the next launch of the same segment (same arrays,
+                               start and end times) continues it. Set status to ODEVEC_STATUS_FRESH
+                               before the first launch; cells with a final status are left as they
+                               are. The result equals the solve in one launch bit for bit. 0: off */
   double* hist;             /* device: warm-restart history of these cells (odevec_history_planes x ncells
                                doubles, zero-filled before first use) or NULL = cold start; see
                                odevec_set_history for the rules */

@@ -13,7 +13,8 @@ LIBPATH = os.environ.get("ODEVEC_LIB", os.path.join(HERE, "libodevec.so"))
 
 OK = 0
 EINVAL, ENOMEM, EDEVICE, ENOTFOUND, ESOLVE = -1, -2, -3, -4, -5
-STATUS = {0: "OK", 1: "DT_MIN", 2: "CONV_FAIL", 3: "ORDER", 4: "MAX_STEPS", 5: "SKIPPED"}
+STATUS = {0: "OK", 1: "DT_MIN", 2: "CONV_FAIL", 3: "ORDER", 4: "MAX_STEPS", 5: "SKIPPED", 8: "PAUSED"}
+PAUSED, FRESH = 8, 9   # batch segments with a launch quantum
 FP64, MIXED = 0, 1
 PERCELL, LOCKSTEP, PERSISTENT, COOP = 0, 1, 2, 3
 
@@ -51,7 +52,7 @@ class Segment(C.Structure):
     _fields_ = [("network", C.c_char_p), ("precision", _I32), ("check_negatives", _I32), ("ncells", _I64),
                 ("y", _VP), ("t0", _VP), ("t_stop", _VP), ("dt0", _VP), ("params", _VP), ("status", _VP),
                 ("dt_out", _VP), ("counters", _VP), ("flops", _VP), ("rtol", C.c_double), ("atol", C.c_double),
-                ("dt_min", C.c_double), ("max_attempts", _I32), ("reserved", _I32), ("hist", _VP)]
+                ("dt_min", C.c_double), ("max_attempts", _I32), ("quantum", _I32), ("hist", _VP)]
 
 
 class OdevecError(RuntimeError):

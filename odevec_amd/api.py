@@ -324,7 +324,7 @@ class Batch:
                                _ptr(sg.get("params")), _ptr(sg.get("status")), _ptr(sg.get("dt_out")),
                                _ptr(sg.get("counters")), _ptr(sg.get("flops")), float(sg.get("rtol", 1e-6)),
                                float(sg.get("atol", 1e-10)), float(sg.get("dt_min", float(np.float32(1e-10)))),
-                               int(sg.get("max_attempts", 0)), 0, _ptr(sg.get("hist")))
+                               int(sg.get("max_attempts", 0)), int(sg.get("quantum", 0)), _ptr(sg.get("hist")))
         self._keep = (arr, names)
         L.check(L.lib().odevec_batch_solve_async(self._h, arr, n, stream))
 

@@ -1156,6 +1156,7 @@ int batch_fallback_passes(odevec_batch_s* b, const odevec_segment_t* segs, int n
     }
     odv::KArgs a2 = b->h_segs[k].a;
     a2.ctl.promote_cf = 0;
+    a2.quantum = 0;   // handed-back cells run to the end
     a2.order = a2.promo;
     a2.qlen = a2.promo_n;
     a2.ho_mode = 2;
@@ -1181,6 +1182,7 @@ int batch_fallback_lanes(odevec_batch_s* b, const odevec_segment_t* segs, int ns
     if (!fbnet[k]) continue;
     odv::KArgs c = b->h_segs[k].a;
     c.ctl.promote_cf = 0;
+    c.quantum = 0;
     c.order = c.promo;
     c.qlen = c.promo_n;
     c.promo = nullptr;
@@ -1286,6 +1288,8 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     if (g.ncells < 0) return fail(ODEVEC_EINVAL, "segment ncells < 0");
     if (g.ncells > 0 && (!g.y || !g.t_stop)) return fail(ODEVEC_EINVAL, "segment y / t_stop is NULL");
     if (!(g.rtol > 0.0) || !(g.atol >= 0.0)) return fail(ODEVEC_EINVAL, "segment tolerances");
+    if (g.quantum < 0 || (g.quantum > 0 && (!g.hist || !g.status || g.flops)))
+      return fail(ODEVEC_EINVAL, "segment quantum: > 0 needs hist and status, and no flops");
     kind[k] = kk;
     total += g.ncells;
     if (g.hist) hist = 1;
@@ -1412,6 +1416,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
     a.ctl.promote_inv = b->promote_inv;
     a.group = 1;
     a.hist = g.hist;
+    a.quantum = g.quantum;
     if (fbnet[k]) {
       a.ctl.promote_cf = b->promote_cf;
       a.promo = b->d_promo + fboff[k];

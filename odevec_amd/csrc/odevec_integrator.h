@@ -63,6 +63,11 @@ enum Status : int {
                        // (never returned to the caller)
   ST_PROMOTE_HIST = 7, // mixed form with a warm-restart history: the cell went on in FP64 in the
                        // last call, so FP64 lanes continue it from its history (never returned)
+  ST_PAUSED = 8,       // launch quantum used up: the next launch of the same solve continues the
+                       // cell from its history
+  ST_FRESH = 9,        // set by the caller before a solve's first quantum launch: not yet started
+  ST_DONE_EARLIER = 10,  // finished in an earlier quantum launch of the solve: left as it is
+                         // (never written)
 };
 
 constexpr int MAXORD = 5;          // pre_odevec.py:83

@@ -74,6 +74,11 @@ struct KArgs {
   // persistent mode without an order: the arrays are already in queue order
   // (k_gather), and lanes start spread over the wavefronts as with an order
   int spread;
+  // > 0 (persistent and batch launches with a history): a cell runs at most
+  // this many attempts per launch, then pauses (ST_PAUSED, its state in its
+  // history) for the next launch of the same solve; status[] carries each
+  // cell's phase between the launches (ST_FRESH before the first)
+  int quantum;
 };
 
 constexpr int WAVE = 64;         // lanes per wavefront: the column stride of a lane's external state
@@ -191,7 +196,9 @@ struct Hist {
   static constexpr int EO = W + N;           // previous accepted correction
   static constexpr int A = EO + N;           // P's LU factors
   static constexpr int K = A + Net::NNZ;     // rate coefficients
-  static constexpr int PLANES = K + Net::NRATE;
+  static constexpr int T0 = K + Net::NRATE;  // a paused cell (VALID 2): its solve's start time
+  static constexpr int CNT = T0 + 1;         // ... and its 8 event counters so far
+  static constexpr int PLANES = CNT + 8;
 };
 
 // Hand-off record of the mixed form's FP64 fallback (no counterpart in the
@@ -394,15 +401,26 @@ __device__ inline void odv_hist_resume(const KArgs& a, long long i, L& s, double
 // after odv_store: a.y holds the state returned to the caller; fp64: the
 // cell ran on an FP64 lane of the fallback
 template <class L>
+__device__ inline void odv_hist_write(const KArgs& a, long long i, const L& s, bool fp64);
+
+template <class L>
 __device__ inline void odv_hist_save(const KArgs& a, long long i, const L& s, bool fp64) {
   using H = Hist<typename L::Net>;
-  const long long n = a.ncells;
   double* h = a.hist + i * H::PLANES;
   if (s.status == ST_SKIPPED) return;   // nothing integrated: the saved state stays usable
   if (s.status != ST_OK) {
     h[H::VALID] = 0.0;
     return;
   }
+  odv_hist_write(a, i, s, fp64);
+}
+
+// the cell's whole integrator state, the state in a.y and its parameters
+template <class L>
+__device__ inline void odv_hist_write(const KArgs& a, long long i, const L& s, bool fp64) {
+  using H = Hist<typename L::Net>;
+  const long long n = a.ncells;
+  double* h = a.hist + i * H::PLANES;
   h[H::TSTOP] = s.tstop;
   h[H::T] = s.t;
   h[H::DT] = s.dt;
@@ -438,6 +456,43 @@ __device__ inline void odv_hist_save(const KArgs& a, long long i, const L& s, bo
   h[H::VALID] = 1.0;
 }
 
+// A cell whose launch quantum ran out: its whole state (the input state
+// and parameters it started from, which stay in place, and its counters so
+// far) into its history as a pause record (VALID 2), which the next launch
+// of the same solve continues.
+template <class L>
+__device__ inline void odv_hist_pause(const KArgs& a, long long i, const L& s) {
+  using H = Hist<typename L::Net>;
+  odv_hist_write(a, i, s, false);
+  double* h = a.hist + i * H::PLANES;
+  h[H::T0] = a.t0[i * a.t0_s];
+  const Counters& c = s.cnt;
+  const int cv[8] = {c.attempts, c.steps, c.newton, c.jac, c.errfail, c.convfail, c.stepsel, c.qsum};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) h[H::CNT + k] = (double)cv[k];
+  h[H::VALID] = 2.0;
+}
+
+// the pause record of cell i continues this launch (the same solve: start
+// and end time, state and parameters unchanged)
+template <class L>
+__device__ inline bool odv_hist_unpause(const KArgs& a, long long i, L& s, const double* y, const double* prm,
+                                        double t0, double ts) {
+  using H = Hist<typename L::Net>;
+  const double* h = a.hist + i * H::PLANES;
+  if (h[H::VALID] != 2.0 || h[H::T0] != t0 || h[H::TSTOP] != ts) return false;
+  bool same = true;
+#pragma unroll
+  for (int j = 0; j < H::NF; ++j) same = same && (h[H::Y + j] == y[j]);
+#pragma unroll
+  for (int p = 0; p < H::NP; ++p) same = same && (h[H::P + p] == prm[p]);
+  if (!same) return false;
+  odv_hist_resume(a, i, s, ts);
+  s.cnt = Counters{(int)h[H::CNT], (int)h[H::CNT + 1], (int)h[H::CNT + 2], (int)h[H::CNT + 3], (int)h[H::CNT + 4],
+                   (int)h[H::CNT + 5], (int)h[H::CNT + 6], (int)h[H::CNT + 7], (int)h[H::INTS + 7]};
+  return true;
+}
+
 // Load cell i into the lane and start its solve; returns true if there is
 // nothing to integrate (masked out or t_stop <= t0; y is then left as is).
 // HIST: the kernel variant that reads and writes a warm-restart history (a
@@ -460,6 +515,17 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
   s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
   s.dt = dh;
   if (a.times) a.times[i] = (long long)wall_clock64();
+  if constexpr (HIST) {
+    if (a.quantum > 0 && a.hist && a.status) {
+      // a solve in quantum launches: the cell's phase is its status entry
+      const int ph = a.status[i];
+      if (ph == ST_PAUSED && odv_hist_unpause(a, i, s, y, prm, t0, ts)) return false;
+      if (ph != ST_FRESH && ph != ST_PAUSED) {
+        s.status = ST_DONE_EARLIER;
+        return true;
+      }
+    }
+  }
   if ((a.mask && !a.mask[i]) || !(t0 < ts)) {
     s.status = ST_SKIPPED;
     return true;
@@ -503,6 +569,14 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
 
 template <class L, bool HIST = false>
 __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
+  if constexpr (HIST) {
+    if (s.status == ST_DONE_EARLIER) return;
+    if (s.status == ST_PAUSED) {
+      odv_hist_pause(a, i, s);
+      if (a.status) a.status[i] = ST_PAUSED;
+      return;
+    }
+  }
   if (a.times) a.times[a.ncells + i] = (long long)wall_clock64();
   if (s.status == ST_PROMOTE || s.status == ST_PROMOTE_HIST) {
     // the caller's state stays untouched for the FP64 pass over promo[],
@@ -593,6 +667,7 @@ __device__ __forceinline__ void odv_persistent_run(const KArgs& a, double* lds, 
   bool have = p0 < nq && i >= 0;
   bool done = true;
   bool exhausted = nlanes >= nq;   // wave-uniform: no cell left to hand out
+  int left = a.quantum;            // attempts left in this launch for the lane's cell (quantum launches)
   if (have) done = odv_load_begin<L, HIST>(a, i, s);
   while (true) {
     const bool need = !have || done;
@@ -631,13 +706,22 @@ __device__ __forceinline__ void odv_persistent_run(const KArgs& a, double* lds, 
           i = a.order ? (long long)a.order[ni] : ni;
           if (i >= 0) {
             have = true;
+            left = a.quantum;
             done = odv_load_begin<L, HIST>(a, i, s);
           }
         }
       }
     }
     if (__ballot(have) == 0ull) break;
-    if (have && !done) done = odv_attempt(s, a.ctl, g);
+    if (have && !done) {
+      done = odv_attempt(s, a.ctl, g);
+      if constexpr (HIST) {
+        if (!done && a.quantum > 0 && --left <= 0) {
+          s.status = ST_PAUSED;   // the quantum is used up: the next launch continues the cell
+          done = true;
+        }
+      }
+    }
   }
   if (a.flops) {
     fl64 = odv_wave_sum(fl64);

@@ -334,3 +334,53 @@ def test_batch_history_with_fallback_equals_one_solve():
     assert np.array_equal(yb[:, ok], y1[:, ok]) and np.array_equal(db[ok], d1[ok])
     for k, name in enumerate(("attempts", "steps", "newton", "jac")):
         assert np.array_equal(ca[k][ok] + cb[k][ok], c1[k][ok]), name
+
+
+def test_batch_quantum_launches_equal_one_launch(oracle):
+    """Launch quantum (continuous batching): a batch whose cells may run at
+    most 12 attempts per launch, relaunched until no cell is paused -- with
+    a second segment joining at the third launch -- gives every cell the
+    result of the solve in one launch bit for bit (states, status, last
+    steps, counters), FP64 and mixed with its FP64 fallback (limit 2); the
+    FP64 segment also equals the oracle."""
+    from odevec_amd.api import Batch
+    import torch
+    pri = W.primordial(1200, seed=23)
+    rob = W.rober(900, seed=24)
+    pri["t_stop"][::17] = 0.0
+    limit = 2
+    say("GPU batch quantum")
+
+    def seg(w, prec, quantum):
+        s = to_dev(w, prec)
+        s["hist"] = torch.zeros(_lib.history_planes(w["network"]) * w["y"].shape[1], dtype=torch.float64,
+                                device="cuda")
+        if quantum:
+            s["quantum"] = quantum
+            s["status"].fill_(_lib.FRESH)
+        return s
+
+    b = Batch("MIX3+REPRO", fp64_fallback=limit)
+    one = run("MIX3+REPRO", [seg(pri, _lib.MIXED, 0), seg(rob, _lib.FP64, 0)], batch=b)
+    sp, sr = seg(pri, _lib.MIXED, 12), seg(rob, _lib.FP64, 12)
+    launches, active = 0, [sp]
+    while True:
+        if launches == 2:
+            active.append(sr)          # joins the running work
+        run("MIX3+REPRO", active, batch=b)
+        launches += 1
+        paused = sum(int((s["status"] == _lib.PAUSED).sum()) for s in active)
+        if paused == 0 and len(active) == 2:
+            break
+        assert launches < 10000
+    b.close()
+    say("quantum launches", launches)
+    assert launches > 20
+    for got, ref in zip((host(sp), host(sr)), one):
+        assert all(np.array_equal(x, y) for x, y in zip(got, ref))
+    ys, ss, ds, _ = host(sr)
+    yo, so, do, _ = oracle.Network.load(rob["network"]).solve_cells(
+        rob["y"], rob["t_stop"], t0=rob["t0"], dt0=rob["dt0"], params=rob["params"], rtol=rob["rtol"],
+        atol=rob["atol"], dt_min=rob["dt_min"])
+    assert np.array_equal(ys, yo) and np.array_equal(ss, so) and np.array_equal(ds, do)
+    assert (one[0][1][::17] == 5).all() and (one[0][1] == 0).mean() > 0.9
