@@ -302,7 +302,8 @@ typedef struct {
                                and its state in hist; the next launch of the same segment (same arrays,
                                start and end times) continues it. Set status to ODEVEC_STATUS_FRESH
                                before the first launch; cells with a final status are left as they
-                               are. The result equals the solve in one launch bit for bit. 0: off */
+                               are. The result equals the solve in one launch bit for bit. hist then
+                               holds pause records only (needs no zero fill; no warm restart). 0: off */
   double* hist;             /* device: warm-restart history of these cells (odevec_history_planes x ncells
                                doubles, zero-filled before first use) or NULL = cold start; see
                                odevec_set_history for the rules */

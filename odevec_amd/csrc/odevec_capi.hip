@@ -1156,7 +1156,9 @@ int batch_fallback_passes(odevec_batch_s* b, const odevec_segment_t* segs, int n
     }
     odv::KArgs a2 = b->h_segs[k].a;
     a2.ctl.promote_cf = 0;
-    a2.quantum = 0;   // handed-back cells run to the end
+    // handed-back cells run to the end (-1: a quantum segment's, whose
+    // history holds pause records only)
+    a2.quantum = a2.quantum > 0 ? -1 : 0;
     a2.order = a2.promo;
     a2.qlen = a2.promo_n;
     a2.ho_mode = 2;
@@ -1182,7 +1184,7 @@ int batch_fallback_lanes(odevec_batch_s* b, const odevec_segment_t* segs, int ns
     if (!fbnet[k]) continue;
     odv::KArgs c = b->h_segs[k].a;
     c.ctl.promote_cf = 0;
-    c.quantum = 0;
+    c.quantum = c.quantum > 0 ? -1 : 0;
     c.order = c.promo;
     c.qlen = c.promo_n;
     c.promo = nullptr;

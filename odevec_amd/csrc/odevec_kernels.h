@@ -77,7 +77,9 @@ struct KArgs {
   // > 0 (persistent and batch launches with a history): a cell runs at most
   // this many attempts per launch, then pauses (ST_PAUSED, its state in its
   // history) for the next launch of the same solve; status[] carries each
-  // cell's phase between the launches (ST_FRESH before the first)
+  // cell's phase between the launches (ST_FRESH before the first). The
+  // history then holds pause records only (no warm restart; -1: the FP64
+  // fallback's passes over such a solve, which run cells to the end)
   int quantum;
 };
 
@@ -531,7 +533,9 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
     return true;
   }
   if constexpr (HIST) {
-    if (a.hist && odv_hist_match<Net>(a, i, y, prm, t0)) {
+    // (in quantum launches the history holds pause records only: a fresh
+    // cell starts cold)
+    if (a.hist && a.quantum == 0 && odv_hist_match<Net>(a, i, y, prm, t0)) {
       const bool f64 = odv_hist_fp64<Net>(a, i);
       if (a.ho_mode == 1 && f64) {
         // FP64 lanes continued the cell last call: they continue it again
@@ -557,7 +561,7 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
         // a cell this call continued from its history before handing it
         // back: its rates are the history's (the rates come from the state
         // the integration started from, not from this call's input state)
-        if (a.hist && odv_hist_match<Net>(a, i, y, prm, t0)) odv_hist_rates(a, i, s);
+        if (a.hist && a.quantum == 0 && odv_hist_match<Net>(a, i, y, prm, t0)) odv_hist_rates(a, i, s);
       }
       return false;
     }
@@ -603,7 +607,7 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   odv_result(s, y);
   odv_store(a, i, s, y);
   if constexpr (HIST) {
-    if (a.hist) odv_hist_save(a, i, s, a.ho_mode == 2);
+    if (a.hist && a.quantum == 0) odv_hist_save(a, i, s, a.ho_mode == 2);
   }
 }
 

@@ -68,6 +68,20 @@ def is_coop(network: str) -> bool:
     return network.split("+")[0] in COOP_NETWORKS
 
 
+# Attempts per cell and launch in the batch slots, 0 = no limit. With a
+# quantum q > 0 a cell that has not finished after q attempts pauses (its
+# state in a history record) and goes on in the slot's next launch, which
+# also takes the requests that arrived meanwhile: new work joins running
+# work at every launch, with results equal to the solve in one launch. On
+# the bench's serving workload it raised latencies at every q tried (256 to
+# 4096, one or two slots): the slowest PRIMORDIAL cells need up to ~40,000
+# attempts, i.e. tens to hundreds of relaunches, each of which rescans every
+# cell of the slot's requests, while without a quantum a slot's requests
+# finish in one launch and new ones take the other slot. Hence off.
+QUANTUM = 0
+L_MAXSEG = 1024   # segments per batch launch
+
+
 @dataclass
 class Session:
     """A block of cells resident on the device with its warm-restart history."""
@@ -101,6 +115,8 @@ class Request:
     result: Optional[np.ndarray] = None
     dev: dict = field(default_factory=dict)
     session: Optional[Session] = None
+    hist_planes: int = 0            # > 0: solved in quantum launches, with a history of this many planes
+    launches: int = 0               # launches the request took part in
 
     @property
     def ncells(self) -> int:
@@ -113,12 +129,13 @@ class Request:
         if self.session is not None:
             return 8 * n
         p = 0 if self.params is None else self.params.shape[0]
-        return n * (8 * self.y.shape[0] + 8 + 8 + 8 * p + 4)
+        return n * (8 * self.y.shape[0] + 8 + 8 + 8 * p + 4 + 8 * self.hist_planes)
 
 
 class Server:
     def __init__(self, bundle: str = "MIX3", slots: int = 2, hbm_budget: int = 16 << 30, block: int = 256,
-                 precision: Optional[dict] = None, fp64_fallback: int = 0, coop_chunk: int = 16384):
+                 precision: Optional[dict] = None, fp64_fallback: int = 0, coop_chunk: int = 16384,
+                 quantum: int = QUANTUM):
         import torch
         from .api import Batch
         self.torch = torch
@@ -131,7 +148,9 @@ class Server:
         self.grid = max(1, full // slots)
         self.slots = [{"stream": torch.cuda.Stream(),
                        "batch": Batch(bundle, block=block, grid=self.grid, fp64_fallback=fp64_fallback),
-                       "lane": {}, "reqs": None, "event": None} for _ in range(slots)]
+                       "lane": {}, "reqs": None, "event": None, "running": [], "finishing": []}
+                      for _ in range(slots)]
+        self.quantum = int(quantum)
         # the cooperative kernel's slot: one request at a time, chunk by chunk
         self.coop_slot = {"stream": torch.cuda.Stream(), "coop": {}, "req": None, "next": 0, "event": None}
         self.coop_chunk = int(coop_chunk)
@@ -206,17 +225,18 @@ class Server:
                 r.precision = self.precision.get(r.network, L.FP64)
             if (r.network, r.precision) not in self.entries:
                 raise ValueError("network %s precision %d not in bundle %s" % (r.network, r.precision, self.bundle))
+            r.hist_planes = L.history_planes(r.network) if self.quantum > 0 and r.session is None else 0
         if r.footprint() > self.budget:
             raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.footprint()))
         (self.coop_waiting if is_coop(r.network) else self.waiting).append(r)
 
-    def _admit(self, stream) -> List[Request]:
+    def _admit(self, stream, room: int = L_MAXSEG) -> List[Request]:
         """Upload waiting requests that fit the budget (arrival order, at most
-        one launch's worth) on `stream`; returns them."""
+        `room` segments) on `stream`; returns them."""
         torch = self.torch
         new = []
         skipped = deque()
-        while (self.waiting and len(new) < L_MAXSEG and
+        while (self.waiting and len(new) < room and
                self.in_use + self.waiting[0].footprint() <= self.budget):
             r = self.waiting.popleft()
             dev = torch.device("cuda:0")
@@ -237,6 +257,11 @@ class Server:
                      "status": torch.empty(r.ncells, dtype=torch.int32, device=dev)}
                 if r.params is not None:
                     d["params"] = torch.from_numpy(np.ascontiguousarray(r.params)).pin_memory().to(dev, non_blocking=True)
+                if r.hist_planes:
+                    # quantum launches: the cells' phase (all fresh) and room for
+                    # their pause records (read only where a pause wrote one)
+                    d["status"].fill_(L.FRESH)
+                    d["hist"] = torch.empty(r.hist_planes * r.ncells, dtype=torch.float64, device=dev)
             r.dev = d
             self.in_use += r.footprint()
             new.append(r)
@@ -378,7 +403,10 @@ class Server:
                  "dt0": r.dev["dt0"], "status": r.dev["status"], "rtol": r.rtol, "atol": r.atol, "dt_min": r.dt_min}
             if r.params is not None:
                 s["params"] = r.dev["params"]
+            if r.hist_planes:
+                s["hist"], s["quantum"] = r.dev["hist"], self.quantum
             segs.append(s)
+            r.launches += 1
         torch = self.torch
         if segs:
             slot["batch"].solve_async(segs, st.cuda_stream)
@@ -386,34 +414,65 @@ class Server:
             for r in advanced:     # the session's next advance starts where this one ends
                 r.session.dev["t"].copy_(r.dev["t_stop"], non_blocking=True)
             for r in reqs:
-                r.dev["y_host"] = torch.empty(tuple(r.dev["y"].shape), dtype=torch.float64, pin_memory=True)
-                r.dev["st_host"] = torch.empty(r.ncells, dtype=torch.int32, pin_memory=True)
-                r.dev["y_host"].copy_(r.dev["y"], non_blocking=True)
+                if "st_host" not in r.dev:
+                    r.dev["st_host"] = torch.empty(r.ncells, dtype=torch.int32, pin_memory=True)
                 r.dev["st_host"].copy_(r.dev["status"], non_blocking=True)
+                if not r.hist_planes:
+                    self._copy_y(r)
             ev = torch.cuda.Event()
             ev.record(st)
         slot["reqs"], slot["event"] = reqs, ev
+
+    def _copy_y(self, r: Request):
+        """Enqueue the copy of a request's results to pinned host memory (on
+        the current stream)."""
+        r.dev["y_host"] = self.torch.empty(tuple(r.dev["y"].shape), dtype=self.torch.float64, pin_memory=True)
+        r.dev["y_host"].copy_(r.dev["y"], non_blocking=True)
+
+    def _complete(self, r: Request, t: float, done: List[Request]):
+        r.result = r.dev["y_host"].numpy()
+        r.status = r.dev["st_host"].numpy()
+        r.done = t
+        self.in_use -= r.footprint()
+        r.dev = {}
+        if r.session is not None:
+            r.session.busy = False
+        done.append(r)
 
     def poll(self) -> List[Request]:
         """Complete finished launches, admit waiting requests, launch on free
         slots. Returns the requests completed in this call."""
         done = []
+        torch = self.torch
         for slot in self.slots:
+            # requests whose results are on their way to the host
+            still = []
+            for r, ev in slot["finishing"]:
+                if ev.query():
+                    self._complete(r, self.now(), done)
+                else:
+                    still.append((r, ev))
+            slot["finishing"] = still
             if slot["reqs"] is not None and slot["event"].query():
                 t = self.now()
                 for r in slot["reqs"]:
-                    r.result = r.dev["y_host"].numpy()
-                    r.status = r.dev["st_host"].numpy()
-                    r.done = t
-                    self.in_use -= r.footprint()
-                    r.dev = {}
-                    if r.session is not None:
-                        r.session.busy = False
-                    done.append(r)
+                    if not r.hist_planes:
+                        self._complete(r, t, done)
+                    elif (r.dev["st_host"].numpy() == L.PAUSED).any():
+                        slot["running"].append(r)      # continues in the slot's next launch
+                    else:
+                        with torch.cuda.stream(slot["stream"]):
+                            self._copy_y(r)
+                            ev = torch.cuda.Event()
+                            ev.record(slot["stream"])
+                        slot["finishing"].append((r, ev))
                 slot["reqs"], slot["event"] = None, None
         for slot in self.slots:
-            if slot["reqs"] is None and self.waiting:
-                reqs = self._admit(slot["stream"])
+            if slot["reqs"] is None and (self.waiting or slot["running"]):
+                # the slot's unfinished requests and the waiting ones that fit:
+                # new requests join the running work at every launch
+                reqs = slot["running"] + self._admit(slot["stream"], L_MAXSEG - len(slot["running"]))
+                slot["running"] = []
                 if reqs:
                     self._launch(slot, reqs)
                     self.launches += 1
@@ -425,7 +484,7 @@ class Server:
 
     def busy(self) -> bool:
         return bool(self.waiting or self.coop_waiting or self.coop_slot["req"] is not None or
-                    any(s["reqs"] is not None for s in self.slots))
+                    any(s["reqs"] is not None or s["running"] or s["finishing"] for s in self.slots))
 
     def drain(self):
         while self.busy():
@@ -441,8 +500,6 @@ class Server:
         for h in self.coop_slot["coop"].values():
             h.CloseOdevec()
 
-
-L_MAXSEG = 1024
 
 
 def make_trace(rate: float, seconds: float, seed: int = 5, networks=None, lo: float = 1e3, hi: float = 2.56e5,
