@@ -1294,7 +1294,8 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
       return fail(ODEVEC_EINVAL, "segment quantum: > 0 needs hist and status, and no flops");
     kind[k] = kk;
     total += g.ncells;
-    if (g.hist) hist = 1;
+    if (g.hist && hist < 1) hist = 1;
+    if (g.quantum > 0) hist = 2;
   }
   if (total == 0) return ODEVEC_OK;
   // FP64 fallback: mixed segments hand back the cells that reach

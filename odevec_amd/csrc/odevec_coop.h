@@ -436,7 +436,7 @@ constexpr int COOP_WAVES_F32 = 3;
 
 // HIST: the variant that continues cells from a warm-restart history and
 // saves it (for networks that only this kernel runs)
-template <class Net, class M, bool RCP, bool HIST = false>
+template <class Net, class M, bool RCP, int HIST = 0>
 __global__ void __launch_bounds__(COOP_BLOCK)
 __attribute__((amdgpu_waves_per_eu(CoopWidth<Net::NACT>::G < 64 ? 1 : (sizeof(M) == 8 ? COOP_WAVES_F64 : COOP_WAVES_F32))))
 odv_k_coop(KArgs a) {

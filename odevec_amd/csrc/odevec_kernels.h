@@ -499,8 +499,11 @@ __device__ inline bool odv_hist_unpause(const KArgs& a, long long i, L& s, const
 // nothing to integrate (masked out or t_stop <= t0; y is then left as is).
 // HIST: the kernel variant that reads and writes a warm-restart history (a
 // separate instantiation: the history code raises the register pressure of
-// the refill path, which the kernels without it should not pay).
-template <class L, bool HIST = false>
+// the refill path, which the kernels without it should not pay); 2: also
+// the launch quantum's pause and resume (the batch kernel's third variant:
+// that code more than tripled the history variant's spills, 552 -> 1,928 B
+// per lane for the MIX3 bundle).
+template <class L, int HIST = 0>
 __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
   using Net = typename L::Net;
   constexpr int NF = Net::NEQ;
@@ -517,7 +520,7 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
   s.cnt = Counters{0, 0, 0, 0, 0, 0, 0, 0};
   s.dt = dh;
   if (a.times) a.times[i] = (long long)wall_clock64();
-  if constexpr (HIST) {
+  if constexpr (HIST == 2) {
     if (a.quantum > 0 && a.hist && a.status) {
       // a solve in quantum launches: the cell's phase is its status entry
       const int ph = a.status[i];
@@ -571,9 +574,9 @@ __device__ inline bool odv_load_begin(const KArgs& a, long long i, L& s) {
   return false;
 }
 
-template <class L, bool HIST = false>
+template <class L, int HIST = 0>
 __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
-  if constexpr (HIST) {
+  if constexpr (HIST == 2) {
     if (s.status == ST_DONE_EARLIER) return;
     if (s.status == ST_PAUSED) {
       odv_hist_pause(a, i, s);
@@ -611,7 +614,7 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
   }
 }
 
-template <class Net, class M, bool RCP, bool HIST = false>
+template <class Net, class M, bool RCP, int HIST = 0>
 __global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
@@ -652,7 +655,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) odv_k_percell(KArgs a) {
 // (forced inline: called from more than one kernel of a unit, the compiler
 // would otherwise emit it as a function, with the arguments in scratch memory
 // and a call per launch)
-template <class Net, class M, bool RCP, bool HIST = false>
+template <class Net, class M, bool RCP, int HIST = 0>
 __device__ __forceinline__ void odv_persistent_run(const KArgs& a, double* lds, long long nl = 0) {
   using P = LdsPolicy<Net, M>;
   using L = PerCellLane<Net, M, RCP>;
@@ -719,7 +722,7 @@ __device__ __forceinline__ void odv_persistent_run(const KArgs& a, double* lds, 
     if (__ballot(have) == 0ull) break;
     if (have && !done) {
       done = odv_attempt(s, a.ctl, g);
-      if constexpr (HIST) {
+      if constexpr (HIST == 2) {
         if (!done && a.quantum > 0 && --left <= 0) {
           s.status = ST_PAUSED;   // the quantum is used up: the next launch continues the cell
           done = true;
@@ -760,7 +763,7 @@ __device__ __forceinline__ void odv_persistent_run(const KArgs& a, double* lds, 
 // in the build without contraction: 29 of 113 PRIMORDIAL cells, the same
 // ones at every run); without the read they agree bit for bit. Columns: b.ws
 // (a workspace for the FP64 layout) and the wavefront's LDS slice.
-template <class Net, bool HIST = false>
+template <class Net, int HIST = 0>
 __device__ __forceinline__ void odv_fb_consume(const KArgs& b, double* lds, const unsigned* done, unsigned nprod,
                                                long long idle_rounds) {
   using P = LdsPolicy<Net, double>;
@@ -861,7 +864,7 @@ __device__ __forceinline__ void odv_fb_consume(const KArgs& b, double* lds, cons
 // The FP64 lanes of the fallback as a kernel of their own, launched on a
 // second stream beside the persistent kernel whose hand-backs they take
 // (nprod: that kernel's wavefronts); one wavefront per workgroup.
-template <class Net, bool HIST = false>
+template <class Net, int HIST = 0>
 __global__ void __launch_bounds__(WAVE) odv_k_fb_consume(KArgs b, unsigned* done, unsigned nprod,
                                                          long long idle_rounds) {
   odv_fb_consume<Net, HIST>(b, odv_lane_lds, done, nprod, idle_rounds);
@@ -869,7 +872,7 @@ __global__ void __launch_bounds__(WAVE) odv_k_fb_consume(KArgs b, unsigned* done
 
 // With a.order and a.qlen the queue is the first *a.qlen entries of a.order
 // (the FP64 pass over the mixed form's promoted cells).
-template <class Net, class M, bool RCP, bool HIST = false>
+template <class Net, class M, bool RCP, int HIST = 0>
 __global__ void __launch_bounds__(MAX_BLOCK) odv_k_persistent(KArgs a) {
   odv_persistent_run<Net, M, RCP, HIST>(a, odv_lane_lds);
 }
@@ -1109,7 +1112,7 @@ struct MaxWords<E, F, R...> {
   static constexpr long long WS = E::WS_PER_SLOT > MaxWords<F, R...>::WS ? E::WS_PER_SLOT : MaxWords<F, R...>::WS;
 };
 
-template <int K, bool HIST, class E, class... R>
+template <int K, int HIST, class E, class... R>
 __device__ inline void odv_batch_dispatch(int kind, const KArgs& a, double* lds) {
   if (kind == K) {
     odv_persistent_run<typename E::Net, typename E::M, E::RCP, HIST>(a, lds);
@@ -1119,8 +1122,9 @@ __device__ inline void odv_batch_dispatch(int kind, const KArgs& a, double* lds)
 }
 
 // HIST: the variant for tables where segments carry warm-restart histories
-// (segments without one start cold in it too)
-template <bool HIST, class... E>
+// (segments without one start cold in it too); 2: some segment runs with a
+// launch quantum
+template <int HIST, class... E>
 __global__ void __launch_bounds__(MAX_BLOCK) odv_k_batch(const SegDev* segs, const int* nseg) {
   const int n = *nseg;
   for (int k = 0; k < n; ++k) {

@@ -184,7 +184,7 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
       if (a.hist && Net::LANE) return hipErrorInvalidValue;
       hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * GPB));
       if (a.hist)
-        hipLaunchKernelGGL((odv_k_coop<Net, M, RCP, true>), dim3((unsigned)p.grid), dim3(COOP_BLOCK),
+        hipLaunchKernelGGL((odv_k_coop<Net, M, RCP, 1>), dim3((unsigned)p.grid), dim3(COOP_BLOCK),
                            coop_lds_bytes<Net>(), st, a);
       else
         hipLaunchKernelGGL((odv_k_coop<Net, M, RCP>), dim3((unsigned)p.grid), dim3(COOP_BLOCK), coop_lds_bytes<Net>(),
@@ -200,13 +200,13 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
     if (a.hist && mode != MODE_PERCELL && mode != MODE_PERSISTENT) return hipErrorInvalidValue;
     if (mode == MODE_PERCELL) {
       if (a.hist)
-        hipLaunchKernelGGL((odv_k_percell<Net, M, RCP, true>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
+        hipLaunchKernelGGL((odv_k_percell<Net, M, RCP, 1>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
       else
         hipLaunchKernelGGL((odv_k_percell<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
     } else if (mode == MODE_PERSISTENT) {
       hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * p.block));
       if (a.hist)
-        hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP, true>), dim3((unsigned)p.grid), dim3(p.block), lds, st,
+        hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP, 1>), dim3((unsigned)p.grid), dim3(p.block), lds, st,
                            a);
       else
         hipLaunchKernelGGL((odv_k_persistent<Net, M, RCP>), dim3((unsigned)p.grid), dim3(p.block), lds, st, a);
@@ -234,11 +234,11 @@ hipError_t launch_fbc_net(const KArgs& b, unsigned* done, int nprod, long long g
     if ((LdsPolicy<Net, double>::WS_PER_SLOT > 0 && !b.ws) || !b.order || !b.qlen || !b.queue || !b.ho || !done)
       return hipErrorInvalidValue;
     if (b.hist)
-      hipLaunchKernelGGL((odv_k_fb_consume<Net, true>), dim3((unsigned)grid), dim3(WAVE),
+      hipLaunchKernelGGL((odv_k_fb_consume<Net, 1>), dim3((unsigned)grid), dim3(WAVE),
                          lane_lds_bytes(LdsPolicy<Net, double>::WORDS, WAVE), st, b, done, (unsigned)nprod,
                          FB_IDLE_ROUNDS);
     else
-      hipLaunchKernelGGL((odv_k_fb_consume<Net, false>), dim3((unsigned)grid), dim3(WAVE),
+      hipLaunchKernelGGL((odv_k_fb_consume<Net, 0>), dim3((unsigned)grid), dim3(WAVE),
                          lane_lds_bytes(LdsPolicy<Net, double>::WORDS, WAVE), st, b, done, (unsigned)nprod,
                          FB_IDLE_ROUNDS);
     return hipGetLastError();
@@ -342,7 +342,7 @@ struct Bundle {
       hipError_t e = hipGetDevice(&dev);
       if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
       if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(odv_k_batch<false, E...>), b,
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(odv_k_batch<0, E...>), b,
                                                          lane_lds_bytes(MaxWords<E...>::WORDS, b));
       if (e != hipSuccess) return e;
       g = (long long)ncu * (nb > 0 ? nb : 1);
@@ -355,18 +355,22 @@ struct Bundle {
     return hipSuccess;
   }
 
-  // hist: launch the variant that reads and writes the segments' warm-restart histories
+  // hist: 1 launch the variant that reads and writes the segments'
+  // warm-restart histories, 2 the one that also runs launch quanta
   static hipError_t launch(void* segs, const int* nseg, int maxseg, const Plan& p, int hist, hipStream_t st) {
     if (p.block <= 0 || p.block > MAX_BLOCK || p.block % WAVE != 0 || p.grid <= 0 || p.grid > 0x7fffffffLL)
       return hipErrorInvalidValue;
     SegDev* sd = static_cast<SegDev*>(segs);
     hipLaunchKernelGGL(k_batch_reset<0>, dim3((unsigned)((maxseg + 255) / 256)), dim3(256), 0, st, sd, nseg,
                        (unsigned long long)(p.grid * p.block));
-    if (hist)
-      hipLaunchKernelGGL((odv_k_batch<true, E...>), dim3((unsigned)p.grid), dim3(p.block),
+    if (hist == 2)
+      hipLaunchKernelGGL((odv_k_batch<2, E...>), dim3((unsigned)p.grid), dim3(p.block),
+                         lane_lds_bytes(MaxWords<E...>::WORDS, p.block), st, static_cast<const SegDev*>(sd), nseg);
+    else if (hist)
+      hipLaunchKernelGGL((odv_k_batch<1, E...>), dim3((unsigned)p.grid), dim3(p.block),
                          lane_lds_bytes(MaxWords<E...>::WORDS, p.block), st, static_cast<const SegDev*>(sd), nseg);
     else
-      hipLaunchKernelGGL((odv_k_batch<false, E...>), dim3((unsigned)p.grid), dim3(p.block),
+      hipLaunchKernelGGL((odv_k_batch<0, E...>), dim3((unsigned)p.grid), dim3(p.block),
                          lane_lds_bytes(MaxWords<E...>::WORDS, p.block), st, static_cast<const SegDev*>(sd), nseg);
     return hipGetLastError();
   }
@@ -374,7 +378,7 @@ struct Bundle {
   static hipError_t info(int block, KernelInfo* out) {
     const int b = block > 0 ? block : BLOCK;
     if (b > MAX_BLOCK || b % WAVE != 0) return hipErrorInvalidValue;
-    return kinfo(odv_k_batch<false, E...>, b, out, lane_lds_bytes(MaxWords<E...>::WORDS, b));
+    return kinfo(odv_k_batch<0, E...>, b, out, lane_lds_bytes(MaxWords<E...>::WORDS, b));
   }
 };
 
