@@ -585,13 +585,17 @@ __device__ inline void odv_finish(const KArgs& a, long long i, const L& s) {
     }
   }
   if (a.times) a.times[a.ncells + i] = (long long)wall_clock64();
-  if (s.status == ST_PROMOTE || s.status == ST_PROMOTE_HIST) {
+  if (s.status == ST_PROMOTE || (HIST && s.status == ST_PROMOTE_HIST)) {
     // the caller's state stays untouched for the FP64 pass over promo[],
     // which continues the cell from its hand-off record (or, ST_PROMOTE_HIST,
     // from its history; the record then only says that no work came with it)
     if (a.ho) {
-      if (s.status == ST_PROMOTE) odv_handoff_save(a, i, s);
-      else odv_handoff_no_counters<typename L::Net>(a, i);
+      if constexpr (HIST) {
+        if (s.status == ST_PROMOTE) odv_handoff_save(a, i, s);
+        else odv_handoff_no_counters<typename L::Net>(a, i);
+      } else {
+        odv_handoff_save(a, i, s);
+      }
     }
     // the entry is published once the record's stores have completed (an
     // FP64 lane of the same launch may be waiting for it)
