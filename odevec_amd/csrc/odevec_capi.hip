@@ -1300,7 +1300,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
   if (total == 0) return ODEVEC_OK;
   // FP64 fallback: mixed segments hand back the cells that reach
   // promote_cf convergence failures; FP64 lanes beside the batch launch (one
-  // kernel per such segment, on a second stream, in 1 in FB_SHARE of the
+  // kernel per such segment, on a second stream, in 1 in FB_SHARE_BATCH of the
   // machine's wavefront slots) continue them from their hand-off records, and
   // after both the network's FP64 persistent kernel takes what they left
   std::vector<const odv::NetEntry*> fbnet(nseg, nullptr);
@@ -1331,7 +1331,7 @@ int32_t odevec_batch_solve_async(odevec_batch_t* b, const odevec_segment_t* segs
   int nprod = 0;
   if (fbtotal > 0) {
     const long long wpb = p.block / 64, waves = p.grid * wpb;
-    nfb = waves / odv::FB_SHARE > 0 ? waves / odv::FB_SHARE : 1;
+    nfb = waves / odv::FB_SHARE_BATCH > 0 ? waves / odv::FB_SHARE_BATCH : 1;
     const long long keep = (waves - nfb) / wpb;
     p.grid = keep > 0 ? keep : 1;
     nprod = (int)(p.grid * wpb);

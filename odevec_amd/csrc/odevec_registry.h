@@ -220,6 +220,10 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
 // FP64 lanes of the fallback beside a persistent launch: 1 in FB_SHARE of
 // the machine's wavefront slots (at least one).
 constexpr int FB_SHARE = 32;
+// ... and beside a batch launch, 1 in FB_SHARE_BATCH (2,097,152-cell MIX3_MIXED
+// batch, ms per solve at 1 in 8 / 16 / 32: 117.6 / 110.0 / 119.5; the 1M-cell
+// PRIMORDIAL handle at 1 in 16 / 32: 119.5 / 115.7)
+constexpr int FB_SHARE_BATCH = 16;
 // an FP64 wavefront that has looked at the list this many times in a row with
 // nothing to run (a sleep and a few loads each, about 1 us: about a second)
 // leaves the rest of the fallback to the pass after the launch
