@@ -541,3 +541,43 @@ def test_krome_setupfile_rhs_jacobian_and_conservation(oracle):
         e = np.zeros(21)
         e[i] = 1.0
         assert np.allclose(net.rhs(y, e), np.array(f_sym(y, e), dtype=float), rtol=1e-14, atol=0.0), i
+
+
+@pytest.mark.parametrize("t_stop", [2.0 ** 10, 2.0 ** 14, 3.0])
+def test_step_controller_on_a_constant_solution(oracle, t_stop):
+    """The step-size controller where its outcome is fixed in closed form:
+    with every ROBER rate 0 the right-hand side is 0, so every predictor is
+    exact, every correction 0, and the controller's candidates are all
+    1/(factor * 0 + 1e-6) = 1e6. The reference's rules then fix the whole
+    step sequence: the initial step sqrt(tol / (W0^-2 + 0)) with W0 =
+    max(|t0|, |t_stop|) (src/odevec.F90:224-236, capped at the span), a step
+    selection after every q + 1 = 2 successful steps (:447-450) that keeps
+    order 1 (up does not beat same on a tie: :570-600) and scales the step
+    by the cap, 1e4 at the first selection and 10 after (:605-611), the
+    last step overshooting t_stop (no clamping). A mis-stated cap, cadence,
+    tie rule or initial-step formula changes the steps, the selections or
+    the last step size."""
+    net = _net(oracle, "ROBER/NONE/DENSE")
+    rtol = 2.0 ** -40                       # powers of two: the initial step is exact
+    y0 = np.array([[1.0], [0.0], [0.0]])
+    y, st, dto, cnt = net.solve_cells(y0, t_stop, params=np.zeros((3, 1)), rtol=rtol, atol=1e-10,
+                                      dt_min=1e-300)
+    # the sequence those rules give
+    W0 = max(0.0, t_stop)
+    h = min(math.sqrt(rtol / W0 ** -2.0), t_stop)
+    t, steps, sel, swu, first, jac = 0.0, 0, 0, 0, True, 1
+    while t < t_stop:
+        t += h
+        steps += 1
+        swu += 1
+        if swu == 2:
+            h = h * (1e4 if first else 10.0)
+            first, swu, sel = False, 0, sel + 1
+            # the step change (>= 10, far past the 0.3 threshold, :454-458)
+            # refreshes the Jacobian at the next attempt, if there is one
+            jac += 1 if t < t_stop else 0
+    assert st[0] == 0 and np.array_equal(y, y0)
+    # attempts, steps, Jacobian refreshes, selections
+    assert (cnt[0][0], cnt[1][0], cnt[3][0], cnt[6][0]) == (steps, steps, jac, sel)
+    # W0^-2 goes through the restated pow (exp(-2 log W0)), within an ulp of 2^-2k
+    assert dto[0] == pytest.approx(h, rel=1e-14, abs=0.0)
