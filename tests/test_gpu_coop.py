@@ -58,6 +58,7 @@ CASES = {
     "PRIMORDIAL": lambda: W.primordial(300, seed=33),
     "PRIMORDIAL_FF": lambda: W.primordial(130, seed=34, network="PRIMORDIAL/FF/CSC"),
     "SYNTH64": lambda: W.synth64(40, seed=35),
+    "KROME": lambda: W.krome_ism(120, seed=38),     # 12 species, 21 rate parameters (groups of 16)
 }
 
 
