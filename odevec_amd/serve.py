@@ -629,12 +629,14 @@ def main(argv=None) -> int:
     p.add_argument("--seconds", type=float, default=10.0)
     p.add_argument("--slots", type=int, default=2)
     p.add_argument("--bundle", default="MIX3")
+    p.add_argument("--quantum", type=int, default=QUANTUM, help="attempts per cell and batch launch (0: no limit)")
+    p.add_argument("--coop_chunk", type=int, default=16384, help="cells per cooperative-kernel launch")
     a = p.parse_args(argv)
     from . import autotune as A
     tuned = A.lookup(list(SERVE_SPANS)) or {}
     prec = tuned.get("precision", {})
     for rate in [float(x) for x in a.rates.split(",")]:
-        srv = Server(a.bundle, slots=a.slots, precision=prec)
+        srv = Server(a.bundle, slots=a.slots, precision=prec, quantum=a.quantum, coop_chunk=a.coop_chunk)
         res = run_trace(srv, make_trace(rate, a.seconds))
         srv.close()
         print(json.dumps(dict(rate_per_s=rate, **res)), flush=True)
