@@ -270,9 +270,14 @@ def synth64(steps, warmup, cells=65536):
     from odevec_amd import _lib as L
     from odevec_amd import workloads as W
     w = W.synth64(cells, seed=7)
-    out = {"workload": "SYNTH64 %d cells (seed 7), t_stop 1e6 s, rtol %g, cooperative kernel" % (cells, w["rtol"])}
+    out = {"workload": "SYNTH64 %d cells (seed 7), t_stop 1e6 s, rtol %g, cooperative kernel; mixed form with the "
+                       "reference's RMS corrector test" % (cells, w["rtol"])}
     for name, prec in (("mixed", 1), ("fp64", 0)):
         r = Runner(w, precision=prec, mode=L.COOP)
+        if prec == 1:
+            # the reference's RMS corrector test: SYNTH64's FP32 factors
+            # contract every component (the max norm adds 35% iterations)
+            r.ode.set_mixed_convergence_test("rms")
         wall, ms = r.timed(steps, warmup)
         out[name] = {"value": steps * cells / wall, "unit": "cell-solves/s", "kernel_ms": ms,
                      "cells_ok": int((r.st == 0).sum()),

@@ -156,6 +156,20 @@ int32_t odevec_set_fp64_fallback(odevec_t* h, int32_t convfail_limit);
 #define ODEVEC_FALLBACK_INVARIANT_FAILS 32
 int32_t odevec_set_invariant_test(odevec_t* h, double tolerances, int32_t fallback_fails);
 
+/* Convergence test of the mixed form's corrector (no reference counterpart;
+ * the reference's CheckConvergence is src/odevec.F90:622-665).
+ * ODEVEC_MIXED_TEST_MAX (the default): every weighted component of the last
+ * correction within the divisor, with no rate discount -- FP32 factors of a
+ * very stiff P contract trace species and invariants more slowly than the
+ * RMS norm shows (PRIMORDIAL). ODEVEC_MIXED_TEST_RMS: the reference's test,
+ * the RMS norm times min(1, 1.5 rate), as the FP64 form uses it -- for
+ * networks whose FP32 factors contract every component (SYNTH64: the same
+ * corrector iterations as FP64, 35% fewer than under the max norm, at the
+ * same distance to an rtol-1e-12 solution). The FP64 form is unaffected. */
+#define ODEVEC_MIXED_TEST_MAX 0
+#define ODEVEC_MIXED_TEST_RMS 1
+int32_t odevec_set_mixed_convergence_test(odevec_t* h, int32_t test);
+
 /* Replaces SolveODE(this,time,dt,t_stop,y,GetRHS,GetJac,GetLU,Mask)
  * (src/odevec.F90:158-214) for the handle's nvector cells: integrate y from
  * *time to t_stop; *dt is the initial-step hint on entry and the last step

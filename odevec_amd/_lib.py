@@ -99,6 +99,7 @@ def lib():
         "odevec_set_fp64_fallback": (_I32, [_VP, _I32]),
         "odevec_batch_set_fp64_fallback": (_I32, [_VP, _I32]),
         "odevec_set_invariant_test": (_I32, [_VP, C.c_double, _I32]),
+        "odevec_set_mixed_convergence_test": (_I32, [_VP, _I32]),
         "odevec_batch_set_invariant_test": (_I32, [_VP, C.c_double, _I32]),
         "odevec_solve": (_I32, [_VP, _DP, _DP, C.c_double, _VP, _VP, _VP]),
         "odevec_solve_cells": (_I32, [_VP, _I64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(Stats)]),
@@ -133,6 +134,7 @@ EXPORTED = [
     "odevec_bundle_name", "odevec_bundle_entry", "odevec_batch_create", "odevec_batch_solve_async",
     "odevec_batch_kernel_info", "odevec_batch_destroy", "odevec_batch_set_fp64_fallback",
     "odevec_set_invariant_test", "odevec_batch_set_invariant_test", "odevec_solve_cells_noalloc",
+    "odevec_set_mixed_convergence_test",
 ]
 
 

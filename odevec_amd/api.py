@@ -136,6 +136,16 @@ class Odevec:
         `fallback_fails` such failures (0: never)."""
         L.check(L.lib().odevec_set_invariant_test(self._h, float(tolerances), int(fallback_fails)))
 
+    def set_mixed_convergence_test(self, test: str = "max"):
+        """Mixed form's corrector test: "max" (the default) asks every
+        weighted component of the last correction to be within the divisor;
+        "rms" is the reference's test (RMS norm times min(1, 1.5 rate),
+        src/odevec.F90:622-665), for networks whose FP32 factors contract
+        every component (SYNTH64). The FP64 form is unaffected."""
+        if test not in ("max", "rms"):
+            raise ValueError("test must be 'max' or 'rms'")
+        L.check(L.lib().odevec_set_mixed_convergence_test(self._h, 1 if test == "rms" else 0))
+
     def disable_history(self):
         L.check(L.lib().odevec_set_history(self._h, None, 0))
         self._hist = None

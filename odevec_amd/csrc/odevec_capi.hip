@@ -117,6 +117,7 @@ struct odevec_s {
   int promote_cf = 0;                // mixed form: FP64 fallback after this many convergence failures (0: off)
   double inv_tol = ODEVEC_MIXED_INVARIANT_TOL;   // mixed form: invariant test of the corrector (0: off)
   int promote_inv = ODEVEC_FALLBACK_INVARIANT_FAILS;   // after this many failed tests: FP64 fallback, or no test (0: neither)
+  int mixed_rms = 0;                 // mixed form: the reference's RMS corrector test instead of the max norm
   int* d_promo = nullptr;            // cells handed to the FP64 pass
   unsigned* d_promo_n = nullptr;     // their count
   int64_t promo_cap = 0;
@@ -231,6 +232,7 @@ int launch(odevec_s* h, const Req& r) {
   a.ctl.max_attempts = h->opts.max_attempts > 0 ? h->opts.max_attempts : 1000000;
   a.ctl.inv_tol = h->inv_tol;
   a.ctl.promote_inv = h->promote_inv;
+  a.ctl.mixed_rms = h->mixed_rms;
   int grp = h->opts.group;
   if (grp <= 0) grp = (int)(h->nvector < 256 ? h->nvector : 256);
   if (grp < 1) grp = 1;
@@ -663,6 +665,14 @@ int32_t odevec_set_fp64_fallback(odevec_t* h, int32_t convfail_limit) {
   if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
   if (convfail_limit < 0) return fail(ODEVEC_EINVAL, "convfail_limit must be >= 0");
   h->promote_cf = convfail_limit;
+  return ODEVEC_OK;
+}
+
+int32_t odevec_set_mixed_convergence_test(odevec_t* h, int32_t test) {
+  if (!h) return fail(ODEVEC_EINVAL, "handle is NULL");
+  if (test != ODEVEC_MIXED_TEST_MAX && test != ODEVEC_MIXED_TEST_RMS)
+    return fail(ODEVEC_EINVAL, "test must be ODEVEC_MIXED_TEST_MAX (0) or ODEVEC_MIXED_TEST_RMS (1)");
+  h->mixed_rms = test == ODEVEC_MIXED_TEST_RMS ? 1 : 0;
   return ODEVEC_OK;
 }
 

@@ -62,6 +62,9 @@ module odevec_main
     ! ... cells failing it this many times go to FP64 (FP64Fallback on) or are no
     ! longer tested (0: neither)
     integer :: InvariantFallbackFails = 32
+    ! mixed form's corrector test: 0 every weighted component (max norm), 1 the
+    ! RMS norm with the rate discount of CheckConvergence (as in FP64)
+    integer :: MixedConvergenceTest = 0
     real(dp), allocatable :: y(:,:)
     integer(c_int32_t), allocatable :: status(:)
     type(c_ptr) :: handle = c_null_ptr
@@ -109,6 +112,12 @@ module odevec_main
       type(c_ptr), value :: h
       real(c_double), value :: tol
       integer(c_int32_t), value :: fails
+      integer(c_int32_t) :: rc
+    end function
+    function c_set_mixed_test(h, test) bind(C, name='odevec_set_mixed_convergence_test') result(rc)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: test
       integer(c_int32_t) :: rc
     end function
     function c_solve(h, time, dt, t_stop, y, mask, status) bind(C, name='odevec_solve') result(rc)
@@ -197,6 +206,7 @@ contains
     if (rc == 0) rc = c_set_fp64_fallback(this%handle, int(this%FP64Fallback, c_int32_t))
     if (rc == 0) rc = c_set_invariant_test(this%handle, real(this%InvariantTolerance, c_double), &
                                            int(this%InvariantFallbackFails, c_int32_t))
+    if (rc == 0) rc = c_set_mixed_test(this%handle, int(this%MixedConvergenceTest, c_int32_t))
     if (rc /= 0) call fail(rc, "SolveODE (settings)")
     mp = c_null_ptr
     if (present(Mask)) then

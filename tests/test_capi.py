@@ -53,6 +53,7 @@ def test_create_argument_checks():
     assert L.odevec_set_tolerances(None, 1e-6, 1e-10) == _lib.EINVAL
     assert L.odevec_set_fp64_fallback(None, 128) == _lib.EINVAL
     assert L.odevec_set_invariant_test(None, 3.0, 32) == _lib.EINVAL
+    assert L.odevec_set_mixed_convergence_test(None, 1) == _lib.EINVAL
     assert L.odevec_destroy(None) == _lib.OK
 
 

@@ -21,12 +21,14 @@ def say(*a):
     print(*a, flush=True)
 
 
-def coop_solve(nid, w, precision=_lib.FP64, grid=0):
+def coop_solve(nid, w, precision=_lib.FP64, grid=0, mixed_test=None):
     from odevec_amd.api import Odevec
     n = w["y"].shape[1]
-    say("GPU coop solve", nid, "cells", n, "precision", precision, "grid", grid)
+    say("GPU coop solve", nid, "cells", n, "precision", precision, "grid", grid, "mixed test", mixed_test)
     ode = Odevec(nid, max(n, 1), precision=precision, mode=_lib.COOP, grid=grid)
     ode.RelativeTolerance, ode.AbsoluteTolerance, ode.MinimalTimestep = w["rtol"], w["atol"], w["dt_min"]
+    if mixed_test:
+        ode.set_mixed_convergence_test(mixed_test)
     y = np.array(w["y"], order="C", copy=True)
     st, dto, stats = ode.solve_cells(y, w["t_stop"], t0=w["t0"], dt0=w["dt0"], params=w["params"], stats=True,
                                      raise_on_fail=False)
@@ -93,6 +95,29 @@ def test_coop_default_build_accuracy(oracle, case, precision):
     print("coop %s precision %d: median %.3g, within 10 tolerances %.4f" % (case, precision, np.median(e),
                                                                          (e <= 10).mean()), flush=True)
     assert np.median(e) < 1.0 and (e <= 10).mean() > 0.8
+
+
+def test_coop_mixed_rms_test(oracle):
+    """SYNTH64's mixed form under the reference's RMS corrector test: every
+    cell finishes within the accuracy bound of the max-norm form, with fewer
+    corrector iterations; on an FP64 handle the setting changes nothing."""
+    w = W.synth64(256, seed=36)
+    yr, sr, dr, cr = oracle_cells(oracle, w["network"], w)
+    res = {}
+    for t in ("max", "rms"):
+        y, st, dto, stats = coop_solve(w["network"], w, _lib.MIXED, mixed_test=t)
+        ok = (sr == 0) & (st == 0)
+        assert (st[sr == 0] == 0).all(), t
+        e = (np.abs(y - yr)[:, ok] / (w["rtol"] * np.abs(yr[:, ok]) + w["atol"])).max(axis=0)
+        say("mixed test %s: newton %d, median %.3g, within 10 tolerances %.4f" % (t, stats["newton"], np.median(e),
+                                                                                 (e <= 10).mean()))
+        assert np.median(e) < 1.0 and (e <= 10).mean() > 0.8, t
+        res[t] = stats["newton"]
+    assert res["rms"] < res["max"]
+    w8 = W.subset(w, np.arange(32))
+    a = coop_solve(w["network"] + "+REPRO", w8)
+    b = coop_solve(w["network"] + "+REPRO", w8, mixed_test="rms")
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
 
 
 @pytest.mark.parametrize("case", ["ROBER", "SYNTH64"])
