@@ -45,6 +45,7 @@ struct odv_net_OREGO_NONE_CSC {
 
   // cooperative kernels (species on lanes): tables in the oracle's order
   static constexpr int CO_MAXT = 4;   // most RHS terms of one species
+  static constexpr int CO_MAXJ = 5;   // most Jacobian contributions of one row
   static constexpr short RT_OFF[4] = {
       0, 4, 7, 9,
   };
@@ -87,6 +88,34 @@ struct odv_net_OREGO_NONE_CSC {
   };
   static constexpr unsigned long long LU_MASK[3] = {
       0x3ull, 0x7ull, 0x7ull,
+  };
+  static constexpr unsigned char RL_N[3] = {
+      4, 3, 2,
+  };
+  static constexpr unsigned RL_W[12] = {
+      131071u, 196607u, 131071u, 196607u, 131071u, 131071u, 196607u, 131071u, 0u, 131071u, 0u, 0u,
+  };
+  static constexpr unsigned RL_F[12] = {
+      0u, 256u, 0u, 0u, 1u, 2u, 256u, 2u, 0u, 1u, 0u, 0u,
+  };
+  static constexpr double RL_C[12] = {
+      77.27, -0.012941633234114146, 0.161, -0.00064713625, -0.012941633234114146, -0.161, -77.27, 0.012941633234114146,
+      0.0, 77.27, 0.0, 0.0,
+  };
+  static constexpr unsigned char JL_N[3] = {
+      5, 4, 2,
+  };
+  static constexpr unsigned JL_W[15] = {
+      65535u, 131071u, 65535u, 131071u, 16908287u, 33619967u, 131071u, 16842751u, 0u, 16908287u, 33619967u, 0u,
+      16842751u, 0u, 0u,
+  };
+  static constexpr unsigned JL_F[15] = {
+      0u, 1u, 0u, 0u, 0u, 0u, 1u, 0u, 0u, 0u, 0u, 0u,
+      0u, 0u, 0u,
+  };
+  static constexpr double JL_C[15] = {
+      77.27, -0.012941633234114146, 0.161, -0.0012942725, -0.012941633234114146, -0.161, -77.27, -0.012941633234114146,
+      0.0, -77.27, 0.012941633234114146, 0.0, 77.27, 0.0, 0.0,
   };
   static constexpr bool LANE = true;   // per-lane kernels available
   // the reference's GetJac layout from this storage (see odevec_get_jac)

@@ -50,6 +50,7 @@ struct odv_net_ROBER_NONE_CSC {
 
   // cooperative kernels (species on lanes): tables in the oracle's order
   static constexpr int CO_MAXT = 3;   // most RHS terms of one species
+  static constexpr int CO_MAXJ = 4;   // most Jacobian contributions of one row
   static constexpr short RT_OFF[4] = {
       0, 2, 5, 6,
   };
@@ -88,6 +89,32 @@ struct odv_net_ROBER_NONE_CSC {
   };
   static constexpr unsigned long long LU_MASK[3] = {
       0x7ull, 0x7ull, 0x6ull,
+  };
+  static constexpr unsigned char RL_N[3] = {
+      2, 3, 1,
+  };
+  static constexpr unsigned RL_W[9] = {
+      65536u, 65536u, 131073u, 131074u, 131073u, 0u, 0u, 131074u, 0u,
+  };
+  static constexpr unsigned RL_F[9] = {
+      0u, 0u, 257u, 513u, 257u, 0u, 0u, 513u, 0u,
+  };
+  static constexpr double RL_C[9] = {
+      -1.0, 1.0, 1.0, 1.0, -1.0, 0.0, 0.0, -1.0,
+      0.0,
+  };
+  static constexpr unsigned char JL_N[3] = {
+      3, 4, 1,
+  };
+  static constexpr unsigned JL_W[12] = {
+      0u, 0u, 16842753u, 16842754u, 16842753u, 0u, 33619970u, 16842754u, 0u, 0u, 33619970u, 0u,
+  };
+  static constexpr unsigned JL_F[12] = {
+      0u, 0u, 1u, 2u, 1u, 0u, 1u, 2u, 0u, 0u, 1u, 0u,
+  };
+  static constexpr double JL_C[12] = {
+      -1.0, 1.0, 2.0, 1.0, -2.0, 0.0, 1.0, -1.0,
+      0.0, 0.0, -1.0, 0.0,
   };
   static constexpr bool LANE = true;   // per-lane kernels available
   // the reference's GetJac layout from this storage (see odevec_get_jac)

@@ -7,7 +7,9 @@
 //
 //  * lane l holds species l's Nordsieck history, correction, weight and
 //    previous correction, and row l of the permuted iteration matrix P and
-//    its factors (dense row, structural zeros kept as exact zeros);
+//    its factors (dense row, structural zeros kept as exact zeros) -- in its
+//    registers, or, for one-wavefront groups in the mixed form, in the
+//    group's LDS block (CoopKind below);
 //  * the cell's rates, full state, parameters, the active species' current
 //    values and one row of U (the broadcast row of the factorization) sit in
 //    the group's LDS block;
@@ -24,6 +26,7 @@
 // order (only exact zeros of the dense storage are skipped), so the
 // reproducible build equals the oracle bit for bit here too.
 #pragma once
+#include <type_traits>
 #include "odevec_kernels.h"
 
 namespace odv {
@@ -77,8 +80,13 @@ struct CoopLayout {
   static constexpr int WORDS = UR + Net::NACT;
 };
 
-// The generated network seen by a one-species lane.
-template <class Net_, int G_>
+// The generated network seen by a one-species lane. MR_: void, each lane's
+// row of P and the factors in its registers (a[0..NB-1]); else their element
+// type, the rows in the group's LDS block: entry (row l, column j) at
+// rows()[j*RS + l] (column stride G + 1: a lane's own entries and a row read
+// across the lanes both fall in distinct banks), over the U-row broadcast
+// slot, which this form does not need.
+template <class Net_, int G_, class MR_ = void>
 struct CoopNet {
   using Base = Net_;
   using Y = CoopLayout<Net_>;
@@ -88,7 +96,12 @@ struct CoopNet {
   static constexpr int NEQ = Net_::NEQ;     // the norms divide by the caller's species count
   static constexpr int NACT = 1;            // species per lane
   static constexpr int ACT[1] = {0};
-  static constexpr int NNZ = NB;            // a lane's row of P / the factors
+  static constexpr bool LROWS = !std::is_void<MR_>::value;
+  using MR = typename std::conditional<LROWS, MR_, float>::type;
+  static constexpr int RS = G + 1;
+  // a group's LDS block (doubles)
+  static constexpr int STRIDE = LROWS ? Y::UR + (NB * RS * (int)sizeof(MR) + 7) / 8 : Y::WORDS;
+  static constexpr int NNZ = LROWS ? 1 : NB;   // a lane's row of P / the factors in registers
   static constexpr bool FD_JAC = false;     // the symbolic Jacobian (tables below)
   static constexpr int NINV = Net_::NINV;   // the cell's linear invariants (mixed form's convergence test)
   static constexpr int NRATE = 1;           // the rates live in the group's LDS block
@@ -100,7 +113,8 @@ struct CoopNet {
   static constexpr int FLOPS_LU_R = Net_::FLOPS_LU, FLOPS_SOLVE_R = Net_::FLOPS_SOLVE;
   static_assert(NB <= G && G <= 64, "a cell must fit one wavefront");
 
-  __device__ static double* blk() { return odv_coop_lds + coop_grp<G>() * Y::WORDS; }
+  __device__ static double* blk() { return odv_coop_lds + coop_grp<G>() * STRIDE; }
+  __device__ static MR* rows() { return reinterpret_cast<MR*>(blk() + Y::UR); }
 
   // coefficient of this lane's species in invariant k (0 on padding lanes)
   __device__ static double inv(int k, int) {
@@ -128,28 +142,61 @@ struct CoopNet {
     odv_wave_sync();
     const double* kk = b + Y::K;
     const double* ya = b + Y::YA;
+    // the lane-major tables: term q of every lane's species in consecutive words
     double acc = 0.0;
     if (l < NB) {
-      const int t0 = Base::RT_OFF[l], t1 = Base::RT_OFF[l + 1];
-      for (int t = t0; t < t1; ++t) {
-        const int r = Base::RT_RATE[t];
-        const int nf = Base::RT_NF[t];
-        const unsigned char* fc = Base::RT_F + 4 * t;
-        double m;
-        int q = 0;
-        if (r >= 0) m = kk[r];
-        else if (nf > 0) { m = ya[fc[0]]; q = 1; }
-        else m = 1.0;
-        for (; q < nf; ++q) m = m * ya[fc[q]];
-        const double v = Base::RT_COEF[t] * m;
-        acc = (t == t0) ? v : acc + v;
+      const int nt = Base::RL_N[l];
+      for (int q = 0; q < Base::CO_MAXT; ++q) {
+        if (q < nt) {
+          const unsigned w = Base::RL_W[q * NB + l];
+          const unsigned fw = Base::RL_F[q * NB + l];
+          const int r = (int)(short)(w & 0xffffu);
+          const int nf = (int)(w >> 16);
+          double m;
+          int p0 = 0;
+          if (r >= 0) m = kk[r];
+          else if (nf > 0) { m = ya[fw & 0xffu]; p0 = 1; }
+          else m = 1.0;
+#pragma unroll
+          for (int p = 0; p < 4; ++p)
+            if (p >= p0 && p < nf) m = m * ya[(fw >> (8 * p)) & 0xffu];
+          const double v = Base::RL_C[q * NB + l] * m;
+          acc = (q == 0) ? v : acc + v;
+        }
       }
     }
     f[0] = acc;
     odv_wave_sync();
   }
 
-  // row l of P = I - hb*J (permuted), from the Jacobian contributions of its entries
+  // entry (row, jj) of P = I - hb*J (permuted), from its Jacobian contributions
+  __device__ __forceinline__ static double pentry(int row, int jj, const double* kk, const double* ya, double hb) {
+    double acc = 0.0;
+    bool has = false;
+    const int c0 = Base::JT_OFF[row * NB + jj], c1 = Base::JT_OFF[row * NB + jj + 1];
+    for (int c = c0; c < c1; ++c) {
+      const int r = Base::JT_RATE[c];
+      const int nr = Base::JT_NR[c];
+      const unsigned char* fr = Base::JT_R + 3 * c;
+      double m;
+      int q = 0;
+      if (r >= 0) m = kk[r];
+      else if (nr > 0) { m = ya[fr[0]]; q = 1; }
+      else m = 1.0;
+      for (; q < nr; ++q) m = m * ya[fr[q]];
+      const double v = Base::JT_CE[c] * m;
+      acc = has ? acc + v : v;
+      has = true;
+    }
+    const bool diag = row == jj;
+    return has ? (diag ? 1.0 - hb * acc : -(hb * acc)) : (diag ? 1.0 : 0.0);
+  }
+  // entry (row, jj) of P from its accumulated Jacobian contributions
+  __device__ __forceinline__ static double pfinal(int row, int jj, double acc, double hb) {
+    return row == jj ? 1.0 - hb * acc : -(hb * acc);
+  }
+
+  // row l of P
   template <class M, int S = 1>
   __device__ __forceinline__ static void jac(const double* y, const double* k, double hb, M* a) {
     (void)k;
@@ -160,28 +207,44 @@ struct CoopNet {
     const double* kk = b + Y::K;
     const double* ya = b + Y::YA;
     const int row = l < NB ? l : 0;
-#pragma unroll
-    for (int jj = 0; jj < NB; ++jj) {
+    if constexpr (LROWS) {
+      // the row's contributions in one pass (lane-major tables, in the
+      // order of its entries, then term order); entries without any keep
+      // 1 on the diagonal and 0 elsewhere
+      MR* A = rows();
+      for (int jj = 0; jj < NB; ++jj) A[jj * RS + l] = (MR)(jj == row ? 1.0 : 0.0);
+      const int nc = Base::JL_N[row];
       double acc = 0.0;
-      bool has = false;
-      const int c0 = Base::JT_OFF[row * NB + jj], c1 = Base::JT_OFF[row * NB + jj + 1];
-      for (int c = c0; c < c1; ++c) {
-        const int r = Base::JT_RATE[c];
-        const int nr = Base::JT_NR[c];
-        const unsigned char* fr = Base::JT_R + 3 * c;
-        double m;
-        int q = 0;
-        if (r >= 0) m = kk[r];
-        else if (nr > 0) { m = ya[fr[0]]; q = 1; }
-        else m = 1.0;
-        for (; q < nr; ++q) m = m * ya[fr[q]];
-        const double v = Base::JT_CE[c] * m;
-        acc = has ? acc + v : v;
-        has = true;
+      int cur = -1;
+      for (int q = 0; q < Base::CO_MAXJ; ++q) {
+        if (q < nc) {
+          const unsigned w = Base::JL_W[q * NB + row];
+          const unsigned fw = Base::JL_F[q * NB + row];
+          const int r = (int)(short)(w & 0xffffu);
+          const int nr = (int)((w >> 16) & 0xffu);
+          const int jj = (int)(w >> 24);
+          double m;
+          int p0 = 0;
+          if (r >= 0) m = kk[r];
+          else if (nr > 0) { m = ya[fw & 0xffu]; p0 = 1; }
+          else m = 1.0;
+#pragma unroll
+          for (int p = 0; p < 3; ++p)
+            if (p >= p0 && p < nr) m = m * ya[(fw >> (8 * p)) & 0xffu];
+          const double v = Base::JL_C[q * NB + row] * m;
+          if (jj != cur) {
+            if (cur >= 0) A[cur * RS + l] = (MR)pfinal(row, cur, acc, hb);
+            acc = v;
+            cur = jj;
+          } else {
+            acc = acc + v;
+          }
+        }
       }
-      const bool diag = row == jj;
-      const double v = has ? (diag ? 1.0 - hb * acc : -(hb * acc)) : (diag ? 1.0 : 0.0);
-      a[jj] = (M)v;
+      if (cur >= 0) A[cur * RS + l] = (MR)pfinal(row, cur, acc, hb);
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj) a[jj] = (M)pentry(row, jj, kk, ya, hb);
     }
     odv_wave_sync();
   }
@@ -191,8 +254,54 @@ struct CoopNet {
   // U(k,j) = A(k,j) / L(k,k), lane k keeps row k of U, and every lane i > k
   // with L(i,k) in the pattern updates A(i,j) -= L(i,k) U(k,j) over row k's
   // pattern (per entry, in increasing k: the oracle's sequence).
+  // (rows in LDS: the same steps, with row k read in place -- entry (k, j)
+  // at rows()[j*RS + k] -- so no row is published; step k's updates run over
+  // row k's pattern bits in batches of 8 entries, loads before stores)
   template <class M>
   __device__ __forceinline__ static void lu(M* a) {
+    if constexpr (LROWS) {
+      (void)a;
+      lu_rows();
+    } else {
+      lu_regs(a);
+    }
+  }
+  __device__ __forceinline__ static void lu_rows() {
+    MR* A = rows();
+    const int l = coop_l<G>();
+    const unsigned long long mine = l < NB ? Base::LU_MASK[l] : 0ull;
+    for (int k = 0; k < NB; ++k) {
+      const unsigned long long rowk = Base::LU_MASK[k];
+      const MR lkk = A[k * RS + k];
+      if (l > k && l < NB && ((rowk >> l) & 1ull)) A[l * RS + k] = A[l * RS + k] / lkk;
+      odv_wave_sync();
+      if (l > k && ((mine >> k) & 1ull)) {
+        const MR lik = A[k * RS + l];
+        unsigned long long m = rowk & ~((2ull << k) - 1ull);   // row k's pattern right of k (k = 63: none)
+        while (m) {
+          int js[8];
+          MR u[8], v[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            js[q] = m ? (int)__builtin_ctzll(m) : -1;
+            m = m ? (m & (m - 1ull)) : 0ull;
+          }
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (js[q] >= 0) {
+              u[q] = A[js[q] * RS + k];
+              v[q] = A[js[q] * RS + l];
+            }
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (js[q] >= 0) A[js[q] * RS + l] = v[q] - lik * u[q];
+        }
+      }
+      odv_wave_sync();
+    }
+  }
+  template <class M>
+  __device__ __forceinline__ static void lu_regs(M* a) {
     double* b = blk();
     const int l = coop_l<G>();
     const int base = coop_base<G>();
@@ -231,6 +340,23 @@ struct CoopNet {
     const int base = coop_base<G>();
     const unsigned long long mine = l < NB ? Base::LU_MASK[l] : 0ull;
     V xp = (V)__shfl((double)x[0], base + (l < NB ? (int)Base::PERM[l] : 0), 64);
+    if constexpr (LROWS) {
+      (void)a;
+      const MR* A = rows();
+#pragma unroll 8
+      for (int kk = 0; kk < NB; ++kk) {
+        if (l == kk) xp = xp / (V)A[kk * RS + kk];
+        const V mult = (V)coop_bcast<G>((double)xp, base, kk);
+        if (l > kk && ((mine >> kk) & 1ull)) xp = xp - mult * (V)A[kk * RS + l];
+      }
+#pragma unroll 8
+      for (int kk = NB - 1; kk >= 0; --kk) {
+        const V mult = (V)coop_bcast<G>((double)xp, base, kk);
+        if (l < kk && ((mine >> kk) & 1ull)) xp = xp - mult * (V)A[kk * RS + l];
+      }
+      x[0] = (V)__shfl((double)xp, base + (l < NB ? (int)Base::IPERM[l] : 0), 64);
+      return;
+    }
 #pragma unroll
     for (int kk = 0; kk < NB; ++kk) {
       if (l == kk) xp = xp / (V)a[kk];
@@ -248,8 +374,8 @@ struct CoopNet {
   __device__ __forceinline__ static void solve_r(const M* a, V* x) { solve<M, V>(a, x); }
 };
 
-template <class B, int G>
-struct SpeciesMap<CoopNet<B, G>> {
+template <class B, int G, class MR>
+struct SpeciesMap<CoopNet<B, G, MR>> {
   static constexpr int NACT_ALL = B::NACT;
   static ODV_HD int all(int a) { return B::ACT[a]; }
   __device__ static int lane(int) {
@@ -287,8 +413,37 @@ struct CoopGroup {
   }
 };
 
+// Where a network's cooperative kernel keeps P: one-wavefront groups (more
+// than 32 species) keep the rows in LDS (16.6 KB of FP32 rows per wavefront,
+// 8 one-wavefront workgroups per CU; 33.3 KB of FP64 rows, 4 per CU), so a
+// lane's registers hold no row, the Jacobian is one pass over a row's
+// contributions and the factorisation and solves run as loops rather than
+// unrolled code; narrower groups keep their short rows in registers.
+// Measured, SYNTH64, 65,536 cells, ms per launch, rows in registers / in
+// LDS: mixed 198 / 138, FP64 283 / 253 (same step and iteration counts;
+// in registers the kernels spilled 1.3 KB per lane and ran 36k
+// instructions of unrolled code).
+template <class Net, class M>
+struct CoopKind {
+  static constexpr int G = CoopWidth<Net::NACT>::G;
+  static constexpr bool LROWS = G == 64;
+  using CN = CoopNet<Net, G, typename std::conditional<LROWS, M, void>::type>;
+  static constexpr int BLOCK = LROWS ? 64 : COOP_BLOCK;   // lanes per workgroup
+  static constexpr int LDS_BYTES = (BLOCK / G) * CN::STRIDE * 8;
+};
+
 template <class Net, class M, bool RCP>
-using CoopLane = Lane<Cfg<CoopNet<Net, CoopWidth<Net::NACT>::G>, M, RCP, 0, false>>;
+using CoopLane = Lane<Cfg<typename CoopKind<Net, M>::CN, M, RCP, 0, false>>;
+
+template <class Net, class M>
+constexpr int coop_lds_bytes() {
+  return CoopKind<Net, M>::LDS_BYTES;
+}
+
+template <class Net, class M>
+constexpr int coop_block() {
+  return CoopKind<Net, M>::BLOCK;
+}
 
 // Warm restart of the cooperative kernel (no reference counterpart; the
 // semantics of the per-lane kernels' Hist): cell i's record at
@@ -362,8 +517,13 @@ __device__ inline bool coop_hist_resume(const KArgs& a, long long i, L& s, const
   s.set_eo(0, h[H::EO + l]);
   s.en[0] = 0.0;
   s.fz = (int)h[H::FZ + l];
+  if constexpr (L::Net::LROWS) {
+    typename L::Net::MR* A = L::Net::rows();
+    for (int j = 0; j < H::NB; ++j) A[j * L::Net::RS + l] = (typename L::Net::MR)h[H::A + j * G + l];
+  } else {
 #pragma unroll
-  for (int j = 0; j < H::NB; ++j) s.a[j] = (M)h[H::A + j * G + l];
+    for (int j = 0; j < H::NB; ++j) s.a[j] = (M)h[H::A + j * G + l];
+  }
   s.kl[0] = 0.0;   // the rates live in the group's LDS block
   double* bw = const_cast<double*>(b);
   for (int r = l; r < Net::NRATE; r += G) bw[Y::K + r] = h[H::K + r];
@@ -394,8 +554,13 @@ __device__ inline void coop_hist_save(const KArgs& a, long long i, const L& s, d
   h[H::W + l] = s.w(0);
   h[H::EO + l] = s.eo(0);
   h[H::FZ + l] = (double)s.fz;
+  if constexpr (L::Net::LROWS) {
+    const typename L::Net::MR* A = L::Net::rows();
+    for (int j = 0; j < H::NB; ++j) h[H::A + j * G + l] = (double)A[j * L::Net::RS + l];
+  } else {
 #pragma unroll
-  for (int j = 0; j < H::NB; ++j) h[H::A + j * G + l] = (double)s.a[j];
+    for (int j = 0; j < H::NB; ++j) h[H::A + j * G + l] = (double)s.a[j];
+  }
   if (l == 0) {
     h[H::TSTOP] = s.tstop;
     h[H::T] = s.t;
@@ -422,26 +587,26 @@ __device__ inline void coop_hist_save(const KArgs& a, long long i, const L& s, d
 // group-wide decision is a group vote, so a group's lanes follow one control
 // path. Each group's loop ends: every cell is bounded by max_attempts.
 // The network functions above are inlined (__forceinline__): called out of
-// line, a lane's P row (passed by pointer) lives in scratch memory and every
-// factorization step goes through it (SYNTH64: 1.9e12 B of HBM traffic per
-// FP64 launch). Inlined, the row stays in registers, and the kernel is
-// compiled for COOP_WAVES_F64 / COOP_WAVES_F32 wavefronts per SIMD (512 /
-// waves registers per lane; the FP64 row alone is 128 registers, the FP32
-// one 64). Measured, SYNTH64, 65,536 cells, ms per launch at 1 / 2 / 3 / 4 /
-// 6 wavefronts per SIMD: FP64 312 / 276 / 386 / 464 / 764, mixed 307 / 212 /
-// 200 / 236 / 297 (out of line, the best were 495 and 270). Narrower groups
-// (<= 32 species) keep the compiler's own budget: their rows are short.
-constexpr int COOP_WAVES_F64 = 2;
-constexpr int COOP_WAVES_F32 = 3;
+// line, a lane's register row (passed by pointer) lives in scratch memory.
+// Narrower groups (<= 32 species) keep the compiler's own register budget:
+// their rows are short. One-wavefront groups keep the rows in LDS
+// (CoopKind), which admits 2 (FP32 rows) or 1 (FP64) wavefronts per SIMD;
+// they are compiled for a 256-register budget.
+constexpr int COOP_WAVES_LROWS = 2;
+
+template <class Net, class M>
+constexpr int coop_waves() {
+  return CoopKind<Net, M>::LROWS ? COOP_WAVES_LROWS : 1;
+}
 
 // HIST: the variant that continues cells from a warm-restart history and
 // saves it (for networks that only this kernel runs)
 template <class Net, class M, bool RCP, int HIST = 0>
-__global__ void __launch_bounds__(COOP_BLOCK)
-__attribute__((amdgpu_waves_per_eu(CoopWidth<Net::NACT>::G < 64 ? 1 : (sizeof(M) == 8 ? COOP_WAVES_F64 : COOP_WAVES_F32))))
+__global__ void __launch_bounds__((coop_block<Net, M>()))
+__attribute__((amdgpu_waves_per_eu(coop_waves<Net, M>())))
 odv_k_coop(KArgs a) {
   constexpr int G = CoopWidth<Net::NACT>::G;
-  using CN = CoopNet<Net, G>;
+  using CN = typename CoopKind<Net, M>::CN;
   using L = CoopLane<Net, M, RCP>;
   using Y = CoopLayout<Net>;
   const int l = coop_l<G>();
@@ -510,10 +675,6 @@ odv_k_coop(KArgs a) {
   }
 }
 
-template <class Net>
-constexpr int coop_lds_bytes() {
-  return (COOP_BLOCK / CoopWidth<Net::NACT>::G) * CoopLayout<Net>::WORDS * 8;
-}
 
 }  // namespace ODV_VARIANT_NS
 }  // namespace odv

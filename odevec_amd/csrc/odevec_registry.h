@@ -115,7 +115,7 @@ inline hipError_t plan_t(long long ncells, int mode, int block, int grid, int gr
       return hipErrorInvalidValue;
     } else {
       // persistent groups: one wave of resident workgroups (CUs x occupancy)
-      constexpr int GPB = COOP_BLOCK / CoopWidth<Net::NACT>::G;
+      constexpr int GPB = coop_block<Net, M>() / CoopWidth<Net::NACT>::G;
       long long g = grid;
       if (g <= 0) {
         int dev = 0, ncu = 0, nb = 0;
@@ -123,13 +123,14 @@ inline hipError_t plan_t(long long ncells, int mode, int block, int grid, int gr
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e == hipSuccess)
           e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &nb, reinterpret_cast<const void*>(odv_k_coop<Net, M, RCP>), COOP_BLOCK, coop_lds_bytes<Net>());
+              &nb, reinterpret_cast<const void*>(odv_k_coop<Net, M, RCP>), coop_block<Net, M>(),
+              coop_lds_bytes<Net, M>());
         if (e != hipSuccess) return e;
         g = (long long)ncu * (nb > 0 ? nb : 1);
       }
       const long long need = (ncells + GPB - 1) / GPB;
       if (g > need) g = need;
-      p->block = COOP_BLOCK;
+      p->block = coop_block<Net, M>();
       p->grid = g;
       p->ws_doubles = 0;
       return hipSuccess;
@@ -177,18 +178,17 @@ inline hipError_t launch_t(const KArgs& a, int mode, const Plan& p, hipStream_t 
   if (p.ws_doubles > 0 && !a.ws) return hipErrorInvalidValue;
   if (mode == MODE_COOP) {
     if constexpr (Net::COOP) {
-      if (p.block != COOP_BLOCK) return hipErrorInvalidValue;
-      constexpr long long GPB = COOP_BLOCK / CoopWidth<Net::NACT>::G;
+      constexpr int CB = coop_block<Net, M>(), LDS = coop_lds_bytes<Net, M>();
+      if (p.block != CB) return hipErrorInvalidValue;
+      constexpr long long GPB = CB / CoopWidth<Net::NACT>::G;
       // a history in this mode only for networks no per-lane kernel runs
       // (a network has one history layout)
       if (a.hist && Net::LANE) return hipErrorInvalidValue;
       hipLaunchKernelGGL(k_setq, dim3(1), dim3(1), 0, st, a.queue, (unsigned long long)(p.grid * GPB));
       if (a.hist)
-        hipLaunchKernelGGL((odv_k_coop<Net, M, RCP, 1>), dim3((unsigned)p.grid), dim3(COOP_BLOCK),
-                           coop_lds_bytes<Net>(), st, a);
+        hipLaunchKernelGGL((odv_k_coop<Net, M, RCP, 1>), dim3((unsigned)p.grid), dim3(CB), LDS, st, a);
       else
-        hipLaunchKernelGGL((odv_k_coop<Net, M, RCP>), dim3((unsigned)p.grid), dim3(COOP_BLOCK), coop_lds_bytes<Net>(),
-                           st, a);
+        hipLaunchKernelGGL((odv_k_coop<Net, M, RCP>), dim3((unsigned)p.grid), dim3(CB), LDS, st, a);
       return hipGetLastError();
     }
     return hipErrorInvalidValue;
@@ -311,7 +311,8 @@ hipError_t launch_net(const KArgs& a, int mode, int prec, const Plan& p, hipStre
 template <class Net, class M, bool RCP>
 inline hipError_t info_t(int mode, int block, KernelInfo* out) {
   if (mode == MODE_COOP) {
-    if constexpr (Net::COOP) return kinfo(odv_k_coop<Net, M, RCP>, COOP_BLOCK, out, coop_lds_bytes<Net>());
+    if constexpr (Net::COOP)
+      return kinfo(odv_k_coop<Net, M, RCP>, coop_block<Net, M>(), out, coop_lds_bytes<Net, M>());
     return hipErrorInvalidValue;
   }
   if constexpr (!Net::LANE) {

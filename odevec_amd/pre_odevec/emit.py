@@ -357,7 +357,14 @@ def _emit_coop(st: Structure) -> List[str]:
       Jacobian, per permuted P entry (i, jj) (row-major, offsets JT_OFF): the
         contributions in term order, v = (coef*e) * (k_rate * reduced monomial);
       PERM / IPERM: permuted position <-> species; LU_MASK: bit jj of row i set
-        when (i, jj) is in the factors' pattern (with fill)."""
+        when (i, jj) is in the factors' pattern (with fill);
+      the same RHS terms and Jacobian contributions lane-major (RL_*, JL_*):
+        entry q of row c at [q*n + c] (q < RL_N[c] / JL_N[c]; up to CO_MAXT /
+        CO_MAXJ), so the lanes of a group read consecutive words; a term's
+        rate index and factor count in one word (RL_W: rate & 0xffff | nf << 16),
+        its factors in another (RL_F: byte p = factor p); a contribution's
+        also carries its permuted column (JL_W: ... | col << 24), in the
+        order of the permuted row's entries, then term order."""
     n = st.neq
     rows = [[] for _ in range(n)]
     for t in st.terms:
@@ -405,8 +412,31 @@ def _emit_coop(st: Structure) -> List[str]:
                 m |= 1 << jj
         mask.append("0x%xull" % m)
     maxr = max((len(r) for r in rows), default=0)
+    # lane-major copies
+    rl_n = [len(rows[c]) for c in range(n)]
+    rl_w, rl_f, rl_c = [0] * max(maxr * n, 1), [0] * max(maxr * n, 1), ["0.0"] * max(maxr * n, 1)
+    for c in range(n):
+        for q, t in enumerate(rows[c]):
+            assert len(t.factors) <= 4
+            rate = -1 if t.rate is None or t.rate < 0 else int(t.rate)
+            i = q * n + c
+            rl_w[i] = (rate & 0xffff) | (len(t.factors) << 16)
+            rl_f[i] = sum(int(f) << (8 * p) for p, f in enumerate(t.factors))
+            rl_c[i] = _lit(float(t.coef))
+    jl_rows = [[(jj, rate, ce, red) for jj in range(n) for rate, ce, red in ent[i][jj]] for i in range(n)]
+    maxj = max((len(r) for r in jl_rows), default=0)
+    jl_n = [len(r) for r in jl_rows]
+    jl_w, jl_f, jl_c = [0] * max(maxj * n, 1), [0] * max(maxj * n, 1), ["0.0"] * max(maxj * n, 1)
+    for i in range(n):
+        for q, (jj, rate, ce, red) in enumerate(jl_rows[i]):
+            assert len(red) <= 3 and jj < 256
+            k = q * n + i
+            jl_w[k] = (rate & 0xffff) | (len(red) << 16) | (jj << 24)
+            jl_f[k] = sum(int(f) << (8 * p) for p, f in enumerate(red))
+            jl_c[k] = _lit(ce)
     out = ["", "  // cooperative kernels (species on lanes): tables in the oracle's order",
-           "  static constexpr int CO_MAXT = %d;   // most RHS terms of one species" % maxr]
+           "  static constexpr int CO_MAXT = %d;   // most RHS terms of one species" % maxr,
+           "  static constexpr int CO_MAXJ = %d;   // most Jacobian contributions of one row" % maxj]
     out += _arr("short", "RT_OFF", rt_off)
     out += _arr("short", "RT_RATE", rt_rate)
     out += _arr("double", "RT_COEF", rt_coef, 8)
@@ -420,6 +450,14 @@ def _emit_coop(st: Structure) -> List[str]:
     out += _arr("unsigned char", "PERM", perm)
     out += _arr("unsigned char", "IPERM", ip)
     out += _arr("unsigned long long", "LU_MASK", mask, 6)
+    out += _arr("unsigned char", "RL_N", rl_n)
+    out += _arr("unsigned", "RL_W", ["%du" % v for v in rl_w], 12)
+    out += _arr("unsigned", "RL_F", ["%du" % v for v in rl_f], 12)
+    out += _arr("double", "RL_C", rl_c, 8)
+    out += _arr("unsigned char", "JL_N", jl_n)
+    out += _arr("unsigned", "JL_W", ["%du" % v for v in jl_w], 12)
+    out += _arr("unsigned", "JL_F", ["%du" % v for v in jl_f], 12)
+    out += _arr("double", "JL_C", jl_c, 8)
     return out
 
 
