@@ -100,7 +100,9 @@ struct CoopNet {
   using MR = typename std::conditional<LROWS, MR_, float>::type;
   static constexpr int RS = G + 1;
   // a group's LDS block (doubles)
-  static constexpr int STRIDE = LROWS ? Y::UR + (NB * RS * (int)sizeof(MR) + 7) / 8 : Y::WORDS;
+  // (NB + 1 columns: column NB is a spare that the factorisation's batches
+  // use for their empty slots)
+  static constexpr int STRIDE = LROWS ? Y::UR + ((NB + 1) * RS * (int)sizeof(MR) + 7) / 8 : Y::WORDS;
   static constexpr int NNZ = LROWS ? 1 : NB;   // a lane's row of P / the factors in registers
   static constexpr bool FD_JAC = false;     // the symbolic Jacobian (tables below)
   static constexpr int NINV = Net_::NINV;   // the cell's linear invariants (mixed form's convergence test)
@@ -143,27 +145,30 @@ struct CoopNet {
     const double* kk = b + Y::K;
     const double* ya = b + Y::YA;
     // the lane-major tables: term q of every lane's species in consecutive words
+    // (every lane runs CO_MAXT slots without branches; a slot past its
+    // species' count reads the zero padding and is not selected)
     double acc = 0.0;
-    if (l < NB) {
-      const int nt = Base::RL_N[l];
-      for (int q = 0; q < Base::CO_MAXT; ++q) {
-        if (q < nt) {
-          const unsigned w = Base::RL_W[q * NB + l];
-          const unsigned fw = Base::RL_F[q * NB + l];
-          const int r = (int)(short)(w & 0xffffu);
-          const int nf = (int)(w >> 16);
-          double m;
-          int p0 = 0;
-          if (r >= 0) m = kk[r];
-          else if (nf > 0) { m = ya[fw & 0xffu]; p0 = 1; }
-          else m = 1.0;
+    const int row = l < NB ? l : 0;
+    const int nt = l < NB ? (int)Base::RL_N[row] : 0;
+    for (int q = 0; q < Base::CO_MAXT; ++q) {
+      const unsigned w = Base::RL_W[q * NB + row];
+      const unsigned fw = Base::RL_F[q * NB + row];
+      const double ce = Base::RL_C[q * NB + row];
+      const int r = (int)(short)(w & 0xffffu);
+      const int nf = (int)(w >> 16);
+      const double kr = kk[r >= 0 ? r : 0];
+      const double y0 = ya[fw & 0xffu];
+      double m = r >= 0 ? kr : (nf > 0 ? y0 : 1.0);
+      const int p0 = (r < 0 && nf > 0) ? 1 : 0;
 #pragma unroll
-          for (int p = 0; p < 4; ++p)
-            if (p >= p0 && p < nf) m = m * ya[(fw >> (8 * p)) & 0xffu];
-          const double v = Base::RL_C[q * NB + l] * m;
-          acc = (q == 0) ? v : acc + v;
-        }
+      for (int p = 0; p < 4; ++p) {
+        const double yp = ya[(fw >> (8 * p)) & 0xffu];
+        const double mp = m * yp;
+        m = (p >= p0 && p < nf) ? mp : m;
       }
+      const double v = ce * m;
+      const double sum = acc + v;
+      acc = q < nt ? ((q == 0) ? v : sum) : acc;
     }
     f[0] = acc;
     odv_wave_sync();
@@ -213,33 +218,38 @@ struct CoopNet {
       // 1 on the diagonal and 0 elsewhere
       MR* A = rows();
       for (int jj = 0; jj < NB; ++jj) A[jj * RS + l] = (MR)(jj == row ? 1.0 : 0.0);
+      // every lane runs CO_MAXJ slots without branches: a slot past the
+      // row's count reads the tables' zero padding and changes nothing (its
+      // results are not selected; a store it does not need goes to the
+      // spare column NB), so the slots' loads can issue ahead
       const int nc = Base::JL_N[row];
       double acc = 0.0;
       int cur = -1;
       for (int q = 0; q < Base::CO_MAXJ; ++q) {
-        if (q < nc) {
-          const unsigned w = Base::JL_W[q * NB + row];
-          const unsigned fw = Base::JL_F[q * NB + row];
-          const int r = (int)(short)(w & 0xffffu);
-          const int nr = (int)((w >> 16) & 0xffu);
-          const int jj = (int)(w >> 24);
-          double m;
-          int p0 = 0;
-          if (r >= 0) m = kk[r];
-          else if (nr > 0) { m = ya[fw & 0xffu]; p0 = 1; }
-          else m = 1.0;
+        const bool valid = q < nc;
+        const unsigned w = Base::JL_W[q * NB + row];
+        const unsigned fw = Base::JL_F[q * NB + row];
+        const double ce = Base::JL_C[q * NB + row];
+        const int r = (int)(short)(w & 0xffffu);
+        const int nr = (int)((w >> 16) & 0xffu);
+        const int jj = (int)(w >> 24);
+        const double kr = kk[r >= 0 ? r : 0];
+        const double y0 = ya[fw & 0xffu];
+        double m = r >= 0 ? kr : (nr > 0 ? y0 : 1.0);
+        const int p0 = (r < 0 && nr > 0) ? 1 : 0;
 #pragma unroll
-          for (int p = 0; p < 3; ++p)
-            if (p >= p0 && p < nr) m = m * ya[(fw >> (8 * p)) & 0xffu];
-          const double v = Base::JL_C[q * NB + row] * m;
-          if (jj != cur) {
-            if (cur >= 0) A[cur * RS + l] = (MR)pfinal(row, cur, acc, hb);
-            acc = v;
-            cur = jj;
-          } else {
-            acc = acc + v;
-          }
+        for (int p = 0; p < 3; ++p) {
+          const double yp = ya[(fw >> (8 * p)) & 0xffu];
+          const double mp = m * yp;
+          m = (p >= p0 && p < nr) ? mp : m;
         }
+        const double v = ce * m;
+        const bool fresh = valid && jj != cur;
+        const int to = (fresh && cur >= 0) ? cur : NB;
+        A[to * RS + l] = (MR)pfinal(row, cur, acc, hb);
+        const double sum = acc + v;
+        acc = fresh ? v : (valid ? sum : acc);
+        cur = fresh ? jj : cur;
       }
       if (cur >= 0) A[cur * RS + l] = (MR)pfinal(row, cur, acc, hb);
     } else {
@@ -278,23 +288,23 @@ struct CoopNet {
       if (l > k && ((mine >> k) & 1ull)) {
         const MR lik = A[k * RS + l];
         unsigned long long m = rowk & ~((2ull << k) - 1ull);   // row k's pattern right of k (k = 63: none)
+        // batches of 8 entries without branches (an empty slot works on the
+        // spare column NB), so a batch's loads issue together
         while (m) {
           int js[8];
           MR u[8], v[8];
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
-            js[q] = m ? (int)__builtin_ctzll(m) : -1;
+            js[q] = m ? (int)__builtin_ctzll(m) : NB;
             m = m ? (m & (m - 1ull)) : 0ull;
           }
 #pragma unroll
-          for (int q = 0; q < 8; ++q)
-            if (js[q] >= 0) {
-              u[q] = A[js[q] * RS + k];
-              v[q] = A[js[q] * RS + l];
-            }
+          for (int q = 0; q < 8; ++q) {
+            u[q] = A[js[q] * RS + k];
+            v[q] = A[js[q] * RS + l];
+          }
 #pragma unroll
-          for (int q = 0; q < 8; ++q)
-            if (js[q] >= 0) A[js[q] * RS + l] = v[q] - lik * u[q];
+          for (int q = 0; q < 8; ++q) A[js[q] * RS + l] = v[q] - lik * u[q];
         }
       }
       odv_wave_sync();
@@ -343,16 +353,24 @@ struct CoopNet {
     if constexpr (LROWS) {
       (void)a;
       const MR* A = rows();
+      // loads outside the lanes' conditions (selects, no branches), so the
+      // unrolled steps' loads can issue ahead of the dependent chain
 #pragma unroll 8
       for (int kk = 0; kk < NB; ++kk) {
-        if (l == kk) xp = xp / (V)A[kk * RS + kk];
+        const V dk = (V)A[kk * RS + kk];
+        const V lk = (V)A[kk * RS + l];
+        const V xd = xp / dk;
+        xp = (l == kk) ? xd : xp;
         const V mult = (V)coop_bcast<G>((double)xp, base, kk);
-        if (l > kk && ((mine >> kk) & 1ull)) xp = xp - mult * (V)A[kk * RS + l];
+        const V xu = xp - mult * lk;
+        xp = (l > kk && ((mine >> kk) & 1ull)) ? xu : xp;
       }
 #pragma unroll 8
       for (int kk = NB - 1; kk >= 0; --kk) {
+        const V uk = (V)A[kk * RS + l];
         const V mult = (V)coop_bcast<G>((double)xp, base, kk);
-        if (l < kk && ((mine >> kk) & 1ull)) xp = xp - mult * (V)A[kk * RS + l];
+        const V xu = xp - mult * uk;
+        xp = (l < kk && ((mine >> kk) & 1ull)) ? xu : xp;
       }
       x[0] = (V)__shfl((double)xp, base + (l < NB ? (int)Base::IPERM[l] : 0), 64);
       return;
