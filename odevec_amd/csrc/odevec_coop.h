@@ -432,15 +432,16 @@ struct CoopGroup {
 };
 
 // Where a network's cooperative kernel keeps P: one-wavefront groups (more
-// than 32 species) keep the rows in LDS (16.6 KB of FP32 rows per wavefront,
-// 8 one-wavefront workgroups per CU; 33.3 KB of FP64 rows, 4 per CU), so a
+// than 32 species) keep the rows in LDS (16.9 KB of FP32 rows per wavefront,
+// 8 one-wavefront workgroups per CU; 33.8 KB of FP64 rows, 4 per CU), so a
 // lane's registers hold no row, the Jacobian is one pass over a row's
 // contributions and the factorisation and solves run as loops rather than
 // unrolled code; narrower groups keep their short rows in registers.
 // Measured, SYNTH64, 65,536 cells, ms per launch, rows in registers / in
-// LDS: mixed 198 / 138, FP64 283 / 253 (same step and iteration counts;
-// in registers the kernels spilled 1.3 KB per lane and ran 36k
-// instructions of unrolled code).
+// LDS: mixed 198 / 98, FP64 283 / 185 (same step and iteration counts; in
+// registers the kernels spilled 1.3 KB per lane and ran 36k instructions of
+// unrolled code; in LDS, the loops' slots are branch-free so that their
+// loads issue together: with a branch per slot, 137 / 250).
 template <class Net, class M>
 struct CoopKind {
   static constexpr int G = CoopWidth<Net::NACT>::G;
