@@ -200,3 +200,68 @@ def test_linear_invariants():
             assert abs(c @ f) <= 1e-12 * (np.abs(c) @ terms)
     hdr = emit.emit_header(S.analyse(N.load("PRIMORDIAL"), "CMK", "CSC"))
     assert "static constexpr int NINV = 4;" in hdr
+
+
+def _coop_tables(name, ordering):
+    """The cooperative-kernel tables the generator emits, as lists of the
+    literals in the header (unsigned suffixes dropped)."""
+    st = S.reduce_frozen(S.analyse(N.load(name), ordering, "CSC"))
+    out, cur = {}, None
+    for ln in emit._emit_coop(st):
+        t = ln.strip()
+        if t.startswith("static constexpr") and t.endswith("{"):
+            cur = t.split("[")[0].split()[-1]
+            out[cur] = []
+        elif t == "};":
+            cur = None
+        elif cur is not None:
+            out[cur] += [v.strip().rstrip("u") for v in t.split(",") if v.strip()]
+    return out
+
+
+@pytest.mark.parametrize("name,ordering", [("SYNTH64", "FF"), ("PRIMORDIAL", "CMK"), ("ROBER", "None")])
+def test_coop_lane_major_tables_restate_the_row_major_ones(name, ordering):
+    """RL_* / JL_* (term q of row c at [q*n + c], packed words) hold exactly
+    the RHS terms and Jacobian contributions of RT_* / JT_*, row by row in
+    the same order, with zero padding past each row's count."""
+    T = _coop_tables(name, ordering)
+    n = len(T["RL_N"])
+    rt_off = [int(v) for v in T["RT_OFF"]]
+    maxt = max(int(v) for v in T["RL_N"])
+    assert len(rt_off) == n + 1
+    for c in range(n):
+        cnt = rt_off[c + 1] - rt_off[c]
+        assert int(T["RL_N"][c]) == cnt
+        for q in range(maxt):
+            i = q * n + c
+            w, fw, coef = int(T["RL_W"][i]), int(T["RL_F"][i]), T["RL_C"][i]
+            if q >= cnt:
+                assert w == 0 and fw == 0 and float(coef) == 0.0
+                continue
+            t = rt_off[c] + q
+            rate = w & 0xffff
+            rate = rate - 0x10000 if rate >= 0x8000 else rate
+            nf = w >> 16
+            assert rate == int(T["RT_RATE"][t]) and nf == int(T["RT_NF"][t]), (c, q)
+            assert [(fw >> (8 * p)) & 0xff for p in range(nf)] == [int(v) for v in T["RT_F"][4 * t:4 * t + nf]]
+            assert coef == T["RT_COEF"][t]
+    jt_off = [int(v) for v in T["JT_OFF"]]
+    maxj = max(int(v) for v in T["JL_N"])
+    for r in range(n):
+        lo, hi = jt_off[r * n], jt_off[r * n + n]
+        assert int(T["JL_N"][r]) == hi - lo
+        for q in range(maxj):
+            k = q * n + r
+            w, fw, ce = int(T["JL_W"][k]), int(T["JL_F"][k]), T["JL_C"][k]
+            if q >= hi - lo:
+                assert w == 0 and fw == 0 and float(ce) == 0.0
+                continue
+            c = lo + q
+            col = w >> 24
+            assert jt_off[r * n + col] <= c < jt_off[r * n + col + 1], (r, q, col)
+            rate = w & 0xffff
+            rate = rate - 0x10000 if rate >= 0x8000 else rate
+            nr = (w >> 16) & 0xff
+            assert rate == int(T["JT_RATE"][c]) and nr == int(T["JT_NR"][c])
+            assert [(fw >> (8 * p)) & 0xff for p in range(nr)] == [int(v) for v in T["JT_R"][3 * c:3 * c + nr]]
+            assert ce == T["JT_CE"][c]
