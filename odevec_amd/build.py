@@ -35,6 +35,7 @@ BUILTIN_VARIANTS = [
     "PRIMORDIAL/NONE/DENSE",
     "PRIMORDIAL/FF/CSC",
     "SYNTH64/FF/CSC",     # 64 species: cooperative kernels only
+    "SYNTH40/FF/CSC",     # 40 species: cooperative kernels, 24 padding lanes per group
     "ROBER/NONE/DENSE/LU3",        # LUmethod 3: Jacobian by forward differences
     "PRIMORDIAL/NONE/DENSE/LU3",
     "ROBER/NONE/DENSE/LU2",        # LUmethod 2: L and U in closed form

@@ -88,7 +88,7 @@ def main(argv: Optional[List[str]] = None) -> int:
     p = argparse.ArgumentParser(
         description="Preprocessor for the MI355X OdeVec integrator: emits HIP device code "
                     "for a network's RHS, Jacobian and sparse LU.")
-    p.add_argument("--example", default="PRIMORDIAL", help="ROBER, OREGO, PRIMORDIAL, SYNTH64 or KROME")
+    p.add_argument("--example", default="PRIMORDIAL", help="ROBER, OREGO, PRIMORDIAL, SYNTH64, SYNTH40 or KROME")
     p.add_argument("--krome_setupfile", default=None, help="KROME-style network file (Python syntax)")
     p.add_argument("--LUmethod", type=int, default=1)
     p.add_argument("--dt_min", type=float, default=float(__import__("numpy").float32(1e-10)))

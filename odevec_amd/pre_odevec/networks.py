@@ -198,14 +198,24 @@ def synth_reactions(nspec: int = 64, nreac: int = 300, seed: int = SYNTH64_SEED)
     return reactions, [float(10.0 ** v) for v in log10A], [float(v) for v in Ea]
 
 
-def load_synth64(seed: int = SYNTH64_SEED) -> Network:
-    reactions, A, Ea = synth_reactions(64, 300, seed)
-    y = _ysyms(64)
-    k = _ksyms(300)
-    rhs = mass_action_rhs(64, reactions, y, k)
-    return Network("SYNTH64", ["S%d" % i for i in range(64)], y, k, rhs,
+def load_synth(nspec: int = 64, nreac: int = 300, seed: int = SYNTH64_SEED) -> Network:
+    reactions, A, Ea = synth_reactions(nspec, nreac, seed)
+    y = _ysyms(nspec)
+    k = _ksyms(nreac)
+    rhs = mass_action_rhs(nspec, reactions, y, k)
+    return Network("SYNTH%d" % nspec, ["S%d" % i for i in range(nspec)], y, k, rhs,
                    rate_kind="arrhenius", param_names=["T"], param_defaults=[1000.0],
                    arrhenius_A=A, arrhenius_Ea=Ea, reactions=reactions)
+
+
+def load_synth64(seed: int = SYNTH64_SEED) -> Network:
+    return load_synth(64, 300, seed)
+
+
+def load_synth40(seed: int = SYNTH64_SEED) -> Network:
+    """40 species, 180 reactions: a cell fills 40 of a 64-lane group (the
+    cooperative kernel's padding lanes)."""
+    return load_synth(40, 180, seed)
 
 
 # --------------------------------------------------------------------------- KROME
@@ -247,6 +257,7 @@ BUILTIN = {
     "OREGO": load_orego,
     "PRIMORDIAL": load_primordial,
     "SYNTH64": load_synth64,
+    "SYNTH40": load_synth40,
 }
 
 

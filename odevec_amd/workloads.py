@@ -95,7 +95,8 @@ def primordial(n: int, seed: int = 3, network: str = "PRIMORDIAL/CMK/CSC", t_lo:
 def synth64(n: int, seed: int = 7, network: str = "SYNTH64/FF/CSC", t_stop: float = 1e6, rtol: float = 1e-6):
     rng = np.random.default_rng(seed)
     ntot = _logu(rng, 1.0, 1e6, n)
-    y = ntot[None, :] * np.exp(rng.uniform(np.log(1e-6), 0.0, (64, n)))
+    nsp = int(network.split("/")[0].split("+")[0][len("SYNTH"):])   # SYNTH64, SYNTH40
+    y = ntot[None, :] * np.exp(rng.uniform(np.log(1e-6), 0.0, (nsp, n)))
     T = _logu(rng, 100.0, 1e4, n)
     return dict(network=network, y=np.ascontiguousarray(y), t0=np.zeros(n), t_stop=np.full(n, float(t_stop)),
                 dt0=np.full(n, 1e-10), params=T[None, :].copy(), rtol=rtol, atol=1e-20, dt_min=1e-20)

@@ -4,6 +4,7 @@ one species per lane).
 * the +REPRO build equals the oracle bit for bit (states, status, last step,
   counters) for the CSC networks: ROBER, OREGO (groups of 4 lanes),
   PRIMORDIAL CMK and FF (16), SYNTH64 (64 species, one wavefront per cell),
+  SYNTH40 (40 species in a 64-lane group), KROME (16),
   with a group per cell and with one workgroup working through the queue;
 * the default build and the mixed form: every cell finishes and the states
   agree with the oracle's FP64 solution to a bound set by the tolerance.
@@ -61,6 +62,8 @@ CASES = {
     "PRIMORDIAL_FF": lambda: W.primordial(130, seed=34, network="PRIMORDIAL/FF/CSC"),
     "SYNTH64": lambda: W.synth64(40, seed=35),
     "KROME": lambda: W.krome_ism(120, seed=38),     # 12 species, 21 rate parameters (groups of 16)
+    # 40 species in a 64-lane group: 24 padding lanes beside rows held in LDS
+    "SYNTH40": lambda: W.synth64(60, seed=39, network="SYNTH40/FF/CSC"),
 }
 
 
@@ -140,17 +143,19 @@ def test_repro_coop_nan_state_bitwise(oracle, case):
     assert stats["attempts"] == int(cr[0].sum())
 
 
+@pytest.mark.parametrize("net", ["SYNTH64/FF/CSC", "SYNTH40/FF/CSC"])
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
 @pytest.mark.parametrize("grid", [0, 1])
-def test_coop_continue_equals_one_solve(oracle, precision, grid):
-    """Warm restart of the cooperative kernel (SYNTH64, a network no
-    per-lane kernel runs): over [-S, 0] (the first step then does not depend
+def test_coop_continue_equals_one_solve(oracle, precision, grid, net):
+    """Warm restart of the cooperative kernel (SYNTH64 and SYNTH40,
+    networks no per-lane kernel runs; SYNTH40's groups carry padding lanes,
+    whose history planes are kept too): over [-S, 0] (the first step then does not depend
     on t_stop), solve(-S -> t1) continued to 0 equals one solve bit for bit
     (states, status, last steps; attempts, steps, Newton iterations and
     Jacobians add up), and in FP64 equals the oracle's solve; a cell whose
     state changed between the calls starts cold and equals a fresh solve."""
     from odevec_amd.api import Odevec
-    w = W.synth64(48, seed=36)
+    w = W.synth64(48, seed=36, network=net)
     nid = w["network"] + "+REPRO"
     n = w["y"].shape[1]
     span = np.broadcast_to(np.asarray(w["t_stop"], dtype=np.float64), (n,)).copy()
