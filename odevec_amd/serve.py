@@ -61,6 +61,13 @@ COST_ORDER = ["PRIMORDIAL/CMK/CSC", "OREGO/NONE/DENSE", "ROBER/NONE/DENSE"]
 # networks served by the cooperative kernel (one launch per request, on the
 # same slot stream as the slot's batch launch) rather than the bundle
 COOP_NETWORKS = {"SYNTH64/FF/CSC"}
+# cooperative-kernel chunks of one request in flight at once, each on a
+# stream of its own. Measured on the bench's serving workload at 4
+# requests/s: with 2, p50/p99 45/959 ms (SYNTH64's p99 1.05 s, the per-lane
+# networks' up to 317 ms); with 1, 27/1,255 ms (1.36 s; up to 112 ms): a
+# second chunk overlaps the first one's last cells but takes the device
+# from the batch slots, so the server keeps 1
+COOP_INFLIGHT = 1
 # networks whose mixed form runs the reference's RMS corrector test (their
 # FP32 factors contract every component: the max norm only adds iterations)
 MIXED_RMS = {"SYNTH64/FF/CSC"}
@@ -143,7 +150,7 @@ class Request:
 class Server:
     def __init__(self, bundle: str = "MIX3", slots: int = 2, hbm_budget: int = 16 << 30, block: int = 256,
                  precision: Optional[dict] = None, fp64_fallback: int = 0, coop_chunk: int = 16384,
-                 quantum: int = QUANTUM):
+                 quantum: int = QUANTUM, coop_inflight: int = COOP_INFLIGHT):
         import torch
         from .api import Batch
         self.torch = torch
@@ -160,7 +167,14 @@ class Server:
                       for _ in range(slots)]
         self.quantum = int(quantum)
         # the cooperative kernel's slot: one request at a time, chunk by chunk
-        self.coop_slot = {"stream": torch.cuda.Stream(), "coop": {}, "req": None, "next": 0, "event": None}
+        # the cooperative slot keeps coop_inflight chunks of its request in
+        # flight, on streams of their own, so that one chunk's last cells can
+        # overlap the next chunk (uploads on the first stream)
+        if int(coop_inflight) < 1:
+            raise ValueError("coop_inflight must be >= 1")
+        streams = [torch.cuda.Stream() for _ in range(int(coop_inflight))]
+        self.coop_slot = {"stream": streams[0], "streams": streams, "coop": {}, "req": None, "next": 0,
+                          "inflight": [], "uploaded": None}
         self.coop_chunk = int(coop_chunk)
         self.coop_waiting: deque = deque()
         self.sessions = {}
@@ -305,9 +319,11 @@ class Server:
                 if r.params is not None:
                     d["params"] = up(r.params[:, c0:c1])
                 chunks.append(d)
+            up_ev = torch.cuda.Event()
+            up_ev.record(self.coop_slot["stream"])
         r.dev = {"chunks": chunks}
         self.in_use += r.footprint()
-        self.coop_slot["req"], self.coop_slot["next"] = r, 0
+        self.coop_slot["req"], self.coop_slot["next"], self.coop_slot["uploaded"] = r, 0, up_ev
         return True
 
     def _coop_launch(self):
@@ -317,7 +333,9 @@ class Server:
         cs = self.coop_slot
         r = cs["req"]
         d = r.dev["chunks"][cs["next"]]
-        st = cs["stream"]
+        st = cs["streams"][cs["next"] % len(cs["streams"])]
+        if st is not cs["stream"]:
+            st.wait_event(cs["uploaded"])
         h = self._coop(r.network, r.precision)
         h.RelativeTolerance, h.AbsoluteTolerance, h.MinimalTimestep = r.rtol, r.atol, r.dt_min
         h.solve_cells_async(d["n"], d["t_stop"].data_ptr(), d["y"].data_ptr(), dt0_ptr=d["dt0"].data_ptr(),
@@ -330,26 +348,34 @@ class Server:
             d["st_host"].copy_(d["status"], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(st)
-        cs["event"] = ev
+        cs["inflight"].append(ev)
         cs["next"] += 1
         self.launches += 1
 
+    def _coop_fill(self):
+        cs = self.coop_slot
+        while len(cs["inflight"]) < len(cs["streams"]) and cs["next"] < len(cs["req"].dev["chunks"]):
+            self._coop_launch()
+
     def _coop_poll(self, done: List[Request]):
         cs = self.coop_slot
-        if cs["req"] is not None and cs["event"].query():
+        if cs["req"] is not None:
+            cs["inflight"] = [ev for ev in cs["inflight"] if not ev.query()]
             r = cs["req"]
             if cs["next"] < len(r.dev["chunks"]):
-                self._coop_launch()
+                self._coop_fill()
+                return
+            if cs["inflight"]:
                 return
             r.result = np.concatenate([d["y_host"].numpy() for d in r.dev["chunks"]], axis=1)
             r.status = np.concatenate([d["st_host"].numpy() for d in r.dev["chunks"]])
             r.done = self.now()
             self.in_use -= r.footprint()
             r.dev = {}
-            cs["req"], cs["event"] = None, None
+            cs["req"], cs["uploaded"] = None, None
             done.append(r)
         if cs["req"] is None and self.coop_waiting and self._coop_admit():
-            self._coop_launch()
+            self._coop_fill()
 
     def _lane(self, slot, network, precision):
         """The slot's persistent-mode handle for (network, precision) (the
@@ -641,12 +667,14 @@ def main(argv=None) -> int:
     p.add_argument("--bundle", default="MIX3")
     p.add_argument("--quantum", type=int, default=QUANTUM, help="attempts per cell and batch launch (0: no limit)")
     p.add_argument("--coop_chunk", type=int, default=16384, help="cells per cooperative-kernel launch")
+    p.add_argument("--coop_inflight", type=int, default=COOP_INFLIGHT, help="cooperative chunks in flight at once")
     a = p.parse_args(argv)
     from . import autotune as A
     tuned = A.lookup(list(SERVE_SPANS)) or {}
     prec = tuned.get("precision", {})
     for rate in [float(x) for x in a.rates.split(",")]:
-        srv = Server(a.bundle, slots=a.slots, precision=prec, quantum=a.quantum, coop_chunk=a.coop_chunk)
+        srv = Server(a.bundle, slots=a.slots, precision=prec, quantum=a.quantum, coop_chunk=a.coop_chunk,
+                     coop_inflight=a.coop_inflight)
         res = run_trace(srv, make_trace(rate, a.seconds))
         srv.close()
         print(json.dumps(dict(rate_per_s=rate, **res)), flush=True)

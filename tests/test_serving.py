@@ -68,9 +68,11 @@ def test_serving_repro_equals_oracle(oracle):
     nets = ["ROBER/NONE/DENSE", "OREGO/NONE/DENSE", "PRIMORDIAL/CMK/CSC", "SYNTH64/FF/CSC+REPRO"]
     trace = S.make_trace(400.0, 0.04, seed=14, networks=nets, lo=1e3, hi=8e3)
     assert len(trace) >= 6 and len({r.network for r in trace}) == 4
-    # SYNTH64 through the cooperative slot (FP64: its default is mixed), chunks of 2048 cells;
-    # the batch slots in quantum launches of 48 attempts per cell (requests join running work)
-    srv = S.Server("MIX3+REPRO", slots=2, coop_chunk=2048, precision={"SYNTH64/FF/CSC+REPRO": 0}, quantum=48)
+    # SYNTH64 through the cooperative slot (FP64: its default is mixed), chunks of 2048 cells,
+    # two in flight; the batch slots in quantum launches of 48 attempts per cell (requests
+    # join running work)
+    srv = S.Server("MIX3+REPRO", slots=2, coop_chunk=2048, precision={"SYNTH64/FF/CSC+REPRO": 0}, quantum=48,
+                   coop_inflight=2)
     S.run_trace(srv, trace)
     srv.close()
     print("requests", len(trace), "launches", srv.launches, flush=True)
